@@ -1,0 +1,356 @@
+"""drawlab_amd -- the MI355X (gfx950) rendering backend for DrawLab.
+
+The product is the native library ``lib/libdrawlab_hip.so`` (hand-written HIP
+kernels + the C ABI declared in ``include/dl_hip.h``) and its C++ host layer.
+This package is a thin ctypes binding used by tests, benches and Python
+callers.  There is no CPU fallback: without the built library (or without a
+GPU) every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .scene import (Bsdf, Camera, EnvMap, Mesh, RFilter, SceneDesc, BSDF_DIFFUSE, BSDF_MIRROR, BSDF_DIELECTRIC,
+                    BSDF_MICROFACET, BSDF_ANISO_GGX, BSDF_CONDUCTOR, FILTER_GAUSSIAN, FILTER_BOX, FILTER_TENT,
+                    FILTER_MITCHELL)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libdrawlab_hip.so")
+REPO_ROOT = os.path.dirname(_HERE)
+
+BUILD_GPU, BUILD_HOST_SAH = 0, 1
+LIGHT_AREA, LIGHT_POINT, LIGHT_ENV = 0, 1, 2
+
+EXPORTED_SYMBOLS = (
+    "dl_hip_create", "dl_hip_destroy", "dl_hip_last_error", "dl_hip_device_name", "dl_hip_upload_scene",
+    "dl_hip_build_accel", "dl_hip_trace_closest", "dl_hip_trace_shadow", "dl_hip_trace_closest_counted",
+    "dl_hip_render", "dl_hip_render_paths", "dl_hip_malloc", "dl_hip_free", "dl_hip_memcpy_h2d",
+    "dl_hip_memcpy_d2h", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_film_filter",
+    "dl_mesh_tangents",
+)
+
+
+class DlError(RuntimeError):
+    pass
+
+
+class dl_mesh_desc(ctypes.Structure):
+    _fields_ = [("positions", ctypes.c_void_p), ("indices", ctypes.c_void_p), ("normals", ctypes.c_void_p),
+                ("texcoords", ctypes.c_void_p), ("tangents", ctypes.c_void_p), ("num_vertices", ctypes.c_uint32),
+                ("num_triangles", ctypes.c_uint32), ("material", ctypes.c_int32), ("light", ctypes.c_int32)]
+
+
+class dl_material_desc(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("params", ctypes.c_float * 16)]
+
+
+class dl_light_desc(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("mesh", ctypes.c_int32), ("radiance", ctypes.c_float * 3),
+                ("position", ctypes.c_float * 3), ("env_rgb", ctypes.c_void_p), ("env_width", ctypes.c_int32),
+                ("env_height", ctypes.c_int32), ("env_scale", ctypes.c_float), ("env_visual", ctypes.c_int32)]
+
+
+class dl_camera_desc(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("eye", ctypes.c_float * 3), ("U", ctypes.c_float * 3), ("V", ctypes.c_float * 3),
+                ("W", ctypes.c_float * 3)]
+
+
+class dl_film_desc(ctypes.Structure):
+    _fields_ = [("radius", ctypes.c_float), ("border", ctypes.c_int32), ("lookup", ctypes.c_float),
+                ("table", ctypes.c_float * 33)]
+
+
+class dl_scene_desc(ctypes.Structure):
+    _fields_ = [("meshes", ctypes.POINTER(dl_mesh_desc)), ("num_meshes", ctypes.c_uint32),
+                ("materials", ctypes.POINTER(dl_material_desc)), ("num_materials", ctypes.c_uint32),
+                ("lights", ctypes.POINTER(dl_light_desc)), ("num_lights", ctypes.c_uint32),
+                ("camera", dl_camera_desc), ("film", dl_film_desc), ("bg_color", ctypes.c_float * 3)]
+
+
+class dl_accel_stats(ctypes.Structure):
+    _fields_ = [("num_nodes", ctypes.c_uint32), ("num_leaves", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
+                ("num_prims", ctypes.c_uint32), ("build_ms", ctypes.c_double), ("sah_cost", ctypes.c_double),
+                ("node_bytes", ctypes.c_uint64), ("tri_bytes", ctypes.c_uint64)]
+
+
+class dl_render_params(ctypes.Structure):
+    _fields_ = [("spp", ctypes.c_int32), ("seed", ctypes.c_uint32), ("max_depth", ctypes.c_int32),
+                ("rr_depth", ctypes.c_int32), ("spp_per_pass", ctypes.c_int32), ("first_sample", ctypes.c_int32),
+                ("flags", ctypes.c_int32)]
+
+
+class dl_render_stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "paths", "closest_rays", "shadow_rays", "invalid_samples", "ms_total", "ms_closest", "ms_shadow", "ms_shade",
+        "ms_film", "ms_other", "launches_closest", "launches_shadow", "nodes_closest", "tris_closest")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libdrawlab_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DlError("libdrawlab_hip.so is not built: run __graft_entry__.build() or make -C drawlab_amd")
+        L = ctypes.CDLL(LIB_PATH)
+        P, i32, u32, c = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes
+        sigs = {
+            "dl_hip_create": (i32, [i32, c.POINTER(P)]),
+            "dl_hip_destroy": (i32, [P]),
+            "dl_hip_last_error": (c.c_char_p, [P]),
+            "dl_hip_device_name": (c.c_char_p, [P]),
+            "dl_hip_upload_scene": (i32, [P, c.POINTER(dl_scene_desc)]),
+            "dl_hip_build_accel": (i32, [P, i32, c.POINTER(dl_accel_stats)]),
+            "dl_hip_trace_closest": (i32, [P, P, u32, P, P, P, P]),
+            "dl_hip_trace_closest_counted": (i32, [P, P, u32, P, P, P, P, P, P]),
+            "dl_hip_trace_shadow": (i32, [P, P, u32, P]),
+            "dl_hip_render": (i32, [P, c.POINTER(dl_render_params), P, c.POINTER(dl_render_stats)]),
+            "dl_hip_render_paths": (i32, [P, c.POINTER(dl_render_params), u32, P]),
+            "dl_hip_malloc": (i32, [P, c.c_size_t, c.POINTER(P)]),
+            "dl_hip_free": (i32, [P, P]),
+            "dl_hip_memcpy_h2d": (i32, [P, P, P, c.c_size_t]),
+            "dl_hip_memcpy_d2h": (i32, [P, P, P, c.c_size_t]),
+            "dl_hip_sync": (i32, [P]),
+            "dl_hip_last_kernel_ms": (i32, [P, c.POINTER(c.c_double)]),
+            "dl_camera_virtual": (None, [P, P, P, c.c_float, i32, i32, c.POINTER(dl_camera_desc)]),
+            "dl_film_filter": (None, [i32, c.c_float, c.c_float, c.c_float, c.POINTER(dl_film_desc)]),
+            "dl_mesh_tangents": (None, [P, u32, P, u32, P, P]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def camera_desc(cam: Camera) -> dl_camera_desc:
+    out = dl_camera_desc()
+    e = np.asarray(cam.eye, np.float32)
+    la = np.asarray(cam.lookat, np.float32)
+    up = np.asarray(cam.up, np.float32)
+    lib().dl_camera_virtual(_p(e), _p(la), _p(up), float(cam.fov), int(cam.width), int(cam.height), ctypes.byref(out))
+    return out
+
+
+def film_desc(f: RFilter) -> dl_film_desc:
+    out = dl_film_desc()
+    lib().dl_film_filter(int(f.type), float(f.radius), float(f.p0), float(f.p1), ctypes.byref(out))
+    return out
+
+
+class DeviceBuffer:
+    """A device allocation owned by a Backend context."""
+
+    def __init__(self, be, nbytes):
+        self.be = be
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        be._check(lib().dl_hip_malloc(be.h, max(1, self.nbytes), ctypes.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.be._check(lib().dl_hip_memcpy_h2d(self.be.h, self.ptr, _p(arr), arr.nbytes))
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        self.be._check(lib().dl_hip_memcpy_d2h(self.be.h, _p(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.be.h:
+            lib().dl_hip_free(self.be.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Backend:
+    """One HIP context on one device (the reference's DeviceContext role)."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        h = ctypes.c_void_p()
+        rc = self.L.dl_hip_create(int(device), ctypes.byref(h))
+        if rc != 0:
+            raise DlError("dl_hip_create failed with status %d (no usable HIP device?)" % rc)
+        self.h = h.value
+        self._keep = []
+        self.desc = None
+        self.accel_stats = None
+
+    def close(self):
+        if self.h:
+            self.L.dl_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.L.dl_hip_last_error(self.h)
+            raise DlError("dl_hip status %d: %s" % (rc, msg.decode() if msg else ""))
+
+    @property
+    def device_name(self):
+        return self.L.dl_hip_device_name(self.h).decode()
+
+    # -- scene -------------------------------------------------------------
+    def upload_scene(self, desc: SceneDesc):
+        keep = []
+        order = desc.emitter_order()
+        light_of_mesh = {}
+        lights = (dl_light_desc * max(1, len(order)))()
+        for li, (kind, idx) in enumerate(order):
+            ld = lights[li]
+            if kind == "area":
+                ld.type = LIGHT_AREA
+                ld.mesh = idx
+                ld.radiance[:] = desc.meshes[idx].radiance
+                light_of_mesh[idx] = li
+            elif kind == "point":
+                ld.type = LIGHT_POINT
+                ld.mesh = -1
+                ld.position[:] = desc.point_lights[idx][0]
+                ld.radiance[:] = desc.point_lights[idx][1]
+            else:
+                env = desc.envmap
+                rgb = np.ascontiguousarray(env.rgb, dtype=np.float32)
+                keep.append(rgb)
+                ld.type = LIGHT_ENV
+                ld.mesh = -1
+                ld.env_rgb = _p(rgb)
+                ld.env_height, ld.env_width = rgb.shape[0], rgb.shape[1]
+                ld.env_scale = float(env.scale)
+                ld.env_visual = int(bool(env.visual))
+        meshes = (dl_mesh_desc * len(desc.meshes))()
+        for i, m in enumerate(desc.meshes):
+            V = np.ascontiguousarray(m.V, dtype=np.float32)
+            F = np.ascontiguousarray(m.F, dtype=np.uint32)
+            N = None if m.N is None else np.ascontiguousarray(m.N, dtype=np.float32)
+            UV = None if m.UV is None else np.ascontiguousarray(m.UV, dtype=np.float32)
+            keep += [V, F, N, UV]
+            md = meshes[i]
+            md.positions, md.indices, md.normals, md.texcoords, md.tangents = _p(V), _p(F), _p(N), _p(UV), None
+            md.num_vertices, md.num_triangles = V.shape[0], F.shape[0]
+            md.material = m.bsdf
+            md.light = light_of_mesh.get(i, -1)
+        mats = (dl_material_desc * max(1, len(desc.bsdfs)))()
+        for i, b in enumerate(desc.bsdfs):
+            mats[i].type = b.type
+            for k, v in enumerate(b.params):
+                mats[i].params[k] = v
+        sd = dl_scene_desc()
+        sd.meshes = meshes
+        sd.num_meshes = len(desc.meshes)
+        sd.materials = mats
+        sd.num_materials = len(desc.bsdfs)
+        sd.lights = lights
+        sd.num_lights = len(order)
+        sd.camera = camera_desc(desc.camera)
+        sd.film = film_desc(desc.rfilter)
+        sd.bg_color[:] = desc.bg_color
+        self._check(self.L.dl_hip_upload_scene(self.h, ctypes.byref(sd)))
+        self.desc = desc
+        return self
+
+    def build_accel(self, builder=BUILD_HOST_SAH):
+        st = dl_accel_stats()
+        self._check(self.L.dl_hip_build_accel(self.h, int(builder), ctypes.byref(st)))
+        self.accel_stats = {n: getattr(st, n) for n, _ in st._fields_}
+        return self.accel_stats
+
+    # -- traversal ---------------------------------------------------------
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def trace_closest_dev(self, d_rays, n, d_prim, d_t, d_u, d_v, d_nodes=None, d_tris=None):
+        if d_nodes is not None:
+            self._check(self.L.dl_hip_trace_closest_counted(self.h, d_rays.ptr, n, d_prim.ptr, d_t.ptr, d_u.ptr,
+                                                            d_v.ptr, d_nodes.ptr, d_tris.ptr))
+        else:
+            self._check(self.L.dl_hip_trace_closest(self.h, d_rays.ptr, n, d_prim.ptr, d_t.ptr, d_u.ptr, d_v.ptr))
+
+    def trace_shadow_dev(self, d_rays, n, d_occ):
+        self._check(self.L.dl_hip_trace_shadow(self.h, d_rays.ptr, n, d_occ.ptr))
+
+    def last_kernel_ms(self):
+        ms = ctypes.c_double()
+        self._check(self.L.dl_hip_last_kernel_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def trace_closest(self, rays, counted=False):
+        """Host convenience wrapper: rays is (8, n) float32 planes."""
+        rays = np.ascontiguousarray(np.asarray(rays, dtype=np.float32).reshape(8, -1))
+        n = rays.shape[1]
+        d_rays = self.alloc(rays.nbytes)
+        d_rays.upload(rays)
+        bufs = [self.alloc(4 * max(1, n)) for _ in range(4)]
+        cnt = [self.alloc(4 * max(1, n)) for _ in range(2)] if counted else [None, None]
+        self.trace_closest_dev(d_rays, n, *bufs, *cnt)
+        prim = bufs[0].download(np.uint32, n)
+        t = bufs[1].download(np.float32, n)
+        u = bufs[2].download(np.float32, n)
+        v = bufs[3].download(np.float32, n)
+        if counted:
+            return prim, t, u, v, cnt[0].download(np.uint32, n), cnt[1].download(np.uint32, n)
+        return prim, t, u, v
+
+    def trace_shadow(self, rays):
+        rays = np.ascontiguousarray(np.asarray(rays, dtype=np.float32).reshape(8, -1))
+        n = rays.shape[1]
+        d_rays = self.alloc(rays.nbytes)
+        d_rays.upload(rays)
+        d_occ = self.alloc(max(1, n))
+        self.trace_shadow_dev(d_rays, n, d_occ)
+        return d_occ.download(np.uint8, n)
+
+    # -- rendering ---------------------------------------------------------
+    def _params(self, spp, seed, max_depth, rr_depth, spp_per_pass=0, first_sample=0, flags=0):
+        p = dl_render_params()
+        p.spp, p.seed, p.max_depth, p.rr_depth = int(spp), int(seed), int(max_depth), int(rr_depth)
+        p.spp_per_pass, p.first_sample, p.flags = int(spp_per_pass), int(first_sample), int(flags)
+        return p
+
+    def render(self, spp=None, seed=0, max_depth=None, rr_depth=None, spp_per_pass=0, want_image=True,
+               count_traversal=False, first_sample=0):
+        d = self.desc
+        spp = d.spp if spp is None else spp
+        max_depth = d.max_depth if max_depth is None else max_depth
+        rr_depth = d.rr_depth if rr_depth is None else rr_depth
+        p = self._params(spp, seed, max_depth, rr_depth, spp_per_pass, first_sample, 1 if count_traversal else 0)
+        img = np.zeros((d.camera.height, d.camera.width, 3), np.float32) if want_image else None
+        st = dl_render_stats()
+        self._check(self.L.dl_hip_render(self.h, ctypes.byref(p), _p(img), ctypes.byref(st)))
+        return img, st.as_dict()
+
+    def render_paths(self, sample, seed=0, max_depth=None, rr_depth=None):
+        d = self.desc
+        max_depth = d.max_depth if max_depth is None else max_depth
+        rr_depth = d.rr_depth if rr_depth is None else rr_depth
+        p = self._params(sample + 1, seed, max_depth, rr_depth)
+        out = np.zeros((d.camera.height * d.camera.width, 5), np.float32)
+        self._check(self.L.dl_hip_render_paths(self.h, ctypes.byref(p), int(sample), _p(out)))
+        return out

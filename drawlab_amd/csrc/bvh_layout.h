@@ -1,0 +1,48 @@
+// HBM layout of the acceleration structure used by the gfx950 traversal
+// kernels.  Shared by the host builder and the device code.
+//
+// Node: a binary "child-pair" node, 64 bytes = 4 x 16-byte records, so one
+// node visit is four dwordx4 loads from one 64-byte-aligned line:
+//   r0 = (c0.lo.x, c0.hi.x, c1.lo.x, c1.hi.x)
+//   r1 = (c0.lo.y, c0.hi.y, c1.lo.y, c1.hi.y)
+//   r2 = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
+//   r3 = (c0.ref, c1.ref, c0.count, c1.count)  as int32
+// ref >= 0 is an interior node index; ref < 0 is a leaf whose triangles are
+// tris[~ref .. ~ref + count).  Node 0 is the root.  Child boxes are padded
+// outward (relative 2^-16 of their magnitude), so the slab test never
+// rejects a box that holds a triangle Moller-Trumbore accepts, including
+// rays with zero direction components (no 0 * inf).
+//
+// Triangle: 48 bytes = 3 x 16-byte records, pre-gathered in leaf order:
+//   t0 = (p0.x, p0.y, p0.z, e1.x)
+//   t1 = (e1.y, e1.z, e2.x, e2.y)
+//   t2 = (e2.z, prim_id (bits), 0, 0)
+// with e1 = p1 - p0 and e2 = p2 - p0 computed once in binary32, which is
+// exactly what the reference computes per test (src/tracer/mesh/mesh.cpp:187).
+#pragma once
+#include <stdint.h>
+
+namespace dl {
+
+constexpr int kNodeBytes = 64;
+constexpr int kTriBytes = 48;
+constexpr int kMaxLeafSize = 4;
+// Traversal stack: kLdsStack entries in LDS per lane, the rest in a global
+// overflow buffer; no tree is deeper than kMaxStack internal levels (the
+// host builder caps depth at kLdsStack, an LBVH over 64-bit keys has at
+// most 64 levels).
+constexpr int kLdsStack = 48;
+constexpr int kMaxStack = 64;
+constexpr int kMaxDepth = kLdsStack;
+
+struct BvhNodeH {
+    float r0[4], r1[4], r2[4];
+    int32_t r3[4];
+};
+struct BvhTriH {
+    float t0[4], t1[4], t2[4];
+};
+static_assert(sizeof(BvhNodeH) == kNodeBytes, "node layout");
+static_assert(sizeof(BvhTriH) == kTriBytes, "tri layout");
+
+}  // namespace dl

@@ -1,0 +1,362 @@
+// C ABI entry points (include/dl_hip.h): context, scene upload, acceleration
+// structure, traversal batches, device memory helpers.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "../host/bvh_build.h"
+#include "../host/scene_prep.h"
+#include "backend.h"
+
+#include <algorithm>
+
+namespace dl {
+
+dl_status set_error(Ctx* c, dl_status s, const std::string& msg) {
+    if (c) c->err = msg;
+    return s;
+}
+
+dl_status hip_fail(Ctx* c, hipError_t e, const char* what) {
+    char buf[512];
+    snprintf(buf, sizeof(buf), "%s failed: %s (%d)", what, hipGetErrorString(e), (int)e);
+    return set_error(c, e == hipErrorOutOfMemory ? DL_ERR_OOM : DL_ERR_HIP, buf);
+}
+
+namespace {
+
+template <typename T>
+dl_status upload(Ctx* c, T** dst, const T* src, size_t count) {
+    *dst = nullptr;
+    if (count == 0) return DL_OK;
+    void* p = nullptr;
+    DL_HIP_CHECK(c, hipMalloc(&p, count * sizeof(T)));
+    c->scene_allocs.push_back(p);
+    if (src) DL_HIP_CHECK(c, hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    *dst = (T*)p;
+    return DL_OK;
+}
+
+void free_scene(Ctx* c) {
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void* p : c->scene_allocs) hipFree(p);
+    c->scene_allocs.clear();
+    c->ds = DevScene();
+    c->has_scene = false;
+    c->has_accel = false;
+}
+
+#define DL_TRY(expr)                     \
+    do {                                 \
+        dl_status s_ = (expr);           \
+        if (s_ != DL_OK) return s_;      \
+    } while (0)
+
+}  // namespace
+}  // namespace dl
+
+using namespace dl;
+
+extern "C" {
+
+dl_status dl_hip_create(int device, void** out) {
+    if (!out) return DL_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return DL_ERR_NO_DEVICE;
+    if (device < 0 || device >= count) return DL_ERR_INVALID;
+    Ctx* c = new Ctx();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        delete c;
+        return DL_ERR_HIP;
+    }
+    snprintf(c->devname, sizeof(c->devname), "%s (%s)", prop.name, prop.gcnArchName);
+    c->num_cus = prop.multiProcessorCount;
+    c->trace_blocks = (uint32_t)std::max(1, c->num_cus) * 16u;
+    e = hipMalloc(&c->stack_ovf, sizeof(int) * (size_t)c->trace_blocks * kTraceBlockHost * (kMaxStack - kLdsStack));
+    if (e != hipSuccess) {
+        delete c;
+        return DL_ERR_OOM;
+    }
+    *out = c;
+    return DL_OK;
+}
+
+dl_status dl_hip_destroy(void* h) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    hipSetDevice(c->device);
+    free_render_state(c);
+    free_scene(c);
+    if (c->stack_ovf) hipFree(c->stack_ovf);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return DL_OK;
+}
+
+const char* dl_hip_last_error(void* h) { return h ? ((Ctx*)h)->err.c_str() : "null context"; }
+const char* dl_hip_device_name(void* h) { return h ? ((Ctx*)h)->devname : ""; }
+
+dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !sd) return DL_ERR_INVALID;
+    hipSetDevice(c->device);
+    free_render_state(c);
+    free_scene(c);
+    if (sd->num_meshes == 0) return set_error(c, DL_ERR_INVALID, "scene has no meshes");
+    if (sd->camera.width <= 0 || sd->camera.height <= 0) return set_error(c, DL_ERR_INVALID, "bad camera size");
+    for (uint32_t i = 0; i < sd->num_materials; i++)
+        if (sd->materials[i].type < 0 || sd->materials[i].type >= DL_BSDF_COUNT)
+            return set_error(c, DL_ERR_INVALID, "unknown material type");
+    uint64_t nv = 0, nf = 0;
+    for (uint32_t m = 0; m < sd->num_meshes; m++) {
+        const dl_mesh_desc& md = sd->meshes[m];
+        if (!md.positions || !md.indices) return set_error(c, DL_ERR_INVALID, "mesh without positions/indices");
+        if (md.material < 0 || (uint32_t)md.material >= sd->num_materials)
+            return set_error(c, DL_ERR_INVALID, "mesh material index out of range");
+        if (md.light >= (int32_t)sd->num_lights) return set_error(c, DL_ERR_INVALID, "mesh light index out of range");
+        for (uint64_t k = 0; k < 3ull * md.num_triangles; k++)
+            if (md.indices[k] >= md.num_vertices) return set_error(c, DL_ERR_INVALID, "vertex index out of range");
+        nv += md.num_vertices;
+        nf += md.num_triangles;
+    }
+    if (nf >= 0x7fffffffull || nv >= 0x7fffffffull) return set_error(c, DL_ERR_INVALID, "scene too large");
+    c->nprims = (uint32_t)nf;
+    c->nverts = (uint32_t)nv;
+
+    std::vector<float> pos(3 * nv), nrm(3 * nv, 0.0f), tan(3 * nv, 0.0f);
+    std::vector<uint32_t> idx(3 * nf), prim_mesh(nf);
+    c->tri_verts.assign(9 * nf, 0.0f);
+    c->mesh_recs.assign(sd->num_meshes, MeshRec{});
+    std::vector<float> cdf_all;
+    uint64_t vo = 0, fo = 0;
+    for (uint32_t m = 0; m < sd->num_meshes; m++) {
+        const dl_mesh_desc& md = sd->meshes[m];
+        MeshRec& r = c->mesh_recs[m];
+        r.prim_offset = (uint32_t)fo;
+        r.num_prims = md.num_triangles;
+        r.material = md.material;
+        r.light = md.light;
+        r.has_normals = md.normals != nullptr;
+        r.has_texcoords = md.texcoords != nullptr;
+        r.has_tangents = md.tangents != nullptr || md.texcoords != nullptr;
+        std::memcpy(&pos[3 * vo], md.positions, sizeof(float) * 3 * md.num_vertices);
+        if (md.normals) std::memcpy(&nrm[3 * vo], md.normals, sizeof(float) * 3 * md.num_vertices);
+        if (md.tangents)
+            std::memcpy(&tan[3 * vo], md.tangents, sizeof(float) * 3 * md.num_vertices);
+        else if (md.texcoords)
+            dl_mesh_tangents(md.positions, md.num_vertices, md.indices, md.num_triangles, md.texcoords, &tan[3 * vo]);
+        for (uint32_t f = 0; f < md.num_triangles; f++) {
+            prim_mesh[fo + f] = m;
+            for (int k = 0; k < 3; k++) {
+                uint32_t vi = md.indices[3 * f + k];
+                idx[3 * (fo + f) + k] = (uint32_t)(vo + vi);
+                std::memcpy(&c->tri_verts[9 * (fo + f) + 3 * k], md.positions + 3 * (size_t)vi, 3 * sizeof(float));
+            }
+        }
+        if (md.light >= 0) {
+            std::vector<float> cdf;
+            r.area_norm = area_cdf(md.positions, md.indices, md.num_triangles, cdf);
+            r.cdf_offset = (uint32_t)cdf_all.size();
+            cdf_all.insert(cdf_all.end(), cdf.begin(), cdf.end());
+        }
+        vo += md.num_vertices;
+        fo += md.num_triangles;
+    }
+    c->mats.assign(sd->materials, sd->materials + sd->num_materials);
+    c->light_recs.assign(sd->num_lights, LightRec{});
+    c->env_light = -1;
+    std::vector<float4> env_rgb;
+    std::vector<float> env_q, env_pdf;
+    std::vector<uint32_t> env_alias;
+    for (uint32_t l = 0; l < sd->num_lights; l++) {
+        const dl_light_desc& ld = sd->lights[l];
+        LightRec& r = c->light_recs[l];
+        r.type = ld.type;
+        r.mesh = ld.mesh;
+        for (int k = 0; k < 3; k++) {
+            r.radiance[k] = ld.radiance[k];
+            r.position[k] = ld.position[k];
+        }
+        if (ld.type == DL_LIGHT_AREA) {
+            if (ld.mesh < 0 || (uint32_t)ld.mesh >= sd->num_meshes || sd->meshes[ld.mesh].light != (int32_t)l)
+                return set_error(c, DL_ERR_INVALID, "area light must reference the mesh that references it");
+        } else if (ld.type == DL_LIGHT_ENV) {
+            if (c->env_light >= 0) return set_error(c, DL_ERR_INVALID, "There is more than one envmap in the scene!");
+            if (!ld.env_rgb || ld.env_width <= 0 || ld.env_height <= 0)
+                return set_error(c, DL_ERR_INVALID, "envmap without pixels");
+            c->env_light = (int)l;
+            r.env_w = ld.env_width;
+            r.env_h = ld.env_height;
+            r.env_scale = ld.env_scale;
+            r.env_visual = ld.env_visual;
+            const size_t n = (size_t)ld.env_width * ld.env_height;
+            env_rgb.resize(n);
+            for (size_t i = 0; i < n; i++)
+                env_rgb[i] = make_float4(ld.env_rgb[3 * i], ld.env_rgb[3 * i + 1], ld.env_rgb[3 * i + 2], 0.0f);
+            env_tables(ld.env_rgb, ld.env_width, ld.env_height, env_q, env_alias, env_pdf);
+        } else if (ld.type != DL_LIGHT_POINT) {
+            return set_error(c, DL_ERR_INVALID, "unknown light type");
+        }
+    }
+    c->cam = sd->camera;
+    c->film = sd->film;
+    for (int k = 0; k < 3; k++) c->bg[k] = sd->bg_color[k];
+
+    DevScene& ds = c->ds;
+    DL_TRY(upload(c, &ds.pos, pos.data(), pos.size()));
+    DL_TRY(upload(c, &ds.nrm, nrm.data(), nrm.size()));
+    DL_TRY(upload(c, &ds.tan, tan.data(), tan.size()));
+    DL_TRY(upload(c, &ds.idx, idx.data(), idx.size()));
+    DL_TRY(upload(c, &ds.prim_mesh, prim_mesh.data(), prim_mesh.size()));
+    MeshRec* dm = nullptr;
+    DL_TRY(upload(c, &dm, c->mesh_recs.data(), c->mesh_recs.size()));
+    ds.meshes = dm;
+    dl_material_desc* dmat = nullptr;
+    DL_TRY(upload(c, &dmat, c->mats.data(), c->mats.size()));
+    ds.materials = dmat;
+    LightRec* dl_ = nullptr;
+    DL_TRY(upload(c, &dl_, c->light_recs.data(), c->light_recs.size()));
+    ds.lights = dl_;
+    DL_TRY(upload(c, &ds.light_cdf, cdf_all.data(), cdf_all.size()));
+    DL_TRY(upload(c, &ds.env_rgb, env_rgb.data(), env_rgb.size()));
+    DL_TRY(upload(c, &ds.env_q, env_q.data(), env_q.size()));
+    DL_TRY(upload(c, &ds.env_alias, env_alias.data(), env_alias.size()));
+    DL_TRY(upload(c, &ds.env_pdf, env_pdf.data(), env_pdf.size()));
+    DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->has_scene = true;
+    return DL_OK;
+}
+
+dl_status dl_hip_build_accel(void* h, int builder, dl_accel_stats* stats) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    if (!c->has_scene) return set_error(c, DL_ERR_STATE, "build_accel before upload_scene");
+    hipSetDevice(c->device);
+    if (c->ds.nodes) {
+        hipFree(c->ds.nodes);
+        hipFree(c->ds.tris);
+        c->ds.nodes = nullptr;
+        c->ds.tris = nullptr;
+    }
+    dl_accel_stats st{};
+    if (builder == DL_BUILD_GPU) {
+        DL_TRY(build_bvh_gpu(c, &st));
+    } else if (builder == DL_BUILD_HOST_SAH) {
+        auto t0 = std::chrono::steady_clock::now();
+        HostBvh bvh;
+        build_bvh_host(c->tri_verts.data(), c->nprims, bvh);
+        auto t1 = std::chrono::steady_clock::now();
+        void* pn = nullptr;
+        void* pt = nullptr;
+        DL_HIP_CHECK(c, hipMalloc(&pn, bvh.nodes.size() * sizeof(BvhNodeH)));
+        DL_HIP_CHECK(c, hipMalloc(&pt, std::max<size_t>(1, bvh.tris.size()) * sizeof(BvhTriH)));
+        DL_HIP_CHECK(c, hipMemcpy(pn, bvh.nodes.data(), bvh.nodes.size() * sizeof(BvhNodeH), hipMemcpyHostToDevice));
+        if (!bvh.tris.empty())
+            DL_HIP_CHECK(c, hipMemcpy(pt, bvh.tris.data(), bvh.tris.size() * sizeof(BvhTriH), hipMemcpyHostToDevice));
+        c->ds.nodes = (float4*)pn;
+        c->ds.tris = (float4*)pt;
+        st.num_nodes = (uint32_t)bvh.nodes.size();
+        st.num_leaves = bvh.num_leaves;
+        st.max_depth = bvh.max_depth;
+        st.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    } else {
+        return set_error(c, DL_ERR_INVALID, "unknown builder");
+    }
+    st.num_prims = c->nprims;
+    st.node_bytes = (uint64_t)st.num_nodes * kNodeBytes;
+    st.tri_bytes = (uint64_t)c->nprims * kTriBytes;
+    c->accel = st;
+    c->has_accel = true;
+    if (stats) *stats = st;
+    return DL_OK;
+}
+
+dl_status dl_hip_trace_closest(void* h, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u, float* v) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !rays || !prim || !t || !u || !v) return DL_ERR_INVALID;
+    if (!c->has_accel) return set_error(c, DL_ERR_STATE, "trace before build_accel");
+    return launch_trace_closest(c, rays, n, prim, t, u, v, nullptr, nullptr);
+}
+
+dl_status dl_hip_trace_closest_counted(void* h, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u,
+                                       float* v, uint32_t* nodes, uint32_t* tris) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !rays || !prim || !t || !u || !v || !nodes || !tris) return DL_ERR_INVALID;
+    if (!c->has_accel) return set_error(c, DL_ERR_STATE, "trace before build_accel");
+    return launch_trace_closest(c, rays, n, prim, t, u, v, nodes, tris);
+}
+
+dl_status dl_hip_trace_shadow(void* h, const float* rays, uint32_t n, uint8_t* occ) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !rays || !occ) return DL_ERR_INVALID;
+    if (!c->has_accel) return set_error(c, DL_ERR_STATE, "trace before build_accel");
+    return launch_trace_shadow(c, rays, n, occ);
+}
+
+dl_status dl_hip_render(void* h, const dl_render_params* p, float* rgb_host, dl_render_stats* stats) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !p) return DL_ERR_INVALID;
+    if (!c->has_accel) return set_error(c, DL_ERR_STATE, "render before build_accel");
+    return render(c, p, rgb_host, stats, -1, nullptr);
+}
+
+dl_status dl_hip_render_paths(void* h, const dl_render_params* p, uint32_t sample, float* out) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !p || !out) return DL_ERR_INVALID;
+    if (!c->has_accel) return set_error(c, DL_ERR_STATE, "render before build_accel");
+    return render(c, p, nullptr, nullptr, (int)sample, out);
+}
+
+dl_status dl_hip_malloc(void* h, size_t bytes, void** dptr) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !dptr) return DL_ERR_INVALID;
+    hipSetDevice(c->device);
+    DL_HIP_CHECK(c, hipMalloc(dptr, bytes));
+    return DL_OK;
+}
+dl_status dl_hip_free(void* h, void* p) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    DL_HIP_CHECK(c, hipFree(p));
+    return DL_OK;
+}
+dl_status dl_hip_memcpy_h2d(void* h, void* dst, const void* src, size_t bytes) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    DL_HIP_CHECK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return DL_OK;
+}
+dl_status dl_hip_memcpy_d2h(void* h, void* dst, const void* src, size_t bytes) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    DL_HIP_CHECK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return DL_OK;
+}
+dl_status dl_hip_sync(void* h) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return DL_OK;
+}
+dl_status dl_hip_last_kernel_ms(void* h, double* ms) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !ms) return DL_ERR_INVALID;
+    *ms = c->last_kernel_ms;
+    return DL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+// Internal state of a dl_hip context (one device, one stream).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../../include/dl_hip.h"
+#include "../bvh_layout.h"
+
+namespace dl {
+
+constexpr int kTraceBlockHost = 128;  // threads per traversal block (trace.h kTraceBlock)
+
+// Device copy of the scene in the layouts the kernels read.  All arrays are
+// indexed by global primitive id (mesh order, then face order) or global
+// vertex id.
+struct DevScene {
+    // geometry
+    float* pos = nullptr;        // 3 * nv   (global vertex positions)
+    float* nrm = nullptr;        // 3 * nv   (zeros for meshes without normals)
+    float* tan = nullptr;        // 3 * nv   (zeros for meshes without tangents)
+    uint32_t* idx = nullptr;     // 3 * nprims (global vertex indices)
+    uint32_t* prim_mesh = nullptr;  // nprims -> mesh index
+    // per mesh records (MeshRec)
+    void* meshes = nullptr;
+    // materials (dl_material_desc) and lights (LightRec)
+    void* materials = nullptr;
+    void* lights = nullptr;
+    float* light_cdf = nullptr;  // concatenated area CDFs (dpdf.h)
+    // environment map
+    float4* env_rgb = nullptr;   // w * h texels (rgb, 0)
+    float* env_q = nullptr;
+    uint32_t* env_alias = nullptr;
+    float* env_pdf = nullptr;
+    // acceleration structure
+    float4* nodes = nullptr;
+    float4* tris = nullptr;
+};
+
+// Per-mesh record read by the shading kernels.
+struct MeshRec {
+    uint32_t prim_offset;
+    uint32_t num_prims;
+    int32_t material;
+    int32_t light;
+    int32_t has_normals;
+    int32_t has_tangents;
+    int32_t has_texcoords;
+    float area_norm;   // 1 / total area (DiscretePDF::getNormalization)
+    uint32_t cdf_offset;  // into light_cdf (num_prims + 1 entries), emitters only
+    int32_t pad[3];
+};
+
+struct LightRec {
+    int32_t type;
+    int32_t mesh;
+    float radiance[3];
+    float position[3];
+    int32_t env_w, env_h;
+    float env_scale;
+    int32_t env_visual;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    char devname[256] = {0};
+    int num_cus = 0;
+    double last_kernel_ms = 0.0;
+    // traversal stack overflow (entries kLdsStack..kMaxStack of every lane of
+    // a persistent traversal grid)
+    int* stack_ovf = nullptr;
+    uint32_t trace_blocks = 0;
+
+    // scene
+    bool has_scene = false;
+    bool has_accel = false;
+    uint32_t nprims = 0, nverts = 0;
+    std::vector<float> tri_verts;  // 9 floats per prim (host copy for builders)
+    std::vector<MeshRec> mesh_recs;
+    std::vector<dl_material_desc> mats;
+    std::vector<LightRec> light_recs;
+    int env_light = -1;
+    dl_camera_desc cam{};
+    dl_film_desc film{};
+    float bg[3] = {0, 0, 0};
+    DevScene ds;
+    dl_accel_stats accel{};
+    std::vector<void*> scene_allocs;
+
+    // render scratch (reused across calls, grown on demand)
+    std::vector<void*> render_allocs;
+    size_t render_capacity = 0;
+    void* rs = nullptr;  // RenderState*
+};
+
+dl_status set_error(Ctx* c, dl_status s, const std::string& msg);
+dl_status hip_fail(Ctx* c, hipError_t e, const char* what);
+
+#define DL_HIP_CHECK(ctx, call)                                    \
+    do {                                                           \
+        hipError_t e_ = (call);                                    \
+        if (e_ != hipSuccess) return ::dl::hip_fail((ctx), e_, #call); \
+    } while (0)
+
+// trace.hip launchers
+dl_status launch_trace_closest(Ctx* c, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u, float* v,
+                               uint32_t* cn, uint32_t* ct);
+dl_status launch_trace_shadow(Ctx* c, const float* rays, uint32_t n, uint8_t* occ);
+
+// integrator.hip
+dl_status render(Ctx* c, const dl_render_params* p, float* rgb_host, dl_render_stats* stats, int paths_sample,
+                 float* paths_out);
+void free_render_state(Ctx* c);
+
+// bvh_gpu.hip
+dl_status build_bvh_gpu(Ctx* c, dl_accel_stats* stats);
+
+}  // namespace dl

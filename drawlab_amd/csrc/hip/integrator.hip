@@ -1,0 +1,760 @@
+// Wavefront unidirectional MIS path integrator for gfx950.
+//
+// Restates PathIntegrator::Li (src/tracer/integrator/path.cpp:13-100) as a
+// sequence of kernels over compacted queues of path indices, one bounce per
+// iteration:
+//
+//   raygen          camera rays for every (pixel, sample) of the pass
+//                   (CPURenderer::renderBlock, cpu_renderer.cpp:36-48;
+//                   VirtualCamera::sampleRay, virtual.cpp:52-65)
+//   trace_closest   closest hits of the queued rays
+//   post_camera     first-hit emission / environment on a camera-ray miss
+//   loop:
+//     shade         per path, material-sorted: emitter sampling (NEE) with
+//                   a shadow ray, BSDF sampling, continuation ray
+//     trace_closest + trace_shadow
+//     post          NEE resolution, emitter hit with MIS, miss with the
+//                   environment, Russian roulette, depth cap; survivors are
+//                   appended to the next material queues
+//   film            deterministic gather of the pass's samples through the
+//                   reconstruction filter (ImageBlock::put, block.cpp:81-113)
+//
+// Queues are appended with wave-aggregated atomics (one atomic per wave and
+// queue).  Stage kernels are persistent grid-stride loops whose trip counts
+// come from device-side counters, so no launch depends on a host readback
+// of its size.
+#include <algorithm>
+#include <cstring>
+
+#include "backend.h"
+#include "shading.h"
+#include "trace.h"
+
+namespace dl {
+
+constexpr int kStageBlock = 256;
+constexpr int kTypes = 8;  // material queue segments (DL_BSDF_COUNT rounded up)
+
+enum : uint32_t { kFlagDiffuse = 1u, kFlagNee = 2u, kFlagCont = 4u };
+
+struct Counters {
+    uint32_t mat[2][kTypes];
+    uint32_t ray;
+    uint32_t shadow;
+    uint32_t pad[14];
+    unsigned long long nodes;
+    unsigned long long tris;
+    unsigned long long invalid;
+    unsigned long long pad2;
+};
+
+// Device buffers of a render (plain pointers: passed by value to kernels).
+struct RenderBufs {
+    float4 *ray_o, *ray_d, *hit, *sh_o, *sh_d;
+    float4 *thr, *rad, *nee;
+    int4* aux;
+    uint64_t* rng;
+    float2* pos;
+    uint8_t* occ;
+    uint32_t* matq[2];
+    uint32_t* rayq;
+    uint32_t* shq;
+    Counters* cnt;
+    float4* film;
+    float* rgb;
+};
+
+struct RenderState {
+    uint32_t capacity = 0;
+    uint32_t film_pixels = 0;
+    RenderBufs b{};
+    Counters* cnt_host = nullptr;  // pinned
+    std::vector<void*> allocs;
+    std::vector<hipEvent_t> evs;
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (!pred) return 0;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ vec3 f3(const float4& a) { return v3(a.x, a.y, a.z); }
+__device__ __forceinline__ float4 f4(vec3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// Locate entry i of the concatenated per-type queues.
+__device__ __forceinline__ bool queue_entry(const uint32_t* counts, const uint32_t* q, uint32_t cap, uint32_t i,
+                                            uint32_t& path, int& type) {
+    uint32_t base = 0;
+#pragma unroll
+    for (int t = 0; t < kTypes; t++) {
+        const uint32_t c = counts[t];
+        if (i < base + c) {
+            path = q[(size_t)t * cap + (i - base)];
+            type = t;
+            return true;
+        }
+        base += c;
+    }
+    return false;
+}
+
+__device__ __forceinline__ uint32_t queue_total(const uint32_t* counts) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int t = 0; t < kTypes; t++) s += counts[t];
+    return s;
+}
+
+struct PassParams {
+    uint32_t P;           // paths in this pass
+    uint32_t W, H;
+    uint32_t sample_base;
+    uint32_t seed;
+    int max_depth;
+    int rr_depth;
+    uint32_t cap;         // queue segment capacity
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kStageBlock) void k_reset(Counters* cnt, int next) {
+    const int t = threadIdx.x;
+    if (t < kTypes) cnt->mat[next][t] = 0;
+    if (t == 0) {
+        cnt->ray = 0;
+        cnt->shadow = 0;
+    }
+}
+
+__global__ __launch_bounds__(kStageBlock) void k_reset_all(Counters* cnt) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(cnt);
+    for (uint32_t i = threadIdx.x; i < sizeof(Counters) / 4; i += blockDim.x) w[i] = 0;
+}
+
+__global__ __launch_bounds__(kStageBlock) void k_clear_film(float4* film, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        film[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Camera sample: pixel 2D, aperture 2D, VirtualCamera::sampleRay.
+__global__ __launch_bounds__(kStageBlock) void k_raygen(PassParams pp, dl_camera_desc cam, RenderBufs rs) {
+    const uint32_t WH = pp.W * pp.H;
+    const float invx = 1.f / (float)cam.width, invy = 1.f / (float)cam.height;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < pp.P; p += gridDim.x * blockDim.x) {
+        const uint32_t pix = p % WH, j = p / WH;
+        const uint32_t x = pix % pp.W, y = pix / pp.W;
+        Pcg32 rng;
+        rng.seed((uint64_t)pix + ((uint64_t)pp.seed << 32), (uint64_t)(pp.sample_base + j));
+        const float ax = rng.next_float();
+        const float ay = rng.next_float();
+        const float px = (float)x + ax, py = (float)y + ay;
+        rng.next_float();  // aperture sample (unused by the virtual camera)
+        rng.next_float();
+        const float lx = px, ly = ((float)cam.height - py) - 1;
+        const float dx = lx * invx * 2 - 1, dy = ly * invy * 2 - 1;
+        const vec3 d = normalized(add(add(scale(pcol(cam.U), dx), scale(pcol(cam.V), dy)), pcol(cam.W)));
+        rs.ray_o[p] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], kEpsilon);
+        rs.ray_d[p] = f4(d, INFINITY);
+        rs.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
+        rs.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        rs.aux[p] = make_int4(0, 0, 0, 0);
+        rs.rng[p] = rng.state;
+        rs.pos[p] = make_float2(px, py);
+        rs.rayq[p] = p;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) rs.cnt->ray = pp.P;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_queue(BvhView bvh, RenderBufs rs, int* __restrict__ ovf) {
+    __shared__ int lds[kLdsStack * kTraceBlock];
+    const uint32_t n = rs.cnt->ray;
+    const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x, G = gridDim.x * kTraceBlock;
+    LaneStack stk{lds, (int)threadIdx.x, kTraceBlock, ovf + g, G};
+    TraceCounters cnt;
+    for (uint32_t i = g; i < n; i += G) {
+        const uint32_t p = rs.rayq[i];
+        const float4 o = rs.ray_o[p], d = rs.ray_d[p];
+        uint32_t prim;
+        float t, u, v;
+        trace_closest<COUNT>(bvh, f3(o), f3(d), o.w, d.w, stk, prim, t, u, v, cnt);
+        rs.hit[p] = make_float4(__uint_as_float(prim), t, u, v);
+    }
+    if (COUNT) {
+        atomicAdd(&rs.cnt->nodes, (unsigned long long)cnt.nodes);
+        atomicAdd(&rs.cnt->tris, (unsigned long long)cnt.tris);
+    }
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_shadow_queue(BvhView bvh, RenderBufs rs, int* __restrict__ ovf) {
+    __shared__ int lds[kLdsStack * kTraceBlock];
+    const uint32_t n = rs.cnt->shadow;
+    const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x, G = gridDim.x * kTraceBlock;
+    LaneStack stk{lds, (int)threadIdx.x, kTraceBlock, ovf + g, G};
+    TraceCounters cnt;
+    for (uint32_t i = g; i < n; i += G) {
+        const uint32_t p = rs.shq[i];
+        const float4 o = rs.sh_o[p], d = rs.sh_d[p];
+        rs.occ[p] = trace_any<false>(bvh, f3(o), f3(d), o.w, d.w, stk, cnt) ? 1 : 0;
+    }
+}
+
+__device__ __forceinline__ int material_of(const SceneView& s, uint32_t prim, int& mat) {
+    const uint32_t m = s.prim_mesh[prim];
+    mat = s.meshes[m].material;
+    return s.mats[mat].type;
+}
+
+// First intersection (path.cpp:21-29) and a camera-ray miss (miss.cu:23-29).
+__global__ __launch_bounds__(kStageBlock) void k_post_camera(PassParams pp, SceneView s, RenderBufs rs, int next) {
+    const uint32_t n = rs.cnt->ray;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        bool alive = false;
+        int type = 0;
+        uint32_t p = 0;
+        if (i < n) {
+            p = rs.rayq[i];
+            const float4 h = rs.hit[p];
+            const uint32_t prim = __float_as_uint(h.x);
+            const vec3 d = f3(rs.ray_d[p]);
+            vec3 rad = v3(0, 0, 0);
+            if (prim == 0xffffffffu) {
+                if (s.env_light >= 0) {
+                    const LightRec& e = s.lights[s.env_light];
+                    rad = e.env_visual ? env_eval(s, e, d) : v3(s.bg[0], s.bg[1], s.bg[2]);
+                }
+            } else {
+                const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
+                if (mr.light >= 0) {
+                    vec3 hp, hn;
+                    hit_point_normal(s, prim, h.z, h.w, hp, hn);
+                    const vec3 le = dot(hn, neg(d)) > 0.f ? pcol(s.lights[mr.light].radiance) : v3(0, 0, 0);
+                    rad = add(rad, mul(v3(1.f, 1.f, 1.f), le));
+                }
+                alive = !(pp.max_depth >= 0 && 0 + 2 > pp.max_depth);
+                int mat;
+                type = material_of(s, prim, mat);
+            }
+            rs.rad[p] = f4(rad, 0.f);
+        }
+#pragma unroll
+        for (int t = 0; t < DL_BSDF_COUNT; t++) {
+            const bool mine = alive && type == t;
+            const uint32_t slot = wave_append(&rs.cnt->mat[next][t], mine);
+            if (mine) rs.matq[next][(size_t)t * pp.cap + slot] = p;
+        }
+    }
+}
+
+// One bounce at the current vertex: emitter sampling + BSDF sampling
+// (path.cpp:32-69).
+__global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView s, RenderBufs rs, int cur) {
+    const uint32_t* counts = rs.cnt->mat[cur];
+    const uint32_t n = queue_total(counts);
+    const uint32_t WH = pp.W * pp.H;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        bool want_shadow = false, want_cont = false;
+        uint32_t p = 0;
+        if (i < n) {
+            int type;
+            queue_entry(counts, rs.matq[cur], pp.cap, i, p, type);
+            const float4 h = rs.hit[p];
+            const uint32_t prim = __float_as_uint(h.x);
+            const vec3 d = f3(rs.ray_d[p]);
+            const float4 th4 = rs.thr[p];
+            vec3 thr = f3(th4);
+            float eta = th4.w;
+            vec3 rad = f3(rs.rad[p]);
+            int4 aux = rs.aux[p];
+            Pcg32 rng;
+            rng.state = rs.rng[p];
+            rng.inc = ((uint64_t)(pp.sample_base + p / WH) << 1u) | 1u;
+            Its its;
+            make_its(s, prim, h.z, h.w, type == DL_BSDF_ANISO_GGX, its);
+            const int mat = s.meshes[its.mesh].material;
+            const float* prm = s.mats[mat].params;
+            uint32_t flags = 0;
+            // ---- emitter sampling (Scene::sampleEmitterDirection, scene.cpp:139-170)
+            float lx = rng.next_float();
+            const float ly = rng.next_float();
+            DirSample ds;
+            ds.pdf = 0.0f;
+            ds.delta = false;
+            vec3 spectrum = v3(0, 0, 0);
+            const int L = s.num_lights;
+            if (L == 1) {
+                spectrum = emitter_sample(s, s.lights[0], lx, ly, its, ds, rng);
+            } else if (L > 1) {
+                const float epdf = 1.f / (float)L;
+                uint32_t idx = (uint32_t)(lx * (float)L);
+                idx = idx < (uint32_t)(L - 1) ? idx : (uint32_t)(L - 1);
+                lx = lx * (float)L - (float)idx;
+                spectrum = emitter_sample(s, s.lights[idx], lx, ly, its, ds, rng);
+                spectrum = scale(spectrum, (float)L);
+                ds.pdf *= epdf;
+            }
+            const vec3 wi = to_local(its.sh, neg(normalized(d)));
+            if (ds.pdf > 0) {
+                BRec q;
+                q.wi = wi;
+                q.wo = to_local(its.sh, ds.d);
+                q.eta = 1.f;
+                q.measure = kMeasureSolidAngle;
+                const vec3 bv = bsdf_eval(type, prm, q);
+                const float bp = bsdf_pdf(type, prm, q);
+                const float mis = ds.delta ? 1.0f : mis_weight(ds.pdf, bp);
+                const vec3 contrib = mul(mul(scale(thr, mis), bv), spectrum);
+                if (csum(spectrum) > 0) {
+                    rs.nee[p] = f4(contrib, 0.f);
+                    rs.sh_o[p] = f4(its.p, kEpsilon);
+                    rs.sh_d[p] = f4(ds.d, ds.dist - kEpsilon);
+                    flags |= kFlagNee;
+                    want_shadow = true;
+                } else {
+                    rad = add(rad, contrib);
+                }
+            }
+            // ---- BSDF sampling
+            BRec br;
+            br.wi = wi;
+            br.wo = v3(0, 0, 0);
+            br.eta = 1.f;
+            br.measure = kMeasureUnknown;
+            const float bx = rng.next_float();
+            const float by = rng.next_float();
+            const vec3 fr = bsdf_sample(type, prm, br, bx, by);
+            thr = mul(thr, fr);
+            eta *= br.eta;
+            const float prev_pdf = bsdf_pdf(type, prm, br);
+            if (bsdf_is_diffuse(type)) flags |= kFlagDiffuse;
+            if (!is_zero(thr)) {
+                rs.ray_o[p] = f4(its.p, kEpsilon);
+                rs.ray_d[p] = f4(to_world(its.sh, br.wo), INFINITY);
+                flags |= kFlagCont;
+                want_cont = true;
+            }
+            rs.thr[p] = f4(thr, eta);
+            rs.rad[p] = f4(rad, 0.f);
+            rs.rng[p] = rng.state;
+            aux.x = __float_as_int(prev_pdf);
+            aux.y = (int)flags;
+            rs.aux[p] = aux;
+        }
+        const uint32_t ss = wave_append(&rs.cnt->shadow, want_shadow);
+        if (want_shadow) rs.shq[ss] = p;
+        const uint32_t cs = wave_append(&rs.cnt->ray, want_cont);
+        if (want_cont) rs.rayq[cs] = p;
+    }
+}
+
+// NEE resolution, emitter hit / miss with MIS, Russian roulette
+// (path.cpp:46-56, 71-97; miss.cu:31-42).
+__global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s, RenderBufs rs, int cur) {
+    const uint32_t* counts = rs.cnt->mat[cur];
+    const uint32_t n = queue_total(counts);
+    const int next = cur ^ 1;
+    const uint32_t WH = pp.W * pp.H;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        bool alive = false;
+        int ntype = 0;
+        uint32_t p = 0;
+        if (i < n) {
+            int type;
+            queue_entry(counts, rs.matq[cur], pp.cap, i, p, type);
+            int4 aux = rs.aux[p];
+            const uint32_t flags = (uint32_t)aux.y;
+            vec3 rad = f3(rs.rad[p]);
+            if ((flags & kFlagNee) && rs.occ[p] == 0) rad = add(rad, f3(rs.nee[p]));
+            if (flags & kFlagCont) {
+                const float4 th4 = rs.thr[p];
+                vec3 thr = f3(th4);
+                const float eta = th4.w;
+                const float prev_pdf = __int_as_float(aux.x);
+                const bool diffuse = (flags & kFlagDiffuse) != 0;
+                const float4 h = rs.hit[p];
+                const uint32_t prim = __float_as_uint(h.x);
+                const vec3 o = f3(rs.ray_o[p]);
+                const vec3 d = f3(rs.ray_d[p]);
+                if (prim == 0xffffffffu) {
+                    if (s.env_light >= 0) {
+                        const LightRec& e = s.lights[s.env_light];
+                        const vec3 ev = env_eval(s, e, d);
+                        DirSample hs;
+                        hs.d = d;
+                        const float epdf = pdf_emitter_direction(s, s.env_light, hs);
+                        const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
+                        rad = add(rad, mul(scale(thr, mis), ev));
+                    }
+                } else {
+                    const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
+                    if (mr.light >= 0) {
+                        vec3 hp, hn;
+                        hit_point_normal(s, prim, h.z, h.w, hp, hn);
+                        const vec3 ev = dot(hn, neg(d)) > 0.f ? pcol(s.lights[mr.light].radiance) : v3(0, 0, 0);
+                        const DirSample hs = dir_sample(o, hp, hn);
+                        const float epdf = pdf_emitter_direction(s, mr.light, hs);
+                        const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
+                        rad = add(rad, mul(scale(thr, mis), ev));
+                    }
+                    alive = true;
+                    const int depth = aux.z;
+                    if (depth >= pp.rr_depth) {
+                        Pcg32 rng;
+                        rng.state = rs.rng[p];
+                        rng.inc = ((uint64_t)(pp.sample_base + p / WH) << 1u) | 1u;
+                        const float pr = fmin0(cmax(scale(scale(thr, eta), eta)), 0.99f);
+                        if (rng.next_float() > pr) {
+                            alive = false;
+                        } else {
+                            thr = divs(thr, pr);
+                            rs.thr[p] = f4(thr, eta);
+                        }
+                        rs.rng[p] = rng.state;
+                    }
+                    if (alive) {
+                        const int nd = depth + 1;
+                        if (pp.max_depth >= 0 && nd + 2 > pp.max_depth) alive = false;
+                        aux.z = nd;
+                        rs.aux[p] = aux;
+                        int mat;
+                        ntype = material_of(s, prim, mat);
+                    }
+                }
+            }
+            rs.rad[p] = f4(rad, 0.f);
+        }
+#pragma unroll
+        for (int t = 0; t < DL_BSDF_COUNT; t++) {
+            const bool mine = alive && ntype == t;
+            const uint32_t slot = wave_append(&rs.cnt->mat[next][t], mine);
+            if (mine) rs.matq[next][(size_t)t * pp.cap + slot] = p;
+        }
+    }
+}
+
+__device__ __forceinline__ bool sample_valid(const float4& c) {  // Color3f::isValid  color.h:50-58
+    return !(c.x < 0 || !isfinite(c.x) || c.y < 0 || !isfinite(c.y) || c.z < 0 || !isfinite(c.z));
+}
+
+// Deterministic gather of one pass's samples into the film: pixel q receives
+// every sample whose filter footprint, computed exactly as ImageBlock::put
+// computes it inside the sample's 32x32 block, covers q.
+__global__ __launch_bounds__(kStageBlock) void k_film(PassParams pp, uint32_t k, dl_film_desc film, RenderBufs rs) {
+    const uint32_t W = pp.W, H = pp.H, WH = W * H;
+    const int b = film.border;
+    const float r = film.radius;
+    const int R = (int)ceilf(r) + 1;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < WH; q += gridDim.x * blockDim.x) {
+        const int qx = (int)(q % W), qy = (int)(q / W);
+        float4 acc = rs.film[q];
+        uint32_t invalid = 0;
+        for (int py = max(0, qy - R); py <= min((int)H - 1, qy + R); ++py) {
+            const int oy = (py / 32) * 32;
+            const int sy = min(32, (int)H - oy);
+            const int yl = qy - (oy - b);
+            if (yl < 0 || yl > sy + 2 * b - 1) continue;
+            for (int px = max(0, qx - R); px <= min((int)W - 1, qx + R); ++px) {
+                const int ox = (px / 32) * 32;
+                const int sx = min(32, (int)W - ox);
+                const int xl = qx - (ox - b);
+                if (xl < 0 || xl > sx + 2 * b - 1) continue;
+                for (uint32_t j = 0; j < k; ++j) {
+                    const uint32_t sp = j * WH + (uint32_t)py * W + (uint32_t)px;
+                    const float4 L = rs.rad[sp];
+                    if (!sample_valid(L)) {
+                        if (px == qx && py == qy) invalid++;
+                        continue;
+                    }
+                    const float2 P = rs.pos[sp];
+                    const float plx = P.x - 0.5f - (float)(ox - b);
+                    const float ply = P.y - 0.5f - (float)(oy - b);
+                    const int x0 = max((int)ceilf(plx - r), 0), x1 = min((int)floorf(plx + r), sx + 2 * b - 1);
+                    const int y0 = max((int)ceilf(ply - r), 0), y1 = min((int)floorf(ply + r), sy + 2 * b - 1);
+                    if (xl < x0 || xl > x1 || yl < y0 || yl > y1) continue;
+                    const float wx = film.table[(int)(fabsf((float)xl - plx) * film.lookup)];
+                    const float wy = film.table[(int)(fabsf((float)yl - ply) * film.lookup)];
+                    acc.x += (L.x * wx) * wy;
+                    acc.y += (L.y * wx) * wy;
+                    acc.z += (L.z * wx) * wy;
+                    acc.w += (1.0f * wx) * wy;
+                }
+            }
+        }
+        rs.film[q] = acc;
+        if (invalid) atomicAdd(&rs.cnt->invalid, (unsigned long long)invalid);
+    }
+}
+
+// ImageBlock::toBitmap / Color4f::divideByFilterWeight  block.cpp:58-66, color.h:120-125
+__global__ __launch_bounds__(kStageBlock) void k_resolve(const float4* film, float* rgb, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 c = film[i];
+        if (c.w != 0) {
+            rgb[3 * i] = c.x / c.w;
+            rgb[3 * i + 1] = c.y / c.w;
+            rgb[3 * i + 2] = c.z / c.w;
+        } else {
+            rgb[3 * i] = rgb[3 * i + 1] = rgb[3 * i + 2] = 0.0f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+void free_render_state(Ctx* c) {
+    RenderState* rs = (RenderState*)c->rs;
+    if (!rs) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : rs->allocs) (void)hipFree(p);
+    if (rs->cnt_host) (void)hipHostFree(rs->cnt_host);
+    for (hipEvent_t e : rs->evs) (void)hipEventDestroy(e);
+    delete rs;
+    c->rs = nullptr;
+}
+
+namespace {
+
+template <typename T>
+dl_status alloc(Ctx* c, RenderState* rs, T** p, size_t count) {
+    void* q = nullptr;
+    DL_HIP_CHECK(c, hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T)));
+    rs->allocs.push_back(q);
+    *p = (T*)q;
+    return DL_OK;
+}
+
+#define DL_TRY(expr)                \
+    do {                            \
+        dl_status s_ = (expr);      \
+        if (s_ != DL_OK) return s_; \
+    } while (0)
+
+dl_status ensure_state(Ctx* c, uint32_t P, uint32_t pixels) {
+    RenderState* rs = (RenderState*)c->rs;
+    if (rs && rs->capacity >= P && rs->film_pixels >= pixels) return DL_OK;
+    free_render_state(c);
+    rs = new RenderState();
+    c->rs = rs;
+    rs->capacity = P;
+    rs->film_pixels = pixels;
+    DL_TRY(alloc(c, rs, &rs->b.ray_o, P));
+    DL_TRY(alloc(c, rs, &rs->b.ray_d, P));
+    DL_TRY(alloc(c, rs, &rs->b.hit, P));
+    DL_TRY(alloc(c, rs, &rs->b.sh_o, P));
+    DL_TRY(alloc(c, rs, &rs->b.sh_d, P));
+    DL_TRY(alloc(c, rs, &rs->b.thr, P));
+    DL_TRY(alloc(c, rs, &rs->b.rad, P));
+    DL_TRY(alloc(c, rs, &rs->b.nee, P));
+    DL_TRY(alloc(c, rs, &rs->b.aux, P));
+    DL_TRY(alloc(c, rs, &rs->b.rng, P));
+    DL_TRY(alloc(c, rs, &rs->b.pos, P));
+    DL_TRY(alloc(c, rs, &rs->b.occ, P));
+    DL_TRY(alloc(c, rs, &rs->b.matq[0], (size_t)P * kTypes));
+    DL_TRY(alloc(c, rs, &rs->b.matq[1], (size_t)P * kTypes));
+    DL_TRY(alloc(c, rs, &rs->b.rayq, P));
+    DL_TRY(alloc(c, rs, &rs->b.shq, P));
+    DL_TRY(alloc(c, rs, &rs->b.cnt, 1));
+    DL_TRY(alloc(c, rs, &rs->b.film, pixels));
+    DL_TRY(alloc(c, rs, &rs->b.rgb, (size_t)pixels * 3));
+    DL_HIP_CHECK(c, hipHostMalloc((void**)&rs->cnt_host, sizeof(Counters), hipHostMallocDefault));
+    return DL_OK;
+}
+
+SceneView scene_view(Ctx* c) {
+    SceneView s;
+    const DevScene& d = c->ds;
+    s.pos = d.pos;
+    s.nrm = d.nrm;
+    s.tan = d.tan;
+    s.idx = d.idx;
+    s.prim_mesh = d.prim_mesh;
+    s.meshes = (const MeshRec*)d.meshes;
+    s.mats = (const dl_material_desc*)d.materials;
+    s.lights = (const LightRec*)d.lights;
+    s.light_cdf = d.light_cdf;
+    s.env_rgb = d.env_rgb;
+    s.env_q = d.env_q;
+    s.env_alias = d.env_alias;
+    s.env_pdf = d.env_pdf;
+    s.num_lights = (int)c->light_recs.size();
+    s.env_light = c->env_light;
+    for (int k = 0; k < 3; k++) s.bg[k] = c->bg[k];
+    return s;
+}
+
+struct Timer {
+    Ctx* c;
+    RenderState* rs;
+    std::vector<int>* tags;
+    dl_status start(int tag) {
+        const size_t k = tags->size();
+        while (rs->evs.size() < 2 * (k + 1)) {
+            hipEvent_t e;
+            DL_HIP_CHECK(c, hipEventCreate(&e));
+            rs->evs.push_back(e);
+        }
+        tags->push_back(tag);
+        DL_HIP_CHECK(c, hipEventRecord(rs->evs[2 * k], c->stream));
+        return DL_OK;
+    }
+    dl_status stop() {
+        const size_t k = tags->size() - 1;
+        DL_HIP_CHECK(c, hipEventRecord(rs->evs[2 * k + 1], c->stream));
+        return DL_OK;
+    }
+};
+
+enum { kTagClosest = 0, kTagShadow = 1, kTagShade = 2, kTagFilm = 3, kTagOther = 4 };
+
+}  // namespace
+
+dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render_stats* stats, int paths_sample,
+                 float* paths_out) {
+    if (prm->spp <= 0) return set_error(c, DL_ERR_INVALID, "spp must be positive");
+    if (c->light_recs.empty()) return set_error(c, DL_ERR_INVALID, "There is not emitter in the scene.");
+    (void)hipSetDevice(c->device);
+    const uint32_t W = (uint32_t)c->cam.width, H = (uint32_t)c->cam.height, WH = W * H;
+    uint32_t k = prm->spp_per_pass > 0 ? (uint32_t)prm->spp_per_pass : std::max(1u, (2u << 20) / WH);
+    k = std::min<uint32_t>(k, (uint32_t)prm->spp);
+    if (paths_sample >= 0) k = 1;
+    if ((uint64_t)WH * k > (1ull << 27)) return set_error(c, DL_ERR_INVALID, "pass too large");
+    const uint32_t cap = WH * k;
+    DL_TRY(ensure_state(c, cap, WH));
+    RenderState* rs = (RenderState*)c->rs;
+    const SceneView sv = scene_view(c);
+    const BvhView bvh{c->ds.nodes, c->ds.tris};
+    const bool count = stats && (prm->flags & DL_RENDER_COUNT_TRAVERSAL);
+    const int stage_grid = std::max(1, c->num_cus) * 8;
+    const int trace_grid = (int)c->trace_blocks;
+    std::vector<int> tags;
+    Timer tm{c, rs, &tags};
+    double closest = 0, shadow = 0, paths = 0;
+
+    DL_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
+    k_reset_all<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt);
+    k_clear_film<<<stage_grid, kStageBlock, 0, c->stream>>>(rs->b.film, WH);
+    const uint32_t total_spp = paths_sample >= 0 ? 1u : (uint32_t)prm->spp;
+    for (uint32_t done = 0; done < total_spp;) {
+        const uint32_t kk = std::min(k, total_spp - done);
+        PassParams pp;
+        pp.P = WH * kk;
+        pp.W = W;
+        pp.H = H;
+        pp.sample_base = paths_sample >= 0 ? (uint32_t)paths_sample : (uint32_t)prm->first_sample + done;
+        pp.seed = prm->seed;
+        pp.max_depth = prm->max_depth;
+        pp.rr_depth = prm->rr_depth;
+        pp.cap = cap;
+        paths += pp.P;
+        closest += pp.P;
+        int cur = 0;
+        k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, 0);
+        k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, 1);
+        DL_TRY(tm.start(kTagOther));
+        k_raygen<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, c->cam, rs->b);
+        DL_TRY(tm.stop());
+        DL_TRY(tm.start(kTagClosest));
+        if (count)
+            k_trace_queue<true><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
+        else
+            k_trace_queue<false><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
+        DL_TRY(tm.stop());
+        DL_TRY(tm.start(kTagShade));
+        k_post_camera<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
+        DL_TRY(tm.stop());
+        DL_HIP_CHECK(c, hipGetLastError());
+        for (int iter = 0;; ++iter) {
+            DL_HIP_CHECK(c, hipMemcpyAsync(rs->cnt_host, rs->b.cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+            DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+            uint32_t live = 0;
+            for (int t = 0; t < kTypes; t++) live += rs->cnt_host->mat[cur][t];
+            if (iter > 0) {
+                closest += rs->cnt_host->ray;
+                shadow += rs->cnt_host->shadow;
+            }
+            if (live == 0) break;
+            k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, cur ^ 1);
+            DL_TRY(tm.start(kTagShade));
+            k_shade<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
+            DL_TRY(tm.stop());
+            DL_TRY(tm.start(kTagClosest));
+            if (count)
+                k_trace_queue<true><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
+            else
+                k_trace_queue<false><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
+            DL_TRY(tm.stop());
+            DL_TRY(tm.start(kTagShadow));
+            k_shadow_queue<<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
+            DL_TRY(tm.stop());
+            DL_TRY(tm.start(kTagShade));
+            k_post<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
+            DL_TRY(tm.stop());
+            DL_HIP_CHECK(c, hipGetLastError());
+            cur ^= 1;
+        }
+        if (paths_sample >= 0) break;
+        DL_TRY(tm.start(kTagFilm));
+        k_film<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, kk, c->film, rs->b);
+        DL_TRY(tm.stop());
+        done += kk;
+    }
+    if (paths_sample < 0) k_resolve<<<stage_grid, kStageBlock, 0, c->stream>>>(rs->b.film, rs->b.rgb, WH);
+    DL_HIP_CHECK(c, hipGetLastError());
+    DL_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
+    DL_HIP_CHECK(c, hipEventSynchronize(c->ev1));
+    if (paths_out) {
+        std::vector<float2> pos(WH);
+        std::vector<float4> rad(WH);
+        DL_HIP_CHECK(c, hipMemcpy(pos.data(), rs->b.pos, WH * sizeof(float2), hipMemcpyDeviceToHost));
+        DL_HIP_CHECK(c, hipMemcpy(rad.data(), rs->b.rad, WH * sizeof(float4), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < WH; i++) {
+            paths_out[5 * i] = pos[i].x;
+            paths_out[5 * i + 1] = pos[i].y;
+            paths_out[5 * i + 2] = rad[i].x;
+            paths_out[5 * i + 3] = rad[i].y;
+            paths_out[5 * i + 4] = rad[i].z;
+        }
+    }
+    if (rgb_host) DL_HIP_CHECK(c, hipMemcpy(rgb_host, rs->b.rgb, (size_t)WH * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        float ms = 0.f;
+        DL_HIP_CHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        stats->ms_total = ms;
+        for (size_t i = 0; i < tags.size(); i++) {
+            float e = 0.f;
+            DL_HIP_CHECK(c, hipEventElapsedTime(&e, rs->evs[2 * i], rs->evs[2 * i + 1]));
+            switch (tags[i]) {
+                case kTagClosest: stats->ms_closest += e; stats->launches_closest += 1; break;
+                case kTagShadow: stats->ms_shadow += e; stats->launches_shadow += 1; break;
+                case kTagShade: stats->ms_shade += e; break;
+                case kTagFilm: stats->ms_film += e; break;
+                default: stats->ms_other += e; break;
+            }
+        }
+        DL_HIP_CHECK(c, hipMemcpy(rs->cnt_host, rs->b.cnt, sizeof(Counters), hipMemcpyDeviceToHost));
+        stats->paths = paths;
+        stats->closest_rays = closest;
+        stats->shadow_rays = shadow;
+        stats->invalid_samples = (double)rs->cnt_host->invalid;
+        stats->nodes_closest = (double)rs->cnt_host->nodes;
+        stats->tris_closest = (double)rs->cnt_host->tris;
+    }
+    return DL_OK;
+}
+
+}  // namespace dl

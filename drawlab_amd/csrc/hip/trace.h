@@ -1,0 +1,203 @@
+// BVH traversal for gfx950: one ray per lane, child-pair nodes (bvh_layout.h),
+// per-lane traversal stack staged in LDS with a lane-interleaved layout
+// (entry k of lane l at stack[k * blockDim + l]: every wave touches 64
+// consecutive dwords whatever the stack depths, so the accesses are
+// bank-conflict free).
+//
+// Closest hit: the binary32 Moller-Trumbore test of the reference
+// (src/tracer/mesh/mesh.cpp:180-220), accepting t in [tmin, t_best] and
+// keeping the lowest primitive id on equal t, so the result equals the
+// brute-force argmin over all triangles bit-for-bit.  Any hit: the first
+// accepted triangle (src/tracer/accel/octree.cpp:112-133).
+#pragma once
+#include "../bvh_layout.h"
+#include "backend.h"
+#include "dl_math.h"
+
+namespace dl {
+
+constexpr int kTraceBlock = kTraceBlockHost;
+// Slack on the far slab distance: absorbs the rounding of the slab
+// arithmetic and of Moller-Trumbore's t (Ize, "Robust BVH ray traversal", 2013).
+constexpr float kFarSlack = 1.0f + 1.0f / 65536.0f;
+
+struct BvhView {
+    const float4* __restrict__ nodes;
+    const float4* __restrict__ tris;
+};
+
+struct TraceCounters {
+    uint32_t nodes = 0;
+    uint32_t tris = 0;
+};
+
+// Lane-private traversal stack: entries [0, kLdsStack) in LDS
+// (lane-interleaved), [kLdsStack, kMaxStack) in a global overflow buffer
+// (entry k of grid thread g at ovf[k * grid_threads + g], coalesced).
+struct LaneStack {
+    int* __restrict__ lds;
+    int lane, stride;
+    int* __restrict__ ovf;
+    uint32_t ostride;
+    // No tree is deeper than kMaxStack, so the last branch is never taken;
+    // it keeps every access in bounds regardless.
+    __device__ __forceinline__ void push(int& sp, int v) {
+        if (sp < kLdsStack)
+            lds[sp * stride + lane] = v;
+        else if (sp < kMaxStack)
+            ovf[(size_t)(sp - kLdsStack) * ostride] = v;
+        else
+            return;
+        ++sp;
+    }
+    __device__ __forceinline__ int pop(int& sp) {
+        --sp;
+        return sp < kLdsStack ? lds[sp * stride + lane] : ovf[(size_t)(sp - kLdsStack) * ostride];
+    }
+};
+
+__device__ __forceinline__ void slab2(const float4& r0, const float4& r1, const float4& r2, const vec3& o,
+                                      const vec3& rcp, float tmin, float tmax, bool& h0, float& n0, bool& h1,
+                                      float& n1) {
+    float a0 = (r0.x - o.x) * rcp.x, b0 = (r0.y - o.x) * rcp.x;
+    float a1 = (r1.x - o.y) * rcp.y, b1 = (r1.y - o.y) * rcp.y;
+    float a2 = (r2.x - o.z) * rcp.z, b2 = (r2.y - o.z) * rcp.z;
+    n0 = fmaxf(fmaxf(fminf(a0, b0), fminf(a1, b1)), fmaxf(fminf(a2, b2), tmin));
+    float f0 = fminf(fminf(fmaxf(a0, b0), fmaxf(a1, b1)), fminf(fmaxf(a2, b2), tmax));
+    h0 = n0 <= f0 * kFarSlack;
+    float c0 = (r0.z - o.x) * rcp.x, d0 = (r0.w - o.x) * rcp.x;
+    float c1 = (r1.z - o.y) * rcp.y, d1 = (r1.w - o.y) * rcp.y;
+    float c2 = (r2.z - o.z) * rcp.z, d2 = (r2.w - o.z) * rcp.z;
+    n1 = fmaxf(fmaxf(fminf(c0, d0), fminf(c1, d1)), fmaxf(fminf(c2, d2), tmin));
+    float f1 = fminf(fminf(fmaxf(c0, d0), fmaxf(c1, d1)), fminf(fmaxf(c2, d2), tmax));
+    h1 = n1 <= f1 * kFarSlack;
+}
+
+// Moller-Trumbore on a pre-gathered triangle; returns false on a miss.
+__device__ __forceinline__ bool mt_test(const float4& a, const float4& b, const float4& c, const vec3& o,
+                                        const vec3& d, float& u, float& v, float& t) {
+    vec3 p0 = v3(a.x, a.y, a.z);
+    vec3 e1 = v3(a.w, b.x, b.y);
+    vec3 e2 = v3(b.z, b.w, c.x);
+    vec3 pvec = cross(d, e2);
+    float det = dot(e1, pvec);
+    if (det > -1e-8f && det < 1e-8f) return false;
+    float inv = 1.0f / det;
+    vec3 tvec = sub(o, p0);
+    u = dot(tvec, pvec) * inv;
+    if (u < 0.0f || u > 1.0f) return false;
+    vec3 qvec = cross(tvec, e1);
+    v = dot(d, qvec) * inv;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = dot(e2, qvec) * inv;
+    return true;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void trace_closest(const BvhView& bvh, vec3 o, vec3 d, float tmin, float tmax,
+                                              LaneStack& stk, uint32_t& hit_prim, float& hit_t, float& hit_u,
+                                              float& hit_v, TraceCounters& cnt) {
+    const vec3 rcp = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    float best_t = tmax, bu = 0.0f, bv = 0.0f;
+    uint32_t best = 0xffffffffu;
+    int sp = 0;
+    int cur = 0;
+    for (;;) {
+        const float4* np = bvh.nodes + 4 * cur;
+        const float4 r0 = np[0], r1 = np[1], r2 = np[2];
+        const int4 r3 = reinterpret_cast<const int4*>(np)[3];
+        if (COUNT) cnt.nodes++;
+        bool h0, h1;
+        float n0, n1;
+        slab2(r0, r1, r2, o, rcp, tmin, best_t, h0, n0, h1, n1);
+        // Leaf children are intersected at once.
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const bool h = side ? h1 : h0;
+            const int ref = side ? r3.y : r3.x;
+            if (h && ref < 0) {
+                const int first = ~ref, count = side ? r3.w : r3.z;
+                for (int k = first; k < first + count; ++k) {
+                    const float4* tp = bvh.tris + 3 * k;
+                    const float4 ta = tp[0], tb = tp[1], tc = tp[2];
+                    if (COUNT) cnt.tris++;
+                    float u, v, t;
+                    if (mt_test(ta, tb, tc, o, d, u, v, t) && t >= tmin && t <= best_t) {
+                        const uint32_t id = __float_as_uint(tc.y);
+                        if (t < best_t || id < best) {
+                            best_t = t;
+                            best = id;
+                            bu = u;
+                            bv = v;
+                        }
+                    }
+                }
+                if (side) h1 = false; else h0 = false;
+            }
+        }
+        if (h0 && h1) {
+            const bool first0 = n0 <= n1;
+            const int nearc = first0 ? r3.x : r3.y, farc = first0 ? r3.y : r3.x;
+            stk.push(sp, farc);
+            cur = nearc;
+        } else if (h0) {
+            cur = r3.x;
+        } else if (h1) {
+            cur = r3.y;
+        } else {
+            if (sp == 0) break;
+            cur = stk.pop(sp);
+        }
+    }
+    hit_prim = best;
+    hit_t = best == 0xffffffffu ? tmax : best_t;
+    hit_u = bu;
+    hit_v = bv;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ bool trace_any(const BvhView& bvh, vec3 o, vec3 d, float tmin, float tmax,
+                                          LaneStack& stk, TraceCounters& cnt) {
+    const vec3 rcp = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    int sp = 0;
+    int cur = 0;
+    for (;;) {
+        const float4* np = bvh.nodes + 4 * cur;
+        const float4 r0 = np[0], r1 = np[1], r2 = np[2];
+        const int4 r3 = reinterpret_cast<const int4*>(np)[3];
+        if (COUNT) cnt.nodes++;
+        bool h0, h1;
+        float n0, n1;
+        slab2(r0, r1, r2, o, rcp, tmin, tmax, h0, n0, h1, n1);
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const bool h = side ? h1 : h0;
+            const int ref = side ? r3.y : r3.x;
+            if (h && ref < 0) {
+                const int first = ~ref, count = side ? r3.w : r3.z;
+                for (int k = first; k < first + count; ++k) {
+                    const float4* tp = bvh.tris + 3 * k;
+                    const float4 ta = tp[0], tb = tp[1], tc = tp[2];
+                    if (COUNT) cnt.tris++;
+                    float u, v, t;
+                    if (mt_test(ta, tb, tc, o, d, u, v, t) && t >= tmin && t <= tmax) return true;
+                }
+                if (side) h1 = false; else h0 = false;
+            }
+        }
+        if (h0 && h1) {
+            stk.push(sp, r3.y);
+            cur = r3.x;
+        } else if (h0) {
+            cur = r3.x;
+        } else if (h1) {
+            cur = r3.y;
+        } else {
+            if (sp == 0) break;
+            cur = stk.pop(sp);
+        }
+    }
+    return false;
+}
+
+}  // namespace dl

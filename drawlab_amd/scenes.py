@@ -1,0 +1,331 @@
+"""Seeded procedural scenes for the benchmark workloads.
+
+The reference ships no scenes, so every workload is generated in code:
+  cornell_box       Cornell box, 5 walls + 2 axis-aligned boxes + ceiling light
+                    (36 triangles), 512x512, 64 spp, max depth 8, RR from 3
+  displaced_sphere  2,097,152-triangle octahedral sphere displaced by 4-octave
+                    value noise (GGX conductor), 20,480-triangle glass
+                    icosphere, diffuse ground, 1024x512 sky + sun env map,
+                    1920x1080, 256 spp, max depth 16
+  mis_plates        4 GGX plates (last anisotropic), 4 icosphere lights of equal
+                    power, dark backdrop, 1280x720, 512 spp, max depth 6
+  random_scene      displaced spheres and boxes with random materials and
+                    1-8 area lights (render-service job mix)
+"""
+import math
+
+import numpy as np
+
+from .scene import Bsdf, Camera, EnvMap, RFilter, SceneDesc, FILTER_GAUSSIAN
+
+
+# ---------------------------------------------------------------------------
+# primitives
+# ---------------------------------------------------------------------------
+def quad(p, e0, e1, with_uv=False):
+    """Two triangles (p, p+e0, p+e1), (p+e0+e1, p+e1, p+e0); normal = e0 x e1
+    (the reference's `rectangle` mesh, src/tracer/mesh/rectangle.cpp:11-40)."""
+    p, e0, e1 = (np.asarray(a, np.float64) for a in (p, e0, e1))
+    V = np.array([p, p + e0, p + e1, p + e0 + e1], np.float32)
+    F = np.array([[0, 1, 2], [3, 2, 1]], np.uint32)
+    UV = np.array([[1, 0], [1, 1], [0, 0], [0, 1]], np.float32) if with_uv else None
+    return V, F, UV
+
+
+def merge(parts):
+    Vs, Fs, off = [], [], 0
+    for V, F in parts:
+        Vs.append(V)
+        Fs.append(F + off)
+        off += V.shape[0]
+    return np.concatenate(Vs).astype(np.float32), np.concatenate(Fs).astype(np.uint32)
+
+
+def box(lo, hi):
+    """Axis-aligned box as 6 outward-facing quads (12 triangles)."""
+    lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    d = hi - lo
+    X, Y, Z = np.array([d[0], 0, 0]), np.array([0, d[1], 0]), np.array([0, 0, d[2]])
+    faces = [
+        (lo, X, Z),            # bottom, -y
+        (lo + Y, Z, X),        # top, +y
+        (lo, Y, X),            # front (z = lo), -z
+        (lo + Z, X, Y),        # back (z = hi), +z
+        (lo, Z, Y),            # x = lo, -x
+        (lo + X, Y, Z),        # x = hi, +x
+    ]
+    return merge([quad(p, a, b)[:2] for p, a, b in faces])
+
+
+def icosphere(level, radius=1.0, center=(0.0, 0.0, 0.0)):
+    """Subdivided icosahedron: 20 * 4**level triangles, outward winding."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.array(verts) * radius + np.asarray(center, np.float64)
+    F = np.array(faces, np.uint32)
+    return _orient_outward(V.astype(np.float32), F, center)
+
+
+def _orient_outward(V, F, center):
+    p0, p1, p2 = V[F[:, 0]].astype(np.float64), V[F[:, 1]].astype(np.float64), V[F[:, 2]].astype(np.float64)
+    n = np.cross(p1 - p0, p2 - p0)
+    c = (p0 + p1 + p2) / 3.0 - np.asarray(center, np.float64)
+    flip = (n * c).sum(1) < 0
+    F = F.copy()
+    F[flip] = F[flip][:, [0, 2, 1]]
+    return V, F
+
+
+def value_noise(p, seed, octaves=4, base_freq=2.0):
+    """Seeded 3-D value noise (smoothstep-interpolated lattice values in
+    [-1, 1]) summed over octaves with halving amplitude, normalised to [-1, 1]."""
+    p = np.asarray(p, np.float64)
+    total = np.zeros(p.shape[0])
+    amp, norm = 1.0, 0.0
+    for o in range(octaves):
+        f = base_freq * (2 ** o)
+        q = p * f
+        i0 = np.floor(q).astype(np.int64)
+        fr = q - i0
+        w = fr * fr * (3.0 - 2.0 * fr)
+        acc = np.zeros(p.shape[0])
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    h = _hash3(i0[:, 0] + dx, i0[:, 1] + dy, i0[:, 2] + dz, seed + 7919 * o)
+                    wx = w[:, 0] if dx else 1 - w[:, 0]
+                    wy = w[:, 1] if dy else 1 - w[:, 1]
+                    wz = w[:, 2] if dz else 1 - w[:, 2]
+                    acc += wx * wy * wz * h
+        total += amp * acc
+        norm += amp
+        amp *= 0.5
+    return total / norm
+
+
+def _hash3(x, y, z, seed):
+    h = (x * 73856093) ^ (y * 19349663) ^ (z * 83492791) ^ (seed * 2654435761)
+    h = h & 0xFFFFFFFF
+    h = (h ^ (h >> 16)) * 0x7FEB352D & 0xFFFFFFFF
+    h = (h ^ (h >> 15)) * 0x846CA68B & 0xFFFFFFFF
+    h = h ^ (h >> 16)
+    return (h.astype(np.float64) / 4294967295.0) * 2.0 - 1.0
+
+
+def octa_sphere(n, radius=1.0, center=(0.0, 0.0, 0.0), displace=0.0, seed=0):
+    """Octahedral-parametrised sphere on an n x n grid: 2 n^2 triangles
+    (n = 1024 gives 2,097,152).  Quads are split along the diagonal parallel
+    to the octahedron's edges so no triangle straddles one.  Optional radial
+    displacement radius * (1 + displace * noise)."""
+    g = np.linspace(-1.0, 1.0, n + 1)
+    u, v = np.meshgrid(g, g, indexing="ij")
+    u, v = u.reshape(-1), v.reshape(-1)
+    z = 1.0 - np.abs(u) - np.abs(v)
+    x, y = u.copy(), v.copy()
+    neg = z < 0
+    x[neg] = (1.0 - np.abs(v[neg])) * np.sign(u[neg])
+    y[neg] = (1.0 - np.abs(u[neg])) * np.sign(v[neg])
+    # sign(0) = 0 would collapse points on the axes; those lie on |u|+|v|=... only when z<0 with u or v = 0
+    x[neg & (u == 0)] = 0.0
+    y[neg & (v == 0)] = 0.0
+    P = np.stack([x, y, z], 1)
+    P /= np.linalg.norm(P, axis=1, keepdims=True)
+    # boundary points of the square map to the same sphere point: snap them
+    # through the same formula so duplicates are bit-identical
+    r = np.full(P.shape[0], radius, np.float64)
+    if displace:
+        r = radius * (1.0 + displace * value_noise(P, seed))
+    V = (P * r[:, None] + np.asarray(center, np.float64)).astype(np.float32)
+    # canonicalise duplicates (boundary of the octahedral map)
+    key = np.round(P * 1e9).astype(np.int64)
+    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    V = V[first[inv.reshape(-1)]]
+    i = np.arange(n)
+    I, J = np.meshgrid(i, i, indexing="ij")
+    I, J = I.reshape(-1), J.reshape(-1)
+    a = I * (n + 1) + J
+    b = (I + 1) * (n + 1) + J
+    c = I * (n + 1) + J + 1
+    d = (I + 1) * (n + 1) + J + 1
+    uc = (I + 0.5) / n * 2 - 1
+    vc = (J + 0.5) / n * 2 - 1
+    same = (uc * vc) > 0  # quadrants where the edge line has slope -1 in (u, v)
+    t1 = np.where(same[:, None], np.stack([a, b, c], 1), np.stack([a, b, d], 1))
+    t2 = np.where(same[:, None], np.stack([b, d, c], 1), np.stack([a, d, c], 1))
+    F = np.concatenate([t1, t2]).astype(np.uint32)
+    return _orient_outward(V, F, center)
+
+
+# ---------------------------------------------------------------------------
+# scenes
+# ---------------------------------------------------------------------------
+def cornell_box(width=512, height=512, spp=64, max_depth=8, rr_depth=3):
+    """Cornell box in metres (x to the left of the camera, y up, z into the
+    scene), the classic camera (eye (0.278, 0.273, -0.8), fov 39.3)."""
+    X, Y, Z = 0.556, 0.549, 0.559
+    s = SceneDesc(name="cornell_box")
+    white = s.add_bsdf(Bsdf.diffuse((0.73, 0.73, 0.73)))
+    red = s.add_bsdf(Bsdf.diffuse((0.63, 0.06, 0.05)))
+    green = s.add_bsdf(Bsdf.diffuse((0.14, 0.45, 0.09)))
+    walls = [
+        ((0, 0, 0), (0, 0, Z), (X, 0, 0), white),   # floor
+        ((0, Y, 0), (X, 0, 0), (0, 0, Z), white),   # ceiling
+        ((0, 0, Z), (0, Y, 0), (X, 0, 0), white),   # back wall
+        ((X, 0, 0), (0, 0, Z), (0, Y, 0), red),     # left wall (x = X)
+        ((0, 0, 0), (0, Y, 0), (0, 0, Z), green),   # right wall (x = 0)
+    ]
+    for p, e0, e1, m in walls:
+        V, F, _ = quad(p, e0, e1)
+        s.add_mesh(V, F, m, name="wall")
+    V, F = box((0.130, 0.0, 0.065), (0.295, 0.165, 0.230))
+    s.add_mesh(V, F, white, name="short_box")
+    V, F = box((0.265, 0.0, 0.295), (0.430, 0.330, 0.460))
+    s.add_mesh(V, F, white, name="tall_box")
+    lw, ld = 0.130, 0.105
+    V, F, _ = quad((X / 2 - lw / 2, Y - 1e-4, Z / 2 - ld / 2), (lw, 0, 0), (0, 0, ld))
+    s.add_mesh(V, F, white, radiance=(17.0, 12.0, 4.0), name="light")
+    s.camera = Camera(eye=(0.278, 0.273, -0.8), lookat=(0.278, 0.273, 0.0), up=(0, 1, 0), fov=39.3,
+                      width=width, height=height)
+    s.rfilter = RFilter(FILTER_GAUSSIAN, 2.0, 0.5)
+    s.spp, s.max_depth, s.rr_depth = spp, max_depth, rr_depth
+    return s
+
+
+def sky_envmap(width=1024, height=512, sun_dir=(0.5, 0.7071, 0.5), sun_peak=2e4, sun_sigma_deg=0.5):
+    """Lat-long sky: gradient 0.1 (horizon and below) to 1.0 (zenith) plus a
+    Gaussian sun lobe (peak sun_peak, sigma sun_sigma_deg)."""
+    y = (np.arange(height) + 0.5) / height
+    x = (np.arange(width) + 0.5) / width
+    theta = y * math.pi
+    phi = x * 2 * math.pi - math.pi
+    T, P = np.meshgrid(theta, phi, indexing="ij")
+    d = np.stack([np.cos(P) * np.sin(T), np.cos(T), np.sin(P) * np.sin(T)], -1)
+    elev = np.clip(d[..., 1], 0.0, 1.0)
+    g = 0.1 + 0.9 * elev
+    rgb = np.stack([g * 0.55, g * 0.75, g * 1.0], -1)
+    sd = np.asarray(sun_dir, np.float64)
+    sd /= np.linalg.norm(sd)
+    ang = np.arccos(np.clip((d * sd).sum(-1), -1.0, 1.0))
+    sig = math.radians(sun_sigma_deg)
+    sun = sun_peak * np.exp(-0.5 * (ang / sig) ** 2)
+    rgb += sun[..., None] * np.array([1.0, 0.95, 0.85])
+    return rgb.astype(np.float32)
+
+
+def displaced_sphere(width=1920, height=1080, spp=256, max_depth=16, grid=1024, glass_level=5, env_size=(1024, 512),
+                     seed=0x5EED):
+    """2,097,152-triangle displaced sphere (grid=1024) under a sky env map."""
+    s = SceneDesc(name="displaced_sphere")
+    cond = s.add_bsdf(Bsdf.conductor(0.2, (0.2, 0.92, 1.1), (3.912, 2.452, 2.142)))
+    glass = s.add_bsdf(Bsdf.dielectric(1.5, 1.0))
+    ground = s.add_bsdf(Bsdf.diffuse((0.5, 0.5, 0.5)))
+    V, F = octa_sphere(grid, radius=1.0, displace=0.1, seed=seed)
+    s.add_mesh(V, F, cond, name="displaced_sphere")
+    V, F = icosphere(glass_level, radius=0.5, center=(1.7, -0.6, 0.9))
+    s.add_mesh(V, F, glass, name="glass_sphere")
+    V, F, _ = quad((-10.0, -1.1, -10.0), (0, 0, 20.0), (20.0, 0, 0))
+    s.add_mesh(V, F, ground, name="ground")
+    s.envmap = EnvMap(sky_envmap(*env_size), 1.0, True)
+    s.camera = Camera(eye=(0.4, 0.9, 5.0), lookat=(0.4, -0.1, 0.0), up=(0, 1, 0), fov=38.0, width=width,
+                      height=height)
+    s.rfilter = RFilter(FILTER_GAUSSIAN, 2.0, 0.5)
+    s.spp, s.max_depth, s.rr_depth = spp, max_depth, 4
+    return s
+
+
+def mis_plates(width=1280, height=720, spp=512, max_depth=6):
+    """Veach-style MIS scene: glossy plates of increasing roughness lit by
+    four spherical lights of equal power."""
+    s = SceneDesc(name="mis_plates")
+    back = s.add_bsdf(Bsdf.diffuse((0.1, 0.1, 0.1)))
+    lamp = s.add_bsdf(Bsdf.diffuse((0.0, 0.0, 0.0)))
+    rough = [(0.005, 0.005), (0.02, 0.02), (0.05, 0.05), (0.1, 0.01)]
+    plates = [s.add_bsdf(Bsdf.aniso_ggx((0, 0, 0), (1.0, 1.0, 1.0), r)) for r in rough]
+    # plates: 8 x 0.9 strips stepping down and toward the camera, tilted up
+    for i, m in enumerate(plates):
+        y = -0.7 - 0.65 * i
+        z = 0.9 * i
+        ang = math.radians(38.0 - 7.0 * i)
+        e1 = (0.0, 0.9 * math.sin(ang), 0.9 * math.cos(ang))
+        V, F, UV = quad((4.0, y, z), (-8.0, 0.0, 0.0), e1, with_uv=True)
+        s.add_mesh(V, F, m, UV=UV, name="plate%d" % i)
+    V, F, _ = quad((-20.0, -10.0, -3.0), (40.0, 0, 0), (0, 30.0, 0))
+    s.add_mesh(V, F, back, name="backdrop")
+    V, F, _ = quad((-20.0, -5.0, -3.0), (0, 0, 25.0), (40.0, 0, 0))
+    s.add_mesh(V, F, back, name="floor")
+    radii = [0.03, 0.1, 0.3, 0.9]
+    for i, r in enumerate(radii):
+        V, F = icosphere(2, radius=r, center=(-3.75 + 2.5 * i, 1.6, -1.2))
+        L = 0.8 / (r * r)
+        s.add_mesh(V, F, lamp, radiance=(L, L, L), name="light%d" % i)
+    s.camera = Camera(eye=(0.0, 2.2, 9.0), lookat=(0.0, -1.6, 1.0), up=(0, 1, 0), fov=38.0, width=width,
+                      height=height)
+    s.rfilter = RFilter(FILTER_GAUSSIAN, 2.0, 0.5)
+    s.spp, s.max_depth, s.rr_depth = spp, max_depth, 4
+    return s
+
+
+def random_scene(seed, target_tris=50_000, width=512, height=512, spp=64):
+    """Procedural job scene: displaced spheres and boxes with random diffuse /
+    GGX-conductor / dielectric materials and 1-8 spherical area lights."""
+    rng = np.random.default_rng(seed)
+    s = SceneDesc(name="random_%d" % seed)
+    floor = s.add_bsdf(Bsdf.diffuse((0.4, 0.4, 0.4)))
+    V, F, _ = quad((-8.0, 0.0, -8.0), (0, 0, 16.0), (16.0, 0, 0))
+    s.add_mesh(V, F, floor, name="floor")
+    budget = int(target_tris)
+    nobj = int(rng.integers(3, 9))
+    per = max(200, budget // nobj)
+    for k in range(nobj):
+        kind = rng.integers(0, 3)
+        if kind == 0:
+            m = s.add_bsdf(Bsdf.diffuse(tuple(rng.uniform(0.1, 0.9, 3))))
+        elif kind == 1:
+            m = s.add_bsdf(Bsdf.conductor(float(rng.uniform(0.05, 0.5))))
+        else:
+            m = s.add_bsdf(Bsdf.dielectric(float(rng.uniform(1.3, 1.8)), 1.0))
+        c = (float(rng.uniform(-3, 3)), float(rng.uniform(0.5, 1.5)), float(rng.uniform(-3, 3)))
+        if rng.random() < 0.7:
+            n = max(4, int(math.sqrt(per / 2)))
+            V, F = octa_sphere(n, radius=float(rng.uniform(0.3, 0.8)), center=c, displace=float(rng.uniform(0, 0.15)),
+                               seed=int(rng.integers(1 << 30)))
+        else:
+            h = rng.uniform(0.2, 0.7, 3)
+            V, F = box(np.array(c) - h, np.array(c) + h)
+        s.add_mesh(V, F, m, name="obj%d" % k)
+    lamp = s.add_bsdf(Bsdf.diffuse((0.5, 0.5, 0.5)))
+    for k in range(int(rng.integers(1, 9))):
+        r = float(rng.uniform(0.1, 0.4))
+        V, F = icosphere(2, radius=r, center=(float(rng.uniform(-4, 4)), float(rng.uniform(3, 5)),
+                                               float(rng.uniform(-4, 4))))
+        L = tuple(float(x) for x in rng.uniform(2.0, 10.0, 3) / (r * r) * 0.05)
+        s.add_mesh(V, F, lamp, radiance=L, name="light%d" % k)
+    s.camera = Camera(eye=(0.0, 4.0, 10.0), lookat=(0.0, 0.8, 0.0), up=(0, 1, 0), fov=40.0, width=width,
+                      height=height)
+    s.spp, s.max_depth, s.rr_depth = spp, 8, 4
+    return s
+
+
+def from_name(name, **kw):
+    return {"cornell_box": cornell_box, "displaced_sphere": displaced_sphere, "mis_plates": mis_plates}[name](**kw)

@@ -1,0 +1,220 @@
+/*
+ * dl_hip.h -- C ABI of the DrawLab MI355X (gfx950) rendering backend.
+ *
+ * A third backend beside the reference's CPU (TBB) and OptiX ones.  Host
+ * code (the C++ plugin layer in drawlab_amd/csrc/host, or any FFI caller)
+ * exports the scene as plain descriptors and drives HIP through these
+ * entry points.  Conventions: every call returns a dl_status; on failure
+ * dl_hip_last_error(ctx) describes it.  Host memory is caller-owned and only
+ * read during the call; device memory is owned by the context.  A context is
+ * used by one host thread at a time.
+ *
+ * Entry points and the reference interfaces they replace:
+ *   dl_hip_create / dl_hip_destroy
+ *       DeviceContext ctor/dtor        src/optix/host/device_context.cpp:29-120
+ *   dl_hip_upload_scene
+ *       OptixRenderer::initOptixScene  src/tracer/backend/optix_renderer.cpp:25-82
+ *       (addMesh/addLight/addMaterial/updateCamera, include/optix/host/optix_scene.h:16-76)
+ *   dl_hip_build_accel
+ *       Accel::build / OCTree::build   include/tracer/accel.h:13-58,
+ *                                      src/tracer/accel/octree.cpp:135-151
+ *       OptixAccel::build              src/optix/host/optix_accel.cpp:26-152
+ *   dl_hip_trace_closest / dl_hip_trace_shadow
+ *       Accel::rayIntersect(ray, its, shadowRay)  include/tracer/accel.h:50-57,
+ *                                      src/tracer/accel/octree.cpp:153-212
+ *   dl_hip_render
+ *       CPURenderer::renderAsync       src/tracer/backend/cpu_renderer.cpp:57-135
+ *       OptixScene::render + launch    src/optix/host/optix_scene.cpp:125-140,
+ *                                      src/optix/host/device_context.cpp:303-309
+ *   dl_hip_render_paths
+ *       PathIntegrator::Li per camera sample (cpu_renderer.cpp:36-51,
+ *       src/tracer/integrator/path.cpp:13-100), for per-path parity checks.
+ */
+#ifndef DL_HIP_H
+#define DL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dl_status {
+    DL_OK = 0,
+    DL_ERR_INVALID = 1, /* bad argument or descriptor */
+    DL_ERR_HIP = 2,     /* a HIP runtime call failed */
+    DL_ERR_OOM = 3,     /* device allocation failed */
+    DL_ERR_STATE = 4,   /* call out of order (e.g. render before upload) */
+    DL_ERR_NO_DEVICE = 5
+} dl_status;
+
+/* BSDF types and the meaning of dl_material_desc.params (reference plugin names) */
+enum {
+    DL_BSDF_DIFFUSE = 0,    /* "diffuse":    albedo[3]                      diffuse.cpp:16-19  */
+    DL_BSDF_MIRROR = 1,     /* "mirror":     -                              mirror.cpp         */
+    DL_BSDF_DIELECTRIC = 2, /* "dielectric": intIOR, extIOR                 dielectric.cpp:26-32 */
+    DL_BSDF_MICROFACET = 3, /* "microfacet": alpha, intIOR, extIOR, kd[3]   microfacet.cpp:13-33 */
+    DL_BSDF_ANISO_GGX = 4,  /* "aniso_ggx":  pd[3], ps[3], ax, ay           aniso_ggx.cpp:12-20 */
+    DL_BSDF_CONDUCTOR = 5,  /* "conductor":  alpha, eta[3], k[3] (GGX, complex-IOR Fresnel;
+                               no reference counterpart) */
+    DL_BSDF_COUNT = 6
+};
+
+enum { DL_LIGHT_AREA = 0, DL_LIGHT_POINT = 1, DL_LIGHT_ENV = 2 };
+enum { DL_CAMERA_VIRTUAL = 0 };
+
+typedef struct dl_mesh_desc {
+    const float* positions;    /* 3 * num_vertices, world space */
+    const uint32_t* indices;   /* 3 * num_triangles */
+    const float* normals;      /* 3 * num_vertices, or NULL */
+    const float* texcoords;    /* 2 * num_vertices, or NULL */
+    const float* tangents;     /* 3 * num_vertices, or NULL (Mesh::activate, mesh.cpp:30-44) */
+    uint32_t num_vertices;
+    uint32_t num_triangles;
+    int32_t material;          /* index into dl_scene_desc.materials */
+    int32_t light;             /* index into dl_scene_desc.lights, -1 if not an emitter */
+} dl_mesh_desc;
+
+typedef struct dl_material_desc {
+    int32_t type;              /* DL_BSDF_* */
+    float params[16];
+} dl_material_desc;
+
+typedef struct dl_light_desc {
+    int32_t type;              /* DL_LIGHT_* */
+    int32_t mesh;              /* area: the emitting mesh */
+    float radiance[3];         /* area: radiance; point: intensity */
+    float position[3];         /* point */
+    const float* env_rgb;      /* env: height x width x 3, row 0 = +y (theta = 0) */
+    int32_t env_width, env_height;
+    float env_scale;
+    int32_t env_visual;
+} dl_light_desc;
+
+/* VirtualCamera (src/tracer/camera/virtual.cpp:16-65) after its constructor:
+   eye, U, V, W as the reference computes them, and the output size. */
+typedef struct dl_camera_desc {
+    int32_t type;              /* DL_CAMERA_VIRTUAL */
+    int32_t width, height;
+    float eye[3], U[3], V[3], W[3];
+} dl_camera_desc;
+
+/* ImageBlock filter tabulation (src/core/bitmap/block.cpp:12-27):
+   table[i] = filter(radius * i / 32) for i < 32, table[32] = 0. */
+typedef struct dl_film_desc {
+    float radius;
+    int32_t border;            /* ceil(radius - 0.5) */
+    float lookup;              /* 32 / radius */
+    float table[33];
+} dl_film_desc;
+
+typedef struct dl_scene_desc {
+    const dl_mesh_desc* meshes;
+    uint32_t num_meshes;
+    const dl_material_desc* materials;
+    uint32_t num_materials;
+    const dl_light_desc* lights;  /* in Scene::m_emitters order (scene.cpp:36-81) */
+    uint32_t num_lights;
+    dl_camera_desc camera;
+    dl_film_desc film;
+    float bg_color[3];
+} dl_scene_desc;
+
+typedef struct dl_accel_stats {
+    uint32_t num_nodes;
+    uint32_t num_leaves;
+    uint32_t max_depth;
+    uint32_t num_prims;
+    double build_ms;
+    double sah_cost;
+    uint64_t node_bytes;       /* S_node * num_nodes */
+    uint64_t tri_bytes;        /* S_tri * num_prims */
+} dl_accel_stats;
+
+enum { DL_BUILD_GPU = 0, DL_BUILD_HOST_SAH = 1 };
+
+typedef struct dl_render_params {
+    int32_t spp;               /* samples per pixel (the sampler's sampleCount) */
+    uint32_t seed;             /* stream key: initstate = pixel + (seed << 32) */
+    int32_t max_depth;         /* -1 = unlimited (reference behaviour) */
+    int32_t rr_depth;          /* RR when bounce index >= rr_depth; 4 = reference */
+    int32_t spp_per_pass;      /* samples per pixel per wavefront pass (0 = auto) */
+    int32_t first_sample;      /* index of the first sample (progressive rendering) */
+    int32_t flags;             /* DL_RENDER_* */
+} dl_render_params;
+
+enum { DL_RENDER_COUNT_TRAVERSAL = 1 /* count BVH node visits / triangle tests of closest-hit rays */ };
+
+typedef struct dl_render_stats {
+    double paths;              /* camera samples traced */
+    double closest_rays;       /* closest-hit queries issued */
+    double shadow_rays;        /* shadow queries issued */
+    double invalid_samples;    /* NaN / Inf / negative samples dropped (block.cpp:82-88) */
+    double ms_total;           /* wall time of the render loop (hipEvents) */
+    double ms_closest;         /* summed kernel time per stage */
+    double ms_shadow;
+    double ms_shade;
+    double ms_film;
+    double ms_other;
+    double launches_closest;
+    double launches_shadow;
+    double nodes_closest;      /* with DL_RENDER_COUNT_TRAVERSAL: node visits of closest-hit rays */
+    double tris_closest;       /* with DL_RENDER_COUNT_TRAVERSAL: triangle tests of closest-hit rays */
+} dl_render_stats;
+
+/* ---- context ---------------------------------------------------------- */
+dl_status dl_hip_create(int device, void** ctx);
+dl_status dl_hip_destroy(void* ctx);
+const char* dl_hip_last_error(void* ctx);
+const char* dl_hip_device_name(void* ctx);
+
+/* ---- scene and acceleration structure ---------------------------------- */
+dl_status dl_hip_upload_scene(void* ctx, const dl_scene_desc* scene);
+dl_status dl_hip_build_accel(void* ctx, int builder, dl_accel_stats* stats);
+
+/* ---- traversal on device-resident batches ------------------------------
+ * rays: 8 planes of n floats (ox, oy, oz, dx, dy, dz, tmin, tmax);
+ * hits: prim (global primitive index, 0xffffffff on a miss), t, u, v.
+ * Pointers are device pointers (dl_hip_malloc); the call is asynchronous on
+ * the context's stream unless synchronised with dl_hip_sync. */
+dl_status dl_hip_trace_closest(void* ctx, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u, float* v);
+dl_status dl_hip_trace_shadow(void* ctx, const float* rays, uint32_t n, uint8_t* occluded);
+/* Same with counters: per-ray node visits and triangle tests (uint32 each). */
+dl_status dl_hip_trace_closest_counted(void* ctx, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u,
+                                       float* v, uint32_t* node_visits, uint32_t* tri_tests);
+
+/* ---- rendering --------------------------------------------------------- */
+/* Renders spp samples per pixel; writes W*H*3 floats (row-major, row 0 =
+ * top) to rgb_host if non-NULL. */
+dl_status dl_hip_render(void* ctx, const dl_render_params* params, float* rgb_host, dl_render_stats* stats);
+/* Radiance of every path of one sample index: out_host = W*H*5 floats
+ * (sample position x, y, radiance r, g, b) for sample `sample`. */
+dl_status dl_hip_render_paths(void* ctx, const dl_render_params* params, uint32_t sample, float* out_host);
+
+/* ---- device memory and timing helpers ---------------------------------- */
+dl_status dl_hip_malloc(void* ctx, size_t bytes, void** dptr);
+dl_status dl_hip_free(void* ctx, void* dptr);
+dl_status dl_hip_memcpy_h2d(void* ctx, void* dst, const void* src, size_t bytes);
+dl_status dl_hip_memcpy_d2h(void* ctx, void* dst, const void* src, size_t bytes);
+dl_status dl_hip_sync(void* ctx);
+/* Kernel time of the last dl_hip_trace_* call (hipEvents on the context
+ * stream, milliseconds). */
+dl_status dl_hip_last_kernel_ms(void* ctx, double* ms);
+
+/* ---- host-side helpers shared with the plugin layer -------------------- */
+/* VirtualCamera constructor arithmetic (virtual.cpp:16-42). */
+void dl_camera_virtual(const float eye[3], const float lookat[3], const float up[3], float fov, int width, int height,
+                       dl_camera_desc* out);
+/* ReconstructionFilter + ImageBlock tabulation (rfilter.cpp:13-98,
+ * block.cpp:12-27).  type: 0 gaussian(radius, stddev) 1 box 2 tent
+ * 3 mitchell(radius, B, C). */
+void dl_film_filter(int type, float radius, float p0, float p1, dl_film_desc* out);
+/* Mesh::activate tangent computation (mesh.cpp:30-44, 144-177). */
+void dl_mesh_tangents(const float* positions, uint32_t nv, const uint32_t* indices, uint32_t nf,
+                      const float* texcoords, float* tangents_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DL_HIP_H */

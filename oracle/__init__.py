@@ -1,0 +1,269 @@
+"""CPU oracle for the DrawLab hot path -- TEST INFRASTRUCTURE ONLY.
+
+A restatement of the reference CPU path (see oracle/oracle.cpp).  Only the
+test suite, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
+leg import this package, and only as the checker or as the timed CPU
+baseline; the product (``drawlab_amd``) never does.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "liboracle.so")
+REF_PATH = os.path.join(_HERE, "_ref", "libref.so")
+
+BSDF_DIFFUSE, BSDF_MIRROR, BSDF_DIELECTRIC, BSDF_MICROFACET, BSDF_ANISO_GGX, BSDF_CONDUCTOR = range(6)
+FILTER_GAUSSIAN, FILTER_BOX, FILTER_TENT, FILTER_MITCHELL = range(4)
+ACCEL_BRUTE, ACCEL_OCTREE, ACCEL_OCTREE_LOWEST, ACCEL_OCTREE_ROBUST = range(4)
+
+_lib = None
+_ref = None
+
+_P = ctypes.c_void_p
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        u32, i32, f32 = ctypes.c_uint32, ctypes.c_int, ctypes.c_float
+        sig = {
+            "dlo_scene_new": (_P, []),
+            "dlo_scene_free": (None, [_P]),
+            "dlo_scene_add_bsdf": (i32, [_P, i32, _f32p, i32]),
+            "dlo_scene_add_mesh": (i32, [_P, _f32p, u32, _u32p, u32, _P, _P, i32, _P]),
+            "dlo_scene_add_point_light": (i32, [_P, _f32p, _f32p]),
+            "dlo_scene_add_envmap": (i32, [_P, _f32p, i32, i32, f32, i32]),
+            "dlo_scene_set_camera_virtual": (None, [_P, _f32p, _f32p, _f32p, f32, i32, i32]),
+            "dlo_scene_get_camera": (None, [_P, _f32p]),
+            "dlo_scene_set_filter": (None, [_P, i32, f32, f32, f32]),
+            "dlo_scene_get_filter_table": (None, [_P, _f32p]),
+            "dlo_scene_set_background": (None, [_P, _f32p]),
+            "dlo_scene_build": (i32, [_P]),
+            "dlo_scene_num_prims": (u32, [_P]),
+            "dlo_trace_closest": (None, [_P, i32, _f32p, u32, _u32p, _f32p, _f32p, _f32p]),
+            "dlo_trace_shadow": (None, [_P, i32, _f32p, u32, _u8p]),
+            "dlo_camera_rays": (None, [_P, _f32p, u32, _f32p]),
+            "dlo_render_samples": (None, [_P, _i32p, u32, u32, i32, i32, i32, _f32p]),
+            "dlo_render": (i32, [_P, i32, u32, i32, i32, i32, i32, i32, _P, _P]),
+            "dlo_pcg32": (None, [ctypes.c_uint64, ctypes.c_uint64, u32, _u32p]),
+            "dlo_pcg32_float": (None, [ctypes.c_uint64, ctypes.c_uint64, u32, _f32p]),
+            "dlo_warp": (None, [i32, _f32p, u32, f32, f32, _f32p]),
+            "dlo_optics": (None, [i32, _f32p, u32, _f32p]),
+            "dlo_bsdf_query": (None, [_P, i32, _f32p, u32, _f32p]),
+            "dlo_env_tables": (None, [_P, _f32p, _u32p, _f32p]),
+            "dlo_spiral_blocks": (u32, [i32, i32, _P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's own wrap.cpp/math.cpp (oracle/_ref), or None if not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        L = ctypes.CDLL(REF_PATH)
+        for name in ("ref_warp", "ref_optics", "ref_vec"):
+            getattr(L, name).restype = None
+        L.ref_warp.argtypes = [ctypes.c_int, _f32p, ctypes.c_uint32, ctypes.c_float, ctypes.c_float, _f32p]
+        L.ref_optics.argtypes = [ctypes.c_int, _f32p, ctypes.c_uint32, _f32p]
+        L.ref_vec.argtypes = [ctypes.c_int, _f32p, ctypes.c_uint32, _f32p]
+        _ref = L
+    return _ref
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f(a, n=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1))
+    if n is not None:
+        assert a.size == n
+    return a
+
+
+class Scene:
+    """Oracle scene built from a plain scene description (drawlab_amd.scene.SceneDesc-like)."""
+
+    def __init__(self):
+        self.L = lib()
+        self.h = self.L.dlo_scene_new()
+        self.width = self.height = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.dlo_scene_free(self.h)
+            self.h = None
+
+    def add_bsdf(self, btype, params=()):
+        p = _f(list(params) + [0.0] * (16 - len(params)))
+        return self.L.dlo_scene_add_bsdf(self.h, int(btype), p, len(params))
+
+    def add_mesh(self, V, F, bsdf, N=None, UV=None, radiance=None):
+        V = _f(V)
+        F = np.ascontiguousarray(np.asarray(F, dtype=np.uint32).reshape(-1))
+        N = None if N is None else _f(N)
+        UV = None if UV is None else _f(UV)
+        rad = None if radiance is None else _f(radiance, 3)
+        return self.L.dlo_scene_add_mesh(self.h, V, V.size // 3, F, F.size // 3, _ptr(N), _ptr(UV), int(bsdf), _ptr(rad))
+
+    def add_point_light(self, position, intensity):
+        return self.L.dlo_scene_add_point_light(self.h, _f(position, 3), _f(intensity, 3))
+
+    def add_envmap(self, rgb, scale=1.0, visual=True):
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        h, w = rgb.shape[:2]
+        return self.L.dlo_scene_add_envmap(self.h, rgb.reshape(-1), w, h, float(scale), int(bool(visual)))
+
+    def set_camera(self, eye, lookat, up, fov, width, height):
+        self.width, self.height = int(width), int(height)
+        self.L.dlo_scene_set_camera_virtual(self.h, _f(eye, 3), _f(lookat, 3), _f(up, 3), float(fov), int(width), int(height))
+
+    def camera(self):
+        out = np.zeros(12, np.float32)
+        self.L.dlo_scene_get_camera(self.h, out)
+        return out.reshape(4, 3)
+
+    def set_filter(self, ftype=FILTER_GAUSSIAN, radius=2.0, p0=0.5, p1=0.0):
+        self.L.dlo_scene_set_filter(self.h, int(ftype), float(radius), float(p0), float(p1))
+
+    def filter_table(self):
+        out = np.zeros(33, np.float32)
+        self.L.dlo_scene_get_filter_table(self.h, out)
+        return out
+
+    def set_background(self, rgb):
+        self.L.dlo_scene_set_background(self.h, _f(rgb, 3))
+
+    def build(self):
+        return self.L.dlo_scene_build(self.h)
+
+    def num_prims(self):
+        return self.L.dlo_scene_num_prims(self.h)
+
+    def trace_closest(self, rays, mode=ACCEL_BRUTE):
+        rays = _f(rays)
+        n = rays.size // 8
+        prim = np.zeros(n, np.uint32)
+        t, u, v = (np.zeros(n, np.float32) for _ in range(3))
+        self.L.dlo_trace_closest(self.h, int(mode), rays, n, prim, t, u, v)
+        return prim, t, u, v
+
+    def trace_shadow(self, rays, mode=ACCEL_BRUTE):
+        rays = _f(rays)
+        n = rays.size // 8
+        occ = np.zeros(n, np.uint8)
+        self.L.dlo_trace_shadow(self.h, int(mode), rays, n, occ)
+        return occ
+
+    def camera_rays(self, pos):
+        pos = _f(pos)
+        n = pos.size // 2
+        out = np.zeros(8 * n, np.float32)
+        self.L.dlo_camera_rays(self.h, pos, n, out)
+        return out.reshape(8, n)
+
+    def render_samples(self, xys, seed=0, max_depth=-1, rr_depth=4, accel=ACCEL_OCTREE_ROBUST):
+        xys = np.ascontiguousarray(np.asarray(xys, dtype=np.int32).reshape(-1))
+        n = xys.size // 3
+        out = np.zeros(5 * n, np.float32)
+        self.L.dlo_render_samples(self.h, xys, n, int(seed), int(max_depth), int(rr_depth), int(accel), out)
+        return out.reshape(n, 5)
+
+    def render(self, spp, seed=0, max_depth=-1, rr_depth=4, accel=ACCEL_OCTREE_ROBUST, threads=0, max_blocks=0,
+               want_image=True):
+        img = np.zeros((self.height, self.width, 3), np.float32) if want_image else None
+        stats = np.zeros(8, np.float64)
+        rc = self.L.dlo_render(self.h, int(spp), int(seed), int(max_depth), int(rr_depth), int(accel), int(threads),
+                               int(max_blocks), _ptr(img), _ptr(stats))
+        if rc != 0:
+            raise RuntimeError("oracle render failed (%d)" % rc)
+        return img, {"paths": stats[0], "closest": stats[1], "shadow": stats[2], "seconds": stats[3],
+                     "threads": int(stats[4])}
+
+    def bsdf_query(self, bsdf, wi, wo, s):
+        inp = np.concatenate([np.asarray(wi, np.float32).reshape(-1, 3), np.asarray(wo, np.float32).reshape(-1, 3),
+                              np.asarray(s, np.float32).reshape(-1, 2)], axis=1)
+        inp = np.ascontiguousarray(inp, dtype=np.float32)
+        n = inp.shape[0]
+        out = np.zeros(12 * n, np.float32)
+        self.L.dlo_bsdf_query(self.h, int(bsdf), inp.reshape(-1), n, out)
+        return out.reshape(n, 12)
+
+    def env_tables(self, w, h):
+        q = np.zeros(w * h, np.float32)
+        a = np.zeros(w * h, np.uint32)
+        p = np.zeros(w * h, np.float32)
+        self.L.dlo_env_tables(self.h, q, a, p)
+        return q, a, p
+
+
+def from_desc(desc, build=True):
+    """Build an oracle Scene from a drawlab_amd scene description."""
+    s = Scene()
+    for b in desc.bsdfs:
+        s.add_bsdf(b.type, b.params)
+    # emitters are created in scene order: meshes carry area lights
+    for m in desc.meshes:
+        s.add_mesh(m.V, m.F, m.bsdf, m.N, m.UV, m.radiance)
+    for pl in desc.point_lights:
+        s.add_point_light(pl[0], pl[1])
+    if desc.envmap is not None:
+        s.add_envmap(desc.envmap.rgb, desc.envmap.scale, desc.envmap.visual)
+    c = desc.camera
+    s.set_camera(c.eye, c.lookat, c.up, c.fov, c.width, c.height)
+    f = desc.rfilter
+    s.set_filter(f.type, f.radius, f.p0, f.p1)
+    s.set_background(desc.bg_color)
+    if build:
+        s.build()
+    return s
+
+
+def pcg32(initstate, initseq, n):
+    out = np.zeros(n, np.uint32)
+    lib().dlo_pcg32(initstate, initseq, n, out)
+    return out
+
+
+def pcg32_float(initstate, initseq, n):
+    out = np.zeros(n, np.float32)
+    lib().dlo_pcg32_float(initstate, initseq, n, out)
+    return out
+
+
+def warp(which, inp, a0=0.0, a1=0.0):
+    inp = _f(inp)
+    n = inp.size // 3
+    out = np.zeros(3 * n, np.float32)
+    lib().dlo_warp(int(which), inp, n, float(a0), float(a1), out)
+    return out.reshape(n, 3)
+
+
+def optics(which, inp):
+    inp = _f(inp)
+    n = inp.size // 6
+    out = np.zeros(3 * n, np.float32)
+    lib().dlo_optics(int(which), inp, n, out)
+    return out.reshape(n, 3)
+
+
+def spiral_blocks(W, H):
+    n = lib().dlo_spiral_blocks(W, H, None)
+    out = np.zeros(2 * n, np.int32)
+    lib().dlo_spiral_blocks(W, H, out.ctypes.data_as(ctypes.c_void_p))
+    return out.reshape(n, 2)

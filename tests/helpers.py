@@ -1,0 +1,86 @@
+"""Ray batches and comparison helpers for the parity tests."""
+import numpy as np
+
+import oracle
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def scene_arrays(desc):
+    V = np.concatenate([m.V for m in desc.meshes]).astype(np.float32)
+    offs = np.cumsum([0] + [m.V.shape[0] for m in desc.meshes])[:-1]
+    F = np.concatenate([m.F + o for m, o in zip(desc.meshes, offs)]).astype(np.int64)
+    return V, F
+
+
+def planes(o, d, tmin, tmax):
+    n = o.shape[0]
+    r = np.empty((8, n), np.float32)
+    r[0:3] = o.T
+    r[3:6] = d.T
+    r[6] = tmin
+    r[7] = tmax
+    return r
+
+
+def unit(v):
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+
+def surface_points(desc, n, rng):
+    V, F = scene_arrays(desc)
+    f = rng.integers(0, F.shape[0], n)
+    s = rng.random((n, 2))
+    t = np.sqrt(1 - s[:, 0])
+    b = np.stack([1 - t, t * s[:, 1], t - t * s[:, 1]], 1)
+    P = (V[F[f, 0]] * b[:, :1] + V[F[f, 1]] * b[:, 1:2] + V[F[f, 2]] * b[:, 2:3]).astype(np.float32)
+    return P, f
+
+
+def ray_batches(desc, osc, n, seed=0):
+    """Camera, bounce (surface-to-random), random-in-box, axis-aligned,
+    edge/vertex-grazing and shadow rays."""
+    rng = np.random.default_rng(seed)
+    V, F = scene_arrays(desc)
+    lo, hi = V.min(0), V.max(0)
+    W, H = desc.camera.width, desc.camera.height
+    out = {}
+    pos = np.stack([rng.random(n) * W, rng.random(n) * H], 1).astype(np.float32)
+    out["camera"] = osc.camera_rays(pos)
+    P, _ = surface_points(desc, n, rng)
+    d = unit(rng.normal(size=(n, 3)))
+    out["bounce"] = planes(P, d, np.float32(1e-4), np.float32(np.inf))
+    o = (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+    out["random"] = planes(o, unit(rng.normal(size=(n, 3))), np.float32(1e-4), np.float32(np.inf))
+    axes = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], np.float32)
+    o2 = (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+    out["axis"] = planes(o2, axes[rng.integers(0, 6, n)], np.float32(1e-4), np.float32(np.inf))
+    # aim exactly at vertices and edge midpoints (ties between neighbours)
+    f = rng.integers(0, F.shape[0], n)
+    k = rng.integers(0, 3, n)
+    tgt = V[F[f, k]].astype(np.float32)
+    half = rng.random(n) < 0.5
+    tgt[half] = ((V[F[f[half], k[half]]] + V[F[f[half], (k[half] + 1) % 3]]) * 0.5).astype(np.float32)
+    o3 = (lo + (hi - lo) * rng.random((n, 3)) * 1.5 - (hi - lo) * 0.25).astype(np.float32)
+    out["edges"] = planes(o3, unit(tgt - o3), np.float32(1e-4), np.float32(np.inf))
+    # shadow-style segments between two surface points
+    P1, _ = surface_points(desc, n, rng)
+    P2, _ = surface_points(desc, n, rng)
+    v = (P2 - P1).astype(np.float32)
+    dist = np.linalg.norm(v, axis=1).astype(np.float32)
+    ok = dist > 1e-3
+    out["shadow"] = planes(P1[ok], unit(v[ok]), np.float32(1e-4), (dist[ok] - np.float32(1e-4)).astype(np.float32))
+    return out
+
+
+def compare_hits(g, o):
+    """Returns the number of rays whose (prim, t, u, v) differ bitwise."""
+    gp, gt, gu, gv = g
+    op, ot, ou, ov = o
+    miss = op == 0xFFFFFFFF
+    bad = gp != op
+    hit = ~miss
+    bad |= hit & ((bits(gt) != bits(ot)) | (bits(gu) != bits(ou)) | (bits(gv) != bits(ov)))
+    return int(bad.sum()), np.nonzero(bad)[0]
