@@ -296,6 +296,9 @@ class Backend:
     def trace_shadow_dev(self, d_rays, n, d_occ):
         self._check(self.L.dl_hip_trace_shadow(self.h, d_rays.ptr, n, d_occ.ptr))
 
+    def sync(self):
+        self._check(self.L.dl_hip_sync(self.h))
+
     def last_kernel_ms(self):
         ms = ctypes.c_double()
         self._check(self.L.dl_hip_last_kernel_ms(self.h, ctypes.byref(ms)))
@@ -335,12 +338,13 @@ class Backend:
         return p
 
     def render(self, spp=None, seed=0, max_depth=None, rr_depth=None, spp_per_pass=0, want_image=True,
-               count_traversal=False, first_sample=0):
+               count_traversal=False, first_sample=0, accumulate=False):
         d = self.desc
         spp = d.spp if spp is None else spp
         max_depth = d.max_depth if max_depth is None else max_depth
         rr_depth = d.rr_depth if rr_depth is None else rr_depth
-        p = self._params(spp, seed, max_depth, rr_depth, spp_per_pass, first_sample, 1 if count_traversal else 0)
+        flags = (1 if count_traversal else 0) | (2 if accumulate else 0)
+        p = self._params(spp, seed, max_depth, rr_depth, spp_per_pass, first_sample, flags)
         img = np.zeros((d.camera.height, d.camera.width, 3), np.float32) if want_image else None
         st = dl_render_stats()
         self._check(self.L.dl_hip_render(self.h, ctypes.byref(p), _p(img), ctypes.byref(st)))

@@ -27,13 +27,14 @@ namespace dl {
 constexpr int kNodeBytes = 64;
 constexpr int kTriBytes = 48;
 constexpr int kMaxLeafSize = 4;
-// Traversal stack: kLdsStack entries in LDS per lane, the rest in a global
-// overflow buffer; no tree is deeper than kMaxStack internal levels (the
-// host builder caps depth at kLdsStack, an LBVH over 64-bit keys has at
-// most 64 levels).
-constexpr int kLdsStack = 48;
+// Traversal stack: kLdsStack entries in LDS per lane (8 KB per 128-lane
+// block, so LDS does not limit occupancy below the 8 waves/SIMD the VGPR
+// budget allows), the rest in a global overflow buffer; no tree is deeper
+// than kMaxStack internal levels (the host builder caps depth at kMaxDepth,
+// an LBVH over 64-bit keys has at most 64 levels).
+constexpr int kLdsStack = 16;
 constexpr int kMaxStack = 64;
-constexpr int kMaxDepth = kLdsStack;
+constexpr int kMaxDepth = 48;
 
 struct BvhNodeH {
     float r0[4], r1[4], r2[4];

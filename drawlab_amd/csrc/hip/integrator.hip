@@ -68,6 +68,7 @@ struct RenderState {
     uint32_t capacity = 0;
     uint32_t film_pixels = 0;
     RenderBufs b{};
+    bool film_valid = false;       // film holds the accumulation of earlier calls
     Counters* cnt_host = nullptr;  // pinned
     std::vector<void*> allocs;
     std::vector<hipEvent_t> evs;
@@ -646,7 +647,9 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
 
     DL_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
     k_reset_all<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt);
-    k_clear_film<<<stage_grid, kStageBlock, 0, c->stream>>>(rs->b.film, WH);
+    if (!(prm->flags & DL_RENDER_ACCUMULATE) || !rs->film_valid)
+        k_clear_film<<<stage_grid, kStageBlock, 0, c->stream>>>(rs->b.film, WH);
+    rs->film_valid = paths_sample < 0;
     const uint32_t total_spp = paths_sample >= 0 ? 1u : (uint32_t)prm->spp;
     for (uint32_t done = 0; done < total_spp;) {
         const uint32_t kk = std::min(k, total_spp - done);
@@ -712,7 +715,7 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
         DL_TRY(tm.stop());
         done += kk;
     }
-    if (paths_sample < 0) k_resolve<<<stage_grid, kStageBlock, 0, c->stream>>>(rs->b.film, rs->b.rgb, WH);
+    if (paths_sample < 0 && rgb_host) k_resolve<<<stage_grid, kStageBlock, 0, c->stream>>>(rs->b.film, rs->b.rgb, WH);
     DL_HIP_CHECK(c, hipGetLastError());
     DL_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
     DL_HIP_CHECK(c, hipEventSynchronize(c->ev1));

@@ -144,7 +144,11 @@ typedef struct dl_render_params {
     int32_t flags;             /* DL_RENDER_* */
 } dl_render_params;
 
-enum { DL_RENDER_COUNT_TRAVERSAL = 1 /* count BVH node visits / triangle tests of closest-hit rays */ };
+enum {
+    DL_RENDER_COUNT_TRAVERSAL = 1, /* count BVH node visits / triangle tests of closest-hit rays */
+    DL_RENDER_ACCUMULATE = 2       /* add to the film of the previous call instead of clearing it
+                                      (progressive rendering, src/optix/cuda/integrator/path.cu:86-90) */
+};
 
 typedef struct dl_render_stats {
     double paths;              /* camera samples traced */
