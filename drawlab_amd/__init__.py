@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libdrawlab_hip.so")
 REPO_ROOT = os.path.dirname(_HERE)
 
-BUILD_GPU, BUILD_HOST_SAH = 0, 1
+BUILD_GPU, BUILD_HOST_SAH, BUILD_GPU_LBVH = 0, 1, 2
 LIGHT_AREA, LIGHT_POINT, LIGHT_ENV = 0, 1, 2
 
 EXPORTED_SYMBOLS = (

@@ -82,6 +82,7 @@ dl_status dl_hip_create(int device, void** out) {
     c->num_cus = prop.multiProcessorCount;
     c->trace_blocks = (uint32_t)std::max(1, c->num_cus) * 16u;
     e = hipMalloc(&c->stack_ovf, sizeof(int) * (size_t)c->trace_blocks * kTraceBlockHost * (kMaxStack - kLdsStack));
+    if (e == hipSuccess) e = hipMalloc(&c->work, 4 * sizeof(uint32_t));
     if (e != hipSuccess) {
         delete c;
         return DL_ERR_OOM;
@@ -97,6 +98,7 @@ dl_status dl_hip_destroy(void* h) {
     free_render_state(c);
     free_scene(c);
     if (c->stack_ovf) hipFree(c->stack_ovf);
+    if (c->work) hipFree(c->work);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -250,8 +252,8 @@ dl_status dl_hip_build_accel(void* h, int builder, dl_accel_stats* stats) {
         c->ds.tris = nullptr;
     }
     dl_accel_stats st{};
-    if (builder == DL_BUILD_GPU) {
-        DL_TRY(build_bvh_gpu(c, &st));
+    if (builder == DL_BUILD_GPU || builder == DL_BUILD_GPU_LBVH) {
+        DL_TRY(build_bvh_gpu(c, &st, builder));
     } else if (builder == DL_BUILD_HOST_SAH) {
         auto t0 = std::chrono::steady_clock::now();
         HostBvh bvh;

@@ -74,6 +74,7 @@ struct Ctx {
     // a persistent traversal grid)
     int* stack_ovf = nullptr;
     uint32_t trace_blocks = 0;
+    uint32_t* work = nullptr;  // work counters of the batch traversal kernels
 
     // scene
     bool has_scene = false;
@@ -117,6 +118,6 @@ dl_status render(Ctx* c, const dl_render_params* p, float* rgb_host, dl_render_s
 void free_render_state(Ctx* c);
 
 // bvh_gpu.hip
-dl_status build_bvh_gpu(Ctx* c, dl_accel_stats* stats);
+dl_status build_bvh_gpu(Ctx* c, dl_accel_stats* stats, int method);
 
 }  // namespace dl

@@ -312,25 +312,29 @@ struct Scratch {
 
 }  // namespace
 
-dl_status build_bvh_gpu(Ctx* c, dl_accel_stats* stats) {
+// Scenes of at most kMaxLeafSize triangles: the host builder makes the
+// single node.
+static dl_status build_small(Ctx* c, dl_accel_stats* stats) {
+    HostBvh bvh;
+    build_bvh_host(c->tri_verts.data(), c->nprims, bvh);
+    void *pn = nullptr, *pt = nullptr;
+    DL_HIP_CHECK(c, hipMalloc(&pn, bvh.nodes.size() * sizeof(BvhNodeH)));
+    DL_HIP_CHECK(c, hipMalloc(&pt, std::max<size_t>(1, bvh.tris.size()) * sizeof(BvhTriH)));
+    DL_HIP_CHECK(c, hipMemcpy(pn, bvh.nodes.data(), bvh.nodes.size() * sizeof(BvhNodeH), hipMemcpyHostToDevice));
+    if (!bvh.tris.empty())
+        DL_HIP_CHECK(c, hipMemcpy(pt, bvh.tris.data(), bvh.tris.size() * sizeof(BvhTriH), hipMemcpyHostToDevice));
+    c->ds.nodes = (float4*)pn;
+    c->ds.tris = (float4*)pt;
+    stats->num_nodes = (uint32_t)bvh.nodes.size();
+    stats->num_leaves = bvh.num_leaves;
+    stats->max_depth = bvh.max_depth;
+    return DL_OK;
+}
+
+
+dl_status build_bvh_lbvh(Ctx* c, dl_accel_stats* stats) {
     const int n = (int)c->nprims;
-    if (n <= kMaxLeafSize) {
-        // Too small for a hierarchy: the host builder makes the single node.
-        HostBvh bvh;
-        build_bvh_host(c->tri_verts.data(), c->nprims, bvh);
-        void *pn = nullptr, *pt = nullptr;
-        DL_HIP_CHECK(c, hipMalloc(&pn, bvh.nodes.size() * sizeof(BvhNodeH)));
-        DL_HIP_CHECK(c, hipMalloc(&pt, std::max<size_t>(1, bvh.tris.size()) * sizeof(BvhTriH)));
-        DL_HIP_CHECK(c, hipMemcpy(pn, bvh.nodes.data(), bvh.nodes.size() * sizeof(BvhNodeH), hipMemcpyHostToDevice));
-        if (!bvh.tris.empty())
-            DL_HIP_CHECK(c, hipMemcpy(pt, bvh.tris.data(), bvh.tris.size() * sizeof(BvhTriH), hipMemcpyHostToDevice));
-        c->ds.nodes = (float4*)pn;
-        c->ds.tris = (float4*)pt;
-        stats->num_nodes = (uint32_t)bvh.nodes.size();
-        stats->num_leaves = bvh.num_leaves;
-        stats->max_depth = bvh.max_depth;
-        return DL_OK;
-    }
+    if (n <= kMaxLeafSize) return build_small(c, stats);
     Scratch sc{c};
     float4 *blo, *bhi, *ilo, *ihi;
     unsigned long long *keys, *keys_sorted;
@@ -386,6 +390,342 @@ dl_status build_bvh_gpu(Ctx* c, dl_accel_stats* stats) {
     stats->build_ms = ms;
     if (hb[6] > (uint32_t)kMaxStack) return set_error(c, DL_ERR_STATE, "LBVH deeper than the traversal stack");
     return DL_OK;
+}
+
+
+// ===========================================================================
+// PLOC (Meister and Bittner, "Parallel locally-ordered clustering for
+// bounding volume hierarchy construction", TVCG 2018): starting from the
+// Morton-ordered primitives, every cluster finds its nearest neighbour
+// (smallest merged surface area) within kPlocRadius positions, mutual
+// nearest neighbours merge, and the survivors are compacted in order, until
+// one cluster remains.  Subtree primitive counts are kept at merge time;
+// the triangles are laid out in depth-first leaf order so that every subtree
+// of at most kMaxLeafSize primitives becomes one leaf.
+// ===========================================================================
+namespace {
+
+constexpr int kPlocRadius = 16;
+
+__device__ __forceinline__ float half_area(const float4& lo, const float4& hi) {
+    const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// Cluster references: r >= 0 internal node, r < 0 leaf of sorted position ~r.
+__global__ __launch_bounds__(kB) void k_ploc_init(int n, const uint32_t* __restrict__ sorted,
+                                                  const float4* __restrict__ blo, const float4* __restrict__ bhi,
+                                                  int* __restrict__ cref, float4* __restrict__ clo,
+                                                  float4* __restrict__ chi) {
+    for (int k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB) {
+        const uint32_t p = sorted[k];
+        cref[k] = ~k;
+        clo[k] = blo[p];
+        chi[k] = bhi[p];
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_ploc_nn(int n, const float4* __restrict__ clo, const float4* __restrict__ chi,
+                                                int* __restrict__ nn) {
+    for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+        const float4 a = clo[i], b = chi[i];
+        float best = INFINITY;
+        int bj = i > 0 ? i - 1 : i + 1;
+        const int j0 = max(0, i - kPlocRadius), j1 = min(n - 1, i + kPlocRadius);
+        for (int j = j0; j <= j1; ++j) {
+            if (j == i) continue;
+            const float4 c = clo[j], d = chi[j];
+            const float4 lo = make_float4(fminf(a.x, c.x), fminf(a.y, c.y), fminf(a.z, c.z), 0.f);
+            const float4 hi = make_float4(fmaxf(b.x, d.x), fmaxf(b.y, d.y), fmaxf(b.z, d.z), 0.f);
+            const float ar = half_area(lo, hi);
+            if (ar < best) {  // j increases, so ties keep the lowest index
+                best = ar;
+                bj = j;
+            }
+        }
+        nn[i] = bj;
+    }
+}
+
+// flags: merge[i] = 1 when i starts a merge (mutual pair, i < partner);
+// keep[i] = 0 for the partner that disappears.
+__global__ __launch_bounds__(kB) void k_ploc_flags(int n, const int* __restrict__ nn, int* __restrict__ merge,
+                                                   int* __restrict__ keep) {
+    for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+        const int j = nn[i];
+        const bool mutual = nn[j] == i;
+        merge[i] = (mutual && i < j) ? 1 : 0;
+        keep[i] = (mutual && i > j) ? 0 : 1;
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_ploc_merge(int n, int top, const int* __restrict__ nn,
+                                                   const int* __restrict__ merge, const int* __restrict__ midx,
+                                                   int* __restrict__ cref, float4* __restrict__ clo,
+                                                   float4* __restrict__ chi, int2* __restrict__ child,
+                                                   float4* __restrict__ nlo, float4* __restrict__ nhi,
+                                                   int* __restrict__ ncount, int* __restrict__ ecount,
+                                                   int* __restrict__ parent_int, int* __restrict__ parent_leaf) {
+    for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+        if (!merge[i]) continue;
+        const int j = nn[i];
+        const int node = top - midx[i];
+        const int ra = cref[i], rb = cref[j];
+        const float4 a0 = clo[i], a1 = chi[i], b0 = clo[j], b1 = chi[j];
+        const float4 lo = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0.f);
+        const float4 hi = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0.f);
+        child[node] = make_int2(ra, rb);
+        nlo[node] = lo;
+        nhi[node] = hi;
+        const int cnt = (ra < 0 ? 1 : ncount[ra]) + (rb < 0 ? 1 : ncount[rb]);
+        ncount[node] = cnt;
+        // nodes that survive leaf collapsing in this subtree
+        ecount[node] = (cnt > kMaxLeafSize ? 1 : 0) + (ra < 0 ? 0 : ecount[ra]) + (rb < 0 ? 0 : ecount[rb]);
+        if (ra < 0) parent_leaf[~ra] = node; else parent_int[ra] = node;
+        if (rb < 0) parent_leaf[~rb] = node; else parent_int[rb] = node;
+        cref[i] = node;
+        clo[i] = lo;
+        chi[i] = hi;
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_ploc_compact(int n, const int* __restrict__ keep, const int* __restrict__ kidx,
+                                                     const int* __restrict__ cref, const float4* __restrict__ clo,
+                                                     const float4* __restrict__ chi, int* __restrict__ oref,
+                                                     float4* __restrict__ olo, float4* __restrict__ ohi) {
+    for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+        if (!keep[i]) continue;
+        const int d = kidx[i];
+        oref[d] = cref[i];
+        olo[d] = clo[i];
+        ohi[d] = chi[i];
+    }
+}
+
+// Depth-first position of every leaf (element k < n, sorted position k) and
+// of every internal node (element n + i): the number of primitives to its
+// left, summed over the ancestors it descends from as a right child.
+// Also the depth-first pre-order rank of every surviving internal node among
+// the surviving nodes (its ancestors, plus the surviving nodes of every left
+// sibling subtree), which becomes its index in the emitted node array.
+__global__ __launch_bounds__(kB) void k_ploc_offsets(int n, const int2* __restrict__ child,
+                                                     const int* __restrict__ ncount, const int* __restrict__ ecount,
+                                                     const int* __restrict__ parent_int,
+                                                     const int* __restrict__ parent_leaf, int* __restrict__ leaf_pos,
+                                                     int* __restrict__ node_pos, int* __restrict__ node_rank,
+                                                     uint32_t* __restrict__ max_depth) {
+    uint32_t dmax = 0;
+    for (int e = blockIdx.x * kB + threadIdx.x; e < 2 * n - 1; e += gridDim.x * kB) {
+        int ref, node;
+        if (e < n) {
+            ref = ~e;
+            node = parent_leaf[e];
+        } else {
+            ref = e - n;
+            if (ref == 0) {
+                node_pos[0] = 0;
+                node_rank[0] = 0;
+                continue;
+            }
+            node = parent_int[ref];
+        }
+        int off = 0, rank = 0;
+        uint32_t d = 1;
+        for (;;) {
+            const int2 ch = child[node];
+            if (ch.y == ref && ch.x >= 0) rank += ecount[ch.x];
+            if (ch.y == ref) off += ch.x < 0 ? 1 : ncount[ch.x];
+            rank += 1;  // the ancestor itself precedes its subtree
+            if (node == 0 || d >= 4 * kMaxStack) break;
+            ref = node;
+            node = parent_int[node];
+            ++d;
+        }
+        if (e < n) {
+            leaf_pos[e] = off;
+            dmax = max(dmax, d);
+        } else {
+            node_pos[e - n] = off;
+            node_rank[e - n] = rank;
+        }
+    }
+    atomicMax(max_depth, dmax);
+}
+
+__global__ __launch_bounds__(kB) void k_ploc_emit(int n, const int2* __restrict__ child,
+                                                  const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                                                  const int* __restrict__ ncount, const int* __restrict__ leaf_pos,
+                                                  const int* __restrict__ node_pos, const int* __restrict__ node_rank,
+                                                  const uint32_t* __restrict__ sorted, const float4* __restrict__ blo,
+                                                  const float4* __restrict__ bhi, float4* __restrict__ nodes,
+                                                  uint32_t* __restrict__ leaves) {
+    uint32_t nl = 0;
+    for (int i = blockIdx.x * kB + threadIdx.x; i < n - 1; i += gridDim.x * kB) {
+        if (ncount[i] <= kMaxLeafSize) continue;  // collapsed into a leaf of its parent
+        const int2 ch = child[i];
+        const int refs[2] = {ch.x, ch.y};
+        float lo[2][3], hi[2][3];
+        int ref[2], cnt[2];
+        for (int c = 0; c < 2; c++) {
+            const int r = refs[c];
+            float4 a, b;
+            int count, pos;
+            if (r < 0) {
+                const uint32_t p = sorted[~r];
+                a = blo[p];
+                b = bhi[p];
+                count = 1;
+                pos = leaf_pos[~r];
+            } else {
+                a = nlo[r];
+                b = nhi[r];
+                count = ncount[r];
+                pos = node_pos[r];
+            }
+            pad(a, b, lo[c], hi[c]);
+            if (count <= kMaxLeafSize) {
+                ref[c] = ~pos;
+                cnt[c] = count;
+                nl++;
+            } else {
+                ref[c] = node_rank[r];
+                cnt[c] = 0;
+            }
+        }
+        float4* np = nodes + 4 * (size_t)node_rank[i];
+        np[0] = make_float4(lo[0][0], hi[0][0], lo[1][0], hi[1][0]);
+        np[1] = make_float4(lo[0][1], hi[0][1], lo[1][1], hi[1][1]);
+        np[2] = make_float4(lo[0][2], hi[0][2], lo[1][2], hi[1][2]);
+        reinterpret_cast<int4*>(np)[3] = make_int4(ref[0], ref[1], cnt[0], cnt[1]);
+    }
+    if (nl) atomicAdd(leaves, nl);
+}
+
+// Triangle of sorted position k goes to its depth-first position.
+__global__ __launch_bounds__(kB) void k_ploc_gather(int n, const uint32_t* __restrict__ sorted,
+                                                    const int* __restrict__ leaf_pos, const float* __restrict__ pos,
+                                                    const uint32_t* __restrict__ idx, float4* __restrict__ tris) {
+    for (int k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB) {
+        const uint32_t p = sorted[k];
+        const uint32_t i0 = idx[3 * p], i1 = idx[3 * p + 1], i2 = idx[3 * p + 2];
+        const float p0x = pos[3 * i0], p0y = pos[3 * i0 + 1], p0z = pos[3 * i0 + 2];
+        const float e1x = pos[3 * i1] - p0x, e1y = pos[3 * i1 + 1] - p0y, e1z = pos[3 * i1 + 2] - p0z;
+        const float e2x = pos[3 * i2] - p0x, e2y = pos[3 * i2 + 1] - p0y, e2z = pos[3 * i2 + 2] - p0z;
+        float4* t = tris + 3 * (size_t)leaf_pos[k];
+        t[0] = make_float4(p0x, p0y, p0z, e1x);
+        t[1] = make_float4(e1y, e1z, e2x, e2y);
+        t[2] = make_float4(e2z, __uint_as_float(p), 0.f, 0.f);
+    }
+}
+
+}  // namespace
+
+dl_status build_bvh_ploc(Ctx* c, dl_accel_stats* stats) {
+    const int n = (int)c->nprims;
+    if (n <= kMaxLeafSize) return build_small(c, stats);
+    Scratch sc{c};
+    float4 *blo, *bhi, *clo[2], *chi[2], *nlo, *nhi;
+    unsigned long long *keys, *keys_sorted;
+    uint32_t *bounds, *sorted;
+    int *cref[2], *nn, *merge, *keep, *midx, *kidx, *ncount, *ecount, *parent_int, *parent_leaf, *leaf_pos, *node_pos,
+        *node_rank;
+    int2* child;
+    DL_TRY(sc.get(&blo, n));
+    DL_TRY(sc.get(&bhi, n));
+    for (int b = 0; b < 2; b++) {
+        DL_TRY(sc.get(&clo[b], n));
+        DL_TRY(sc.get(&chi[b], n));
+        DL_TRY(sc.get(&cref[b], n));
+    }
+    DL_TRY(sc.get(&nlo, n));
+    DL_TRY(sc.get(&nhi, n));
+    DL_TRY(sc.get(&keys, n));
+    DL_TRY(sc.get(&keys_sorted, n));
+    DL_TRY(sc.get(&bounds, 8));
+    DL_TRY(sc.get(&sorted, n));
+    DL_TRY(sc.get(&nn, n));
+    DL_TRY(sc.get(&merge, n + 1));
+    DL_TRY(sc.get(&keep, n + 1));
+    DL_TRY(sc.get(&midx, n + 1));
+    DL_TRY(sc.get(&kidx, n + 1));
+    DL_TRY(sc.get(&ncount, n));
+    DL_TRY(sc.get(&ecount, n));
+    DL_TRY(sc.get(&parent_int, n));
+    DL_TRY(sc.get(&parent_leaf, n));
+    DL_TRY(sc.get(&leaf_pos, n));
+    DL_TRY(sc.get(&node_pos, n));
+    DL_TRY(sc.get(&node_rank, n));
+    DL_TRY(sc.get(&child, n));
+    size_t sort_bytes = 0, scan_bytes = 0;
+    DL_HIP_CHECK(c, hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, keys, keys_sorted, n, 0, 64, c->stream));
+    DL_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, merge, midx, n + 1, c->stream));
+    void* temp = nullptr;
+    DL_TRY(sc.get((char**)&temp, std::max(sort_bytes, scan_bytes)));
+    void *pn = nullptr, *pt = nullptr;
+    DL_HIP_CHECK(c, hipMalloc(&pn, (size_t)(n - 1) * kNodeBytes));
+    DL_HIP_CHECK(c, hipMalloc(&pt, (size_t)n * kTriBytes));
+    c->ds.nodes = (float4*)pn;
+    c->ds.tris = (float4*)pt;
+    int* host_cnt = nullptr;
+    DL_HIP_CHECK(c, hipHostMalloc((void**)&host_cnt, 4 * sizeof(int), hipHostMallocDefault));
+    struct HostFree {
+        int* p;
+        ~HostFree() { (void)hipHostFree(p); }
+    } hf{host_cnt};
+
+    const int grid = std::max(1, c->num_cus) * 8;
+    DL_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
+    k_init_bounds<<<1, 64, 0, c->stream>>>(bounds);
+    k_prim_boxes<<<grid, kB, 0, c->stream>>>(c->ds.pos, c->ds.idx, (uint32_t)n, blo, bhi, bounds);
+    k_keys<<<grid, kB, 0, c->stream>>>(blo, bhi, (uint32_t)n, bounds, keys);
+    DL_HIP_CHECK(c, hipcub::DeviceRadixSort::SortKeys(temp, sort_bytes, keys, keys_sorted, n, 0, 64, c->stream));
+    k_low_ids<<<grid, kB, 0, c->stream>>>(keys_sorted, n, sorted);
+    k_ploc_init<<<grid, kB, 0, c->stream>>>(n, sorted, blo, bhi, cref[0], clo[0], chi[0]);
+    int cur = 0, count = n, top = n - 2, iters = 0;
+    while (count > 1) {
+        if (++iters > 4 * n) return set_error(c, DL_ERR_STATE, "PLOC made no progress");
+        k_ploc_nn<<<grid, kB, 0, c->stream>>>(count, clo[cur], chi[cur], nn);
+        k_ploc_flags<<<grid, kB, 0, c->stream>>>(count, nn, merge, keep);
+        DL_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(temp, scan_bytes, merge, midx, count + 1, c->stream));
+        DL_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(temp, scan_bytes, keep, kidx, count + 1, c->stream));
+        DL_HIP_CHECK(c, hipMemcpyAsync(&host_cnt[0], midx + count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        DL_HIP_CHECK(c, hipMemcpyAsync(&host_cnt[1], kidx + count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        k_ploc_merge<<<grid, kB, 0, c->stream>>>(count, top, nn, merge, midx, cref[cur], clo[cur], chi[cur], child, nlo,
+                                                 nhi, ncount, ecount, parent_int, parent_leaf);
+        k_ploc_compact<<<grid, kB, 0, c->stream>>>(count, keep, kidx, cref[cur], clo[cur], chi[cur], cref[cur ^ 1],
+                                                   clo[cur ^ 1], chi[cur ^ 1]);
+        DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        const int merges = host_cnt[0], kept = host_cnt[1];
+        if (merges <= 0 || kept != count - merges) return set_error(c, DL_ERR_STATE, "PLOC merge step inconsistent");
+        top -= merges;
+        count = kept;
+        cur ^= 1;
+    }
+    if (top != -1) return set_error(c, DL_ERR_STATE, "PLOC node count mismatch");
+    k_ploc_offsets<<<grid, kB, 0, c->stream>>>(n, child, ncount, ecount, parent_int, parent_leaf, leaf_pos, node_pos,
+                                               node_rank, bounds + 6);
+    k_ploc_emit<<<grid, kB, 0, c->stream>>>(n, child, nlo, nhi, ncount, leaf_pos, node_pos, node_rank, sorted, blo, bhi,
+                                            (float4*)pn, bounds + 7);
+    k_ploc_gather<<<grid, kB, 0, c->stream>>>(n, sorted, leaf_pos, c->ds.pos, c->ds.idx, (float4*)pt);
+    DL_HIP_CHECK(c, hipGetLastError());
+    DL_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
+    DL_HIP_CHECK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    DL_HIP_CHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    uint32_t hb[8];
+    DL_HIP_CHECK(c, hipMemcpy(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost));
+    int emitted = 0;
+    DL_HIP_CHECK(c, hipMemcpy(&emitted, ecount, sizeof(int), hipMemcpyDeviceToHost));  // root = node 0
+    stats->num_nodes = (uint32_t)emitted;
+    stats->num_leaves = hb[7];
+    stats->max_depth = hb[6];
+    stats->build_ms = ms;
+    if (hb[6] > (uint32_t)kMaxStack) return set_error(c, DL_ERR_STATE, "PLOC tree deeper than the traversal stack");
+    return DL_OK;
+}
+
+dl_status build_bvh_gpu(Ctx* c, dl_accel_stats* stats, int method) {
+    return method == DL_BUILD_GPU_LBVH ? build_bvh_lbvh(c, stats) : build_bvh_ploc(c, stats);
 }
 
 }  // namespace dl

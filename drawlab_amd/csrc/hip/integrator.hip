@@ -41,7 +41,9 @@ struct Counters {
     uint32_t mat[2][kTypes];
     uint32_t ray;
     uint32_t shadow;
-    uint32_t pad[14];
+    uint32_t work_ray;     // rays claimed by the closest-hit traversal
+    uint32_t work_shadow;  // rays claimed by the shadow traversal
+    uint32_t pad[12];
     unsigned long long nodes;
     unsigned long long tris;
     unsigned long long invalid;
@@ -69,6 +71,7 @@ struct RenderState {
     uint32_t film_pixels = 0;
     RenderBufs b{};
     bool film_valid = false;       // film holds the accumulation of earlier calls
+    bool types_present[8] = {false, false, false, false, false, false, false, false};
     Counters* cnt_host = nullptr;  // pinned
     std::vector<void*> allocs;
     std::vector<hipEvent_t> evs;
@@ -134,6 +137,8 @@ __global__ __launch_bounds__(kStageBlock) void k_reset(Counters* cnt, int next) 
     if (t == 0) {
         cnt->ray = 0;
         cnt->shadow = 0;
+        cnt->work_ray = 0;
+        cnt->work_shadow = 0;
     }
 }
 
@@ -176,21 +181,37 @@ __global__ __launch_bounds__(kStageBlock) void k_raygen(PassParams pp, dl_camera
     if (blockIdx.x == 0 && threadIdx.x == 0) rs.cnt->ray = pp.P;
 }
 
+// Queued path rays: the ray of path q[i] lives in ray_o/ray_d (shadow: sh_o/sh_d).
+struct QueueIO {
+    const uint32_t* __restrict__ q;
+    const float4* __restrict__ ro;
+    const float4* __restrict__ rd;
+    float4* __restrict__ hit;
+    uint8_t* __restrict__ occ;
+    __device__ __forceinline__ void load(uint32_t i, vec3& o, vec3& d, float& tmin, float& tmax) const {
+        const uint32_t p = q[i];
+        const float4 a = ro[p], b = rd[p];
+        o = v3(a.x, a.y, a.z);
+        d = v3(b.x, b.y, b.z);
+        tmin = a.w;
+        tmax = b.w;
+    }
+    __device__ __forceinline__ void store_hit(uint32_t i, uint32_t prim, float t, float u, float v, uint32_t,
+                                              uint32_t) const {
+        hit[q[i]] = make_float4(__uint_as_float(prim), t, u, v);
+    }
+    __device__ __forceinline__ void store_occ(uint32_t i, bool o) const { occ[q[i]] = o ? 1 : 0; }
+};
+
 template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(BvhView bvh, RenderBufs rs, int* __restrict__ ovf) {
     __shared__ int lds[kLdsStack * kTraceBlock];
     const uint32_t n = rs.cnt->ray;
     const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x, G = gridDim.x * kTraceBlock;
     LaneStack stk{lds, (int)threadIdx.x, kTraceBlock, ovf + g, G};
+    QueueIO io{rs.rayq, rs.ray_o, rs.ray_d, rs.hit, nullptr};
     TraceCounters cnt;
-    for (uint32_t i = g; i < n; i += G) {
-        const uint32_t p = rs.rayq[i];
-        const float4 o = rs.ray_o[p], d = rs.ray_d[p];
-        uint32_t prim;
-        float t, u, v;
-        trace_closest<COUNT>(bvh, f3(o), f3(d), o.w, d.w, stk, prim, t, u, v, cnt);
-        rs.hit[p] = make_float4(__uint_as_float(prim), t, u, v);
-    }
+    traverse_persistent<false, COUNT>(bvh, n, &rs.cnt->work_ray, stk, io, cnt);
     if (COUNT) {
         atomicAdd(&rs.cnt->nodes, (unsigned long long)cnt.nodes);
         atomicAdd(&rs.cnt->tris, (unsigned long long)cnt.tris);
@@ -202,12 +223,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_queue(BvhView bvh, Rende
     const uint32_t n = rs.cnt->shadow;
     const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x, G = gridDim.x * kTraceBlock;
     LaneStack stk{lds, (int)threadIdx.x, kTraceBlock, ovf + g, G};
+    QueueIO io{rs.shq, rs.sh_o, rs.sh_d, nullptr, rs.occ};
     TraceCounters cnt;
-    for (uint32_t i = g; i < n; i += G) {
-        const uint32_t p = rs.shq[i];
-        const float4 o = rs.sh_o[p], d = rs.sh_d[p];
-        rs.occ[p] = trace_any<false>(bvh, f3(o), f3(d), o.w, d.w, stk, cnt) ? 1 : 0;
-    }
+    traverse_persistent<true, false>(bvh, n, &rs.cnt->work_shadow, stk, io, cnt);
 }
 
 __device__ __forceinline__ int material_of(const SceneView& s, uint32_t prim, int& mat) {
@@ -260,17 +278,20 @@ __global__ __launch_bounds__(kStageBlock) void k_post_camera(PassParams pp, Scen
 
 // One bounce at the current vertex: emitter sampling + BSDF sampling
 // (path.cpp:32-69).
+// One kernel instantiation per BSDF type (the material-sorted queue of that
+// type), so each carries only its own BSDF's code and registers.
+template <int TYPE>
 __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView s, RenderBufs rs, int cur) {
-    const uint32_t* counts = rs.cnt->mat[cur];
-    const uint32_t n = queue_total(counts);
+    const uint32_t n = rs.cnt->mat[cur][TYPE];
+    const uint32_t* __restrict__ queue = rs.matq[cur] + (size_t)TYPE * pp.cap;
     const uint32_t WH = pp.W * pp.H;
+    constexpr int type = TYPE;
     for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
         const uint32_t i = i0 + threadIdx.x;
         bool want_shadow = false, want_cont = false;
         uint32_t p = 0;
         if (i < n) {
-            int type;
-            queue_entry(counts, rs.matq[cur], pp.cap, i, p, type);
+            p = queue[i];
             const float4 h = rs.hit[p];
             const uint32_t prim = __float_as_uint(h.x);
             const vec3 d = f3(rs.ray_d[p]);
@@ -637,6 +658,8 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
     DL_TRY(ensure_state(c, cap, WH));
     RenderState* rs = (RenderState*)c->rs;
     const SceneView sv = scene_view(c);
+    for (int t = 0; t < 8; t++) rs->types_present[t] = false;
+    for (const auto& m : c->mats) rs->types_present[m.type] = true;
     const BvhView bvh{c->ds.nodes, c->ds.tris};
     const bool count = stats && (prm->flags & DL_RENDER_COUNT_TRAVERSAL);
     const int stage_grid = std::max(1, c->num_cus) * 8;
@@ -692,7 +715,17 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
             if (live == 0) break;
             k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, cur ^ 1);
             DL_TRY(tm.start(kTagShade));
-            k_shade<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
+            for (int t = 0; t < DL_BSDF_COUNT; t++) {
+                if (!rs->types_present[t] || rs->cnt_host->mat[cur][t] == 0) continue;
+                switch (t) {
+                    case DL_BSDF_DIFFUSE: k_shade<DL_BSDF_DIFFUSE><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur); break;
+                    case DL_BSDF_MIRROR: k_shade<DL_BSDF_MIRROR><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur); break;
+                    case DL_BSDF_DIELECTRIC: k_shade<DL_BSDF_DIELECTRIC><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur); break;
+                    case DL_BSDF_MICROFACET: k_shade<DL_BSDF_MICROFACET><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur); break;
+                    case DL_BSDF_ANISO_GGX: k_shade<DL_BSDF_ANISO_GGX><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur); break;
+                    case DL_BSDF_CONDUCTOR: k_shade<DL_BSDF_CONDUCTOR><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur); break;
+                }
+            }
             DL_TRY(tm.stop());
             DL_TRY(tm.start(kTagClosest));
             if (count)

@@ -1,4 +1,5 @@
 // BVH traversal for gfx950: one ray per lane, child-pair nodes (bvh_layout.h),
+// persistent lanes with per-lane ray refill,
 // per-lane traversal stack staged in LDS with a lane-interleaved layout
 // (entry k of lane l at stack[k * blockDim + l]: every wave touches 64
 // consecutive dwords whatever the stack depths, so the accesses are
@@ -93,16 +94,76 @@ __device__ __forceinline__ bool mt_test(const float4& a, const float4& b, const 
     return true;
 }
 
-template <bool COUNT>
-__device__ __forceinline__ void trace_closest(const BvhView& bvh, vec3 o, vec3 d, float tmin, float tmax,
-                                              LaneStack& stk, uint32_t& hit_prim, float& hit_t, float& hit_u,
-                                              float& hit_v, TraceCounters& cnt) {
-    const vec3 rcp = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    float best_t = tmax, bu = 0.0f, bv = 0.0f;
+// Wave-aggregated claim of the next work item: one atomic per wave.
+__device__ __forceinline__ uint32_t claim_next(uint32_t* counter, bool want) {
+    const unsigned long long mask = __ballot(want);
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (want && lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// Persistent traversal with per-lane ray refill.  Every lane of a wave stays
+// in the loop until the whole wave has run out of rays; a lane whose ray
+// finishes starts its next ray at once (lane g takes rays g, g + G, g + 2G,
+// ...: no atomic round trip on the refill path), so a wave never idles
+// behind its slowest ray.  Each iteration visits one node.
+//
+// IO provides: load(i, o, d, tmin, tmax) -> bool (false: skip the ray),
+// store_hit(i, prim, t, u, v, nodes, tris) for closest hits and
+// store_occ(i, occluded) for any-hit queries.
+template <bool ANY, bool COUNT, class IO>
+__device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t n, uint32_t* work, LaneStack& stk,
+                                                    IO& io, TraceCounters& total) {
+    (void)work;
+    const uint32_t G = gridDim.x * blockDim.x;
+    uint32_t next = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t ray = 0;
+    bool active = false, exhausted = false;
+    vec3 o = v3(0, 0, 0), d = v3(0, 0, 0), rcp = v3(0, 0, 0);
+    float tmin = 0.f, best_t = 0.f, bu = 0.f, bv = 0.f;
     uint32_t best = 0xffffffffu;
-    int sp = 0;
-    int cur = 0;
+    int sp = 0, cur = 0;
+    TraceCounters cnt;
     for (;;) {
+        const bool need = !active && !exhausted;
+        if (need) {
+            const uint32_t slot = next;
+            next += G;
+            {
+                if (slot < n) {
+                    ray = slot;
+                    float tmax;
+                    io.load(slot, o, d, tmin, tmax);
+                    rcp = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+                    best_t = tmax;
+                    best = 0xffffffffu;
+                    bu = bv = 0.0f;
+                    sp = 0;
+                    cur = 0;
+                    cnt = TraceCounters();
+                    // A NaN anywhere in the ray makes every Moller-Trumbore
+                    // comparison false (no hit), but would pass every slab
+                    // test: finish it at once as a miss.
+                    if (isnan(o.x) || isnan(o.y) || isnan(o.z) || isnan(d.x) || isnan(d.y) || isnan(d.z) || isnan(tmin) ||
+                        isnan(tmax)) {
+                        if (ANY)
+                            io.store_occ(ray, false);
+                        else
+                            io.store_hit(ray, best, best_t, 0.0f, 0.0f, 0, 0);
+                    } else {
+                        active = true;
+                    }
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (!__ballot(active)) break;
+        if (!active) continue;
+
         const float4* np = bvh.nodes + 4 * cur;
         const float4 r0 = np[0], r1 = np[1], r2 = np[2];
         const int4 r3 = reinterpret_cast<const int4*>(np)[3];
@@ -110,12 +171,12 @@ __device__ __forceinline__ void trace_closest(const BvhView& bvh, vec3 o, vec3 d
         bool h0, h1;
         float n0, n1;
         slab2(r0, r1, r2, o, rcp, tmin, best_t, h0, n0, h1, n1);
-        // Leaf children are intersected at once.
+        bool hit_any = false;
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             const bool h = side ? h1 : h0;
             const int ref = side ? r3.y : r3.x;
-            if (h && ref < 0) {
+            if (h && ref < 0 && !hit_any) {
                 const int first = ~ref, count = side ? r3.w : r3.z;
                 for (int k = first; k < first + count; ++k) {
                     const float4* tp = bvh.tris + 3 * k;
@@ -123,6 +184,10 @@ __device__ __forceinline__ void trace_closest(const BvhView& bvh, vec3 o, vec3 d
                     if (COUNT) cnt.tris++;
                     float u, v, t;
                     if (mt_test(ta, tb, tc, o, d, u, v, t) && t >= tmin && t <= best_t) {
+                        if (ANY) {
+                            hit_any = true;
+                            break;
+                        }
                         const uint32_t id = __float_as_uint(tc.y);
                         if (t < best_t || id < best) {
                             best_t = t;
@@ -132,11 +197,15 @@ __device__ __forceinline__ void trace_closest(const BvhView& bvh, vec3 o, vec3 d
                         }
                     }
                 }
-                if (side) h1 = false; else h0 = false;
             }
         }
-        if (h0 && h1) {
-            const bool first0 = n0 <= n1;
+        if (h0 && r3.x < 0) h0 = false;
+        if (h1 && r3.y < 0) h1 = false;
+        bool done = false;
+        if (ANY && hit_any) {
+            done = true;
+        } else if (h0 && h1) {
+            const bool first0 = ANY ? true : n0 <= n1;
             const int nearc = first0 ? r3.x : r3.y, farc = first0 ? r3.y : r3.x;
             stk.push(sp, farc);
             cur = nearc;
@@ -144,60 +213,23 @@ __device__ __forceinline__ void trace_closest(const BvhView& bvh, vec3 o, vec3 d
             cur = r3.x;
         } else if (h1) {
             cur = r3.y;
+        } else if (sp == 0) {
+            done = true;
         } else {
-            if (sp == 0) break;
             cur = stk.pop(sp);
         }
-    }
-    hit_prim = best;
-    hit_t = best == 0xffffffffu ? tmax : best_t;
-    hit_u = bu;
-    hit_v = bv;
-}
-
-template <bool COUNT>
-__device__ __forceinline__ bool trace_any(const BvhView& bvh, vec3 o, vec3 d, float tmin, float tmax,
-                                          LaneStack& stk, TraceCounters& cnt) {
-    const vec3 rcp = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    int sp = 0;
-    int cur = 0;
-    for (;;) {
-        const float4* np = bvh.nodes + 4 * cur;
-        const float4 r0 = np[0], r1 = np[1], r2 = np[2];
-        const int4 r3 = reinterpret_cast<const int4*>(np)[3];
-        if (COUNT) cnt.nodes++;
-        bool h0, h1;
-        float n0, n1;
-        slab2(r0, r1, r2, o, rcp, tmin, tmax, h0, n0, h1, n1);
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            const bool h = side ? h1 : h0;
-            const int ref = side ? r3.y : r3.x;
-            if (h && ref < 0) {
-                const int first = ~ref, count = side ? r3.w : r3.z;
-                for (int k = first; k < first + count; ++k) {
-                    const float4* tp = bvh.tris + 3 * k;
-                    const float4 ta = tp[0], tb = tp[1], tc = tp[2];
-                    if (COUNT) cnt.tris++;
-                    float u, v, t;
-                    if (mt_test(ta, tb, tc, o, d, u, v, t) && t >= tmin && t <= tmax) return true;
-                }
-                if (side) h1 = false; else h0 = false;
+        if (done) {
+            if (ANY)
+                io.store_occ(ray, hit_any);
+            else
+                io.store_hit(ray, best, best_t, bu, bv, cnt.nodes, cnt.tris);  // a miss keeps t = tmax
+            if (COUNT) {
+                total.nodes += cnt.nodes;
+                total.tris += cnt.tris;
             }
-        }
-        if (h0 && h1) {
-            stk.push(sp, r3.y);
-            cur = r3.x;
-        } else if (h0) {
-            cur = r3.x;
-        } else if (h1) {
-            cur = r3.y;
-        } else {
-            if (sp == 0) break;
-            cur = stk.pop(sp);
+            active = false;
         }
     }
-    return false;
 }
 
 }  // namespace dl

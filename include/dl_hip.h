@@ -132,7 +132,9 @@ typedef struct dl_accel_stats {
     uint64_t tri_bytes;        /* S_tri * num_prims */
 } dl_accel_stats;
 
-enum { DL_BUILD_GPU = 0, DL_BUILD_HOST_SAH = 1 };
+/* DL_BUILD_GPU: PLOC clustering on the device (SAH-like quality);
+   DL_BUILD_HOST_SAH: binned SAH on the host; DL_BUILD_GPU_LBVH: Karras LBVH. */
+enum { DL_BUILD_GPU = 0, DL_BUILD_HOST_SAH = 1, DL_BUILD_GPU_LBVH = 2 };
 
 typedef struct dl_render_params {
     int32_t spp;               /* samples per pixel (the sampler's sampleCount) */
