@@ -3,16 +3,24 @@
 
 Workload: the 2,097,152-triangle displaced sphere (GGX conductor) with a
 20,480-triangle glass sphere, a diffuse ground and a 1024x512 sky + sun
-environment map, rendered at 1920x1080 with max depth 16.  One step is one
-wavefront pass of the unidirectional MIS path tracer over the whole frame
-(1 sample per pixel, 2,073,600 paths), accumulated progressively; the
-default 256 steps are the workload's 256 spp.
+environment map, rendered at 1920x1080 with max depth 16, 256 spp.  One step
+is one wavefront pass of the unidirectional MIS path tracer over the whole
+frame at 8 samples per pixel (16,588,800 paths), accumulated progressively;
+the default 32 steps are the workload's 256 spp.
 
 Prints one JSON line: Mpaths/s over the timed steps, the roofline of the
-dominant kernel (closest-hit traversal) and the CPU oracle's throughput on a
-bounded sample of the same workload, timed on this host's cores.
+dominant kernel (closest-hit traversal, k_trace_queue) and the CPU oracle's
+throughput on a bounded sample of the same workload, timed on this host's
+cores.
+
+Profile helpers (they read rocprofv3 CSV output and write the summaries that
+`roofline.traffic` and the kernel-time tables come from):
+  bench.py --summarize-stats DIR OUT.md       kernel time table
+  bench.py --summarize-pmc FETCH_DIR WRITE_DIR OUT.json
 """
 import argparse
+import csv
+import glob
 import json
 import os
 import sys
@@ -22,16 +30,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Mpaths/s + relMSE@equal-time (guided vs plain); p50/p99 job latency"
+WORKLOAD = "displaced_sphere_2M"
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES = 64, 48, 32, 16
+TRACE_KERNEL = "k_trace_queue<false>"
+COPY_KERNEL = "k_copy"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_queue_%s_traffic.json" % WORKLOAD)
 
 
 def traffic_from_profiles():
-    """HBM bytes per trace_closest launch measured with rocprofv3 PMC runs
-    (profiles/trace_closest_c1_traffic.json), or None."""
-    path = os.path.join(ROOT, "profiles", "trace_closest_c1_traffic.json")
+    """HBM bytes per closest-hit traversal launch of this workload, measured
+    by two rocprofv3 PMC runs (FETCH_SIZE, WRITE_SIZE) scaled by a copy
+    kernel of known size; None when the summary is absent."""
     try:
-        with open(path) as f:
+        with open(TRAFFIC_FILE) as f:
             return float(json.load(f)["hbm_bytes_per_launch"])
     except (OSError, KeyError, ValueError):
         return None
@@ -62,14 +74,87 @@ def cpu_baseline(desc, target_s, threads):
     }
 
 
+def short_name(name):
+    n = name.replace("void ", "").replace("dl::(anonymous namespace)::", "").replace("dl::", "")
+    if n.startswith("rocprim::"):
+        return n.split("<")[0] + "<...>" + ("" if "(" not in n else "")
+    return n.split("(")[0]
+
+
+def summarize_stats(d, out_md, command):
+    path = sorted(glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True))[0]
+    rows = list(csv.DictReader(open(path)))
+    lines = ["# Kernel time: %s bench (1920x1080, 8 spp per pass, max depth 16)" % WORKLOAD, "",
+             "Command (repository root, MI355X):", "", "```", command, "```", "",
+             "| kernel | calls | total ms | avg us | min us | max us | % |", "|---|---:|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        lines.append("| %s | %s | %.2f | %.1f | %.1f | %.1f | %.2f |" % (
+            short_name(r["Name"]), r["Calls"], float(r["TotalDurationNs"]) / 1e6, float(r["AverageNs"]) / 1e3,
+            float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3, float(r["Percentage"])))
+    with open(out_md, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines[:20]))
+
+
+def _pmc_means(d, counter):
+    path = sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True))[0]
+    acc = {}
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        acc.setdefault(short_name(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+    return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+
+def summarize_pmc(fetch_dir, write_dir, out_json, copy_bytes, commands):
+    f = _pmc_means(fetch_dir, "FETCH_SIZE")
+    w = _pmc_means(write_dir, "WRITE_SIZE")
+    cf = copy_bytes / (f[COPY_KERNEL][0] * 1024.0)
+    cw = copy_bytes / (w[COPY_KERNEL][0] * 1024.0)
+    fetch = f[TRACE_KERNEL][0] * 1024.0 * cf
+    write = w[TRACE_KERNEL][0] * 1024.0 * cw
+    out = {
+        "kernel": TRACE_KERNEL,
+        "workload": WORKLOAD + " (8 spp per pass)",
+        "dispatches": f[TRACE_KERNEL][1],
+        "fetch_kib_per_dispatch": f[TRACE_KERNEL][0],
+        "write_kib_per_dispatch": w[TRACE_KERNEL][0],
+        "copy_kernel_bytes": copy_bytes,
+        "copy_fetch_kib": f[COPY_KERNEL][0],
+        "copy_write_kib": w[COPY_KERNEL][0],
+        "fetch_scale": cf,
+        "write_scale": cw,
+        "hbm_bytes_per_launch": fetch + write,
+        "commands": commands,
+        "other_kernels_fetch_kib": {k: v[0] for k, v in f.items()},
+        "other_kernels_write_kib": {k: v[0] for k, v in w.items()},
+    }
+    with open(out_json, "w") as fo:
+        json.dump(out, fo, indent=1)
+    print(json.dumps({k: out[k] for k in list(out)[:12]}, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=256)
-    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--spp-per-step", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="after the timed steps, run the plain copy kernel twice over this many MiB")
+    ap.add_argument("--summarize-stats", nargs=2, metavar=("DIR", "OUT_MD"))
+    ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"))
+    ap.add_argument("--command", default="", help="command line recorded in a summary")
+    ap.add_argument("--copy-mib", type=int, default=1024, help="calibration copy size used by the PMC runs")
     args = ap.parse_args()
+    if args.summarize_stats:
+        summarize_stats(args.summarize_stats[0], args.summarize_stats[1], args.command)
+        return
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc, copy_bytes=args.copy_mib << 20, commands=args.command)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X, so --gpus %d is not run "
               "(the frame would shard by sample index across GPUs)" % args.gpus, file=sys.stderr)
@@ -80,24 +165,26 @@ def main():
 
     desc = scenes.displaced_sphere()
     W, H = desc.camera.width, desc.camera.height
+    S = args.spp_per_step
     be = drawlab_amd.Backend(0)
     be.upload_scene(desc)
     accel = be.build_accel(drawlab_amd.BUILD_GPU)
 
     # Untimed pass with in-kernel counters: traversal work per closest-hit ray.
-    _, cst = be.render(spp=1, seed=1 << 20, count_traversal=True, want_image=False)
+    _, cst = be.render(spp=S, spp_per_pass=S, seed=1 << 20, count_traversal=True, want_image=False)
     nodes_per_ray = cst["nodes_closest"] / cst["closest_rays"]
     tris_per_ray = cst["tris_closest"] / cst["closest_rays"]
     bytes_per_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray
 
     for i in range(args.warmup):
-        be.render(spp=1, first_sample=i, accumulate=i > 0, want_image=False)
+        be.render(spp=S, spp_per_pass=S, first_sample=i * S, accumulate=i > 0, want_image=False)
     be.sync()
 
     agg = {}
     t0 = time.perf_counter()
     for k in range(args.steps):
-        _, st = be.render(spp=1, first_sample=args.warmup + k, accumulate=True, want_image=False)
+        _, st = be.render(spp=S, spp_per_pass=S, first_sample=(args.warmup + k) * S, accumulate=True,
+                          want_image=False)
         for key, v in st.items():
             agg[key] = agg.get(key, 0.0) + v
     be.sync()
@@ -105,8 +192,20 @@ def main():
     seconds = t1 - t0
     paths = agg["paths"]
     value = paths / seconds / 1e6
-    achieved = agg["closest_rays"] * bytes_per_ray / (agg["ms_closest"] * 1e-3) / 1e9
+    launches = max(1.0, agg["launches_closest"])
+    bytes_per_launch = agg["closest_rays"] * bytes_per_ray / launches
+    ms_per_launch = agg["ms_closest"] / launches
+    achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
     traffic = traffic_from_profiles()
+
+    if args.calibration_copy_mib > 0:
+        n = args.calibration_copy_mib << 20
+        a, b = drawlab_amd.DeviceBuffer(be, n), drawlab_amd.DeviceBuffer(be, n)
+        for _ in range(2):
+            b.copy_from(a)
+        be.sync()
+        a.free()
+        b.free()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -126,30 +225,32 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene generated in code)",
         "config": {
-            "workload": "displaced_sphere_2M",
+            "workload": WORKLOAD,
             "triangles": int(desc.num_triangles),
             "width": W,
             "height": H,
-            "spp": args.steps,
+            "spp": args.steps * S,
+            "spp_per_step": S,
             "max_depth": desc.max_depth,
             "env_map": "1024x512",
-            "bvh": "gpu-lbvh",
+            "bvh": "gpu-ploc",
             "bvh_build_ms": round(accel["build_ms"], 2),
         },
         "mrays_per_s": round((agg["closest_rays"] + agg["shadow_rays"]) / seconds / 1e6, 2),
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_trace_queue (closest hit)",
+            "kernel": TRACE_KERNEL + " (closest hit)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "algorithmic_bytes_per_launch": round(bytes_per_launch),
             "bytes_per_ray": round(bytes_per_ray, 1),
             "node_visits_per_ray": round(nodes_per_ray, 2),
             "tri_tests_per_ray": round(tris_per_ray, 2),
-            "launches": int(agg["launches_closest"]),
-            "avg_launch_ms": round(agg["ms_closest"] / max(1.0, agg["launches_closest"]), 4),
+            "launches": int(launches),
+            "avg_launch_ms": round(ms_per_launch, 4),
         },
         "stage_ms": {k: round(agg[k], 1) for k in ("ms_closest", "ms_shadow", "ms_shade", "ms_film", "ms_other")},
         "cpu_baseline": cpu,

@@ -26,7 +26,7 @@ EXPORTED_SYMBOLS = (
     "dl_hip_create", "dl_hip_destroy", "dl_hip_last_error", "dl_hip_device_name", "dl_hip_upload_scene",
     "dl_hip_build_accel", "dl_hip_trace_closest", "dl_hip_trace_shadow", "dl_hip_trace_closest_counted",
     "dl_hip_render", "dl_hip_render_paths", "dl_hip_malloc", "dl_hip_free", "dl_hip_memcpy_h2d",
-    "dl_hip_memcpy_d2h", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_film_filter",
+    "dl_hip_memcpy_d2h", "dl_hip_memcpy_d2d", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_film_filter",
     "dl_mesh_tangents",
 )
 
@@ -117,6 +117,7 @@ def lib():
             "dl_hip_free": (i32, [P, P]),
             "dl_hip_memcpy_h2d": (i32, [P, P, P, c.c_size_t]),
             "dl_hip_memcpy_d2h": (i32, [P, P, P, c.c_size_t]),
+            "dl_hip_memcpy_d2d": (i32, [P, P, P, c.c_size_t]),
             "dl_hip_sync": (i32, [P]),
             "dl_hip_last_kernel_ms": (i32, [P, c.POINTER(c.c_double)]),
             "dl_camera_virtual": (None, [P, P, P, c.c_float, i32, i32, c.POINTER(dl_camera_desc)]),
@@ -170,6 +171,12 @@ class DeviceBuffer:
         assert out.nbytes <= self.nbytes
         self.be._check(lib().dl_hip_memcpy_d2h(self.be.h, _p(out), self.ptr, out.nbytes))
         return out
+
+    def copy_from(self, other, nbytes=None):
+        """Device-to-device copy (asynchronous on the context stream)."""
+        n = min(self.nbytes, other.nbytes) if nbytes is None else int(nbytes)
+        assert n <= self.nbytes and n <= other.nbytes
+        self.be._check(lib().dl_hip_memcpy_d2d(self.be.h, self.ptr, other.ptr, n))
 
     def free(self):
         if self.ptr is not None and self.be.h:

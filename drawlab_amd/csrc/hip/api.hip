@@ -53,6 +53,17 @@ void free_scene(Ctx* c) {
         if (s_ != DL_OK) return s_;      \
     } while (0)
 
+// Plain streaming copy: 16 B per lane per iteration, grid-stride; the tail
+// (bytes % 16) is copied byte-wise by the first lanes.
+__global__ __launch_bounds__(256) void k_copy(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n16,
+                                              uint8_t* __restrict__ dtail, const uint8_t* __restrict__ stail,
+                                              uint32_t ntail) {
+    const size_t G = (size_t)gridDim.x * blockDim.x;
+    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t i = i0; i < n16; i += G) dst[i] = src[i];
+    if (i0 < ntail) dtail[i0] = stail[i0];
+}
+
 }  // namespace
 }  // namespace dl
 
@@ -346,6 +357,24 @@ dl_status dl_hip_memcpy_d2h(void* h, void* dst, const void* src, size_t bytes) {
     if (!c) return DL_ERR_INVALID;
     DL_HIP_CHECK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return DL_OK;
+}
+dl_status dl_hip_memcpy_d2d(void* h, void* dst, const void* src, size_t bytes) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    if (bytes == 0) return DL_OK;
+    if (!dst || !src) return set_error(c, DL_ERR_INVALID, "null pointer");
+    if ((((uintptr_t)dst | (uintptr_t)src) & 15) != 0) return set_error(c, DL_ERR_INVALID, "pointers must be 16-byte aligned");
+    (void)hipSetDevice(c->device);
+    const size_t n16 = bytes / 16;
+    const uint32_t ntail = (uint32_t)(bytes % 16);
+    const size_t want = (n16 + 255) / 256;
+    const int grid = (int)std::max<size_t>(1, std::min<size_t>(want, (size_t)std::max(1, c->num_cus) * 32));
+    DL_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
+    k_copy<<<grid, 256, 0, c->stream>>>((uint4*)dst, (const uint4*)src, n16, (uint8_t*)dst + n16 * 16,
+                                        (const uint8_t*)src + n16 * 16, ntail);
+    DL_HIP_CHECK(c, hipGetLastError());
+    DL_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
     return DL_OK;
 }
 dl_status dl_hip_sync(void* h) {

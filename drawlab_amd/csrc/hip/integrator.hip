@@ -19,7 +19,7 @@
 //   film            deterministic gather of the pass's samples through the
 //                   reconstruction filter (ImageBlock::put, block.cpp:81-113)
 //
-// Queues are appended with wave-aggregated atomics (one atomic per wave and
+// Queues are appended with block-aggregated atomics (one atomic per block and
 // queue).  Stage kernels are persistent grid-stride loops whose trip counts
 // come from device-side counters, so no launch depends on a host readback
 // of its size.
@@ -39,8 +39,8 @@ enum : uint32_t { kFlagDiffuse = 1u, kFlagNee = 2u, kFlagCont = 4u };
 
 struct Counters {
     uint32_t mat[2][kTypes];
-    uint32_t ray;
-    uint32_t shadow;
+    uint32_t ray;     // continuation rays queued (block_append index 0)
+    uint32_t shadow;  // shadow rays queued (block_append index 1)
     uint32_t work_ray;     // rays claimed by the closest-hit traversal
     uint32_t work_shadow;  // rays claimed by the shadow traversal
     uint32_t pad[12];
@@ -80,15 +80,45 @@ struct RenderState {
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
-    const unsigned long long mask = __ballot(pred);
-    if (!pred) return 0;
+constexpr int kStageWaves = kStageBlock / 64;
+
+// Block-aggregated queue append.  Every thread of the block calls it (the
+// call sits in block-uniform control flow); bit q of `bits` asks for a slot in
+// queue q, whose fill counter is counters[q].  One atomic per block and
+// non-empty queue: the counters share one cache line that every block of the
+// grid updates, so per-wave atomics serialise there.  Slots of a queue are
+// dense, ordered by (wave, lane) inside the block; emit(q, slot) runs once per
+// set bit.
+template <int NQ, typename Emit>
+__device__ __forceinline__ void block_append(uint32_t* counters, uint32_t bits, Emit emit) {
+    __shared__ uint32_t s_cnt[kStageWaves][NQ];
+    __shared__ uint32_t s_base[kStageWaves][NQ];
     const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    const int wave = threadIdx.x >> 6;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    unsigned long long masks[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        masks[q] = __ballot((bits >> q) & 1u);
+        if (lane == 0) s_cnt[wave][q] = (uint32_t)__popcll(masks[q]);
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        const int q = threadIdx.x;
+        uint32_t tot = 0;
+        for (int w = 0; w < kStageWaves; w++) {
+            const uint32_t c = s_cnt[w][q];
+            s_cnt[w][q] = tot;
+            tot += c;
+        }
+        const uint32_t base = tot ? atomicAdd(&counters[q], tot) : 0u;
+        for (int w = 0; w < kStageWaves; w++) s_base[w][q] = base + s_cnt[w][q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+        if ((bits >> q) & 1u) emit(q, s_base[wave][q] + (uint32_t)__popcll(masks[q] & lt));
+    __syncthreads();
 }
 
 __device__ __forceinline__ vec3 f3(const float4& a) { return v3(a.x, a.y, a.z); }
@@ -267,12 +297,10 @@ __global__ __launch_bounds__(kStageBlock) void k_post_camera(PassParams pp, Scen
             }
             rs.rad[p] = f4(rad, 0.f);
         }
-#pragma unroll
-        for (int t = 0; t < DL_BSDF_COUNT; t++) {
-            const bool mine = alive && type == t;
-            const uint32_t slot = wave_append(&rs.cnt->mat[next][t], mine);
-            if (mine) rs.matq[next][(size_t)t * pp.cap + slot] = p;
-        }
+        uint32_t* const mq = rs.matq[next];
+        const uint32_t cap = pp.cap;
+        block_append<DL_BSDF_COUNT>(rs.cnt->mat[next], alive ? 1u << type : 0u,
+                                    [&](int t, uint32_t slot) { mq[(size_t)t * cap + slot] = p; });
     }
 }
 
@@ -374,10 +402,13 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
             aux.y = (int)flags;
             rs.aux[p] = aux;
         }
-        const uint32_t ss = wave_append(&rs.cnt->shadow, want_shadow);
-        if (want_shadow) rs.shq[ss] = p;
-        const uint32_t cs = wave_append(&rs.cnt->ray, want_cont);
-        if (want_cont) rs.rayq[cs] = p;
+        // counters: ray (continuations), then shadow -- adjacent fields
+        block_append<2>(&rs.cnt->ray, (want_cont ? 1u : 0u) | (want_shadow ? 2u : 0u), [&](int q, uint32_t slot) {
+            if (q == 0)
+                rs.rayq[slot] = p;
+            else
+                rs.shq[slot] = p;
+        });
     }
 }
 
@@ -413,10 +444,8 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
                 if (prim == 0xffffffffu) {
                     if (s.env_light >= 0) {
                         const LightRec& e = s.lights[s.env_light];
-                        const vec3 ev = env_eval(s, e, d);
-                        DirSample hs;
-                        hs.d = d;
-                        const float epdf = pdf_emitter_direction(s, s.env_light, hs);
+                        float epdf;
+                        const vec3 ev = env_eval_pdf(s, e, d, epdf);
                         const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
                         rad = add(rad, mul(scale(thr, mis), ev));
                     }
@@ -458,12 +487,10 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
             }
             rs.rad[p] = f4(rad, 0.f);
         }
-#pragma unroll
-        for (int t = 0; t < DL_BSDF_COUNT; t++) {
-            const bool mine = alive && ntype == t;
-            const uint32_t slot = wave_append(&rs.cnt->mat[next][t], mine);
-            if (mine) rs.matq[next][(size_t)t * pp.cap + slot] = p;
-        }
+        uint32_t* const mq = rs.matq[next];
+        const uint32_t cap = pp.cap;
+        block_append<DL_BSDF_COUNT>(rs.cnt->mat[next], alive ? 1u << ntype : 0u,
+                                    [&](int t, uint32_t slot) { mq[(size_t)t * cap + slot] = p; });
     }
 }
 
@@ -650,7 +677,7 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
     if (c->light_recs.empty()) return set_error(c, DL_ERR_INVALID, "There is not emitter in the scene.");
     (void)hipSetDevice(c->device);
     const uint32_t W = (uint32_t)c->cam.width, H = (uint32_t)c->cam.height, WH = W * H;
-    uint32_t k = prm->spp_per_pass > 0 ? (uint32_t)prm->spp_per_pass : std::max(1u, (2u << 20) / WH);
+    uint32_t k = prm->spp_per_pass > 0 ? (uint32_t)prm->spp_per_pass : std::max(1u, (16u << 20) / WH);
     k = std::min<uint32_t>(k, (uint32_t)prm->spp);
     if (paths_sample >= 0) k = 1;
     if ((uint64_t)WH * k > (1ull << 27)) return set_error(c, DL_ERR_INVALID, "pass too large");

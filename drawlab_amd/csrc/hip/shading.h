@@ -360,6 +360,16 @@ __device__ __forceinline__ DirSample dir_sample(vec3 o, vec3 dst, vec3 n) {  // 
     return ds;
 }
 
+// Wrap addressing: i mod n in [0, n).  Lookups at u, v in [0, 1] only ever
+// need the in-range test; the 32-bit remainder covers every other index.
+__device__ __forceinline__ int wrap_index(int i, int n) {
+    if ((unsigned)i < (unsigned)n) return i;
+    if (i == -1) return n - 1;
+    if (i == n) return 0;
+    const int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
 // Bilinear lookup with wrap addressing on the lat-long image (row 0 = top).
 __device__ __forceinline__ vec3 env_lookup(const SceneView& s, const LightRec& e, float u, float v) {
     const int w = e.env_w, h = e.env_h;
@@ -367,9 +377,9 @@ __device__ __forceinline__ vec3 env_lookup(const SceneView& s, const LightRec& e
     const float y = v * (float)h - 0.5f;
     const float fx0 = floorf(x), fy0 = floorf(y);
     const float ax = x - fx0, ay = y - fy0;
-    const long ix = (long)fx0, iy = (long)fy0;
-    const long x0 = ((ix % w) + w) % w, x1 = (((ix + 1) % w) + w) % w;
-    const long y0 = ((iy % h) + h) % h, y1 = (((iy + 1) % h) + h) % h;
+    const int ix = (int)fx0, iy = (int)fy0;
+    const int x0 = wrap_index(ix, w), x1 = wrap_index(ix + 1, w);
+    const int y0 = wrap_index(iy, h), y1 = wrap_index(iy + 1, h);
     const float4 c00 = s.env_rgb[y0 * w + x0], c10 = s.env_rgb[y0 * w + x1];
     const float4 c01 = s.env_rgb[y1 * w + x0], c11 = s.env_rgb[y1 * w + x1];
     const float w00 = (1.0f - ax) * (1.0f - ay), w10 = ax * (1.0f - ay);
@@ -391,12 +401,25 @@ __device__ __forceinline__ vec3 env_eval(const SceneView& s, const LightRec& e, 
     env_coords(wi, u, v);
     return env_lookup(s, e, u, v);
 }
-__device__ __forceinline__ float env_pdf(const SceneView& s, const LightRec& e, vec3 d) {
-    float u, v;
-    env_coords(d, u, v);
+__device__ __forceinline__ float env_pdf_uv(const SceneView& s, const LightRec& e, float u, float v) {
     const int x = min((int)(u * (float)e.env_w), e.env_w - 1);
     const int y = min((int)(v * (float)e.env_h), e.env_h - 1);
     return s.env_pdf[(size_t)y * e.env_w + x];
+}
+__device__ __forceinline__ float env_pdf(const SceneView& s, const LightRec& e, vec3 d) {
+    float u, v;
+    env_coords(d, u, v);
+    return env_pdf_uv(s, e, u, v);
+}
+// Radiance and the emitter-sampling pdf (scene level, divided by the emitter
+// count as Scene::pdfEmitterDirection does) of direction d, from one
+// evaluation of the lat-long coordinates.
+__device__ __forceinline__ vec3 env_eval_pdf(const SceneView& s, const LightRec& e, vec3 d, float& pdf) {
+    float u, v;
+    env_coords(d, u, v);
+    const float p = env_pdf_uv(s, e, u, v);
+    pdf = s.num_lights == 1 ? p : p / (float)s.num_lights;
+    return env_lookup(s, e, u, v);
 }
 
 // DiscretePDF::sampleReuse on a mesh CDF (include/core/math/dpdf.h:95-136).

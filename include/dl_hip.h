@@ -203,6 +203,12 @@ dl_status dl_hip_malloc(void* ctx, size_t bytes, void** dptr);
 dl_status dl_hip_free(void* ctx, void* dptr);
 dl_status dl_hip_memcpy_h2d(void* ctx, void* dst, const void* src, size_t bytes);
 dl_status dl_hip_memcpy_d2h(void* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel on the context stream
+ * (asynchronous; both pointers 16-byte aligned).  Its known byte count also
+ * calibrates hardware traffic counters.  (No reference counterpart: the
+ * reference's CUDABuffer copies host<->device only,
+ * include/optix/host/cuda_buffer.h:46-57.) */
+dl_status dl_hip_memcpy_d2d(void* ctx, void* dst, const void* src, size_t bytes);
 dl_status dl_hip_sync(void* ctx);
 /* Kernel time of the last dl_hip_trace_* call (hipEvents on the context
  * stream, milliseconds). */
