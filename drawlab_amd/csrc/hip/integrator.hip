@@ -81,26 +81,41 @@ struct RenderState {
 // helpers
 // ---------------------------------------------------------------------------
 constexpr int kStageWaves = kStageBlock / 64;
+constexpr int kStageItems = 4;                          // items per thread per block-wide append
+
+// Items per thread per loop trip: up to kStageItems, but no more than spread
+// the queue over the whole grid (short queues keep every block busy).
+__device__ __forceinline__ uint32_t stage_items(uint32_t n) {
+    const uint32_t per_trip = gridDim.x * kStageBlock;
+    return min((uint32_t)kStageItems, max(1u, (n + per_trip - 1) / per_trip));
+}
 
 // Block-aggregated queue append.  Every thread of the block calls it (the
-// call sits in block-uniform control flow); bit q of `bits` asks for a slot in
-// queue q, whose fill counter is counters[q].  One atomic per block and
-// non-empty queue: the counters share one cache line that every block of the
-// grid updates, so per-wave atomics serialise there.  Slots of a queue are
-// dense, ordered by (wave, lane) inside the block; emit(q, slot) runs once per
-// set bit.
-template <int NQ, typename Emit>
-__device__ __forceinline__ void block_append(uint32_t* counters, uint32_t bits, Emit emit) {
+// call sits in block-uniform control flow) with N items; bit q of bits[u]
+// asks for a slot in queue q for item u, whose fill counter is counters[q].
+// One atomic per block and non-empty queue: the counters share one cache
+// line that every block of the grid updates, so finer-grained atomics
+// serialise there.  Slots of a queue are dense; emit(q, u, slot) runs once
+// per set bit.
+template <int NQ, int N, typename Emit>
+__device__ __forceinline__ void block_append(uint32_t* counters, const uint32_t (&bits)[N], Emit emit) {
     __shared__ uint32_t s_cnt[kStageWaves][NQ];
     __shared__ uint32_t s_base[kStageWaves][NQ];
     const int lane = __lane_id();
     const int wave = threadIdx.x >> 6;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    unsigned long long masks[NQ];
+    unsigned long long m[NQ][N];
+    uint32_t pre[NQ][N];
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
-        masks[q] = __ballot((bits >> q) & 1u);
-        if (lane == 0) s_cnt[wave][q] = (uint32_t)__popcll(masks[q]);
+        uint32_t tot = 0;
+#pragma unroll
+        for (int u = 0; u < N; u++) {
+            m[q][u] = __ballot((bits[u] >> q) & 1u);
+            pre[q][u] = tot;
+            tot += (uint32_t)__popcll(m[q][u]);
+        }
+        if (lane == 0) s_cnt[wave][q] = tot;
     }
     __syncthreads();
     if (threadIdx.x < NQ) {
@@ -117,7 +132,9 @@ __device__ __forceinline__ void block_append(uint32_t* counters, uint32_t bits, 
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < NQ; q++)
-        if ((bits >> q) & 1u) emit(q, s_base[wave][q] + (uint32_t)__popcll(masks[q] & lt));
+#pragma unroll
+        for (int u = 0; u < N; u++)
+            if ((bits[u] >> q) & 1u) emit(q, u, s_base[wave][q] + pre[q][u] + (uint32_t)__popcll(m[q][u] & lt));
     __syncthreads();
 }
 
@@ -267,40 +284,57 @@ __device__ __forceinline__ int material_of(const SceneView& s, uint32_t prim, in
 // First intersection (path.cpp:21-29) and a camera-ray miss (miss.cu:23-29).
 __global__ __launch_bounds__(kStageBlock) void k_post_camera(PassParams pp, SceneView s, RenderBufs rs, int next) {
     const uint32_t n = rs.cnt->ray;
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
-        const uint32_t i = i0 + threadIdx.x;
-        bool alive = false;
-        int type = 0;
-        uint32_t p = 0;
-        if (i < n) {
-            p = rs.rayq[i];
-            const float4 h = rs.hit[p];
-            const uint32_t prim = __float_as_uint(h.x);
-            const vec3 d = f3(rs.ray_d[p]);
-            vec3 rad = v3(0, 0, 0);
-            if (prim == 0xffffffffu) {
-                if (s.env_light >= 0) {
-                    const LightRec& e = s.lights[s.env_light];
-                    rad = e.env_visual ? env_eval(s, e, d) : v3(s.bg[0], s.bg[1], s.bg[2]);
+    __shared__ uint32_t s_p[kStageItems][kStageBlock];
+    __shared__ uint32_t s_b[kStageItems][kStageBlock];
+    const uint32_t items = stage_items(n);
+    for (uint32_t i0 = blockIdx.x * items * kStageBlock; i0 < n; i0 += gridDim.x * items * kStageBlock) {
+        // items are staged in LDS (each thread reads back only its own) so the
+        // per-item body is not unrolled
+#pragma unroll 1
+        for (int u = 0; u < kStageItems; u++) {
+            const uint32_t i = (uint32_t)u < items ? i0 + (uint32_t)u * kStageBlock + threadIdx.x : n;
+            bool alive = false;
+            int type = 0;
+            uint32_t p = 0;
+            if (i < n) {
+                p = rs.rayq[i];
+                const float4 h = rs.hit[p];
+                const uint32_t prim = __float_as_uint(h.x);
+                const vec3 d = f3(rs.ray_d[p]);
+                vec3 rad = v3(0, 0, 0);
+                if (prim == 0xffffffffu) {
+                    if (s.env_light >= 0) {
+                        const LightRec& e = s.lights[s.env_light];
+                        rad = e.env_visual ? env_eval(s, e, d) : v3(s.bg[0], s.bg[1], s.bg[2]);
+                    }
+                } else {
+                    const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
+                    if (mr.light >= 0) {
+                        vec3 hp, hn;
+                        hit_point_normal(s, prim, h.z, h.w, hp, hn);
+                        const vec3 le = dot(hn, neg(d)) > 0.f ? pcol(s.lights[mr.light].radiance) : v3(0, 0, 0);
+                        rad = add(rad, mul(v3(1.f, 1.f, 1.f), le));
+                    }
+                    alive = !(pp.max_depth >= 0 && 0 + 2 > pp.max_depth);
+                    int mat;
+                    type = material_of(s, prim, mat);
                 }
-            } else {
-                const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
-                if (mr.light >= 0) {
-                    vec3 hp, hn;
-                    hit_point_normal(s, prim, h.z, h.w, hp, hn);
-                    const vec3 le = dot(hn, neg(d)) > 0.f ? pcol(s.lights[mr.light].radiance) : v3(0, 0, 0);
-                    rad = add(rad, mul(v3(1.f, 1.f, 1.f), le));
-                }
-                alive = !(pp.max_depth >= 0 && 0 + 2 > pp.max_depth);
-                int mat;
-                type = material_of(s, prim, mat);
+                rs.rad[p] = f4(rad, 0.f);
             }
-            rs.rad[p] = f4(rad, 0.f);
+            s_p[u][threadIdx.x] = p;
+            s_b[u][threadIdx.x] = alive ? 1u << type : 0u;
+        }
+        uint32_t pv[kStageItems], bv[kStageItems];
+#pragma unroll
+        for (int u = 0; u < kStageItems; u++) {
+            pv[u] = s_p[u][threadIdx.x];
+            bv[u] = s_b[u][threadIdx.x];
         }
         uint32_t* const mq = rs.matq[next];
         const uint32_t cap = pp.cap;
-        block_append<DL_BSDF_COUNT>(rs.cnt->mat[next], alive ? 1u << type : 0u,
-                                    [&](int t, uint32_t slot) { mq[(size_t)t * cap + slot] = p; });
+        block_append<DL_BSDF_COUNT, kStageItems>(rs.cnt->mat[next], bv, [&](int t, int u, uint32_t slot) {
+            mq[(size_t)t * cap + slot] = pv[u];
+        });
     }
 }
 
@@ -314,100 +348,116 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
     const uint32_t* __restrict__ queue = rs.matq[cur] + (size_t)TYPE * pp.cap;
     const uint32_t WH = pp.W * pp.H;
     constexpr int type = TYPE;
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
-        const uint32_t i = i0 + threadIdx.x;
-        bool want_shadow = false, want_cont = false;
-        uint32_t p = 0;
-        if (i < n) {
-            p = queue[i];
-            const float4 h = rs.hit[p];
-            const uint32_t prim = __float_as_uint(h.x);
-            const vec3 d = f3(rs.ray_d[p]);
-            const float4 th4 = rs.thr[p];
-            vec3 thr = f3(th4);
-            float eta = th4.w;
-            vec3 rad = f3(rs.rad[p]);
-            int4 aux = rs.aux[p];
-            Pcg32 rng;
-            rng.state = rs.rng[p];
-            rng.inc = ((uint64_t)(pp.sample_base + p / WH) << 1u) | 1u;
-            Its its;
-            make_its(s, prim, h.z, h.w, type == DL_BSDF_ANISO_GGX, its);
-            const int mat = s.meshes[its.mesh].material;
-            const float* prm = s.mats[mat].params;
-            uint32_t flags = 0;
-            // ---- emitter sampling (Scene::sampleEmitterDirection, scene.cpp:139-170)
-            float lx = rng.next_float();
-            const float ly = rng.next_float();
-            DirSample ds;
-            ds.pdf = 0.0f;
-            ds.delta = false;
-            vec3 spectrum = v3(0, 0, 0);
-            const int L = s.num_lights;
-            if (L == 1) {
-                spectrum = emitter_sample(s, s.lights[0], lx, ly, its, ds, rng);
-            } else if (L > 1) {
-                const float epdf = 1.f / (float)L;
-                uint32_t idx = (uint32_t)(lx * (float)L);
-                idx = idx < (uint32_t)(L - 1) ? idx : (uint32_t)(L - 1);
-                lx = lx * (float)L - (float)idx;
-                spectrum = emitter_sample(s, s.lights[idx], lx, ly, its, ds, rng);
-                spectrum = scale(spectrum, (float)L);
-                ds.pdf *= epdf;
-            }
-            const vec3 wi = to_local(its.sh, neg(normalized(d)));
-            if (ds.pdf > 0) {
-                BRec q;
-                q.wi = wi;
-                q.wo = to_local(its.sh, ds.d);
-                q.eta = 1.f;
-                q.measure = kMeasureSolidAngle;
-                const vec3 bv = bsdf_eval(type, prm, q);
-                const float bp = bsdf_pdf(type, prm, q);
-                const float mis = ds.delta ? 1.0f : mis_weight(ds.pdf, bp);
-                const vec3 contrib = mul(mul(scale(thr, mis), bv), spectrum);
-                if (csum(spectrum) > 0) {
-                    rs.nee[p] = f4(contrib, 0.f);
-                    rs.sh_o[p] = f4(its.p, kEpsilon);
-                    rs.sh_d[p] = f4(ds.d, ds.dist - kEpsilon);
-                    flags |= kFlagNee;
-                    want_shadow = true;
-                } else {
-                    rad = add(rad, contrib);
+    __shared__ uint32_t s_p[kStageItems][kStageBlock];
+    __shared__ uint32_t s_b[kStageItems][kStageBlock];
+    const uint32_t items = stage_items(n);
+    for (uint32_t i0 = blockIdx.x * items * kStageBlock; i0 < n; i0 += gridDim.x * items * kStageBlock) {
+        // items are staged in LDS (each thread reads back only its own) so the
+        // per-item body is not unrolled
+#pragma unroll 1
+        for (int u = 0; u < kStageItems; u++) {
+            const uint32_t i = (uint32_t)u < items ? i0 + (uint32_t)u * kStageBlock + threadIdx.x : n;
+            bool want_shadow = false, want_cont = false;
+            uint32_t p = 0;
+            if (i < n) {
+                p = queue[i];
+                const float4 h = rs.hit[p];
+                const uint32_t prim = __float_as_uint(h.x);
+                const vec3 d = f3(rs.ray_d[p]);
+                const float4 th4 = rs.thr[p];
+                vec3 thr = f3(th4);
+                float eta = th4.w;
+                vec3 rad = f3(rs.rad[p]);
+                int4 aux = rs.aux[p];
+                Pcg32 rng;
+                rng.state = rs.rng[p];
+                rng.inc = ((uint64_t)(pp.sample_base + p / WH) << 1u) | 1u;
+                Its its;
+                make_its(s, prim, h.z, h.w, type == DL_BSDF_ANISO_GGX, its);
+                const int mat = s.meshes[its.mesh].material;
+                const float* prm = s.mats[mat].params;
+                uint32_t flags = 0;
+                // ---- emitter sampling (Scene::sampleEmitterDirection, scene.cpp:139-170)
+                float lx = rng.next_float();
+                const float ly = rng.next_float();
+                DirSample ds;
+                ds.pdf = 0.0f;
+                ds.delta = false;
+                vec3 spectrum = v3(0, 0, 0);
+                const int L = s.num_lights;
+                if (L == 1) {
+                    spectrum = emitter_sample(s, s.lights[0], lx, ly, its, ds, rng);
+                } else if (L > 1) {
+                    const float epdf = 1.f / (float)L;
+                    uint32_t idx = (uint32_t)(lx * (float)L);
+                    idx = idx < (uint32_t)(L - 1) ? idx : (uint32_t)(L - 1);
+                    lx = lx * (float)L - (float)idx;
+                    spectrum = emitter_sample(s, s.lights[idx], lx, ly, its, ds, rng);
+                    spectrum = scale(spectrum, (float)L);
+                    ds.pdf *= epdf;
                 }
+                const vec3 wi = to_local(its.sh, neg(normalized(d)));
+                if (ds.pdf > 0) {
+                    BRec q;
+                    q.wi = wi;
+                    q.wo = to_local(its.sh, ds.d);
+                    q.eta = 1.f;
+                    q.measure = kMeasureSolidAngle;
+                    const vec3 bv = bsdf_eval(type, prm, q);
+                    const float bp = bsdf_pdf(type, prm, q);
+                    const float mis = ds.delta ? 1.0f : mis_weight(ds.pdf, bp);
+                    const vec3 contrib = mul(mul(scale(thr, mis), bv), spectrum);
+                    if (csum(spectrum) > 0) {
+                        rs.nee[p] = f4(contrib, 0.f);
+                        rs.sh_o[p] = f4(its.p, kEpsilon);
+                        rs.sh_d[p] = f4(ds.d, ds.dist - kEpsilon);
+                        flags |= kFlagNee;
+                        want_shadow = true;
+                    } else {
+                        rad = add(rad, contrib);
+                    }
+                }
+                // ---- BSDF sampling
+                BRec br;
+                br.wi = wi;
+                br.wo = v3(0, 0, 0);
+                br.eta = 1.f;
+                br.measure = kMeasureUnknown;
+                const float bx = rng.next_float();
+                const float by = rng.next_float();
+                const vec3 fr = bsdf_sample(type, prm, br, bx, by);
+                thr = mul(thr, fr);
+                eta *= br.eta;
+                const float prev_pdf = bsdf_pdf(type, prm, br);
+                if (bsdf_is_diffuse(type)) flags |= kFlagDiffuse;
+                if (!is_zero(thr)) {
+                    rs.ray_o[p] = f4(its.p, kEpsilon);
+                    rs.ray_d[p] = f4(to_world(its.sh, br.wo), INFINITY);
+                    flags |= kFlagCont;
+                    want_cont = true;
+                }
+                rs.thr[p] = f4(thr, eta);
+                rs.rad[p] = f4(rad, 0.f);
+                rs.rng[p] = rng.state;
+                aux.x = __float_as_int(prev_pdf);
+                aux.y = (int)flags;
+                rs.aux[p] = aux;
             }
-            // ---- BSDF sampling
-            BRec br;
-            br.wi = wi;
-            br.wo = v3(0, 0, 0);
-            br.eta = 1.f;
-            br.measure = kMeasureUnknown;
-            const float bx = rng.next_float();
-            const float by = rng.next_float();
-            const vec3 fr = bsdf_sample(type, prm, br, bx, by);
-            thr = mul(thr, fr);
-            eta *= br.eta;
-            const float prev_pdf = bsdf_pdf(type, prm, br);
-            if (bsdf_is_diffuse(type)) flags |= kFlagDiffuse;
-            if (!is_zero(thr)) {
-                rs.ray_o[p] = f4(its.p, kEpsilon);
-                rs.ray_d[p] = f4(to_world(its.sh, br.wo), INFINITY);
-                flags |= kFlagCont;
-                want_cont = true;
-            }
-            rs.thr[p] = f4(thr, eta);
-            rs.rad[p] = f4(rad, 0.f);
-            rs.rng[p] = rng.state;
-            aux.x = __float_as_int(prev_pdf);
-            aux.y = (int)flags;
-            rs.aux[p] = aux;
+            s_p[u][threadIdx.x] = p;
+            s_b[u][threadIdx.x] = (want_cont ? 1u : 0u) | (want_shadow ? 2u : 0u);
+        }
+        uint32_t pv[kStageItems], bv[kStageItems];
+#pragma unroll
+        for (int u = 0; u < kStageItems; u++) {
+            pv[u] = s_p[u][threadIdx.x];
+            bv[u] = s_b[u][threadIdx.x];
         }
         // counters: ray (continuations), then shadow -- adjacent fields
-        block_append<2>(&rs.cnt->ray, (want_cont ? 1u : 0u) | (want_shadow ? 2u : 0u), [&](int q, uint32_t slot) {
+        block_append<2, kStageItems>(&rs.cnt->ray, bv, [&](int q, int u, uint32_t slot) {
             if (q == 0)
-                rs.rayq[slot] = p;
+                rs.rayq[slot] = pv[u];
             else
-                rs.shq[slot] = p;
+                rs.shq[slot] = pv[u];
         });
     }
 }
@@ -419,78 +469,95 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
     const uint32_t n = queue_total(counts);
     const int next = cur ^ 1;
     const uint32_t WH = pp.W * pp.H;
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
-        const uint32_t i = i0 + threadIdx.x;
-        bool alive = false;
-        int ntype = 0;
-        uint32_t p = 0;
-        if (i < n) {
-            int type;
-            queue_entry(counts, rs.matq[cur], pp.cap, i, p, type);
-            int4 aux = rs.aux[p];
-            const uint32_t flags = (uint32_t)aux.y;
-            vec3 rad = f3(rs.rad[p]);
-            if ((flags & kFlagNee) && rs.occ[p] == 0) rad = add(rad, f3(rs.nee[p]));
-            if (flags & kFlagCont) {
-                const float4 th4 = rs.thr[p];
-                vec3 thr = f3(th4);
-                const float eta = th4.w;
-                const float prev_pdf = __int_as_float(aux.x);
-                const bool diffuse = (flags & kFlagDiffuse) != 0;
-                const float4 h = rs.hit[p];
-                const uint32_t prim = __float_as_uint(h.x);
-                const vec3 o = f3(rs.ray_o[p]);
-                const vec3 d = f3(rs.ray_d[p]);
-                if (prim == 0xffffffffu) {
-                    if (s.env_light >= 0) {
-                        const LightRec& e = s.lights[s.env_light];
-                        float epdf;
-                        const vec3 ev = env_eval_pdf(s, e, d, epdf);
-                        const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
-                        rad = add(rad, mul(scale(thr, mis), ev));
-                    }
-                } else {
-                    const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
-                    if (mr.light >= 0) {
-                        vec3 hp, hn;
-                        hit_point_normal(s, prim, h.z, h.w, hp, hn);
-                        const vec3 ev = dot(hn, neg(d)) > 0.f ? pcol(s.lights[mr.light].radiance) : v3(0, 0, 0);
-                        const DirSample hs = dir_sample(o, hp, hn);
-                        const float epdf = pdf_emitter_direction(s, mr.light, hs);
-                        const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
-                        rad = add(rad, mul(scale(thr, mis), ev));
-                    }
-                    alive = true;
-                    const int depth = aux.z;
-                    if (depth >= pp.rr_depth) {
-                        Pcg32 rng;
-                        rng.state = rs.rng[p];
-                        rng.inc = ((uint64_t)(pp.sample_base + p / WH) << 1u) | 1u;
-                        const float pr = fmin0(cmax(scale(scale(thr, eta), eta)), 0.99f);
-                        if (rng.next_float() > pr) {
-                            alive = false;
-                        } else {
-                            thr = divs(thr, pr);
-                            rs.thr[p] = f4(thr, eta);
+    __shared__ uint32_t s_p[kStageItems][kStageBlock];
+    __shared__ uint32_t s_b[kStageItems][kStageBlock];
+    const uint32_t items = stage_items(n);
+    for (uint32_t i0 = blockIdx.x * items * kStageBlock; i0 < n; i0 += gridDim.x * items * kStageBlock) {
+        // items are staged in LDS (each thread reads back only its own) so the
+        // per-item body is not unrolled
+#pragma unroll 1
+        for (int u = 0; u < kStageItems; u++) {
+            const uint32_t i = (uint32_t)u < items ? i0 + (uint32_t)u * kStageBlock + threadIdx.x : n;
+            bool alive = false;
+            int ntype = 0;
+            uint32_t p = 0;
+            if (i < n) {
+                int type;
+                queue_entry(counts, rs.matq[cur], pp.cap, i, p, type);
+                int4 aux = rs.aux[p];
+                const uint32_t flags = (uint32_t)aux.y;
+                vec3 rad = f3(rs.rad[p]);
+                if ((flags & kFlagNee) && rs.occ[p] == 0) rad = add(rad, f3(rs.nee[p]));
+                if (flags & kFlagCont) {
+                    const float4 th4 = rs.thr[p];
+                    vec3 thr = f3(th4);
+                    const float eta = th4.w;
+                    const float prev_pdf = __int_as_float(aux.x);
+                    const bool diffuse = (flags & kFlagDiffuse) != 0;
+                    const float4 h = rs.hit[p];
+                    const uint32_t prim = __float_as_uint(h.x);
+                    const vec3 o = f3(rs.ray_o[p]);
+                    const vec3 d = f3(rs.ray_d[p]);
+                    if (prim == 0xffffffffu) {
+                        if (s.env_light >= 0) {
+                            const LightRec& e = s.lights[s.env_light];
+                            float epdf;
+                            const vec3 ev = env_eval_pdf(s, e, d, epdf);
+                            const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
+                            rad = add(rad, mul(scale(thr, mis), ev));
                         }
-                        rs.rng[p] = rng.state;
-                    }
-                    if (alive) {
-                        const int nd = depth + 1;
-                        if (pp.max_depth >= 0 && nd + 2 > pp.max_depth) alive = false;
-                        aux.z = nd;
-                        rs.aux[p] = aux;
-                        int mat;
-                        ntype = material_of(s, prim, mat);
+                    } else {
+                        const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
+                        if (mr.light >= 0) {
+                            vec3 hp, hn;
+                            hit_point_normal(s, prim, h.z, h.w, hp, hn);
+                            const vec3 ev = dot(hn, neg(d)) > 0.f ? pcol(s.lights[mr.light].radiance) : v3(0, 0, 0);
+                            const DirSample hs = dir_sample(o, hp, hn);
+                            const float epdf = pdf_emitter_direction(s, mr.light, hs);
+                            const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
+                            rad = add(rad, mul(scale(thr, mis), ev));
+                        }
+                        alive = true;
+                        const int depth = aux.z;
+                        if (depth >= pp.rr_depth) {
+                            Pcg32 rng;
+                            rng.state = rs.rng[p];
+                            rng.inc = ((uint64_t)(pp.sample_base + p / WH) << 1u) | 1u;
+                            const float pr = fmin0(cmax(scale(scale(thr, eta), eta)), 0.99f);
+                            if (rng.next_float() > pr) {
+                                alive = false;
+                            } else {
+                                thr = divs(thr, pr);
+                                rs.thr[p] = f4(thr, eta);
+                            }
+                            rs.rng[p] = rng.state;
+                        }
+                        if (alive) {
+                            const int nd = depth + 1;
+                            if (pp.max_depth >= 0 && nd + 2 > pp.max_depth) alive = false;
+                            aux.z = nd;
+                            rs.aux[p] = aux;
+                            int mat;
+                            ntype = material_of(s, prim, mat);
+                        }
                     }
                 }
+                rs.rad[p] = f4(rad, 0.f);
             }
-            rs.rad[p] = f4(rad, 0.f);
+            s_p[u][threadIdx.x] = p;
+            s_b[u][threadIdx.x] = alive ? 1u << ntype : 0u;
+        }
+        uint32_t pv[kStageItems], bv[kStageItems];
+#pragma unroll
+        for (int u = 0; u < kStageItems; u++) {
+            pv[u] = s_p[u][threadIdx.x];
+            bv[u] = s_b[u][threadIdx.x];
         }
         uint32_t* const mq = rs.matq[next];
         const uint32_t cap = pp.cap;
-        block_append<DL_BSDF_COUNT>(rs.cnt->mat[next], alive ? 1u << ntype : 0u,
-                                    [&](int t, uint32_t slot) { mq[(size_t)t * cap + slot] = p; });
+        block_append<DL_BSDF_COUNT, kStageItems>(rs.cnt->mat[next], bv, [&](int t, int u, uint32_t slot) {
+            mq[(size_t)t * cap + slot] = pv[u];
+        });
     }
 }
 
@@ -501,50 +568,100 @@ __device__ __forceinline__ bool sample_valid(const float4& c) {  // Color3f::isV
 // Deterministic gather of one pass's samples into the film: pixel q receives
 // every sample whose filter footprint, computed exactly as ImageBlock::put
 // computes it inside the sample's 32x32 block, covers q.
-__global__ __launch_bounds__(kStageBlock) void k_film(PassParams pp, uint32_t k, dl_film_desc film, RenderBufs rs) {
-    const uint32_t W = pp.W, H = pp.H, WH = W * H;
+
+// Per-sample contribution of sample (px, py) of one layer to pixel (qx, qy):
+// the block-local footprint test and weights of ImageBlock::put.
+__device__ __forceinline__ void film_add(const dl_film_desc& film, uint32_t W, uint32_t H, int qx, int qy, int px,
+                                         int py, const float4& L, const float2& P, float4& acc) {
     const int b = film.border;
     const float r = film.radius;
-    const int R = (int)ceilf(r) + 1;
+    const int oy = (py / 32) * 32, sy = min(32, (int)H - oy);
+    const int ox = (px / 32) * 32, sx = min(32, (int)W - ox);
+    const int yl = qy - (oy - b), xl = qx - (ox - b);
+    if (yl < 0 || yl > sy + 2 * b - 1 || xl < 0 || xl > sx + 2 * b - 1) return;
+    const float plx = P.x - 0.5f - (float)(ox - b);
+    const float ply = P.y - 0.5f - (float)(oy - b);
+    const int x0 = max((int)ceilf(plx - r), 0), x1 = min((int)floorf(plx + r), sx + 2 * b - 1);
+    const int y0 = max((int)ceilf(ply - r), 0), y1 = min((int)floorf(ply + r), sy + 2 * b - 1);
+    if (xl < x0 || xl > x1 || yl < y0 || yl > y1) return;
+    const float wx = film.table[(int)(fabsf((float)xl - plx) * film.lookup)];
+    const float wy = film.table[(int)(fabsf((float)yl - ply) * film.lookup)];
+    acc.x += (L.x * wx) * wy;
+    acc.y += (L.y * wx) * wy;
+    acc.z += (L.z * wx) * wy;
+    acc.w += (1.0f * wx) * wy;
+}
+
+// Generic gather (any filter radius): sample layers outermost, then rows and
+// columns of the neighbourhood -- the order of the tiled kernel below.
+__global__ __launch_bounds__(kStageBlock) void k_film(PassParams pp, uint32_t k, dl_film_desc film, RenderBufs rs) {
+    const uint32_t W = pp.W, H = pp.H, WH = W * H;
+    const int R = (int)ceilf(film.radius) + 1;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < WH; q += gridDim.x * blockDim.x) {
         const int qx = (int)(q % W), qy = (int)(q / W);
         float4 acc = rs.film[q];
         uint32_t invalid = 0;
-        for (int py = max(0, qy - R); py <= min((int)H - 1, qy + R); ++py) {
-            const int oy = (py / 32) * 32;
-            const int sy = min(32, (int)H - oy);
-            const int yl = qy - (oy - b);
-            if (yl < 0 || yl > sy + 2 * b - 1) continue;
-            for (int px = max(0, qx - R); px <= min((int)W - 1, qx + R); ++px) {
-                const int ox = (px / 32) * 32;
-                const int sx = min(32, (int)W - ox);
-                const int xl = qx - (ox - b);
-                if (xl < 0 || xl > sx + 2 * b - 1) continue;
-                for (uint32_t j = 0; j < k; ++j) {
+        for (uint32_t j = 0; j < k; ++j)
+            for (int py = max(0, qy - R); py <= min((int)H - 1, qy + R); ++py)
+                for (int px = max(0, qx - R); px <= min((int)W - 1, qx + R); ++px) {
                     const uint32_t sp = j * WH + (uint32_t)py * W + (uint32_t)px;
                     const float4 L = rs.rad[sp];
                     if (!sample_valid(L)) {
                         if (px == qx && py == qy) invalid++;
                         continue;
                     }
-                    const float2 P = rs.pos[sp];
-                    const float plx = P.x - 0.5f - (float)(ox - b);
-                    const float ply = P.y - 0.5f - (float)(oy - b);
-                    const int x0 = max((int)ceilf(plx - r), 0), x1 = min((int)floorf(plx + r), sx + 2 * b - 1);
-                    const int y0 = max((int)ceilf(ply - r), 0), y1 = min((int)floorf(ply + r), sy + 2 * b - 1);
-                    if (xl < x0 || xl > x1 || yl < y0 || yl > y1) continue;
-                    const float wx = film.table[(int)(fabsf((float)xl - plx) * film.lookup)];
-                    const float wy = film.table[(int)(fabsf((float)yl - ply) * film.lookup)];
-                    acc.x += (L.x * wx) * wy;
-                    acc.y += (L.y * wx) * wy;
-                    acc.z += (L.z * wx) * wy;
-                    acc.w += (1.0f * wx) * wy;
+                    film_add(film, W, H, qx, qy, px, py, L, rs.pos[sp], acc);
                 }
-            }
-        }
         rs.film[q] = acc;
         if (invalid) atomicAdd(&rs.cnt->invalid, (unsigned long long)invalid);
     }
+}
+
+// Tiled gather for filter radii up to kFilmHalo - 1: one 16x16 pixel tile per
+// block; each sample layer's tile plus halo is staged in LDS once instead of
+// being fetched by each of the (2R+1)^2 pixels it can reach.
+constexpr int kFilmTile = 16;
+constexpr int kFilmHalo = 8;
+constexpr int kFilmSpan = kFilmTile + 2 * kFilmHalo;
+__global__ __launch_bounds__(kFilmTile* kFilmTile) void k_film_tiled(PassParams pp, uint32_t k, dl_film_desc film,
+                                                                     RenderBufs rs) {
+    __shared__ float4 s_rad[kFilmSpan * kFilmSpan];
+    __shared__ float2 s_pos[kFilmSpan * kFilmSpan];
+    const uint32_t W = pp.W, H = pp.H, WH = W * H;
+    const int R = (int)ceilf(film.radius) + 1;
+    const int S = kFilmTile + 2 * R;  // staged span
+    const int tx0 = (int)blockIdx.x * kFilmTile, ty0 = (int)blockIdx.y * kFilmTile;
+    const int qx = tx0 + (int)(threadIdx.x % kFilmTile), qy = ty0 + (int)(threadIdx.x / kFilmTile);
+    const bool inside = qx < (int)W && qy < (int)H;
+    const uint32_t q = inside ? (uint32_t)qy * W + (uint32_t)qx : 0u;
+    float4 acc = inside ? rs.film[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t invalid = 0;
+    for (uint32_t j = 0; j < k; ++j) {
+        for (int e = (int)threadIdx.x; e < S * S; e += (int)blockDim.x) {
+            const int gx = tx0 - R + e % S, gy = ty0 - R + e / S;
+            if (gx >= 0 && gx < (int)W && gy >= 0 && gy < (int)H) {
+                const uint32_t sp = j * WH + (uint32_t)gy * W + (uint32_t)gx;
+                s_rad[e] = rs.rad[sp];
+                s_pos[e] = rs.pos[sp];
+            }
+        }
+        __syncthreads();
+        if (inside) {
+            for (int py = max(0, qy - R); py <= min((int)H - 1, qy + R); ++py)
+                for (int px = max(0, qx - R); px <= min((int)W - 1, qx + R); ++px) {
+                    const int e = (py - (ty0 - R)) * S + (px - (tx0 - R));
+                    const float4 L = s_rad[e];
+                    if (!sample_valid(L)) {
+                        if (px == qx && py == qy) invalid++;
+                        continue;
+                    }
+                    film_add(film, W, H, qx, qy, px, py, L, s_pos[e], acc);
+                }
+        }
+        __syncthreads();
+    }
+    if (inside) rs.film[q] = acc;
+    if (invalid) atomicAdd(&rs.cnt->invalid, (unsigned long long)invalid);
 }
 
 // ImageBlock::toBitmap / Color4f::divideByFilterWeight  block.cpp:58-66, color.h:120-125
@@ -771,7 +888,12 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
         }
         if (paths_sample >= 0) break;
         DL_TRY(tm.start(kTagFilm));
-        k_film<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, kk, c->film, rs->b);
+        if ((int)ceilf(c->film.radius) + 1 <= kFilmHalo) {
+            const dim3 fgrid((W + kFilmTile - 1) / kFilmTile, (H + kFilmTile - 1) / kFilmTile);
+            k_film_tiled<<<fgrid, kFilmTile * kFilmTile, 0, c->stream>>>(pp, kk, c->film, rs->b);
+        } else {
+            k_film<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, kk, c->film, rs->b);
+        }
         DL_TRY(tm.stop());
         done += kk;
     }

@@ -75,23 +75,27 @@ __device__ __forceinline__ void slab2(const float4& r0, const float4& r1, const 
 }
 
 // Moller-Trumbore on a pre-gathered triangle; returns false on a miss.
+// Branch-free: every quantity is computed and the reference's early-out
+// tests (mesh.cpp:195, 204, 213) are combined into the returned predicate,
+// which accepts exactly the rays the early-out sequence accepts.  The triangle
+// is therefore fetched by three loads issued together.
 __device__ __forceinline__ bool mt_test(const float4& a, const float4& b, const float4& c, const vec3& o,
                                         const vec3& d, float& u, float& v, float& t) {
-    vec3 p0 = v3(a.x, a.y, a.z);
-    vec3 e1 = v3(a.w, b.x, b.y);
-    vec3 e2 = v3(b.z, b.w, c.x);
-    vec3 pvec = cross(d, e2);
-    float det = dot(e1, pvec);
-    if (det > -1e-8f && det < 1e-8f) return false;
-    float inv = 1.0f / det;
-    vec3 tvec = sub(o, p0);
+    const vec3 p0 = v3(a.x, a.y, a.z);
+    const vec3 e1 = v3(a.w, b.x, b.y);
+    const vec3 e2 = v3(b.z, b.w, c.x);
+    const vec3 pvec = cross(d, e2);
+    const float det = dot(e1, pvec);
+    const float inv = 1.0f / det;
+    const vec3 tvec = sub(o, p0);
     u = dot(tvec, pvec) * inv;
-    if (u < 0.0f || u > 1.0f) return false;
-    vec3 qvec = cross(tvec, e1);
+    const vec3 qvec = cross(tvec, e1);
     v = dot(d, qvec) * inv;
-    if (v < 0.0f || u + v > 1.0f) return false;
     t = dot(e2, qvec) * inv;
-    return true;
+    const bool parallel = det > -1e-8f && det < 1e-8f;
+    const bool out_u = u < 0.0f || u > 1.0f;
+    const bool out_v = v < 0.0f || u + v > 1.0f;
+    return !(parallel | out_u | out_v);
 }
 
 // Wave-aggregated claim of the next work item: one atomic per wave.
@@ -172,30 +176,30 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
         float n0, n1;
         slab2(r0, r1, r2, o, rcp, tmin, best_t, h0, n0, h1, n1);
         bool hit_any = false;
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            const bool h = side ? h1 : h0;
-            const int ref = side ? r3.y : r3.x;
-            if (h && ref < 0 && !hit_any) {
-                const int first = ~ref, count = side ? r3.w : r3.z;
-                for (int k = first; k < first + count; ++k) {
-                    const float4* tp = bvh.tris + 3 * k;
-                    const float4 ta = tp[0], tb = tp[1], tc = tp[2];
-                    if (COUNT) cnt.tris++;
-                    float u, v, t;
-                    if (mt_test(ta, tb, tc, o, d, u, v, t) && t >= tmin && t <= best_t) {
-                        if (ANY) {
-                            hit_any = true;
-                            break;
-                        }
-                        const uint32_t id = __float_as_uint(tc.y);
-                        if (t < best_t || id < best) {
-                            best_t = t;
-                            best = id;
-                            bu = u;
-                            bv = v;
-                        }
-                    }
+        // Triangles of the hit leaf children, both sides in one loop (the
+        // closest-hit result is the minimum over (t, id) and so does not
+        // depend on the order).
+        const bool l0 = h0 && r3.x < 0, l1 = h1 && r3.y < 0;
+        const int c0 = l0 ? r3.z : 0;
+        const int ntri = c0 + (l1 ? r3.w : 0);
+        const int f0 = ~r3.x, f1 = ~r3.y;
+        for (int j = 0; j < ntri; ++j) {
+            const int k = j < c0 ? f0 + j : f1 + (j - c0);
+            const float4* tp = bvh.tris + 3 * k;
+            const float4 ta = tp[0], tb = tp[1], tc = tp[2];
+            if (COUNT) cnt.tris++;
+            float u, v, t;
+            if (mt_test(ta, tb, tc, o, d, u, v, t) && t >= tmin && t <= best_t) {
+                if (ANY) {
+                    hit_any = true;
+                    break;
+                }
+                const uint32_t id = __float_as_uint(tc.y);
+                if (t < best_t || id < best) {
+                    best_t = t;
+                    best = id;
+                    bu = u;
+                    bv = v;
                 }
             }
         }
