@@ -26,7 +26,7 @@ EXPORTED_SYMBOLS = (
     "dl_hip_create", "dl_hip_destroy", "dl_hip_last_error", "dl_hip_device_name", "dl_hip_upload_scene",
     "dl_hip_build_accel", "dl_hip_trace_closest", "dl_hip_trace_shadow", "dl_hip_trace_closest_counted",
     "dl_hip_render", "dl_hip_render_paths", "dl_hip_malloc", "dl_hip_free", "dl_hip_memcpy_h2d",
-    "dl_hip_memcpy_d2h", "dl_hip_memcpy_d2d", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_film_filter",
+    "dl_hip_memcpy_d2h", "dl_hip_memcpy_d2d", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_camera_perspective", "dl_film_filter",
     "dl_mesh_tangents",
 )
 
@@ -54,7 +54,9 @@ class dl_light_desc(ctypes.Structure):
 class dl_camera_desc(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
                 ("eye", ctypes.c_float * 3), ("U", ctypes.c_float * 3), ("V", ctypes.c_float * 3),
-                ("W", ctypes.c_float * 3)]
+                ("W", ctypes.c_float * 3), ("sample_to_camera", ctypes.c_float * 16),
+                ("camera_to_world", ctypes.c_float * 16), ("inv_size", ctypes.c_float * 2),
+                ("near_clip", ctypes.c_float), ("far_clip", ctypes.c_float)]
 
 
 class dl_film_desc(ctypes.Structure):
@@ -121,6 +123,7 @@ def lib():
             "dl_hip_sync": (i32, [P]),
             "dl_hip_last_kernel_ms": (i32, [P, c.POINTER(c.c_double)]),
             "dl_camera_virtual": (None, [P, P, P, c.c_float, i32, i32, c.POINTER(dl_camera_desc)]),
+            "dl_camera_perspective": (i32, [P, c.c_float, c.c_float, c.c_float, i32, i32, c.POINTER(dl_camera_desc)]),
             "dl_film_filter": (None, [i32, c.c_float, c.c_float, c.c_float, c.POINTER(dl_film_desc)]),
             "dl_mesh_tangents": (None, [P, u32, P, u32, P, P]),
         }
@@ -138,6 +141,14 @@ def _p(a):
 
 def camera_desc(cam: Camera) -> dl_camera_desc:
     out = dl_camera_desc()
+    if cam.type == "perspective":
+        tw = np.ascontiguousarray(np.eye(4, dtype=np.float32) if cam.to_world is None else cam.to_world,
+                                  dtype=np.float32).reshape(16)
+        st = lib().dl_camera_perspective(_p(tw), float(cam.fov), float(cam.near), float(cam.far), int(cam.width),
+                                         int(cam.height), ctypes.byref(out))
+        if st != 0:
+            raise ValueError("perspective camera: projection is not invertible")
+        return out
     e = np.asarray(cam.eye, np.float32)
     la = np.asarray(cam.lookat, np.float32)
     up = np.asarray(cam.up, np.float32)
@@ -276,8 +287,12 @@ class Backend:
         sd.num_materials = len(desc.bsdfs)
         sd.lights = lights
         sd.num_lights = len(order)
-        sd.camera = camera_desc(desc.camera)
-        sd.film = film_desc(desc.rfilter)
+        # exported descriptors (scenes loaded through the plugin layer) are
+        # used as given
+        cd = getattr(desc, "camera_desc", None)
+        sd.camera = cd if cd is not None else camera_desc(desc.camera)
+        fd = getattr(desc, "film", None)
+        sd.film = fd if fd is not None else film_desc(desc.rfilter)
         sd.bg_color[:] = desc.bg_color
         self._check(self.L.dl_hip_upload_scene(self.h, ctypes.byref(sd)))
         self.desc = desc

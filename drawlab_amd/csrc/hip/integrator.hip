@@ -200,6 +200,18 @@ __global__ __launch_bounds__(kStageBlock) void k_clear_film(float4* film, uint32
 }
 
 // Camera sample: pixel 2D, aperture 2D, VirtualCamera::sampleRay.
+// out = M in for a row-major 4x4 M, each row accumulated from zero in index
+// order (the reference's row(j).dot(v), include/core/math/matrix.h:255-262).
+__device__ __forceinline__ void mat4_apply(const float* m, const float in[4], float out[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; k++) s += m[4 * j + k] * in[k];
+        out[j] = s;
+    }
+}
+
 __global__ __launch_bounds__(kStageBlock) void k_raygen(PassParams pp, dl_camera_desc cam, RenderBufs rs) {
     const uint32_t WH = pp.W * pp.H;
     const float invx = 1.f / (float)cam.width, invy = 1.f / (float)cam.height;
@@ -211,13 +223,29 @@ __global__ __launch_bounds__(kStageBlock) void k_raygen(PassParams pp, dl_camera
         const float ax = rng.next_float();
         const float ay = rng.next_float();
         const float px = (float)x + ax, py = (float)y + ay;
-        rng.next_float();  // aperture sample (unused by the virtual camera)
+        rng.next_float();  // aperture sample (unused by both cameras)
         rng.next_float();
-        const float lx = px, ly = ((float)cam.height - py) - 1;
-        const float dx = lx * invx * 2 - 1, dy = ly * invy * 2 - 1;
-        const vec3 d = normalized(add(add(scale(pcol(cam.U), dx), scale(pcol(cam.V), dy)), pcol(cam.W)));
-        rs.ray_o[p] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], kEpsilon);
-        rs.ray_d[p] = f4(d, INFINITY);
+        if (cam.type == DL_CAMERA_PERSPECTIVE) {
+            // PerspectiveCamera::sampleRay  perspective.cpp:75-95
+            const float in[4] = {px * cam.inv_size[0], py * cam.inv_size[1], 0.0f, 1.0f};
+            float q[4];
+            mat4_apply(cam.sample_to_camera, in, q);
+            const vec3 d = normalized(v3(q[0] / q[3], q[1] / q[3], q[2] / q[3]));
+            const float inv_z = 1.0f / d.z;
+            const float o_in[4] = {0.0f, 0.0f, 0.0f, 1.0f}, d_in[4] = {d.x, d.y, d.z, 0.0f};
+            float o4[4], d4[4];
+            mat4_apply(cam.camera_to_world, o_in, o4);
+            mat4_apply(cam.camera_to_world, d_in, d4);
+            rs.ray_o[p] = make_float4(o4[0] / o4[3], o4[1] / o4[3], o4[2] / o4[3], cam.near_clip * inv_z);
+            rs.ray_d[p] = make_float4(d4[0], d4[1], d4[2], cam.far_clip * inv_z);
+        } else {
+            // VirtualCamera::sampleRay  virtual.cpp:52-65
+            const float lx = px, ly = ((float)cam.height - py) - 1;
+            const float dx = lx * invx * 2 - 1, dy = ly * invy * 2 - 1;
+            const vec3 d = normalized(add(add(scale(pcol(cam.U), dx), scale(pcol(cam.V), dy)), pcol(cam.W)));
+            rs.ray_o[p] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], kEpsilon);
+            rs.ray_d[p] = f4(d, INFINITY);
+        }
         rs.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
         rs.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
         rs.aux[p] = make_int4(0, 0, 0, 0);

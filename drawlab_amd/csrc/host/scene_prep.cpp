@@ -17,6 +17,7 @@
 #include <cstring>
 
 #include "../../../include/dl_hip.h"
+#include "mat4.h"
 
 namespace {
 
@@ -139,6 +140,38 @@ void dl_camera_virtual(const float eye[3], const float lookat[3], const float up
         dst[i][1] = src[i].y;
         dst[i][2] = src[i].z;
     }
+}
+
+// PerspectiveCamera (perspective.cpp:21-70): perspective projection onto
+// z = 1 with the cotangent of half the field of view, shifted and scaled into
+// [0, 1]^2 sample space (aspect applied to y), inverted once here.
+dl_status dl_camera_perspective(const float to_world[16], float fov, float near_clip, float far_clip, int width,
+                                int height, dl_camera_desc* out) {
+    std::memset(out, 0, sizeof(*out));
+    if (width <= 0 || height <= 0 || !(far_clip > near_clip)) return DL_ERR_INVALID;
+    out->type = DL_CAMERA_PERSPECTIVE;
+    out->width = width;
+    out->height = height;
+    out->inv_size[0] = 1.f / width;
+    out->inv_size[1] = 1.f / height;
+    out->near_clip = near_clip;
+    out->far_clip = far_clip;
+    const float aspect = width / (float)height;
+    const float recip = 1.0f / (far_clip - near_clip);
+    const float cot = 1.0f / std::tan(fov / 2.0f * (float)(M_PI / 180.0));
+    float persp[4][4] = {{cot, 0, 0, 0}, {0, cot, 0, 0}, {0, 0, far_clip * recip, -near_clip * far_clip * recip}, {0, 0, 1, 0}};
+    float scale[4][4] = {{-0.5f, 0, 0, 0}, {0, -0.5f * aspect, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    float trans[4][4] = {{1, 0, 0, -1.0f}, {0, 1, 0, -1.0f / aspect}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    float st[4][4], stp[4][4], inv[4][4];
+    dlm::mat4_mul(scale, trans, st);
+    dlm::mat4_mul(st, persp, stp);
+    if (!dlm::mat4_inverse(stp, inv)) return DL_ERR_INVALID;
+    for (int j = 0; j < 4; j++)
+        for (int i = 0; i < 4; i++) {
+            out->sample_to_camera[4 * j + i] = inv[j][i];
+            out->camera_to_world[4 * j + i] = to_world ? to_world[4 * j + i] : (i == j ? 1.f : 0.f);
+        }
+    return DL_OK;
 }
 
 void dl_film_filter(int type, float radius, float p0, float p1, dl_film_desc* out) {

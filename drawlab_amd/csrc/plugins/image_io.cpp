@@ -1,0 +1,335 @@
+// Image files of the scene description and the renderer's output, written
+// from the published formats:
+//   Radiance RGBE (.hdr): header, "-Y H +X W" resolution line, scanlines flat
+//     or adaptive run-length encoded; value = mantissa * 2^(e - 136).
+//   PFM: "PF\n W H\n scale\n", little-endian when scale < 0, rows bottom-up.
+//   OpenEXR: scanline images with NO_COMPRESSION and HALF or FLOAT R, G, B
+//     channels (read); written as FLOAT B, G, R, one scanline per chunk.
+// The reference reads images through stb_image / OpenEXR
+// (src/core/bitmap/bitmap.cpp:33-214), neither of which is vendored.
+#include <cmath>
+#include <cstring>
+#include <fstream>
+
+#include "drawlab/plugins.h"
+
+namespace drawlab {
+namespace {
+
+std::vector<unsigned char> read_file(const std::string& fn) {
+    std::ifstream in(fn, std::ios::binary);
+    if (!in) throw Exception("Could not open image \"%s\"", fn);
+    return std::vector<unsigned char>((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+}
+
+Image load_hdr(const std::string& fn) {
+    const std::vector<unsigned char> b = read_file(fn);
+    size_t pos = 0;
+    auto line = [&]() {
+        std::string s;
+        while (pos < b.size() && b[pos] != '\n') s += (char)b[pos++];
+        if (pos < b.size()) ++pos;
+        return s;
+    };
+    const std::string magic = line();
+    if (magic.rfind("#?", 0) != 0) throw Exception("\"%s\" is not a Radiance HDR file", fn);
+    for (;;) {
+        if (pos >= b.size()) throw Exception("\"%s\": truncated header", fn);
+        const std::string l = line();
+        if (l.empty()) break;
+        if (l.rfind("FORMAT=", 0) == 0 && l != "FORMAT=32-bit_rle_rgbe")
+            throw Exception("\"%s\": unsupported format %s", fn, l);
+    }
+    const std::string res = line();
+    Image img;
+    char ya[3], xa[3];
+    if (std::sscanf(res.c_str(), "%2s %d %2s %d", ya, &img.height, xa, &img.width) != 4 || std::string(ya) != "-Y" ||
+        std::string(xa) != "+X")
+        throw Exception("\"%s\": unsupported resolution line \"%s\"", fn, res);
+    const int W = img.width, H = img.height;
+    img.rgb.resize((size_t)W * H * 3);
+    std::vector<unsigned char> scan((size_t)W * 4);
+    auto need = [&](size_t n) {
+        if (pos + n > b.size()) throw Exception("\"%s\": truncated pixel data", fn);
+    };
+    for (int y = 0; y < H; y++) {
+        need(4);
+        if (W >= 8 && W < 32768 && b[pos] == 2 && b[pos + 1] == 2 && ((b[pos + 2] << 8) | b[pos + 3]) == W) {
+            pos += 4;
+            for (int c = 0; c < 4; c++) {
+                int x = 0;
+                while (x < W) {
+                    need(1);
+                    int count = b[pos++];
+                    if (count > 128) {
+                        count -= 128;
+                        need(1);
+                        const unsigned char v = b[pos++];
+                        if (x + count > W) throw Exception("\"%s\": bad run length", fn);
+                        for (int i = 0; i < count; i++) scan[(size_t)4 * (x++) + c] = v;
+                    } else {
+                        if (count == 0 || x + count > W) throw Exception("\"%s\": bad run length", fn);
+                        need(count);
+                        for (int i = 0; i < count; i++) scan[(size_t)4 * (x++) + c] = b[pos++];
+                    }
+                }
+            }
+        } else {
+            need((size_t)W * 4);
+            std::memcpy(scan.data(), &b[pos], (size_t)W * 4);
+            pos += (size_t)W * 4;
+        }
+        for (int x = 0; x < W; x++) {
+            const unsigned char* p = &scan[(size_t)4 * x];
+            float* o = &img.rgb[((size_t)y * W + x) * 3];
+            if (p[3] == 0) {
+                o[0] = o[1] = o[2] = 0.f;
+            } else {
+                const float f = std::ldexp(1.0f, (int)p[3] - (128 + 8));
+                o[0] = p[0] * f;
+                o[1] = p[1] * f;
+                o[2] = p[2] * f;
+            }
+        }
+    }
+    return img;
+}
+
+Image load_pfm(const std::string& fn) {
+    const std::vector<unsigned char> b = read_file(fn);
+    size_t pos = 0;
+    auto token = [&]() {
+        while (pos < b.size() && std::isspace(b[pos])) ++pos;
+        std::string s;
+        while (pos < b.size() && !std::isspace(b[pos])) s += (char)b[pos++];
+        return s;
+    };
+    const std::string magic = token();
+    if (magic != "PF" && magic != "Pf") throw Exception("\"%s\" is not a PFM file", fn);
+    const int channels = magic == "PF" ? 3 : 1;
+    Image img;
+    img.width = std::stoi(token());
+    img.height = std::stoi(token());
+    const float scale = std::stof(token());
+    ++pos;  // the single whitespace byte before the raster
+    const size_t n = (size_t)img.width * img.height * channels;
+    if (pos + n * 4 > b.size()) throw Exception("\"%s\": truncated raster", fn);
+    const bool little = scale < 0;
+    img.rgb.resize((size_t)img.width * img.height * 3);
+    for (int y = 0; y < img.height; y++)
+        for (int x = 0; x < img.width; x++)
+            for (int c = 0; c < 3; c++) {
+                const size_t src = (((size_t)y * img.width + x) * channels + (channels == 3 ? c : 0)) * 4;
+                unsigned char v[4] = {b[pos + src], b[pos + src + 1], b[pos + src + 2], b[pos + src + 3]};
+                if (!little) std::swap(v[0], v[3]), std::swap(v[1], v[2]);
+                float f;
+                std::memcpy(&f, v, 4);
+                // PFM rows run bottom-up
+                img.rgb[(((size_t)(img.height - 1 - y)) * img.width + x) * 3 + c] = f;
+            }
+    return img;
+}
+
+float half_to_float(uint16_t h) {
+    const int s = (h >> 15) & 1, e = (h >> 10) & 31, m = h & 1023;
+    float v;
+    if (e == 0)
+        v = std::ldexp((float)m, -24);
+    else if (e == 31)
+        v = m ? NAN : INFINITY;
+    else
+        v = std::ldexp((float)(m + 1024), e - 25);
+    return s ? -v : v;
+}
+
+template <typename T>
+T rd(const std::vector<unsigned char>& b, size_t& pos) {
+    if (pos + sizeof(T) > b.size()) throw Exception("truncated OpenEXR file");
+    T v;
+    std::memcpy(&v, &b[pos], sizeof(T));
+    pos += sizeof(T);
+    return v;
+}
+
+Image load_exr(const std::string& fn) {
+    const std::vector<unsigned char> b = read_file(fn);
+    size_t pos = 0;
+    if (rd<uint32_t>(b, pos) != 20000630u) throw Exception("\"%s\" is not an OpenEXR file", fn);
+    const uint32_t version = rd<uint32_t>(b, pos);
+    if ((version & 0xff) != 2 || (version & 0x1e00) != 0) throw Exception("\"%s\": only single-part scanline OpenEXR files are supported", fn);
+    struct Ch {
+        std::string name;
+        int type;
+    };
+    std::vector<Ch> chans;
+    int compression = -1, xmin = 0, ymin = 0, xmax = -1, ymax = -1;
+    for (;;) {
+        std::string name, type;
+        while (pos < b.size() && b[pos]) name += (char)b[pos++];
+        ++pos;
+        if (name.empty()) break;
+        while (pos < b.size() && b[pos]) type += (char)b[pos++];
+        ++pos;
+        const int size = rd<int32_t>(b, pos);
+        const size_t end = pos + (size_t)size;
+        if (name == "channels") {
+            while (pos < end && b[pos]) {
+                Ch c;
+                while (b[pos]) c.name += (char)b[pos++];
+                ++pos;
+                c.type = rd<int32_t>(b, pos);
+                pos += 4;  // pLinear + reserved
+                const int xs = rd<int32_t>(b, pos), ys = rd<int32_t>(b, pos);
+                if (xs != 1 || ys != 1) c.type = -1;
+                chans.push_back(c);
+            }
+        } else if (name == "compression") {
+            compression = b[pos];
+        } else if (name == "dataWindow") {
+            xmin = rd<int32_t>(b, pos);
+            ymin = rd<int32_t>(b, pos);
+            xmax = rd<int32_t>(b, pos);
+            ymax = rd<int32_t>(b, pos);
+        }
+        pos = end;
+    }
+    if (compression != 0) throw Exception("\"%s\": only uncompressed OpenEXR files are supported", fn);
+    Image img;
+    img.width = xmax - xmin + 1;
+    img.height = ymax - ymin + 1;
+    if (img.width <= 0 || img.height <= 0) throw Exception("\"%s\": empty data window", fn);
+    int rgb_idx[3] = {-1, -1, -1};
+    for (size_t i = 0; i < chans.size(); i++) {
+        const std::string n = toLower(chans[i].name);
+        const int which = (n == "r" || n == "red" || endsWith(n, ".r")) ? 0
+                          : (n == "g" || n == "green" || endsWith(n, ".g")) ? 1
+                          : (n == "b" || n == "blue" || endsWith(n, ".b")) ? 2 : -1;
+        if (which >= 0 && rgb_idx[which] < 0 && chans[i].type >= 1) rgb_idx[which] = (int)i;
+    }
+    if (rgb_idx[0] < 0 || rgb_idx[1] < 0 || rgb_idx[2] < 0) throw Exception("This is not a standard RGB OpenEXR file!");
+    size_t row_bytes = 0;
+    for (const Ch& c : chans) row_bytes += (size_t)img.width * (c.type == 1 ? 2 : 4);
+    img.rgb.resize((size_t)img.width * img.height * 3);
+    pos += (size_t)img.height * 8;  // offset table
+    for (int k = 0; k < img.height; k++) {
+        const int y = rd<int32_t>(b, pos) - ymin;
+        const int sz = rd<int32_t>(b, pos);
+        if (y < 0 || y >= img.height || (size_t)sz != row_bytes || pos + row_bytes > b.size())
+            throw Exception("\"%s\": malformed scanline chunk", fn);
+        size_t off = pos;
+        for (size_t ci = 0; ci < chans.size(); ci++) {
+            const int bpp = chans[ci].type == 1 ? 2 : 4;
+            for (int c = 0; c < 3; c++)
+                if (rgb_idx[c] == (int)ci)
+                    for (int x = 0; x < img.width; x++) {
+                        float v;
+                        if (bpp == 2) {
+                            uint16_t h;
+                            std::memcpy(&h, &b[off + (size_t)x * 2], 2);
+                            v = half_to_float(h);
+                        } else {
+                            std::memcpy(&v, &b[off + (size_t)x * 4], 4);
+                        }
+                        img.rgb[((size_t)y * img.width + x) * 3 + c] = v;
+                    }
+            off += (size_t)img.width * bpp;
+        }
+        pos += row_bytes;
+    }
+    return img;
+}
+
+}  // namespace
+
+Image loadImage(const std::string& filename) {
+    const std::string l = toLower(filename);
+    if (endsWith(l, ".exr")) return load_exr(filename);
+    if (endsWith(l, ".pfm")) return load_pfm(filename);
+    if (endsWith(l, ".hdr") || endsWith(l, ".pic")) return load_hdr(filename);
+    throw Exception("Unsupported image format \"%s\" (expected .hdr, .pfm or .exr)", filename);
+}
+
+void savePFM(const std::string& filename, const Image& img) {
+    std::ofstream out(filename, std::ios::binary);
+    if (!out) throw Exception("Could not write \"%s\"", filename);
+    out << "PF\n" << img.width << " " << img.height << "\n-1.0\n";
+    for (int y = img.height - 1; y >= 0; y--)
+        out.write(reinterpret_cast<const char*>(&img.rgb[(size_t)y * img.width * 3]), (std::streamsize)img.width * 12);
+}
+
+void saveEXR(const std::string& filename, const Image& img) {
+    std::vector<unsigned char> h;
+    auto put = [&](const void* p, size_t n) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        h.insert(h.end(), c, c + n);
+    };
+    auto put_i32 = [&](int32_t v) { put(&v, 4); };
+    auto put_str = [&](const char* s) { put(s, std::strlen(s) + 1); };
+    const uint32_t magic = 20000630u, version = 2;
+    put(&magic, 4);
+    put(&version, 4);
+    // channels: B, G, R (alphabetical), FLOAT
+    put_str("channels");
+    put_str("chlist");
+    put_i32(3 * (2 + 16) + 1);
+    for (const char* n : {"B", "G", "R"}) {
+        put_str(n);
+        put_i32(2);
+        const unsigned char zero4[4] = {0, 0, 0, 0};
+        put(zero4, 4);
+        put_i32(1);
+        put_i32(1);
+    }
+    h.push_back(0);
+    put_str("compression");
+    put_str("compression");
+    put_i32(1);
+    h.push_back(0);
+    const int32_t box[4] = {0, 0, img.width - 1, img.height - 1};
+    put_str("dataWindow");
+    put_str("box2i");
+    put_i32(16);
+    put(box, 16);
+    put_str("displayWindow");
+    put_str("box2i");
+    put_i32(16);
+    put(box, 16);
+    put_str("lineOrder");
+    put_str("lineOrder");
+    put_i32(1);
+    h.push_back(0);
+    const float one = 1.f, zero2[2] = {0.f, 0.f};
+    put_str("pixelAspectRatio");
+    put_str("float");
+    put_i32(4);
+    put(&one, 4);
+    put_str("screenWindowCenter");
+    put_str("v2f");
+    put_i32(8);
+    put(zero2, 8);
+    put_str("screenWindowWidth");
+    put_str("float");
+    put_i32(4);
+    put(&one, 4);
+    h.push_back(0);
+    const size_t row_bytes = (size_t)img.width * 12;
+    uint64_t off = h.size() + (size_t)img.height * 8;
+    for (int y = 0; y < img.height; y++) {
+        put(&off, 8);
+        off += 8 + row_bytes;
+    }
+    std::ofstream out(filename, std::ios::binary);
+    if (!out) throw Exception("Could not write \"%s\"", filename);
+    out.write(reinterpret_cast<const char*>(h.data()), (std::streamsize)h.size());
+    std::vector<float> row((size_t)img.width * 3);
+    for (int y = 0; y < img.height; y++) {
+        const int32_t yy = y, sz = (int32_t)row_bytes;
+        out.write(reinterpret_cast<const char*>(&yy), 4);
+        out.write(reinterpret_cast<const char*>(&sz), 4);
+        for (int c = 0; c < 3; c++)  // B, G, R planes
+            for (int x = 0; x < img.width; x++) row[(size_t)c * img.width + x] = img.rgb[((size_t)y * img.width + x) * 3 + (2 - c)];
+        out.write(reinterpret_cast<const char*>(row.data()), (std::streamsize)row_bytes);
+    }
+}
+
+}  // namespace drawlab

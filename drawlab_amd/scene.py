@@ -66,12 +66,19 @@ class EnvMap:
 
 @dataclass
 class Camera:
+    """"virtual" (VirtualCamera: eye/lookat/up, vertical fov) or
+    "perspective" (PerspectiveCamera: toWorld 4x4, horizontal fov, clip
+    distances)."""
     eye: Tuple[float, float, float] = (0.0, 0.0, 5.0)
     lookat: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     up: Tuple[float, float, float] = (0.0, 1.0, 0.0)
     fov: float = 30.0
     width: int = 1280
     height: int = 720
+    type: str = "virtual"
+    to_world: Optional[np.ndarray] = None  # (4, 4) float32, row-major
+    near: float = 1e-4
+    far: float = 1e4
 
 
 @dataclass
@@ -95,6 +102,11 @@ class SceneDesc:
     max_depth: int = -1
     rr_depth: int = 4
     name: str = ""
+    seed: int = 0
+    # Scene::m_emitters order as (kind, index) pairs when it is not the
+    # default (area meshes, then point lights, then the environment map), e.g.
+    # for an XML file that declares the environment map first.
+    emitters: Optional[List[Tuple[str, int]]] = None
 
     def add_bsdf(self, b: Bsdf) -> int:
         self.bsdfs.append(b)
@@ -114,9 +126,11 @@ class SceneDesc:
         return int(sum(m.F.shape[0] for m in self.meshes))
 
     def emitter_order(self):
-        """Emitters in Scene::m_emitters order: mesh emitters are registered
-        as their mesh is added; point lights and the environment map after
-        them, as the generators and the XML loader add them last."""
+        """Emitters in Scene::m_emitters order (scene.cpp:36-81): `emitters`
+        when given, else mesh emitters in mesh order, then point lights, then
+        the environment map, as the procedural generators add them."""
+        if self.emitters is not None:
+            return list(self.emitters)
         out = [("area", i) for i, m in enumerate(self.meshes) if m.radiance is not None]
         out += [("point", i) for i in range(len(self.point_lights))]
         if self.envmap is not None:

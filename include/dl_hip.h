@@ -62,7 +62,7 @@ enum {
 };
 
 enum { DL_LIGHT_AREA = 0, DL_LIGHT_POINT = 1, DL_LIGHT_ENV = 2 };
-enum { DL_CAMERA_VIRTUAL = 0 };
+enum { DL_CAMERA_VIRTUAL = 0, DL_CAMERA_PERSPECTIVE = 1 };
 
 typedef struct dl_mesh_desc {
     const float* positions;    /* 3 * num_vertices, world space */
@@ -92,12 +92,19 @@ typedef struct dl_light_desc {
     int32_t env_visual;
 } dl_light_desc;
 
-/* VirtualCamera (src/tracer/camera/virtual.cpp:16-65) after its constructor:
-   eye, U, V, W as the reference computes them, and the output size. */
+/* DL_CAMERA_VIRTUAL: VirtualCamera (src/tracer/camera/virtual.cpp:16-65)
+   after its constructor -- eye, U, V, W as the reference computes them.
+   DL_CAMERA_PERSPECTIVE: PerspectiveCamera (src/tracer/camera/perspective.cpp:
+   21-95) after activate() -- the sample-to-camera and camera-to-world
+   matrices (row-major), 1/size and the clip distances. */
 typedef struct dl_camera_desc {
-    int32_t type;              /* DL_CAMERA_VIRTUAL */
+    int32_t type;              /* DL_CAMERA_* */
     int32_t width, height;
     float eye[3], U[3], V[3], W[3];
+    float sample_to_camera[16];
+    float camera_to_world[16];
+    float inv_size[2];
+    float near_clip, far_clip;
 } dl_camera_desc;
 
 /* ImageBlock filter tabulation (src/core/bitmap/block.cpp:12-27):
@@ -218,6 +225,11 @@ dl_status dl_hip_last_kernel_ms(void* ctx, double* ms);
 /* VirtualCamera constructor arithmetic (virtual.cpp:16-42). */
 void dl_camera_virtual(const float eye[3], const float lookat[3], const float up[3], float fov, int width, int height,
                        dl_camera_desc* out);
+/* PerspectiveCamera constructor + activate (perspective.cpp:21-70): fov is
+   the horizontal field of view in degrees; to_world is row-major (NULL =
+   identity).  Returns DL_ERR_INVALID if the projection is not invertible. */
+dl_status dl_camera_perspective(const float to_world[16], float fov, float near_clip, float far_clip, int width,
+                                int height, dl_camera_desc* out);
 /* ReconstructionFilter + ImageBlock tabulation (rfilter.cpp:13-98,
  * block.cpp:12-27).  type: 0 gaussian(radius, stddev) 1 box 2 tent
  * 3 mitchell(radius, B, C). */

@@ -44,6 +44,9 @@ def lib():
             "dlo_scene_add_point_light": (i32, [_P, _f32p, _f32p]),
             "dlo_scene_add_envmap": (i32, [_P, _f32p, i32, i32, f32, i32]),
             "dlo_scene_set_camera_virtual": (None, [_P, _f32p, _f32p, _f32p, f32, i32, i32]),
+            "dlo_scene_set_camera_perspective": (None, [_P, _f32p, _f32p, i32, i32, f32, f32]),
+            "dlo_scene_set_camera_basis": (None, [_P, _f32p, i32, i32]),
+            "dlo_scene_set_filter_table": (None, [_P, f32, _f32p]),
             "dlo_scene_get_camera": (None, [_P, _f32p]),
             "dlo_scene_set_filter": (None, [_P, i32, f32, f32, f32]),
             "dlo_scene_get_filter_table": (None, [_P, _f32p]),
@@ -133,6 +136,13 @@ class Scene:
         self.width, self.height = int(width), int(height)
         self.L.dlo_scene_set_camera_virtual(self.h, _f(eye, 3), _f(lookat, 3), _f(up, 3), float(fov), int(width), int(height))
 
+    def set_camera_perspective(self, sample_to_camera, camera_to_world, width, height, near, far):
+        """PerspectiveCamera after activate(): the two row-major matrices are
+        inputs."""
+        self.width, self.height = int(width), int(height)
+        self.L.dlo_scene_set_camera_perspective(self.h, _f(sample_to_camera, 16), _f(camera_to_world, 16),
+                                                int(width), int(height), float(near), float(far))
+
     def camera(self):
         out = np.zeros(12, np.float32)
         self.L.dlo_scene_get_camera(self.h, out)
@@ -212,22 +222,85 @@ class Scene:
         return q, a, p
 
 
-def from_desc(desc, build=True):
-    """Build an oracle Scene from a drawlab_amd scene description."""
+def perspective_matrices(fov, near, far, width, height, to_world=None):
+    """PerspectiveCamera::activate (src/tracer/camera/perspective.cpp:42-70):
+    sampleToCamera = (scale(-0.5, -0.5 aspect, 1) * translate(-1, -1/aspect, 0)
+    * perspective(cot, near, far))^-1 and cameraToWorld, both row-major.
+    Evaluated in float32 with a float64 inverse, so it agrees with the
+    product's set-up to rounding (checked by the tests), not bit for bit."""
+    f = np.float32
+    aspect = f(width) / f(height)
+    recip = f(1.0) / (f(far) - f(near))
+    cot = f(1.0) / f(np.tan(f(fov) / f(2.0) * f(np.pi / 180.0)))
+    P = np.array([[cot, 0, 0, 0], [0, cot, 0, 0], [0, 0, f(far) * recip, -f(near) * f(far) * recip], [0, 0, 1, 0]],
+                 dtype=np.float32)
+    S = np.diag(np.array([-0.5, -0.5 * aspect, 1.0, 1.0], dtype=np.float32))
+    T = np.eye(4, dtype=np.float32)
+    T[0, 3], T[1, 3] = -1.0, f(-1.0) / aspect
+    M = (S @ T @ P).astype(np.float32)
+    inv = np.linalg.inv(M.astype(np.float64)).astype(np.float32)
+    C = np.eye(4, dtype=np.float32) if to_world is None else np.asarray(to_world, np.float32).reshape(4, 4)
+    return inv.reshape(16), C.reshape(16)
+
+
+def from_desc(desc, build=True, camera_matrices=None):
+    """Build an oracle Scene from a drawlab_amd scene description.
+    camera_matrices: (sampleToCamera, cameraToWorld) of a perspective camera,
+    used instead of perspective_matrices() (e.g. the values the product's
+    set-up computed, so both sides trace identical rays)."""
     s = Scene()
     for b in desc.bsdfs:
         s.add_bsdf(b.type, b.params)
-    # emitters are created in scene order: meshes carry area lights
-    for m in desc.meshes:
+    # Emitters are registered in the scene's emitter order: an area light
+    # when its mesh is added (meshes keep their order), point lights and the
+    # environment map where the order places them.
+    order = desc.emitter_order() if hasattr(desc, "emitter_order") else None
+    if order is None:
+        order = [("area", i) for i, m in enumerate(desc.meshes) if m.radiance is not None]
+        order += [("point", i) for i in range(len(desc.point_lights))]
+        order += [("env", 0)] if desc.envmap is not None else []
+    pending = list(order)
+
+    def add_other(kind, idx):
+        if kind == "point":
+            s.add_point_light(desc.point_lights[idx][0], desc.point_lights[idx][1])
+        else:
+            s.add_envmap(desc.envmap.rgb, desc.envmap.scale, desc.envmap.visual)
+
+    for mi, m in enumerate(desc.meshes):
+        if m.radiance is not None:
+            while pending and pending[0] != ("area", mi):
+                kind, idx = pending.pop(0)
+                assert kind != "area", "area emitters must follow mesh order"
+                add_other(kind, idx)
+            assert pending and pending[0] == ("area", mi)
+            pending.pop(0)
         s.add_mesh(m.V, m.F, m.bsdf, m.N, m.UV, m.radiance)
-    for pl in desc.point_lights:
-        s.add_point_light(pl[0], pl[1])
-    if desc.envmap is not None:
-        s.add_envmap(desc.envmap.rgb, desc.envmap.scale, desc.envmap.visual)
+    for kind, idx in pending:
+        assert kind != "area"
+        add_other(kind, idx)
     c = desc.camera
-    s.set_camera(c.eye, c.lookat, c.up, c.fov, c.width, c.height)
-    f = desc.rfilter
-    s.set_filter(f.type, f.radius, f.p0, f.p1)
+    cd = getattr(desc, "camera_desc", None)  # an exported camera (XML scenes)
+    if cd is not None and cd.type == 1:
+        s.set_camera_perspective(np.array(cd.sample_to_camera[:], np.float32),
+                                 np.array(cd.camera_to_world[:], np.float32), cd.width, cd.height, cd.near_clip,
+                                 cd.far_clip)
+    elif cd is not None:
+        s.L.dlo_scene_set_camera_basis(s.h, _f(list(cd.eye) + list(cd.U) + list(cd.V) + list(cd.W), 12),
+                                       int(cd.width), int(cd.height))
+        s.width, s.height = int(cd.width), int(cd.height)
+    elif getattr(c, "type", "virtual") == "perspective":
+        if camera_matrices is None:
+            camera_matrices = perspective_matrices(c.fov, c.near, c.far, c.width, c.height, c.to_world)
+        s.set_camera_perspective(camera_matrices[0], camera_matrices[1], c.width, c.height, c.near, c.far)
+    else:
+        s.set_camera(c.eye, c.lookat, c.up, c.fov, c.width, c.height)
+    film = getattr(desc, "film", None)  # an exported filter table (XML scenes)
+    if film is not None:
+        s.L.dlo_scene_set_filter_table(s.h, float(film.radius), _f(film.table[:], 33))
+    else:
+        f = desc.rfilter
+        s.set_filter(f.type, f.radius, f.p0, f.p1)
     s.set_background(desc.bg_color)
     if build:
         s.build()

@@ -103,6 +103,8 @@ struct Camera {
     int width = 0, height = 0;
     V3 eye, U, V, W;
     V2 inv_size;
+    float S[16] = {0}, C[16] = {0};  // perspective: sample->camera, camera->world (row-major)
+    float near_clip = 1e-4f, far_clip = 1e4f;
 };
 
 struct Filter {
@@ -1053,8 +1055,37 @@ static V3 Li(const Scene& s, const RenderOpts& o, Sampler& sampler, Ray ray, uin
     return result;
 }
 
+// Matrix4f * Vector4f: each row accumulated from zero in index order
+// (include/core/math/matrix.h:255-262 via vector.h:329-336).
+static void mat4_apply(const float* m, const float in[4], float out[4]) {
+    for (int j = 0; j < 4; j++) {
+        float s = 0.0f;
+        for (int k = 0; k < 4; k++) s += m[4 * j + k] * in[k];
+        out[j] = s;
+    }
+}
+
+// PerspectiveCamera::sampleRay  src/tracer/camera/perspective.cpp:75-95:
+// near-plane point = sampleToCamera * (pos / size, 0) (homogeneous divide),
+// d = normalize(nearP), o = cameraToWorld * (0,0,0), d' = cameraToWorld * d,
+// [mint, maxt] = [near, far] / d.z.
+static Ray perspective_ray(const Camera& c, V2 pos) {
+    const float in[4] = {pos.x * c.inv_size.x, pos.y * c.inv_size.y, 0.0f, 1.0f};
+    float q[4];
+    mat4_apply(c.S, in, q);
+    const V3 d = normalized(mk(q[0] / q[3], q[1] / q[3], q[2] / q[3]));
+    const float inv_z = 1.0f / d.z;
+    const float o_in[4] = {0.0f, 0.0f, 0.0f, 1.0f}, d_in[4] = {d.x, d.y, d.z, 0.0f};
+    float o4[4], d4[4];
+    mat4_apply(c.C, o_in, o4);
+    mat4_apply(c.C, d_in, d4);
+    return make_ray(mk(o4[0] / o4[3], o4[1] / o4[3], o4[2] / o4[3]), mk(d4[0], d4[1], d4[2]), c.near_clip * inv_z,
+                    c.far_clip * inv_z);
+}
+
 // VirtualCamera::sampleRay  src/tracer/camera/virtual.cpp:52-65
 static Ray camera_ray(const Camera& c, V2 pos) {
+    if (c.type == DLO_CAMERA_PERSPECTIVE) return perspective_ray(c, pos);
     float lx = pos.x, ly = ((float)c.height - pos.y) - 1;
     float px = lx * c.inv_size.x * 2 - 1;
     float py = ly * c.inv_size.y * 2 - 1;
@@ -1376,6 +1407,47 @@ void dlo_scene_set_camera_virtual(void* h, const float* eye, const float* lookat
     c.V = c.V * vlen;
     float ulen = vlen * aspect;
     c.U = c.U * ulen;
+}
+
+// A virtual camera given by its basis (eye, U, V, W as the VirtualCamera
+// constructor leaves them).
+void dlo_scene_set_camera_basis(void* h, const float* eye_u_v_w, int width, int height) {
+    Scene& s = *(Scene*)h;
+    Camera& c = s.cam;
+    c.type = DLO_CAMERA_VIRTUAL;
+    c.width = width;
+    c.height = height;
+    c.inv_size = V2{1.f / width, 1.f / height};
+    V3* dst[4] = {&c.eye, &c.U, &c.V, &c.W};
+    for (int i = 0; i < 4; i++) *dst[i] = mk(eye_u_v_w[3 * i], eye_u_v_w[3 * i + 1], eye_u_v_w[3 * i + 2]);
+}
+
+// The film filter given by its ImageBlock tabulation (radius and the 33
+// table entries).
+void dlo_scene_set_filter_table(void* h, float radius, const float* table33) {
+    Scene& s = *(Scene*)h;
+    Filter& f = s.filter;
+    f.radius = radius;
+    f.border = (int)std::ceil(radius - 0.5f);
+    f.lookup = 32.0f / radius;
+    for (int i = 0; i < 33; i++) f.table[i] = table33[i];
+}
+
+// PerspectiveCamera after activate(): the matrices are inputs (row-major).
+void dlo_scene_set_camera_perspective(void* h, const float* sample_to_camera, const float* camera_to_world,
+                                      int width, int height, float near_clip, float far_clip) {
+    Scene& s = *(Scene*)h;
+    Camera& c = s.cam;
+    c.type = DLO_CAMERA_PERSPECTIVE;
+    c.width = width;
+    c.height = height;
+    c.inv_size = V2{1.f / width, 1.f / height};
+    for (int i = 0; i < 16; i++) {
+        c.S[i] = sample_to_camera[i];
+        c.C[i] = camera_to_world[i];
+    }
+    c.near_clip = near_clip;
+    c.far_clip = far_clip;
 }
 
 void dlo_scene_get_camera(void* h, float* out12) {

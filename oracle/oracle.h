@@ -45,7 +45,7 @@ enum {
     DLO_BSDF_CONDUCTOR = 5,   // alpha, eta[3], k[3]
 };
 enum { DLO_EMITTER_AREA = 0, DLO_EMITTER_POINT = 1, DLO_EMITTER_ENV = 2 };
-enum { DLO_CAMERA_VIRTUAL = 0 };
+enum { DLO_CAMERA_VIRTUAL = 0, DLO_CAMERA_PERSPECTIVE = 1 };
 enum { DLO_FILTER_GAUSSIAN = 0, DLO_FILTER_BOX = 1, DLO_FILTER_TENT = 2, DLO_FILTER_MITCHELL = 3 };
 // BRUTE: every triangle, lowest index on equal t (the semantic definition).
 // OCTREE: the reference octree exactly (last equal hit, unpadded slabs).
@@ -63,6 +63,10 @@ int dlo_scene_add_point_light(void* scene, const float* position, const float* i
 int dlo_scene_add_envmap(void* scene, const float* rgb, int w, int h, float scale, int visual);
 void dlo_scene_set_camera_virtual(void* scene, const float* eye, const float* lookat, const float* up, float fov,
                                   int width, int height);
+void dlo_scene_set_camera_perspective(void* scene, const float* sample_to_camera, const float* camera_to_world,
+                                      int width, int height, float near_clip, float far_clip);
+void dlo_scene_set_camera_basis(void* scene, const float* eye_u_v_w, int width, int height);
+void dlo_scene_set_filter_table(void* scene, float radius, const float* table33);
 void dlo_scene_get_camera(void* scene, float* out12);
 void dlo_scene_set_filter(void* scene, int type, float radius, float p0, float p1);
 void dlo_scene_get_filter_table(void* scene, float* out33);

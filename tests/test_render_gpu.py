@@ -16,33 +16,7 @@ from drawlab_amd import scenes
 pytestmark = pytest.mark.gpu
 
 
-def relmse(x, y):
-    return float(np.mean((x - y) ** 2 / (y * y + 1e-2)))
-
-
-def _paths(backend, desc, samples, seed, builder=drawlab_amd.BUILD_GPU):
-    backend.upload_scene(desc)
-    backend.build_accel(builder)
-    osc = oracle.from_desc(desc)
-    H, W = desc.camera.height, desc.camera.width
-    tot, same_all = 0, 0
-    for s in samples:
-        print("paths sample", s, flush=True)
-        g = backend.render_paths(sample=s, seed=seed)
-        xys = np.array([(x, y, s) for y in range(H) for x in range(W)], np.int32)
-        o = osc.render_samples(xys, seed=seed, max_depth=desc.max_depth, rr_depth=desc.rr_depth)
-        same = (g.view(np.uint32) == o.view(np.uint32)).all(axis=1)
-        tot += same.size
-        same_all += int(same.sum())
-        close = np.allclose(g, o, rtol=1e-3, atol=1e-5)
-        if not same.all():
-            bad = np.nonzero(~same)[0][:5]
-            print("differing paths:", bad, g[bad], o[bad], flush=True)
-        assert close or same.mean() > 0.999
-    frac = same_all / tot
-    print("bit-exact paths: %d / %d (%.5f)" % (same_all, tot, frac), flush=True)
-    assert frac >= 0.999
-    return frac
+from helpers import paths_parity as _paths, relmse
 
 
 def test_cornell_box_paths_bitexact(backend):
