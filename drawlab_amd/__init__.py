@@ -27,7 +27,10 @@ EXPORTED_SYMBOLS = (
     "dl_hip_build_accel", "dl_hip_trace_closest", "dl_hip_trace_shadow", "dl_hip_trace_closest_counted",
     "dl_hip_render", "dl_hip_render_paths", "dl_hip_malloc", "dl_hip_free", "dl_hip_memcpy_h2d",
     "dl_hip_memcpy_d2h", "dl_hip_memcpy_d2d", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_camera_perspective", "dl_film_filter",
-    "dl_mesh_tangents",
+    "dl_mesh_tangents", "dl_hip_memory_usage", "dl_hip_trim",
+    "dl_hip_service_start", "dl_hip_service_add_scene", "dl_hip_service_submit", "dl_hip_service_poll",
+    "dl_hip_service_wait", "dl_hip_service_result", "dl_hip_service_cancel", "dl_hip_service_stats",
+    "dl_hip_service_stop", "dl_hip_service_last_error",
 )
 
 
@@ -201,6 +204,69 @@ class DeviceBuffer:
             pass
 
 
+def scene_descriptor(desc: SceneDesc):
+    """dl_scene_desc of a SceneDesc plus the arrays it points into (keep
+    them alive while the descriptor is used)."""
+    keep = []
+    order = desc.emitter_order()
+    light_of_mesh = {}
+    lights = (dl_light_desc * max(1, len(order)))()
+    for li, (kind, idx) in enumerate(order):
+        ld = lights[li]
+        if kind == "area":
+            ld.type = LIGHT_AREA
+            ld.mesh = idx
+            ld.radiance[:] = desc.meshes[idx].radiance
+            light_of_mesh[idx] = li
+        elif kind == "point":
+            ld.type = LIGHT_POINT
+            ld.mesh = -1
+            ld.position[:] = desc.point_lights[idx][0]
+            ld.radiance[:] = desc.point_lights[idx][1]
+        else:
+            env = desc.envmap
+            rgb = np.ascontiguousarray(env.rgb, dtype=np.float32)
+            keep.append(rgb)
+            ld.type = LIGHT_ENV
+            ld.mesh = -1
+            ld.env_rgb = _p(rgb)
+            ld.env_height, ld.env_width = rgb.shape[0], rgb.shape[1]
+            ld.env_scale = float(env.scale)
+            ld.env_visual = int(bool(env.visual))
+    meshes = (dl_mesh_desc * len(desc.meshes))()
+    for i, m in enumerate(desc.meshes):
+        V = np.ascontiguousarray(m.V, dtype=np.float32)
+        F = np.ascontiguousarray(m.F, dtype=np.uint32)
+        N = None if m.N is None else np.ascontiguousarray(m.N, dtype=np.float32)
+        UV = None if m.UV is None else np.ascontiguousarray(m.UV, dtype=np.float32)
+        keep += [V, F, N, UV]
+        md = meshes[i]
+        md.positions, md.indices, md.normals, md.texcoords, md.tangents = _p(V), _p(F), _p(N), _p(UV), None
+        md.num_vertices, md.num_triangles = V.shape[0], F.shape[0]
+        md.material = m.bsdf
+        md.light = light_of_mesh.get(i, -1)
+    mats = (dl_material_desc * max(1, len(desc.bsdfs)))()
+    for i, b in enumerate(desc.bsdfs):
+        mats[i].type = b.type
+        for k, v in enumerate(b.params):
+            mats[i].params[k] = v
+    sd = dl_scene_desc()
+    sd.meshes = meshes
+    sd.num_meshes = len(desc.meshes)
+    sd.materials = mats
+    sd.num_materials = len(desc.bsdfs)
+    sd.lights = lights
+    sd.num_lights = len(order)
+    # exported descriptors (scenes loaded through the plugin layer) are
+    # used as given
+    cd = getattr(desc, "camera_desc", None)
+    sd.camera = cd if cd is not None else camera_desc(desc.camera)
+    fd = getattr(desc, "film", None)
+    sd.film = fd if fd is not None else film_desc(desc.rfilter)
+    sd.bg_color[:] = desc.bg_color
+    return sd, (keep, meshes, mats, lights)
+
+
 class Backend:
     """One HIP context on one device (the reference's DeviceContext role)."""
 
@@ -237,63 +303,7 @@ class Backend:
 
     # -- scene -------------------------------------------------------------
     def upload_scene(self, desc: SceneDesc):
-        keep = []
-        order = desc.emitter_order()
-        light_of_mesh = {}
-        lights = (dl_light_desc * max(1, len(order)))()
-        for li, (kind, idx) in enumerate(order):
-            ld = lights[li]
-            if kind == "area":
-                ld.type = LIGHT_AREA
-                ld.mesh = idx
-                ld.radiance[:] = desc.meshes[idx].radiance
-                light_of_mesh[idx] = li
-            elif kind == "point":
-                ld.type = LIGHT_POINT
-                ld.mesh = -1
-                ld.position[:] = desc.point_lights[idx][0]
-                ld.radiance[:] = desc.point_lights[idx][1]
-            else:
-                env = desc.envmap
-                rgb = np.ascontiguousarray(env.rgb, dtype=np.float32)
-                keep.append(rgb)
-                ld.type = LIGHT_ENV
-                ld.mesh = -1
-                ld.env_rgb = _p(rgb)
-                ld.env_height, ld.env_width = rgb.shape[0], rgb.shape[1]
-                ld.env_scale = float(env.scale)
-                ld.env_visual = int(bool(env.visual))
-        meshes = (dl_mesh_desc * len(desc.meshes))()
-        for i, m in enumerate(desc.meshes):
-            V = np.ascontiguousarray(m.V, dtype=np.float32)
-            F = np.ascontiguousarray(m.F, dtype=np.uint32)
-            N = None if m.N is None else np.ascontiguousarray(m.N, dtype=np.float32)
-            UV = None if m.UV is None else np.ascontiguousarray(m.UV, dtype=np.float32)
-            keep += [V, F, N, UV]
-            md = meshes[i]
-            md.positions, md.indices, md.normals, md.texcoords, md.tangents = _p(V), _p(F), _p(N), _p(UV), None
-            md.num_vertices, md.num_triangles = V.shape[0], F.shape[0]
-            md.material = m.bsdf
-            md.light = light_of_mesh.get(i, -1)
-        mats = (dl_material_desc * max(1, len(desc.bsdfs)))()
-        for i, b in enumerate(desc.bsdfs):
-            mats[i].type = b.type
-            for k, v in enumerate(b.params):
-                mats[i].params[k] = v
-        sd = dl_scene_desc()
-        sd.meshes = meshes
-        sd.num_meshes = len(desc.meshes)
-        sd.materials = mats
-        sd.num_materials = len(desc.bsdfs)
-        sd.lights = lights
-        sd.num_lights = len(order)
-        # exported descriptors (scenes loaded through the plugin layer) are
-        # used as given
-        cd = getattr(desc, "camera_desc", None)
-        sd.camera = cd if cd is not None else camera_desc(desc.camera)
-        fd = getattr(desc, "film", None)
-        sd.film = fd if fd is not None else film_desc(desc.rfilter)
-        sd.bg_color[:] = desc.bg_color
+        sd, keep = scene_descriptor(desc)
         self._check(self.L.dl_hip_upload_scene(self.h, ctypes.byref(sd)))
         self.desc = desc
         return self
@@ -380,3 +390,136 @@ class Backend:
         out = np.zeros((d.camera.height * d.camera.width, 5), np.float32)
         self._check(self.L.dl_hip_render_paths(self.h, ctypes.byref(p), int(sample), _p(out)))
         return out
+
+
+# ---------------------------------------------------------------------------
+# render service (dl_hip_service_*)
+# ---------------------------------------------------------------------------
+POLICY_FIFO, POLICY_SJF = 0, 1
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
+SERVICE_SYMBOLS = ("dl_hip_service_start", "dl_hip_service_add_scene", "dl_hip_service_submit",
+                   "dl_hip_service_poll", "dl_hip_service_wait", "dl_hip_service_result", "dl_hip_service_cancel",
+                   "dl_hip_service_stats", "dl_hip_service_stop", "dl_hip_service_last_error",
+                   "dl_hip_memory_usage")
+
+
+class dl_service_config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("lanes", ctypes.c_int32), ("policy", ctypes.c_int32),
+                ("builder", ctypes.c_int32), ("budget_bytes", ctypes.c_uint64)]
+
+
+class dl_job_info(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("lane", ctypes.c_int32), ("t_submit", ctypes.c_double),
+                ("t_start", ctypes.c_double), ("t_finish", ctypes.c_double), ("paths", ctypes.c_double),
+                ("ms_gpu", ctypes.c_double), ("reserved_bytes", ctypes.c_uint64), ("peak_bytes", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class dl_service_stats(ctypes.Structure):
+    _fields_ = [("jobs_done", ctypes.c_uint64), ("jobs_failed", ctypes.c_uint64),
+                ("jobs_cancelled", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
+                ("peak_reserved_bytes", ctypes.c_uint64), ("estimate_misses", ctypes.c_uint64),
+                ("scene_uploads", ctypes.c_uint64), ("paths", ctypes.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+def _service_lib():
+    L = lib()
+    if not getattr(L, "_service_sigs", False):
+        P, i32, u64, c = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes
+        sigs = {
+            "dl_hip_service_start": (i32, [c.POINTER(dl_service_config), c.POINTER(P)]),
+            "dl_hip_service_add_scene": (i32, [P, c.POINTER(dl_scene_desc), c.POINTER(u64)]),
+            "dl_hip_service_submit": (i32, [P, u64, c.POINTER(dl_render_params), c.POINTER(u64)]),
+            "dl_hip_service_poll": (i32, [P, u64, c.POINTER(dl_job_info)]),
+            "dl_hip_service_wait": (i32, [P, u64, c.POINTER(dl_job_info)]),
+            "dl_hip_service_result": (i32, [P, u64, P]),
+            "dl_hip_service_cancel": (i32, [P, u64]),
+            "dl_hip_service_stats": (i32, [P, c.POINTER(dl_service_stats)]),
+            "dl_hip_service_stop": (i32, [P]),
+            "dl_hip_service_last_error": (c.c_char_p, [P]),
+            "dl_hip_memory_usage": (i32, [P, c.POINTER(u64), c.POINTER(u64)]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        L._service_sigs = True
+    return L
+
+
+class Service:
+    """Many render jobs on one device: scenes registered once, jobs queued
+    and run on `lanes` worker contexts under a device-memory budget."""
+
+    def __init__(self, device=0, lanes=4, policy=POLICY_FIFO, budget_bytes=200 << 30, builder=BUILD_GPU):
+        self.L = _service_lib()
+        cfg = dl_service_config(int(device), int(lanes), int(policy), int(builder), int(budget_bytes))
+        h = ctypes.c_void_p()
+        st = self.L.dl_hip_service_start(ctypes.byref(cfg), ctypes.byref(h))
+        if st != 0:
+            raise DlError("dl_hip_service_start: %s (status %d)" % (self.L.dl_hip_service_last_error(None).decode(), st))
+        self.h = h
+        self.sizes = {}
+
+    def _check(self, st):
+        if st != 0:
+            raise DlError(self.L.dl_hip_service_last_error(self.h).decode())
+
+    def add_scene(self, desc):
+        """Register a SceneDesc (its arrays are copied); returns the scene id."""
+        sd, keep = scene_descriptor(desc)
+        sid = ctypes.c_uint64()
+        self._check(self.L.dl_hip_service_add_scene(self.h, ctypes.byref(sd), ctypes.byref(sid)))
+        del keep
+        self.sizes[sid.value] = (sd.camera.height, sd.camera.width)
+        return sid.value
+
+    def submit(self, scene_id, spp, seed=0, max_depth=-1, rr_depth=4, spp_per_pass=0, first_sample=0):
+        p = dl_render_params(int(spp), int(seed), int(max_depth), int(rr_depth), int(spp_per_pass),
+                             int(first_sample), 0)
+        jid = ctypes.c_uint64()
+        self._check(self.L.dl_hip_service_submit(self.h, int(scene_id), ctypes.byref(p), ctypes.byref(jid)))
+        return jid.value
+
+    def poll(self, job_id):
+        info = dl_job_info()
+        self._check(self.L.dl_hip_service_poll(self.h, int(job_id), ctypes.byref(info)))
+        return info.as_dict()
+
+    def wait(self, job_id):
+        info = dl_job_info()
+        self._check(self.L.dl_hip_service_wait(self.h, int(job_id), ctypes.byref(info)))
+        return info.as_dict()
+
+    def result(self, job_id, scene_id):
+        H, W = self.sizes[scene_id]
+        img = np.zeros((H, W, 3), np.float32)
+        self._check(self.L.dl_hip_service_result(self.h, int(job_id), _p(img)))
+        return img
+
+    def cancel(self, job_id):
+        self._check(self.L.dl_hip_service_cancel(self.h, int(job_id)))
+
+    def last_error(self):
+        return self.L.dl_hip_service_last_error(self.h).decode()
+
+    def stats(self):
+        st = dl_service_stats()
+        self._check(self.L.dl_hip_service_stats(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def stop(self):
+        if self.h:
+            self.L.dl_hip_service_stop(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass

@@ -31,7 +31,7 @@ dl_status upload(Ctx* c, T** dst, const T* src, size_t count) {
     *dst = nullptr;
     if (count == 0) return DL_OK;
     void* p = nullptr;
-    DL_HIP_CHECK(c, hipMalloc(&p, count * sizeof(T)));
+    DL_HIP_CHECK(c, dmalloc(c, &p, count * sizeof(T)));
     c->scene_allocs.push_back(p);
     if (src) DL_HIP_CHECK(c, hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
     *dst = (T*)p;
@@ -40,7 +40,11 @@ dl_status upload(Ctx* c, T** dst, const T* src, size_t count) {
 
 void free_scene(Ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
-    for (void* p : c->scene_allocs) hipFree(p);
+    // the acceleration structure is allocated by the builders, outside
+    // scene_allocs
+    if (c->ds.nodes) dfree(c, c->ds.nodes);
+    if (c->ds.tris) dfree(c, c->ds.tris);
+    for (void* p : c->scene_allocs) dfree(c, p);
     c->scene_allocs.clear();
     c->ds = DevScene();
     c->has_scene = false;
@@ -92,8 +96,8 @@ dl_status dl_hip_create(int device, void** out) {
     snprintf(c->devname, sizeof(c->devname), "%s (%s)", prop.name, prop.gcnArchName);
     c->num_cus = prop.multiProcessorCount;
     c->trace_blocks = (uint32_t)std::max(1, c->num_cus) * 16u;
-    e = hipMalloc(&c->stack_ovf, sizeof(int) * (size_t)c->trace_blocks * kTraceBlockHost * (kMaxStack - kLdsStack));
-    if (e == hipSuccess) e = hipMalloc(&c->work, 4 * sizeof(uint32_t));
+    e = dmalloc(c, (void**)&c->stack_ovf, sizeof(int) * (size_t)c->trace_blocks * kTraceBlockHost * (kMaxStack - kLdsStack));
+    if (e == hipSuccess) e = dmalloc(c, (void**)&c->work, 4 * sizeof(uint32_t));
     if (e != hipSuccess) {
         delete c;
         return DL_ERR_OOM;
@@ -108,8 +112,8 @@ dl_status dl_hip_destroy(void* h) {
     hipSetDevice(c->device);
     free_render_state(c);
     free_scene(c);
-    if (c->stack_ovf) hipFree(c->stack_ovf);
-    if (c->work) hipFree(c->work);
+    if (c->stack_ovf) dfree(c, c->stack_ovf);
+    if (c->work) dfree(c, c->work);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -257,8 +261,8 @@ dl_status dl_hip_build_accel(void* h, int builder, dl_accel_stats* stats) {
     if (!c->has_scene) return set_error(c, DL_ERR_STATE, "build_accel before upload_scene");
     hipSetDevice(c->device);
     if (c->ds.nodes) {
-        hipFree(c->ds.nodes);
-        hipFree(c->ds.tris);
+        dfree(c, c->ds.nodes);
+        dfree(c, c->ds.tris);
         c->ds.nodes = nullptr;
         c->ds.tris = nullptr;
     }
@@ -272,8 +276,8 @@ dl_status dl_hip_build_accel(void* h, int builder, dl_accel_stats* stats) {
         auto t1 = std::chrono::steady_clock::now();
         void* pn = nullptr;
         void* pt = nullptr;
-        DL_HIP_CHECK(c, hipMalloc(&pn, bvh.nodes.size() * sizeof(BvhNodeH)));
-        DL_HIP_CHECK(c, hipMalloc(&pt, std::max<size_t>(1, bvh.tris.size()) * sizeof(BvhTriH)));
+        DL_HIP_CHECK(c, dmalloc(c, &pn, bvh.nodes.size() * sizeof(BvhNodeH)));
+        DL_HIP_CHECK(c, dmalloc(c, &pt, std::max<size_t>(1, bvh.tris.size()) * sizeof(BvhTriH)));
         DL_HIP_CHECK(c, hipMemcpy(pn, bvh.nodes.data(), bvh.nodes.size() * sizeof(BvhNodeH), hipMemcpyHostToDevice));
         if (!bvh.tris.empty())
             DL_HIP_CHECK(c, hipMemcpy(pt, bvh.tris.data(), bvh.tris.size() * sizeof(BvhTriH), hipMemcpyHostToDevice));
@@ -335,14 +339,14 @@ dl_status dl_hip_malloc(void* h, size_t bytes, void** dptr) {
     Ctx* c = (Ctx*)h;
     if (!c || !dptr) return DL_ERR_INVALID;
     hipSetDevice(c->device);
-    DL_HIP_CHECK(c, hipMalloc(dptr, bytes));
+    DL_HIP_CHECK(c, dmalloc(c, dptr, bytes));
     return DL_OK;
 }
 dl_status dl_hip_free(void* h, void* p) {
     Ctx* c = (Ctx*)h;
     if (!c) return DL_ERR_INVALID;
     DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    DL_HIP_CHECK(c, hipFree(p));
+    DL_HIP_CHECK(c, dfree(c, p));
     return DL_OK;
 }
 dl_status dl_hip_memcpy_h2d(void* h, void* dst, const void* src, size_t bytes) {
@@ -375,6 +379,22 @@ dl_status dl_hip_memcpy_d2d(void* h, void* dst, const void* src, size_t bytes) {
                                         (const uint8_t*)src + n16 * 16, ntail);
     DL_HIP_CHECK(c, hipGetLastError());
     DL_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
+    return DL_OK;
+}
+dl_status dl_hip_memory_usage(void* h, uint64_t* current, uint64_t* peak) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    if (current) *current = c->dev_bytes;
+    if (peak) *peak = c->dev_peak;
+    c->dev_peak = c->dev_bytes;  // the next query reports the peak since this one
+    return DL_OK;
+}
+dl_status dl_hip_trim(void* h) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    free_render_state(c);
+    free_scene(c);
     return DL_OK;
 }
 dl_status dl_hip_sync(void* h) {

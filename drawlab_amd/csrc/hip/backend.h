@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../../include/dl_hip.h"
@@ -96,7 +98,32 @@ struct Ctx {
     std::vector<void*> render_allocs;
     size_t render_capacity = 0;
     void* rs = nullptr;  // RenderState*
+
+    // device memory held by this context (every allocation goes through
+    // dmalloc / dfree): current and peak bytes
+    std::unordered_map<void*, size_t> dev_allocs;
+    size_t dev_bytes = 0, dev_peak = 0;
 };
+
+// Tracked device allocation of a context.
+inline hipError_t dmalloc(Ctx* c, void** p, size_t bytes) {
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) {
+        c->dev_allocs[*p] = bytes;
+        c->dev_bytes += bytes;
+        if (c->dev_bytes > c->dev_peak) c->dev_peak = c->dev_bytes;
+    }
+    return e;
+}
+inline hipError_t dfree(Ctx* c, void* p) {
+    if (!p) return hipSuccess;
+    auto it = c->dev_allocs.find(p);
+    if (it != c->dev_allocs.end()) {
+        c->dev_bytes -= it->second;
+        c->dev_allocs.erase(it);
+    }
+    return hipFree(p);
+}
 
 dl_status set_error(Ctx* c, dl_status s, const std::string& msg);
 dl_status hip_fail(Ctx* c, hipError_t e, const char* what);

@@ -292,7 +292,7 @@ struct Scratch {
     template <typename T>
     dl_status get(T** p, size_t count) {
         void* q = nullptr;
-        hipError_t e = hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T));
+        hipError_t e = dmalloc(c, (void**)&q, std::max<size_t>(1, count) * sizeof(T));
         if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
         ptrs.push_back(q);
         *p = (T*)q;
@@ -300,7 +300,7 @@ struct Scratch {
     }
     ~Scratch() {
         if (c->stream) (void)hipStreamSynchronize(c->stream);
-        for (void* p : ptrs) (void)hipFree(p);
+        for (void* p : ptrs) (void)dfree(c, p);
     }
 };
 
@@ -318,8 +318,8 @@ static dl_status build_small(Ctx* c, dl_accel_stats* stats) {
     HostBvh bvh;
     build_bvh_host(c->tri_verts.data(), c->nprims, bvh);
     void *pn = nullptr, *pt = nullptr;
-    DL_HIP_CHECK(c, hipMalloc(&pn, bvh.nodes.size() * sizeof(BvhNodeH)));
-    DL_HIP_CHECK(c, hipMalloc(&pt, std::max<size_t>(1, bvh.tris.size()) * sizeof(BvhTriH)));
+    DL_HIP_CHECK(c, dmalloc(c, &pn, bvh.nodes.size() * sizeof(BvhNodeH)));
+    DL_HIP_CHECK(c, dmalloc(c, &pt, std::max<size_t>(1, bvh.tris.size()) * sizeof(BvhTriH)));
     DL_HIP_CHECK(c, hipMemcpy(pn, bvh.nodes.data(), bvh.nodes.size() * sizeof(BvhNodeH), hipMemcpyHostToDevice));
     if (!bvh.tris.empty())
         DL_HIP_CHECK(c, hipMemcpy(pt, bvh.tris.data(), bvh.tris.size() * sizeof(BvhTriH), hipMemcpyHostToDevice));
@@ -359,8 +359,8 @@ dl_status build_bvh_lbvh(Ctx* c, dl_accel_stats* stats) {
     void* temp = nullptr;
     DL_TRY(sc.get((char**)&temp, temp_bytes));
     void *pn = nullptr, *pt = nullptr;
-    DL_HIP_CHECK(c, hipMalloc(&pn, (size_t)(n - 1) * kNodeBytes));
-    DL_HIP_CHECK(c, hipMalloc(&pt, (size_t)n * kTriBytes));
+    DL_HIP_CHECK(c, dmalloc(c, &pn, (size_t)(n - 1) * kNodeBytes));
+    DL_HIP_CHECK(c, dmalloc(c, &pt, (size_t)n * kTriBytes));
     c->ds.nodes = (float4*)pn;
     c->ds.tris = (float4*)pt;
 
@@ -662,8 +662,8 @@ dl_status build_bvh_ploc(Ctx* c, dl_accel_stats* stats) {
     void* temp = nullptr;
     DL_TRY(sc.get((char**)&temp, std::max(sort_bytes, scan_bytes)));
     void *pn = nullptr, *pt = nullptr;
-    DL_HIP_CHECK(c, hipMalloc(&pn, (size_t)(n - 1) * kNodeBytes));
-    DL_HIP_CHECK(c, hipMalloc(&pt, (size_t)n * kTriBytes));
+    DL_HIP_CHECK(c, dmalloc(c, &pn, (size_t)(n - 1) * kNodeBytes));
+    DL_HIP_CHECK(c, dmalloc(c, &pt, (size_t)n * kTriBytes));
     c->ds.nodes = (float4*)pn;
     c->ds.tris = (float4*)pt;
     int* host_cnt = nullptr;

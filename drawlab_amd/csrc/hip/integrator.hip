@@ -713,7 +713,7 @@ void free_render_state(Ctx* c) {
     RenderState* rs = (RenderState*)c->rs;
     if (!rs) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (void* p : rs->allocs) (void)hipFree(p);
+    for (void* p : rs->allocs) (void)dfree(c, p);
     if (rs->cnt_host) (void)hipHostFree(rs->cnt_host);
     for (hipEvent_t e : rs->evs) (void)hipEventDestroy(e);
     delete rs;
@@ -725,7 +725,7 @@ namespace {
 template <typename T>
 dl_status alloc(Ctx* c, RenderState* rs, T** p, size_t count) {
     void* q = nullptr;
-    DL_HIP_CHECK(c, hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T)));
+    DL_HIP_CHECK(c, dmalloc(c, (void**)&q, std::max<size_t>(1, count) * sizeof(T)));
     rs->allocs.push_back(q);
     *p = (T*)q;
     return DL_OK;

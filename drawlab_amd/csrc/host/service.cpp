@@ -1,0 +1,475 @@
+// Render service: many jobs on one device (dl_hip_service_* in dl_hip.h).
+//
+// Lanes are worker threads, each owning one backend context (its own HIP
+// stream, scene, BVH and render state); kernels of jobs on different lanes
+// overlap on the device's hardware queues, which is what packs small jobs
+// into the GPU.  Each lane keeps its last scene resident, and a lane prefers
+// a queued job of its resident scene among the next `lanes` queue entries
+// (bounded bypass, so no job waits behind more than `lanes` later arrivals
+// for that reason).
+//
+// Admission: every lane holds a reservation of device bytes -- while it runs
+// a job, a conservative estimate of what its context can reach during that
+// job (fixed context buffers + scene + BVH build + render state); when idle,
+// the bytes its context actually holds.  A job starts on a lane only if the
+// sum of all reservations stays within the budget, so the sum of the lanes'
+// device memory never exceeds it.  After each job the lane's measured peak
+// is compared with its reservation (estimate_misses counts violations).
+//
+// A job renders in passes of the same size dl_hip_render would choose
+// (spp_per_pass = 0 resolves to the same value), accumulating into the
+// lane's film, so its image is bit-identical to a single dl_hip_render
+// call; cancellation takes effect between passes.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../../include/dl_hip.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr uint64_t kPathsPerPass = 16u << 20;  // dl_hip_render's default pass size (paths)
+constexpr uint64_t kStateBytesPerPath = 240;    // path state + queues (integrator RenderState)
+constexpr uint64_t kFilmBytesPerPixel = 28;     // accumulation film (16 B) + resolved RGB (12 B)
+constexpr uint64_t kBytesPerVertex = 36;        // positions, normals, tangents
+constexpr uint64_t kBytesPerPrim = 16 + 112 + 256;  // indices + mesh map, BVH nodes + triangles, build scratch
+constexpr uint64_t kBytesPerTexel = 32;         // environment texels + alias tables
+constexpr uint64_t kSlackBytes = 4u << 20;
+
+// A registered scene: deep copies of every array the descriptor points to.
+struct SceneEntry {
+    std::vector<std::vector<float>> fbuf;
+    std::vector<std::vector<uint32_t>> ubuf;
+    std::vector<dl_mesh_desc> meshes;
+    std::vector<dl_material_desc> materials;
+    std::vector<dl_light_desc> lights;
+    dl_scene_desc desc{};
+    uint64_t verts = 0, prims = 0, texels = 0;
+    uint64_t bytes_estimate() const {
+        return verts * kBytesPerVertex + prims * kBytesPerPrim + texels * kBytesPerTexel + kSlackBytes;
+    }
+};
+
+struct Job {
+    uint64_t id = 0, scene = 0;
+    dl_render_params params{};
+    dl_job_info info{};
+    std::vector<float> rgb;
+    std::string error;
+    std::atomic<bool> cancel{false};  // set by dl_hip_service_cancel / stop, read between passes
+};
+
+struct Lane {
+    int index = 0;
+    void* ctx = nullptr;
+    uint64_t fixed_bytes = 0;     // context buffers that exist without a scene
+    uint64_t resident_scene = 0;  // 0: none
+    uint64_t state_paths = 0, state_pixels = 0;  // render state the context holds
+    uint64_t reservation = 0;
+    std::thread th;
+};
+
+struct Service {
+    dl_service_config cfg{};
+    Clock::time_point t0;
+    std::mutex m;
+    std::condition_variable cv;
+    bool stopping = false;
+    bool pressure = false;  // a queued job waits for budget: idle lanes release their resident bytes
+    uint64_t next_scene = 1, next_job = 1;
+    std::map<uint64_t, std::unique_ptr<SceneEntry>> scenes;
+    std::map<uint64_t, std::unique_ptr<Job>> jobs;
+    std::deque<uint64_t> queue;
+    std::vector<std::unique_ptr<Lane>> lanes;
+    dl_service_stats stats{};
+    std::string err;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+    uint64_t reserved_total() const {
+        uint64_t s = 0;
+        for (const auto& l : lanes) s += l->reservation;
+        return s;
+    }
+};
+
+uint64_t job_pixels(const SceneEntry& s) { return (uint64_t)s.desc.camera.width * s.desc.camera.height; }
+
+// passes of the job: the pass size dl_hip_render uses for spp_per_pass = 0
+uint32_t pass_spp(const dl_render_params& p, uint64_t pixels) {
+    uint64_t k = p.spp_per_pass > 0 ? (uint64_t)p.spp_per_pass : std::max<uint64_t>(1, kPathsPerPass / pixels);
+    return (uint32_t)std::min<uint64_t>(k, (uint64_t)p.spp);
+}
+
+// Device bytes the lane may reach while rendering `job` (see file comment).
+uint64_t reservation_for(const Lane& lane, const Job& job, const SceneEntry& s) {
+    const uint64_t px = job_pixels(s);
+    const uint64_t paths = px * pass_spp(job.params, px);
+    const uint64_t state = std::max(paths, lane.state_paths) * kStateBytesPerPath +
+                           std::max(px, lane.state_pixels) * kFilmBytesPerPixel + kSlackBytes;
+    return lane.fixed_bytes + s.bytes_estimate() + state;
+}
+
+// Pick the next job for `lane` (caller holds the lock): policy order, with a
+// preference for the lane's resident scene among the next `lanes` entries.
+std::deque<uint64_t>::iterator pick(Service& S, Lane& lane) {
+    auto& q = S.queue;
+    if (q.empty()) return q.end();
+    auto best = q.begin();
+    if (S.cfg.policy == DL_POLICY_SJF) {
+        uint64_t bw = ~0ull;
+        for (auto it = q.begin(); it != q.end(); ++it) {
+            const Job& j = *S.jobs[*it];
+            const uint64_t w = job_pixels(*S.scenes[j.scene]) * (uint64_t)j.params.spp;
+            if (w < bw) {
+                bw = w;
+                best = it;
+            }
+        }
+        return best;
+    }
+    const size_t window = std::min<size_t>(q.size(), (size_t)S.cfg.lanes);
+    for (size_t i = 0; i < window; i++)
+        if (S.jobs[q[i]]->scene == lane.resident_scene) return q.begin() + (long)i;
+    return best;
+}
+
+void finish(Service& S, Job& job, int state, const std::string& error) {
+    job.info.state = state;
+    job.info.t_finish = S.now();
+    job.error = error;
+    if (state == DL_JOB_DONE) S.stats.jobs_done++;
+    if (state == DL_JOB_FAILED) S.stats.jobs_failed++;
+    if (state == DL_JOB_CANCELLED) S.stats.jobs_cancelled++;
+}
+
+// Render one job on the lane's context (no lock held).  Returns "" or an error.
+std::string run_job(Service& S, Lane& lane, Job& job, const SceneEntry& scene, bool upload, dl_render_stats& total) {
+    void* c = lane.ctx;
+    auto fail = [&]() { return std::string(dl_hip_last_error(c)); };
+    if (upload) {
+        if (dl_hip_upload_scene(c, &scene.desc) != DL_OK) return fail();
+        if (dl_hip_build_accel(c, S.cfg.builder, nullptr) != DL_OK) return fail();
+    }
+    const uint64_t px = job_pixels(scene);
+    const uint32_t k = pass_spp(job.params, px);
+    job.rgb.assign(px * 3, 0.0f);
+    std::memset(&total, 0, sizeof(total));
+    for (uint32_t done = 0; done < (uint32_t)job.params.spp;) {
+        if (job.cancel.load()) return "cancelled";
+        const uint32_t kk = std::min<uint32_t>(k, (uint32_t)job.params.spp - done);
+        dl_render_params p = job.params;
+        p.spp = (int32_t)kk;
+        p.spp_per_pass = (int32_t)k;
+        p.first_sample = job.params.first_sample + (int32_t)done;
+        p.flags = (job.params.flags & ~DL_RENDER_ACCUMULATE) | (done > 0 ? DL_RENDER_ACCUMULATE : 0);
+        const bool last = done + kk >= (uint32_t)job.params.spp;
+        dl_render_stats st{};
+        if (dl_hip_render(c, &p, last ? job.rgb.data() : nullptr, &st) != DL_OK) return fail();
+        total.paths += st.paths;
+        total.ms_total += st.ms_total;
+        done += kk;
+    }
+    lane.state_paths = std::max<uint64_t>(lane.state_paths, px * k);
+    lane.state_pixels = std::max<uint64_t>(lane.state_pixels, px);
+    return "";
+}
+
+void lane_main(Service* Sp, Lane* lp) {
+    Service& S = *Sp;
+    Lane& lane = *lp;
+    std::unique_lock<std::mutex> lk(S.m);
+    for (;;) {
+        S.cv.wait(lk, [&] {
+            return S.stopping || !S.queue.empty() || (S.pressure && lane.reservation > lane.fixed_bytes);
+        });
+        if (S.stopping) break;
+        if (S.pressure && lane.reservation > lane.fixed_bytes) {
+            // idle under budget pressure: drop the resident scene and state
+            lk.unlock();
+            dl_hip_trim(lane.ctx);
+            uint64_t cur = 0;
+            dl_hip_memory_usage(lane.ctx, &cur, nullptr);
+            lk.lock();
+            lane.resident_scene = 0;
+            lane.state_paths = lane.state_pixels = 0;
+            lane.reservation = cur;
+            S.cv.notify_all();
+        }
+        auto it = pick(S, lane);
+        if (it == S.queue.end()) continue;
+        Job& job = *S.jobs[*it];
+        const SceneEntry& scene = *S.scenes[job.scene];
+        const uint64_t need = reservation_for(lane, job, scene);
+        const uint64_t others = S.reserved_total() - lane.reservation;
+        if (others + need > S.cfg.budget_bytes) {
+            // with every other lane at its floor (fixed buffers only) the job
+            // still would not fit: it never will
+            uint64_t floor_others = 0;
+            for (const auto& l : S.lanes)
+                if (l.get() != &lane) floor_others += l->fixed_bytes;
+            if (floor_others + need > S.cfg.budget_bytes) {
+                S.queue.erase(it);
+                finish(S, job, DL_JOB_FAILED, "job needs an estimated " + std::to_string(need) +
+                                                  " device bytes; the budget is " + std::to_string(S.cfg.budget_bytes));
+                S.cv.notify_all();
+                continue;
+            }
+            // otherwise wait for a lane to finish or to release resident bytes
+            S.pressure = true;
+            S.cv.notify_all();
+            S.cv.wait_for(lk, std::chrono::milliseconds(2));
+            continue;
+        }
+        S.pressure = false;
+        S.queue.erase(it);
+        lane.reservation = need;
+        S.stats.peak_reserved_bytes = std::max<uint64_t>(S.stats.peak_reserved_bytes, S.reserved_total());
+        job.info.state = DL_JOB_RUNNING;
+        job.info.lane = lane.index;
+        job.info.t_start = S.now();
+        job.info.reserved_bytes = need;
+        const bool upload = lane.resident_scene != job.scene;
+        if (upload) S.stats.scene_uploads++;
+        Job* jp = &job;
+        lk.unlock();
+
+        // the context's peak counter restarts at each query: the second
+        // query gives this job's peak
+        dl_hip_memory_usage(lane.ctx, nullptr, nullptr);
+        dl_render_stats total{};
+        std::string e = run_job(S, lane, *jp, scene, upload, total);
+        uint64_t cur = 0, peak = 0;
+        dl_hip_memory_usage(lane.ctx, &cur, &peak);
+
+        lk.lock();
+        lane.resident_scene = e.empty() || e == "cancelled" ? jp->scene : 0;
+        jp->info.paths = total.paths;
+        jp->info.ms_gpu = total.ms_total;
+        jp->info.peak_bytes = peak;
+        if (peak > need) S.stats.estimate_misses++;
+        S.stats.paths += total.paths;
+        if (e.empty())
+            finish(S, *jp, DL_JOB_DONE, "");
+        else if (e == "cancelled")
+            finish(S, *jp, DL_JOB_CANCELLED, "");
+        else
+            finish(S, *jp, DL_JOB_FAILED, e);
+        lane.reservation = cur;
+        S.cv.notify_all();
+    }
+}
+
+thread_local std::string g_service_error;
+
+dl_status err(Service* S, dl_status s, const std::string& msg) {
+    if (S)
+        S->err = msg;
+    else
+        g_service_error = msg;
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+dl_status dl_hip_service_start(const dl_service_config* cfg, void** svc) {
+    if (!cfg || !svc) return err(nullptr, DL_ERR_INVALID, "null argument");
+    *svc = nullptr;
+    if (cfg->lanes < 1 || cfg->lanes > 16) return err(nullptr, DL_ERR_INVALID, "lanes must be in [1, 16]");
+    if (cfg->policy != DL_POLICY_FIFO && cfg->policy != DL_POLICY_SJF)
+        return err(nullptr, DL_ERR_INVALID, "unknown scheduling policy");
+    std::unique_ptr<Service> S(new Service);
+    S->cfg = *cfg;
+    S->stats.budget_bytes = cfg->budget_bytes;
+    for (int i = 0; i < cfg->lanes; i++) {
+        std::unique_ptr<Lane> l(new Lane);
+        l->index = i;
+        const dl_status st = dl_hip_create(cfg->device, &l->ctx);
+        if (st != DL_OK) {
+            const std::string msg = l->ctx ? dl_hip_last_error(l->ctx) : "no HIP device";
+            if (l->ctx) dl_hip_destroy(l->ctx);
+            for (auto& o : S->lanes) dl_hip_destroy(o->ctx);
+            return err(nullptr, st, msg);
+        }
+        dl_hip_memory_usage(l->ctx, &l->fixed_bytes, nullptr);
+        l->reservation = l->fixed_bytes;
+        S->lanes.push_back(std::move(l));
+    }
+    if (S->reserved_total() > cfg->budget_bytes) {
+        for (auto& o : S->lanes) dl_hip_destroy(o->ctx);
+        return err(nullptr, DL_ERR_INVALID, "the lanes' fixed buffers alone exceed the budget");
+    }
+    S->t0 = Clock::now();
+    for (auto& l : S->lanes) l->th = std::thread(lane_main, S.get(), l.get());
+    *svc = S.release();
+    return DL_OK;
+}
+
+dl_status dl_hip_service_add_scene(void* h, const dl_scene_desc* d, uint64_t* scene_id) {
+    Service* S = (Service*)h;
+    if (!S || !d || !scene_id) return err(S, DL_ERR_INVALID, "null argument");
+    if (d->camera.width <= 0 || d->camera.height <= 0) return err(S, DL_ERR_INVALID, "invalid camera size");
+    std::unique_ptr<SceneEntry> e(new SceneEntry);
+    e->desc = *d;
+    e->fbuf.reserve(4 * d->num_meshes + d->num_lights);
+    e->ubuf.reserve(d->num_meshes);
+    for (uint32_t i = 0; i < d->num_meshes; i++) {
+        dl_mesh_desc m = d->meshes[i];
+        if (!m.positions || !m.indices) return err(S, DL_ERR_INVALID, "mesh without positions or indices");
+        auto copyf = [&](const float*& p, size_t n) {
+            if (!p) return;
+            e->fbuf.emplace_back(p, p + n);
+            p = e->fbuf.back().data();
+        };
+        copyf(m.positions, 3 * (size_t)m.num_vertices);
+        copyf(m.normals, 3 * (size_t)m.num_vertices);
+        copyf(m.texcoords, 2 * (size_t)m.num_vertices);
+        copyf(m.tangents, 3 * (size_t)m.num_vertices);
+        e->ubuf.emplace_back(m.indices, m.indices + 3 * (size_t)m.num_triangles);
+        m.indices = e->ubuf.back().data();
+        e->verts += m.num_vertices;
+        e->prims += m.num_triangles;
+        e->meshes.push_back(m);
+    }
+    e->materials.assign(d->materials, d->materials + d->num_materials);
+    for (uint32_t i = 0; i < d->num_lights; i++) {
+        dl_light_desc l = d->lights[i];
+        if (l.type == DL_LIGHT_ENV && l.env_rgb) {
+            const size_t n = (size_t)l.env_width * l.env_height;
+            e->fbuf.emplace_back(l.env_rgb, l.env_rgb + 3 * n);
+            l.env_rgb = e->fbuf.back().data();
+            e->texels += n;
+        }
+        e->lights.push_back(l);
+    }
+    e->desc.meshes = e->meshes.data();
+    e->desc.materials = e->materials.data();
+    e->desc.lights = e->lights.data();
+    std::lock_guard<std::mutex> lk(S->m);
+    *scene_id = S->next_scene++;
+    S->scenes[*scene_id] = std::move(e);
+    return DL_OK;
+}
+
+dl_status dl_hip_service_submit(void* h, uint64_t scene_id, const dl_render_params* p, uint64_t* job_id) {
+    Service* S = (Service*)h;
+    if (!S || !p || !job_id) return err(S, DL_ERR_INVALID, "null argument");
+    if (p->spp <= 0) return err(S, DL_ERR_INVALID, "spp must be positive");
+    std::lock_guard<std::mutex> lk(S->m);
+    if (S->stopping) return err(S, DL_ERR_STATE, "the service is stopping");
+    if (!S->scenes.count(scene_id)) return err(S, DL_ERR_INVALID, "unknown scene id " + std::to_string(scene_id));
+    std::unique_ptr<Job> j(new Job);
+    j->id = S->next_job++;
+    j->scene = scene_id;
+    j->params = *p;
+    j->info.state = DL_JOB_QUEUED;
+    j->info.lane = -1;
+    j->info.t_submit = S->now();
+    *job_id = j->id;
+    S->queue.push_back(j->id);
+    S->jobs[j->id] = std::move(j);
+    S->cv.notify_all();
+    return DL_OK;
+}
+
+dl_status dl_hip_service_poll(void* h, uint64_t job_id, dl_job_info* info) {
+    Service* S = (Service*)h;
+    if (!S || !info) return err(S, DL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(S->m);
+    auto it = S->jobs.find(job_id);
+    if (it == S->jobs.end()) return err(S, DL_ERR_INVALID, "unknown job id " + std::to_string(job_id));
+    *info = it->second->info;
+    if (info->state == DL_JOB_FAILED) S->err = it->second->error;
+    return DL_OK;
+}
+
+dl_status dl_hip_service_wait(void* h, uint64_t job_id, dl_job_info* info) {
+    Service* S = (Service*)h;
+    if (!S) return err(S, DL_ERR_INVALID, "null argument");
+    std::unique_lock<std::mutex> lk(S->m);
+    auto it = S->jobs.find(job_id);
+    if (it == S->jobs.end()) return err(S, DL_ERR_INVALID, "unknown job id " + std::to_string(job_id));
+    Job* j = it->second.get();
+    S->cv.wait(lk, [&] { return j->info.state >= DL_JOB_DONE; });
+    if (info) *info = j->info;
+    if (j->info.state == DL_JOB_FAILED) S->err = j->error;
+    return DL_OK;
+}
+
+dl_status dl_hip_service_result(void* h, uint64_t job_id, float* rgb) {
+    Service* S = (Service*)h;
+    if (!S || !rgb) return err(S, DL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(S->m);
+    auto it = S->jobs.find(job_id);
+    if (it == S->jobs.end()) return err(S, DL_ERR_INVALID, "unknown job id " + std::to_string(job_id));
+    Job& j = *it->second;
+    if (j.info.state != DL_JOB_DONE)
+        return err(S, DL_ERR_STATE, j.info.state == DL_JOB_FAILED ? "job failed: " + j.error : "job is not done");
+    if (j.rgb.empty()) return err(S, DL_ERR_STATE, "the result was already taken");
+    std::memcpy(rgb, j.rgb.data(), j.rgb.size() * sizeof(float));
+    std::vector<float>().swap(j.rgb);
+    return DL_OK;
+}
+
+dl_status dl_hip_service_cancel(void* h, uint64_t job_id) {
+    Service* S = (Service*)h;
+    if (!S) return err(S, DL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(S->m);
+    auto it = S->jobs.find(job_id);
+    if (it == S->jobs.end()) return err(S, DL_ERR_INVALID, "unknown job id " + std::to_string(job_id));
+    Job& j = *it->second;
+    if (j.info.state == DL_JOB_QUEUED) {
+        S->queue.erase(std::find(S->queue.begin(), S->queue.end(), job_id));
+        finish(*S, j, DL_JOB_CANCELLED, "");
+        S->cv.notify_all();
+    } else if (j.info.state == DL_JOB_RUNNING) {
+        j.cancel = true;  // read by the lane before its next pass
+    }
+    return DL_OK;
+}
+
+dl_status dl_hip_service_stats(void* h, dl_service_stats* st) {
+    Service* S = (Service*)h;
+    if (!S || !st) return err(S, DL_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(S->m);
+    *st = S->stats;
+    return DL_OK;
+}
+
+dl_status dl_hip_service_stop(void* h) {
+    Service* S = (Service*)h;
+    if (!S) return DL_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(S->m);
+        S->stopping = true;
+        for (uint64_t id : S->queue) finish(*S, *S->jobs[id], DL_JOB_CANCELLED, "");
+        for (auto& kv : S->jobs)
+            if (kv.second->info.state == DL_JOB_RUNNING) kv.second->cancel = true;
+        S->queue.clear();
+        S->cv.notify_all();
+    }
+    for (auto& l : S->lanes)
+        if (l->th.joinable()) l->th.join();
+    for (auto& l : S->lanes) dl_hip_destroy(l->ctx);
+    delete S;
+    return DL_OK;
+}
+
+const char* dl_hip_service_last_error(void* h) {
+    Service* S = (Service*)h;
+    return S ? S->err.c_str() : g_service_error.c_str();
+}
+
+}  // extern "C"

@@ -221,6 +221,74 @@ dl_status dl_hip_sync(void* ctx);
  * stream, milliseconds). */
 dl_status dl_hip_last_kernel_ms(void* ctx, double* ms);
 
+/* Device memory held by the context (every allocation it made and has not
+   freed): now, and the peak since the previous call (or since creation).
+   (No reference counterpart.) */
+dl_status dl_hip_memory_usage(void* ctx, uint64_t* current_bytes, uint64_t* peak_bytes);
+
+/* Frees the context's scene, acceleration structure and render state (the
+   context keeps only its fixed buffers; upload a scene again to render).
+   (No reference counterpart; used by the render service under memory
+   pressure.) */
+dl_status dl_hip_trim(void* ctx);
+
+/* ---- render service ----------------------------------------------------
+ * Many render jobs on one device: scenes are registered once (their arrays
+ * copied), jobs name a scene and render parameters and run on `lanes`
+ * worker contexts, each with its own HIP stream, so kernels of independent
+ * jobs overlap on the hardware queues.  A lane keeps its last scene (BVH
+ * included) resident for the next job of that scene.  Admission keeps the
+ * device bytes reserved by all lanes within budget_bytes; a job that could
+ * never fit fails.  Every job's image is bit-identical to dl_hip_render of
+ * the same scene and parameters with the same spp_per_pass.
+ * No reference counterpart: the reference renders one scene per process
+ * (src/tracer/main.cpp:34-48); its closest analogue is the progressive
+ * accumulation of src/optix/host/param_buffer.cpp:87-95. */
+enum { DL_POLICY_FIFO = 0, DL_POLICY_SJF = 1 };  /* shortest job (pixels * spp) first */
+enum { DL_JOB_QUEUED = 0, DL_JOB_RUNNING = 1, DL_JOB_DONE = 2, DL_JOB_FAILED = 3, DL_JOB_CANCELLED = 4 };
+
+typedef struct dl_service_config {
+    int32_t device;
+    int32_t lanes;             /* worker contexts (>= 1) */
+    int32_t policy;            /* DL_POLICY_* */
+    int32_t builder;           /* DL_BUILD_* for scene uploads */
+    uint64_t budget_bytes;     /* device memory all lanes together may reserve */
+} dl_service_config;
+
+typedef struct dl_job_info {
+    int32_t state;             /* DL_JOB_* */
+    int32_t lane;              /* lane that ran it, -1 if none */
+    double t_submit, t_start, t_finish;  /* seconds since the service started */
+    double paths;              /* camera samples traced */
+    double ms_gpu;             /* sum of the render calls' event time */
+    uint64_t reserved_bytes;   /* admission estimate for the job's lane */
+    uint64_t peak_bytes;       /* the lane context's device peak while it ran */
+} dl_job_info;
+
+typedef struct dl_service_stats {
+    uint64_t jobs_done, jobs_failed, jobs_cancelled;
+    uint64_t budget_bytes;
+    uint64_t peak_reserved_bytes;  /* largest sum of lane reservations */
+    uint64_t estimate_misses;      /* jobs whose lane peak exceeded its reservation */
+    uint64_t scene_uploads;        /* uploads + BVH builds performed */
+    double paths;
+} dl_service_stats;
+
+dl_status dl_hip_service_start(const dl_service_config* cfg, void** svc);
+/* Copies the scene's arrays; the descriptor may be freed afterwards. */
+dl_status dl_hip_service_add_scene(void* svc, const dl_scene_desc* scene, uint64_t* scene_id);
+dl_status dl_hip_service_submit(void* svc, uint64_t scene_id, const dl_render_params* params, uint64_t* job_id);
+dl_status dl_hip_service_poll(void* svc, uint64_t job_id, dl_job_info* info);
+/* Blocks until the job is done, failed or cancelled. */
+dl_status dl_hip_service_wait(void* svc, uint64_t job_id, dl_job_info* info);
+/* W*H*3 floats of a finished job (row 0 = top); the image is then released. */
+dl_status dl_hip_service_result(void* svc, uint64_t job_id, float* rgb_out);
+dl_status dl_hip_service_cancel(void* svc, uint64_t job_id);
+dl_status dl_hip_service_stats(void* svc, dl_service_stats* stats);
+/* Cancels queued jobs, lets running ones finish their pass, frees all. */
+dl_status dl_hip_service_stop(void* svc);
+const char* dl_hip_service_last_error(void* svc);
+
 /* ---- host-side helpers shared with the plugin layer -------------------- */
 /* VirtualCamera constructor arithmetic (virtual.cpp:16-42). */
 void dl_camera_virtual(const float eye[3], const float lookat[3], const float up[3], float fov, int width, int height,

@@ -73,3 +73,33 @@ def test_lib_has_no_oracle_dependency():
     """The product library must not link the oracle."""
     data = open(drawlab_amd.LIB_PATH, "rb").read()
     assert b"liboracle" not in data and b"dlo_" not in data
+
+
+def test_service_start_without_device_or_bad_config():
+    L = drawlab_amd._service_lib()
+    h = ctypes.c_void_p()
+    cfg = drawlab_amd.dl_service_config(0, 0, 0, 0, 1 << 30)   # lanes = 0
+    assert L.dl_hip_service_start(ctypes.byref(cfg), ctypes.byref(h)) == 1
+    assert b"lanes" in L.dl_hip_service_last_error(None)
+    cfg = drawlab_amd.dl_service_config(0, 1, 7, 0, 1 << 30)   # unknown policy
+    assert L.dl_hip_service_start(ctypes.byref(cfg), ctypes.byref(h)) == 1
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if not has_gpu:
+        cfg = drawlab_amd.dl_service_config(0, 2, 0, 0, 1 << 30)
+        st = L.dl_hip_service_start(ctypes.byref(cfg), ctypes.byref(h))
+        assert st != 0 and not h.value
+
+
+def test_host_library_exports_every_declared_symbol():
+    from drawlab_amd import host
+    text = open(os.path.join(os.path.dirname(drawlab_amd.LIB_PATH), "..", "..", "include", "drawlab_host.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    decl = set(re.findall(r"^\s*(?:int|void|const char\*)\s+(dlx_\w+)\s*\(", text, flags=re.M))
+    assert decl == set(host.HOST_SYMBOLS)
+    L = host.host_lib()
+    for name in decl:
+        assert hasattr(L, name), name

@@ -1,0 +1,129 @@
+"""Render service (dl_hip_service_*): every job's image is bit-identical to
+rendering the job alone with dl_hip_render; queued jobs can be cancelled;
+device memory reserved by all lanes never exceeds the budget, and a job
+that can never fit fails with a message."""
+import ctypes
+import time
+
+import numpy as np
+import pytest
+
+import drawlab_amd
+from drawlab_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _fixed_context_bytes():
+    be = drawlab_amd.Backend(0)
+    cur, peak = ctypes.c_uint64(), ctypes.c_uint64()
+    drawlab_amd._service_lib().dl_hip_memory_usage(be.h, ctypes.byref(cur), ctypes.byref(peak))
+    be.close()
+    return cur.value
+
+
+def test_jobs_match_standalone_renders(backend):
+    cbox = scenes.cornell_box(64, 64, spp=8, max_depth=8, rr_depth=3)
+    plates = scenes.mis_plates(64, 36, max_depth=6)
+    jobs = [(cbox, 8, 1, 0), (plates, 5, 2, 0), (cbox, 3, 3, 2), (plates, 7, 4, 3), (cbox, 16, 5, 0),
+            (cbox, 1, 6, 0)]
+    svc = drawlab_amd.Service(lanes=3, policy=drawlab_amd.POLICY_FIFO)
+    try:
+        sid = {id(cbox): svc.add_scene(cbox), id(plates): svc.add_scene(plates)}
+        ids = [svc.submit(sid[id(d)], spp, seed=seed, max_depth=d.max_depth, rr_depth=d.rr_depth, spp_per_pass=k)
+               for d, spp, seed, k in jobs]
+        print("submitted", ids, flush=True)
+        infos = [svc.wait(j) for j in ids]
+        assert all(i["state"] == drawlab_amd.JOB_DONE for i in infos), infos
+        imgs = [svc.result(j, sid[id(d)]) for j, (d, _, _, _) in zip(ids, jobs)]
+        st = svc.stats()
+        print("stats", st, flush=True)
+        assert st["jobs_done"] == len(jobs) and st["estimate_misses"] == 0
+        assert st["peak_reserved_bytes"] <= st["budget_bytes"]
+    finally:
+        svc.stop()
+    for (d, spp, seed, k), img, info in zip(jobs, imgs, infos):
+        backend.upload_scene(d)
+        backend.build_accel(drawlab_amd.BUILD_GPU)
+        ref, rst = backend.render(spp=spp, seed=seed, spp_per_pass=k)
+        assert info["paths"] == rst["paths"]
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), (spp, seed, k)
+        assert info["t_submit"] <= info["t_start"] <= info["t_finish"]
+
+
+def test_cancel_and_errors():
+    cbox = scenes.cornell_box(64, 64, spp=8, max_depth=8, rr_depth=3)
+    svc = drawlab_amd.Service(lanes=1)
+    try:
+        s = svc.add_scene(cbox)
+        a = svc.submit(s, 256, max_depth=8, rr_depth=3)
+        b = svc.submit(s, 4, max_depth=8, rr_depth=3)
+        svc.cancel(b)
+        assert svc.wait(b)["state"] == drawlab_amd.JOB_CANCELLED
+        with pytest.raises(drawlab_amd.DlError, match="not done"):
+            svc.result(b, s)
+        assert svc.wait(a)["state"] == drawlab_amd.JOB_DONE
+        img = svc.result(a, s)
+        assert img.shape == (64, 64, 3) and np.isfinite(img).all()
+        with pytest.raises(drawlab_amd.DlError, match="already taken"):
+            svc.result(a, s)
+        with pytest.raises(drawlab_amd.DlError, match="unknown scene id"):
+            svc.submit(999, 4)
+        with pytest.raises(drawlab_amd.DlError, match="spp must be positive"):
+            svc.submit(s, 0)
+        with pytest.raises(drawlab_amd.DlError, match="unknown job id"):
+            svc.poll(12345)
+    finally:
+        svc.stop()
+
+
+def test_memory_budget_is_never_exceeded():
+    fixed = _fixed_context_bytes()
+    cbox = scenes.cornell_box(96, 96, spp=8, max_depth=8, rr_depth=3)
+    # room for the two lanes' fixed buffers and one running job, not two
+    # (the service's estimate: scene + state + slack, see service.cpp)
+    one_job = (4 << 20) + 96 * 96 * 8 * 240 + 96 * 96 * 28 + (4 << 20) + 64 * 36 + 40 * 384
+    budget = 2 * fixed + int(1.5 * one_job)
+    svc = drawlab_amd.Service(lanes=2, budget_bytes=budget)
+    try:
+        s = svc.add_scene(cbox)
+        ids = [svc.submit(s, 8, seed=i, max_depth=8, rr_depth=3) for i in range(6)]
+        infos = [svc.wait(j) for j in ids]
+        st = svc.stats()
+        print("budget", budget, "fixed", fixed, "stats", st, flush=True)
+        assert all(i["state"] == drawlab_amd.JOB_DONE for i in infos)
+        assert st["peak_reserved_bytes"] <= budget and st["estimate_misses"] == 0
+        # at most one job in flight at any time
+        spans = sorted((i["t_start"], i["t_finish"]) for i in infos)
+        for (s0, f0), (s1, f1) in zip(spans, spans[1:]):
+            assert s1 >= f0 - 1e-4, spans
+        for i in infos:
+            assert i["peak_bytes"] <= i["reserved_bytes"]
+        # a job larger than the budget can ever hold fails with a message
+        big = svc.add_scene(scenes.cornell_box(1920, 1080, spp=1))
+        j = svc.submit(big, 64)
+        info = svc.wait(j)
+        assert info["state"] == drawlab_amd.JOB_FAILED
+        with pytest.raises(drawlab_amd.DlError, match="budget"):
+            svc.result(j, big)
+    finally:
+        svc.stop()
+
+
+def test_sjf_runs_short_jobs_first():
+    cbox = scenes.cornell_box(256, 256, spp=8, max_depth=8, rr_depth=3)
+    svc = drawlab_amd.Service(lanes=1, policy=drawlab_amd.POLICY_SJF)
+    try:
+        s = svc.add_scene(cbox)
+        first = svc.submit(s, 1024, max_depth=8, rr_depth=3)   # occupies the only lane
+        t0 = time.time()
+        while svc.poll(first)["state"] == drawlab_amd.JOB_QUEUED and time.time() - t0 < 30:
+            time.sleep(0.001)
+        big = svc.submit(s, 64, max_depth=8, rr_depth=3)
+        small = svc.submit(s, 2, max_depth=8, rr_depth=3)
+        ib, is_, i0 = svc.wait(big), svc.wait(small), svc.wait(first)
+        print("first", i0, "big", ib, "small", is_, flush=True)
+        if i0["t_finish"] > ib["t_submit"]:       # both were queued while the lane was busy
+            assert is_["t_start"] <= ib["t_start"]
+    finally:
+        svc.stop()
