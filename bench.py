@@ -13,6 +13,13 @@ dominant kernel (closest-hit traversal, k_trace_queue) and the CPU oracle's
 throughput on a bounded sample of the same workload, timed on this host's
 cores.
 
+`--workload service`: the render-service workload instead -- jobs drawn over
+the Cornell box, MIS plates, the 2M-triangle displaced sphere and procedural
+10k-500k-triangle scenes, resolutions {256^2, 512^2, 1024^2, 1920x1080},
+spp = floor(16 * 64^U); Poisson arrivals at 0.9x the one-at-a-time service
+rate.  Prints p50/p99 job latency of the multi-lane service and of
+one-at-a-time rendering, and both saturated throughputs (jobs/s).
+
 Profile helpers (they read rocprofv3 CSV output and write the summaries that
 `roofline.traffic` and the kernel-time tables come from):
   bench.py --summarize-stats DIR OUT.md       kernel time table
@@ -25,6 +32,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -134,6 +143,101 @@ def summarize_pmc(fetch_dir, write_dir, out_json, copy_bytes, commands):
     print(json.dumps({k: out[k] for k in list(out)[:12]}, indent=1))
 
 
+def _run_jobs(svc, sids, pool, jobs, arrivals):
+    """Submit jobs at their arrival times (seconds from now); wait for all.
+    Returns the per-job latencies, the paths traced and the makespan."""
+    import dataclasses
+    t0 = time.perf_counter()
+    ids = []
+    for (si, w, h, spp), ta in zip(jobs, arrivals):
+        d = time.perf_counter() - t0
+        if ta > d:
+            time.sleep(ta - d)
+        desc = pool[si]
+        cam = dataclasses.replace(desc.camera, width=w, height=h)
+        ids.append(svc.submit(sids[si], spp, seed=len(ids), max_depth=desc.max_depth, rr_depth=desc.rr_depth,
+                              camera=cam))
+    infos = [svc.wait(j) for j in ids]
+    for j in ids:
+        svc.result(j)
+    for (si, w, h, spp), i in zip(jobs, infos):
+        if i["peak_bytes"] > i["reserved_bytes"]:
+            print("reservation exceeded: scene %d %dx%d spp %d lane %d reserved %d peak %d" % (
+                si, w, h, spp, i["lane"], i["reserved_bytes"], i["peak_bytes"]), file=sys.stderr, flush=True)
+    assert all(i["state"] == 2 for i in infos), [i["state"] for i in infos]
+    lat = np.array([i["t_finish"] - i["t_submit"] for i in infos])
+    span = max(i["t_finish"] for i in infos) - min(i["t_submit"] for i in infos)
+    return lat, sum(i["paths"] for i in infos), span
+
+
+def service_bench(args):
+    import drawlab_amd
+    from drawlab_amd import scenes
+
+    pool = scenes.service_scene_pool()
+    jobs = scenes.service_jobs(args.jobs)
+    print("service workload: %d jobs, %.2f Gpaths" % (len(jobs), sum(w * h * s for _, w, h, s in jobs) / 1e9),
+          file=sys.stderr, flush=True)
+    policy = drawlab_amd.POLICY_SJF if args.policy == "sjf" else drawlab_amd.POLICY_FIFO
+    results = {}
+
+    def run(lanes, pol, arrivals):
+        svc = drawlab_amd.Service(lanes=lanes, policy=pol)
+        try:
+            sids = [svc.add_scene(d) for d in pool]
+            lat, paths, span = _run_jobs(svc, sids, pool, jobs, arrivals)
+            st = svc.stats()
+        finally:
+            svc.stop()
+        return lat, paths, span, st
+
+    zero = [0.0] * len(jobs)
+    # saturated (all jobs at t = 0): one-at-a-time, then the service
+    lat1, paths, span1, _ = run(1, drawlab_amd.POLICY_FIFO, zero)
+    print("one-at-a-time saturated: %.2f s, %.2f jobs/s" % (span1, len(jobs) / span1), file=sys.stderr, flush=True)
+    latn, _, spann, stn = run(args.lanes, policy, zero)
+    print("service saturated: %.2f s, %.2f jobs/s" % (spann, len(jobs) / spann), file=sys.stderr, flush=True)
+    # Poisson arrivals at 0.9x the one-at-a-time rate
+    rate = 0.9 * len(jobs) / span1
+    rng = np.random.default_rng(7)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(jobs)))
+    lat1p, _, _, _ = run(1, drawlab_amd.POLICY_FIFO, arrivals)
+    latnp, _, _, stp = run(args.lanes, policy, arrivals)
+
+    def q(a):
+        return {"p50": round(float(np.percentile(a, 50)), 4), "p99": round(float(np.percentile(a, 99)), 4),
+                "mean": round(float(a.mean()), 4)}
+
+    out = {
+        "metric": METRIC,
+        "value": q(latnp)["p99"],
+        "unit": "s (p99 job latency)",
+        "n_gpus": 1,
+        "steps": len(jobs),
+        "warmup": 0,
+        "ms_per_step": round(1e3 * spann / len(jobs), 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scenes and job list)",
+        "config": {"workload": "render_service", "jobs": len(jobs), "lanes": args.lanes, "policy": args.policy,
+                   "arrival_rate_hz": round(rate, 4), "scenes": [d.name for d in pool],
+                   "gpaths": round(paths / 1e9, 3)},
+        "latency_s": {"service": q(latnp), "one_at_a_time": q(lat1p)},
+        "saturated_jobs_per_s": {"service": round(len(jobs) / spann, 4),
+                                 "one_at_a_time": round(len(jobs) / span1, 4)},
+        "saturated_mpaths_per_s": {"service": round(paths / spann / 1e6, 1),
+                                   "one_at_a_time": round(paths / span1 / 1e6, 1)},
+        "service_stats": {"peak_reserved_bytes": int(stp["peak_reserved_bytes"]),
+                          "estimate_misses": int(stp["estimate_misses"] + stn["estimate_misses"]),
+                          "scene_uploads": int(stp["scene_uploads"])},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +252,10 @@ def main():
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"))
     ap.add_argument("--command", default="", help="command line recorded in a summary")
     ap.add_argument("--copy-mib", type=int, default=1024, help="calibration copy size used by the PMC runs")
+    ap.add_argument("--workload", choices=("displaced_sphere", "service"), default="displaced_sphere")
+    ap.add_argument("--jobs", type=int, default=64, help="service workload: number of jobs")
+    ap.add_argument("--lanes", type=int, default=4, help="service workload: worker lanes")
+    ap.add_argument("--policy", choices=("fifo", "sjf"), default="fifo")
     args = ap.parse_args()
     if args.summarize_stats:
         summarize_stats(args.summarize_stats[0], args.summarize_stats[1], args.command)
@@ -159,6 +267,9 @@ def main():
         print("bench.py measures one GPU: every box here has a single MI355X, so --gpus %d is not run "
               "(the frame would shard by sample index across GPUs)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "service":
+        service_bench(args)
+        return
 
     import drawlab_amd
     from drawlab_amd import scenes

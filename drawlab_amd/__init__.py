@@ -27,7 +27,7 @@ EXPORTED_SYMBOLS = (
     "dl_hip_build_accel", "dl_hip_trace_closest", "dl_hip_trace_shadow", "dl_hip_trace_closest_counted",
     "dl_hip_render", "dl_hip_render_paths", "dl_hip_malloc", "dl_hip_free", "dl_hip_memcpy_h2d",
     "dl_hip_memcpy_d2h", "dl_hip_memcpy_d2d", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_camera_perspective", "dl_film_filter",
-    "dl_mesh_tangents", "dl_hip_memory_usage", "dl_hip_trim",
+    "dl_mesh_tangents", "dl_hip_memory_usage", "dl_hip_trim", "dl_hip_set_camera",
     "dl_hip_service_start", "dl_hip_service_add_scene", "dl_hip_service_submit", "dl_hip_service_poll",
     "dl_hip_service_wait", "dl_hip_service_result", "dl_hip_service_cancel", "dl_hip_service_stats",
     "dl_hip_service_stop", "dl_hip_service_last_error",
@@ -434,7 +434,8 @@ def _service_lib():
         sigs = {
             "dl_hip_service_start": (i32, [c.POINTER(dl_service_config), c.POINTER(P)]),
             "dl_hip_service_add_scene": (i32, [P, c.POINTER(dl_scene_desc), c.POINTER(u64)]),
-            "dl_hip_service_submit": (i32, [P, u64, c.POINTER(dl_render_params), c.POINTER(u64)]),
+            "dl_hip_service_submit": (i32, [P, u64, c.POINTER(dl_camera_desc), c.POINTER(dl_render_params),
+                                            c.POINTER(u64)]),
             "dl_hip_service_poll": (i32, [P, u64, c.POINTER(dl_job_info)]),
             "dl_hip_service_wait": (i32, [P, u64, c.POINTER(dl_job_info)]),
             "dl_hip_service_result": (i32, [P, u64, P]),
@@ -465,6 +466,7 @@ class Service:
             raise DlError("dl_hip_service_start: %s (status %d)" % (self.L.dl_hip_service_last_error(None).decode(), st))
         self.h = h
         self.sizes = {}
+        self.job_sizes = {}
 
     def _check(self, st):
         if st != 0:
@@ -479,11 +481,18 @@ class Service:
         self.sizes[sid.value] = (sd.camera.height, sd.camera.width)
         return sid.value
 
-    def submit(self, scene_id, spp, seed=0, max_depth=-1, rr_depth=4, spp_per_pass=0, first_sample=0):
+    def submit(self, scene_id, spp, seed=0, max_depth=-1, rr_depth=4, spp_per_pass=0, first_sample=0, camera=None):
+        """camera: None (the scene's camera) or a drawlab_amd.scene.Camera /
+        dl_camera_desc for this job (e.g. another resolution)."""
         p = dl_render_params(int(spp), int(seed), int(max_depth), int(rr_depth), int(spp_per_pass),
                              int(first_sample), 0)
         jid = ctypes.c_uint64()
-        self._check(self.L.dl_hip_service_submit(self.h, int(scene_id), ctypes.byref(p), ctypes.byref(jid)))
+        cd = None
+        if camera is not None:
+            cd = camera if isinstance(camera, dl_camera_desc) else camera_desc(camera)
+        self._check(self.L.dl_hip_service_submit(self.h, int(scene_id), ctypes.byref(cd) if cd is not None else None,
+                                                 ctypes.byref(p), ctypes.byref(jid)))
+        self.job_sizes[jid.value] = (cd.height, cd.width) if cd is not None else self.sizes[int(scene_id)]
         return jid.value
 
     def poll(self, job_id):
@@ -496,8 +505,8 @@ class Service:
         self._check(self.L.dl_hip_service_wait(self.h, int(job_id), ctypes.byref(info)))
         return info.as_dict()
 
-    def result(self, job_id, scene_id):
-        H, W = self.sizes[scene_id]
+    def result(self, job_id, scene_id=None):
+        H, W = self.job_sizes[int(job_id)]
         img = np.zeros((H, W, 3), np.float32)
         self._check(self.L.dl_hip_service_result(self.h, int(job_id), _p(img)))
         return img

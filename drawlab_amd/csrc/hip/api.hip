@@ -389,6 +389,17 @@ dl_status dl_hip_memory_usage(void* h, uint64_t* current, uint64_t* peak) {
     c->dev_peak = c->dev_bytes;  // the next query reports the peak since this one
     return DL_OK;
 }
+dl_status dl_hip_set_camera(void* h, const dl_camera_desc* cam) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !cam) return DL_ERR_INVALID;
+    if (!c->has_scene) return set_error(c, DL_ERR_STATE, "set_camera before upload_scene");
+    if (cam->width <= 0 || cam->height <= 0) return set_error(c, DL_ERR_INVALID, "bad camera size");
+    if (cam->type != DL_CAMERA_VIRTUAL && cam->type != DL_CAMERA_PERSPECTIVE)
+        return set_error(c, DL_ERR_INVALID, "unknown camera type");
+    c->cam = *cam;
+    invalidate_film(c);
+    return DL_OK;
+}
 dl_status dl_hip_trim(void* h) {
     Ctx* c = (Ctx*)h;
     if (!c) return DL_ERR_INVALID;

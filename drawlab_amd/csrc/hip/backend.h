@@ -143,6 +143,7 @@ dl_status launch_trace_shadow(Ctx* c, const float* rays, uint32_t n, uint8_t* oc
 dl_status render(Ctx* c, const dl_render_params* p, float* rgb_host, dl_render_stats* stats, int paths_sample,
                  float* paths_out);
 void free_render_state(Ctx* c);
+void invalidate_film(Ctx* c);  // the next render clears the film even with DL_RENDER_ACCUMULATE
 
 // bvh_gpu.hip
 dl_status build_bvh_gpu(Ctx* c, dl_accel_stats* stats, int method);

@@ -720,6 +720,10 @@ void free_render_state(Ctx* c) {
     c->rs = nullptr;
 }
 
+void invalidate_film(Ctx* c) {
+    if (c->rs) ((RenderState*)c->rs)->film_valid = false;
+}
+
 namespace {
 
 template <typename T>

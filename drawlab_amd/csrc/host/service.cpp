@@ -9,9 +9,10 @@
 // for that reason).
 //
 // Admission: every lane holds a reservation of device bytes -- while it runs
-// a job, a conservative estimate of what its context can reach during that
-// job (fixed context buffers + scene + BVH build + render state); when idle,
-// the bytes its context actually holds.  A job starts on a lane only if the
+// a job, the larger of what its context already holds and a conservative
+// estimate of what it can reach during the job (fixed context buffers +
+// scene + BVH build scratch + render state); when idle, the bytes its
+// context actually holds.  A job starts on a lane only if the
 // sum of all reservations stays within the budget, so the sum of the lanes'
 // device memory never exceeds it.  After each job the lane's measured peak
 // is compared with its reservation (estimate_misses counts violations).
@@ -63,6 +64,7 @@ struct SceneEntry {
 
 struct Job {
     uint64_t id = 0, scene = 0;
+    dl_camera_desc camera{};  // the scene's camera unless the job gave one
     dl_render_params params{};
     dl_job_info info{};
     std::vector<float> rgb;
@@ -103,7 +105,7 @@ struct Service {
     }
 };
 
-uint64_t job_pixels(const SceneEntry& s) { return (uint64_t)s.desc.camera.width * s.desc.camera.height; }
+uint64_t job_pixels(const Job& j) { return (uint64_t)j.camera.width * j.camera.height; }
 
 // passes of the job: the pass size dl_hip_render uses for spp_per_pass = 0
 uint32_t pass_spp(const dl_render_params& p, uint64_t pixels) {
@@ -113,7 +115,7 @@ uint32_t pass_spp(const dl_render_params& p, uint64_t pixels) {
 
 // Device bytes the lane may reach while rendering `job` (see file comment).
 uint64_t reservation_for(const Lane& lane, const Job& job, const SceneEntry& s) {
-    const uint64_t px = job_pixels(s);
+    const uint64_t px = job_pixels(job);
     const uint64_t paths = px * pass_spp(job.params, px);
     const uint64_t state = std::max(paths, lane.state_paths) * kStateBytesPerPath +
                            std::max(px, lane.state_pixels) * kFilmBytesPerPixel + kSlackBytes;
@@ -130,7 +132,7 @@ std::deque<uint64_t>::iterator pick(Service& S, Lane& lane) {
         uint64_t bw = ~0ull;
         for (auto it = q.begin(); it != q.end(); ++it) {
             const Job& j = *S.jobs[*it];
-            const uint64_t w = job_pixels(*S.scenes[j.scene]) * (uint64_t)j.params.spp;
+            const uint64_t w = job_pixels(j) * (uint64_t)j.params.spp;
             if (w < bw) {
                 bw = w;
                 best = it;
@@ -161,7 +163,8 @@ std::string run_job(Service& S, Lane& lane, Job& job, const SceneEntry& scene, b
         if (dl_hip_upload_scene(c, &scene.desc) != DL_OK) return fail();
         if (dl_hip_build_accel(c, S.cfg.builder, nullptr) != DL_OK) return fail();
     }
-    const uint64_t px = job_pixels(scene);
+    if (dl_hip_set_camera(c, &job.camera) != DL_OK) return fail();
+    const uint64_t px = job_pixels(job);
     const uint32_t k = pass_spp(job.params, px);
     job.rgb.assign(px * 3, 0.0f);
     std::memset(&total, 0, sizeof(total));
@@ -210,7 +213,9 @@ void lane_main(Service* Sp, Lane* lp) {
         if (it == S.queue.end()) continue;
         Job& job = *S.jobs[*it];
         const SceneEntry& scene = *S.scenes[job.scene];
-        const uint64_t need = reservation_for(lane, job, scene);
+        // the lane starts from what it holds (its previous scene stays until
+        // the upload replaces it), then may reach the estimate
+        const uint64_t need = std::max(lane.reservation, reservation_for(lane, job, scene));
         const uint64_t others = S.reserved_total() - lane.reservation;
         if (others + need > S.cfg.budget_bytes) {
             // with every other lane at its floor (fixed buffers only) the job
@@ -363,16 +368,20 @@ dl_status dl_hip_service_add_scene(void* h, const dl_scene_desc* d, uint64_t* sc
     return DL_OK;
 }
 
-dl_status dl_hip_service_submit(void* h, uint64_t scene_id, const dl_render_params* p, uint64_t* job_id) {
+dl_status dl_hip_service_submit(void* h, uint64_t scene_id, const dl_camera_desc* camera, const dl_render_params* p,
+                                uint64_t* job_id) {
     Service* S = (Service*)h;
     if (!S || !p || !job_id) return err(S, DL_ERR_INVALID, "null argument");
     if (p->spp <= 0) return err(S, DL_ERR_INVALID, "spp must be positive");
+    if (camera && (camera->width <= 0 || camera->height <= 0)) return err(S, DL_ERR_INVALID, "bad camera size");
     std::lock_guard<std::mutex> lk(S->m);
     if (S->stopping) return err(S, DL_ERR_STATE, "the service is stopping");
     if (!S->scenes.count(scene_id)) return err(S, DL_ERR_INVALID, "unknown scene id " + std::to_string(scene_id));
+
     std::unique_ptr<Job> j(new Job);
     j->id = S->next_job++;
     j->scene = scene_id;
+    j->camera = camera ? *camera : S->scenes[scene_id]->desc.camera;
     j->params = *p;
     j->info.state = DL_JOB_QUEUED;
     j->info.lane = -1;

@@ -329,3 +329,29 @@ def random_scene(seed, target_tris=50_000, width=512, height=512, spp=64):
 
 def from_name(name, **kw):
     return {"cornell_box": cornell_box, "displaced_sphere": displaced_sphere, "mis_plates": mis_plates}[name](**kw)
+
+
+SERVICE_RESOLUTIONS = ((256, 256), (512, 512), (1024, 1024), (1920, 1080))
+
+
+def service_scene_pool():
+    """Scenes a render service job draws from: the Cornell box, the MIS
+    plates, the 2,097,152-triangle displaced sphere and procedural scenes of
+    10k to 500k triangles."""
+    pool = [cornell_box(), mis_plates(), displaced_sphere()]
+    for k, tris in enumerate((10_000, 50_000, 200_000, 500_000)):
+        pool.append(random_scene(100 + k, tris))
+    return pool
+
+
+def service_jobs(n, seed=0x5EED, pool_size=7):
+    """n jobs (scene index, width, height, spp): resolution uniform over
+    SERVICE_RESOLUTIONS, spp = floor(16 * 64^U) with U uniform on [0, 1)
+    (16 to 1024, mean about 242), scene uniform over the pool."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(n):
+        w, h = SERVICE_RESOLUTIONS[int(rng.integers(0, len(SERVICE_RESOLUTIONS)))]
+        spp = int(16 * 64 ** rng.random())
+        jobs.append((int(rng.integers(0, pool_size)), w, h, spp))
+    return jobs

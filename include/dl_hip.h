@@ -226,6 +226,11 @@ dl_status dl_hip_last_kernel_ms(void* ctx, double* ms);
    (No reference counterpart.) */
 dl_status dl_hip_memory_usage(void* ctx, uint64_t* current_bytes, uint64_t* peak_bytes);
 
+/* Replaces the camera (and output size) of the uploaded scene; the film of
+   the next render starts empty.  Replaces OptixScene::updateCamera / resize
+   (src/optix/host/optix_scene.cpp:43-52). */
+dl_status dl_hip_set_camera(void* ctx, const dl_camera_desc* camera);
+
 /* Frees the context's scene, acceleration structure and render state (the
    context keeps only its fixed buffers; upload a scene again to render).
    (No reference counterpart; used by the render service under memory
@@ -277,11 +282,15 @@ typedef struct dl_service_stats {
 dl_status dl_hip_service_start(const dl_service_config* cfg, void** svc);
 /* Copies the scene's arrays; the descriptor may be freed afterwards. */
 dl_status dl_hip_service_add_scene(void* svc, const dl_scene_desc* scene, uint64_t* scene_id);
-dl_status dl_hip_service_submit(void* svc, uint64_t scene_id, const dl_render_params* params, uint64_t* job_id);
+/* camera: NULL renders the scene's camera; otherwise this job's camera
+   (e.g. another output size). */
+dl_status dl_hip_service_submit(void* svc, uint64_t scene_id, const dl_camera_desc* camera,
+                                const dl_render_params* params, uint64_t* job_id);
 dl_status dl_hip_service_poll(void* svc, uint64_t job_id, dl_job_info* info);
 /* Blocks until the job is done, failed or cancelled. */
 dl_status dl_hip_service_wait(void* svc, uint64_t job_id, dl_job_info* info);
-/* W*H*3 floats of a finished job (row 0 = top); the image is then released. */
+/* W*H*3 floats of a finished job (row 0 = top, W x H of the job's camera);
+   the image is then released. */
 dl_status dl_hip_service_result(void* svc, uint64_t job_id, float* rgb_out);
 dl_status dl_hip_service_cancel(void* svc, uint64_t job_id);
 dl_status dl_hip_service_stats(void* svc, dl_service_stats* stats);

@@ -51,6 +51,29 @@ def test_jobs_match_standalone_renders(backend):
         assert info["t_submit"] <= info["t_start"] <= info["t_finish"]
 
 
+def test_job_camera_override(backend):
+    """A job with its own camera (another resolution) equals rendering the
+    scene with that camera alone."""
+    import dataclasses
+    cbox = scenes.cornell_box(64, 64, spp=8, max_depth=8, rr_depth=3)
+    cam = dataclasses.replace(cbox.camera, width=48, height=40)
+    svc = drawlab_amd.Service(lanes=2)
+    try:
+        s = svc.add_scene(cbox)
+        a = svc.submit(s, 6, seed=3, max_depth=8, rr_depth=3, camera=cam)
+        b = svc.submit(s, 6, seed=3, max_depth=8, rr_depth=3)
+        assert svc.wait(a)["state"] == drawlab_amd.JOB_DONE and svc.wait(b)["state"] == drawlab_amd.JOB_DONE
+        ia, ib = svc.result(a), svc.result(b)
+    finally:
+        svc.stop()
+    assert ia.shape == (40, 48, 3) and ib.shape == (64, 64, 3)
+    small = dataclasses.replace(cbox, camera=cam)
+    backend.upload_scene(small)
+    backend.build_accel(drawlab_amd.BUILD_GPU)
+    ref, _ = backend.render(spp=6, seed=3)
+    assert np.array_equal(ia.view(np.uint32), ref.view(np.uint32))
+
+
 def test_cancel_and_errors():
     cbox = scenes.cornell_box(64, 64, spp=8, max_depth=8, rr_depth=3)
     svc = drawlab_amd.Service(lanes=1)
