@@ -31,6 +31,7 @@ EXPORTED_SYMBOLS = (
     "dl_hip_service_start", "dl_hip_service_add_scene", "dl_hip_service_submit", "dl_hip_service_poll",
     "dl_hip_service_wait", "dl_hip_service_result", "dl_hip_service_cancel", "dl_hip_service_stats",
     "dl_hip_service_stop", "dl_hip_service_last_error",
+    "dl_hip_guide_init", "dl_hip_guide_params", "dl_hip_guide_eval", "dl_hip_guide_train_step",
 )
 
 
@@ -129,6 +130,10 @@ def lib():
             "dl_camera_perspective": (i32, [P, c.c_float, c.c_float, c.c_float, i32, i32, c.POINTER(dl_camera_desc)]),
             "dl_film_filter": (None, [i32, c.c_float, c.c_float, c.c_float, c.POINTER(dl_film_desc)]),
             "dl_mesh_tangents": (None, [P, u32, P, u32, P, P]),
+            "dl_hip_guide_init": (i32, [P, c.c_uint64, c.c_float, P]),
+            "dl_hip_guide_params": (i32, [P, P, P]),
+            "dl_hip_guide_eval": (i32, [P, P, u32, P, P]),
+            "dl_hip_guide_train_step": (i32, [P, P, P, P, u32, i32, c.POINTER(c.c_float)]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -381,6 +386,44 @@ class Backend:
         st = dl_render_stats()
         self._check(self.L.dl_hip_render(self.h, ctypes.byref(p), _p(img), ctypes.byref(st)))
         return img, st.as_dict()
+
+    # ---- learned directional proposal (dl_hip_guide_*) ------------------
+    GUIDE_PARAMS = 21808
+
+    def guide_init(self, seed=0, lr=1e-2, params=None):
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        if p is not None:
+            assert p.size == self.GUIDE_PARAMS
+        self._check(self.L.dl_hip_guide_init(self.h, int(seed), float(lr), _p(p)))
+
+    def guide_params(self):
+        th = np.zeros(self.GUIDE_PARAMS, np.float32)
+        g = np.zeros(self.GUIDE_PARAMS, np.float32)
+        self._check(self.L.dl_hip_guide_params(self.h, _p(th), _p(g)))
+        return th, g
+
+    def guide_eval(self, x):
+        """Raw outputs (n, 48) and mixtures (n, 40) for features x (n, 96)."""
+        x = np.ascontiguousarray(x, np.float32)
+        n = x.shape[0]
+        bx, bo, bm = DeviceBuffer(self, x.nbytes), DeviceBuffer(self, n * 48 * 4), DeviceBuffer(self, n * 40 * 4)
+        bx.upload(x)
+        self._check(self.L.dl_hip_guide_eval(self.h, bx.ptr, n, bo.ptr, bm.ptr))
+        return bo.download(np.float32, n * 48).reshape(n, 48), bm.download(np.float32, n * 40).reshape(n, 40)
+
+    def guide_train_step(self, x, dirs, weights, apply=True):
+        x = np.ascontiguousarray(x, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        w = np.ascontiguousarray(weights, np.float32)
+        n = x.shape[0]
+        bx, bd, bw = DeviceBuffer(self, x.nbytes), DeviceBuffer(self, d.nbytes), DeviceBuffer(self, w.nbytes)
+        bx.upload(x)
+        bd.upload(d)
+        bw.upload(w)
+        loss = ctypes.c_float()
+        self._check(self.L.dl_hip_guide_train_step(self.h, bx.ptr, bd.ptr, bw.ptr, n, int(bool(apply)),
+                                                   ctypes.byref(loss)))
+        return float(loss.value)
 
     def render_paths(self, sample, seed=0, max_depth=None, rr_depth=None):
         d = self.desc

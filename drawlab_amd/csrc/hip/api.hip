@@ -111,6 +111,7 @@ dl_status dl_hip_destroy(void* h) {
     if (!c) return DL_ERR_INVALID;
     hipSetDevice(c->device);
     free_render_state(c);
+    free_guide(c);
     free_scene(c);
     if (c->stack_ovf) dfree(c, c->stack_ovf);
     if (c->work) dfree(c, c->work);
@@ -405,6 +406,7 @@ dl_status dl_hip_trim(void* h) {
     if (!c) return DL_ERR_INVALID;
     (void)hipSetDevice(c->device);
     free_render_state(c);
+    free_guide(c);
     free_scene(c);
     return DL_OK;
 }

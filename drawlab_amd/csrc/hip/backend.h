@@ -98,6 +98,7 @@ struct Ctx {
     std::vector<void*> render_allocs;
     size_t render_capacity = 0;
     void* rs = nullptr;  // RenderState*
+    void* guide = nullptr;  // GuideState* (learned directional proposal)
 
     // device memory held by this context (every allocation goes through
     // dmalloc / dfree): current and peak bytes
@@ -144,6 +145,11 @@ dl_status render(Ctx* c, const dl_render_params* p, float* rgb_host, dl_render_s
                  float* paths_out);
 void free_render_state(Ctx* c);
 void invalidate_film(Ctx* c);  // the next render clears the film even with DL_RENDER_ACCUMULATE
+
+// guide.hip
+struct GuideState;
+void free_guide(Ctx* c);
+dl_status guide_init(Ctx* c, uint64_t seed, float lr, const float* params);
 
 // bvh_gpu.hip
 dl_status build_bvh_gpu(Ctx* c, dl_accel_stats* stats, int method);

@@ -1,0 +1,529 @@
+// Learned directional proposal: network kernels on the matrix cores and the
+// dl_hip_guide_* entry points.
+//
+// Every GEMM of the 96-64-64-64-64-48 network runs as 32x32x16 bfloat16 MFMA
+// tiles with float32 accumulation (v_mfma_f32_32x32x16_bf16).  Operand lane
+// layout (checked on the device): lane l holds A[l%32][(l/32)*8 + 0..7] and
+// B[(l/32)*8 + 0..7][l%32]; accumulator register r of lane l is
+// D[8*(r/4) + 4*(l/32) + (r%4)][l%32].
+//
+//   forward     a wave takes 64 rows (two 32-row blocks); activations live
+//               in the wave's LDS rows (bfloat16, in place layer by layer);
+//               weights are read from global memory (43 KB, L2-resident) as
+//               B operands from W[out][in]
+//   backward    dW = g^T act (tiles of dW per wave, batch chunks reduced by
+//               float32 atomics), dAct = g W (B operands from W^T), ReLU
+//               masks from the stored bfloat16 activations
+//   Adam        float32 master weights; the bfloat16 copies (W and W^T) are
+//               refreshed in the same kernel
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "backend.h"
+#include "guide.h"
+
+namespace dl {
+
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+namespace {
+
+constexpr int kWaves = 4;             // waves per block of the network kernels
+constexpr int kRowsPerWave = 64;
+constexpr int kWSlot = 64 * 96;       // bf16 elements reserved per layer (W or W^T)
+
+__device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 pack8(const float* v) {
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 8; j++) r[j] = (short)f2bf(v[j]);
+    return r;
+}
+
+struct Net {
+    const uint16_t* w;   // W[l]: [64][in(l)] bf16 (layer 4 rows 48..63 zero), slot kWSlot
+    const uint16_t* wt;  // W^T[l]: [in(l)][64] bf16
+    const float* bias;   // [5][64] (layer 4 entries 48..63 zero)
+};
+
+// One layer for the wave's 64 LDS rows (stride kGRowStride): out = act W^T
+// + b.  Hidden layers write ReLU(out) back in place as bfloat16; the last
+// layer calls emit(row, col, value) for col < 48.
+template <typename Emit>
+__device__ __forceinline__ void layer(const Net& net, int l, uint16_t* s, Emit emit) {
+    const int lane = threadIdx.x & 63;
+    const int K = glayer_in(l);
+    const uint16_t* W = net.w + l * kWSlot;
+    const float* bias = net.bias + l * 64;
+#pragma unroll 1
+    for (int rb = 0; rb < 2; rb++) {
+        f32x16 acc0 = {0}, acc1 = {0};
+        const uint16_t* arow = s + (rb * 32 + (lane & 31)) * kGRowStride + (lane >> 5) * 8;
+        const uint16_t* b0 = W + (lane & 31) * K + (lane >> 5) * 8;
+        const uint16_t* b1 = W + (32 + (lane & 31)) * K + (lane >> 5) * 8;
+        for (int kt = 0; kt < K / 16; kt++) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow + kt * 16);
+            acc0 = mfma(a, *reinterpret_cast<const bf16x8*>(b0 + kt * 16), acc0);
+            acc1 = mfma(a, *reinterpret_cast<const bf16x8*>(b1 + kt * 16), acc1);
+        }
+        const int c0 = lane & 31, c1 = 32 + (lane & 31);
+        const float bb0 = bias[c0], bb1 = bias[c1];
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int row = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const float v0 = acc0[r] + bb0, v1 = acc1[r] + bb1;
+            if (l < 4) {
+                s[row * kGRowStride + c0] = f2bf(fmaxf(v0, 0.0f));
+                s[row * kGRowStride + c1] = f2bf(fmaxf(v1, 0.0f));
+            } else {
+                emit(row, c0, v0);
+                if (c1 < kGOut) emit(row, c1, v1);
+            }
+        }
+    }
+}
+
+// Forward pass of rows [row0, row0 + 64) whose bfloat16 inputs are already
+// in the wave's LDS rows.  keep: copy each layer's input rows to act[l].
+template <typename Emit>
+__device__ __forceinline__ void forward_rows(const Net& net, uint16_t* s, uint32_t row0, uint32_t n, bool keep,
+                                             uint16_t* const* act, Emit emit) {
+    const int lane = threadIdx.x & 63;
+    for (int l = 0; l < 5; l++) {
+        if (keep) {
+            const int K = glayer_in(l);
+            const uint32_t row = row0 + lane;
+            if (row < n) {
+                const uint4* src = reinterpret_cast<const uint4*>(s + lane * kGRowStride);
+                uint4* dst = reinterpret_cast<uint4*>(act[l] + (size_t)row * K);
+                for (int q = 0; q < K / 8; q++) dst[q] = src[q];
+            }
+        }
+        layer(net, l, s, emit);
+    }
+}
+
+// Network evaluation / training forward on float32 inputs x (n x 96).
+__global__ __launch_bounds__(kWaves * 64) void k_mlp_forward(Net net, const float* __restrict__ x, uint32_t n,
+                                                             float* __restrict__ out, int keep, uint16_t* act0,
+                                                             uint16_t* act1, uint16_t* act2, uint16_t* act3,
+                                                             uint16_t* act4) {
+    __shared__ uint16_t s_act[kWaves][kRowsPerWave * kGRowStride];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint16_t* s = s_act[wave];
+    uint16_t* const act[5] = {act0, act1, act2, act3, act4};
+    for (uint32_t tile = blockIdx.x * kWaves + wave; (size_t)tile * kRowsPerWave < n; tile += gridDim.x * kWaves) {
+        const uint32_t row0 = tile * kRowsPerWave;
+        const uint32_t row = row0 + lane;
+        float f[8];
+        for (int q = 0; q < kGIn / 8; q++) {
+            for (int j = 0; j < 8; j++) f[j] = row < n ? x[(size_t)row * kGIn + q * 8 + j] : 0.0f;
+            *reinterpret_cast<bf16x8*>(s + lane * kGRowStride + q * 8) = pack8(f);
+        }
+        forward_rows(net, s, row0, n, keep != 0, act, [&](int r, int c, float v) {
+            if (row0 + r < n) out[(size_t)(row0 + r) * kGOut + c] = v;
+        });
+    }
+}
+
+// Raw outputs -> mixture parameters (n x 40).
+__global__ void k_mlp_decode(const float* __restrict__ out, uint32_t n, float* __restrict__ mix) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float o[kGOut], m[kGMix];
+        for (int j = 0; j < kGOut; j++) o[j] = out[(size_t)i * kGOut + j];
+        guide_decode(o, m);
+        for (int j = 0; j < kGMix; j++) mix[(size_t)i * kGMix + j] = m[j];
+    }
+}
+
+__global__ void k_sum(const float* __restrict__ c, uint32_t n, float* __restrict__ total) {
+    float acc = 0.0f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += c[i];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(total, acc);
+}
+
+// dLoss/dout for loss = sum(-c log q(d)) / sum(c); loss accumulated in *loss.
+__global__ void k_loss_grad(const float* __restrict__ out, const float* __restrict__ dirs,
+                            const float* __restrict__ c, uint32_t n, const float* __restrict__ total,
+                            float* __restrict__ g, float* __restrict__ loss) {
+    const float invC = 1.0f / fmaxf(*total, 1e-30f);
+    float lacc = 0.0f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float o[kGOut];
+        for (int j = 0; j < kGOut; j++) o[j] = out[(size_t)i * kGOut + j];
+        float mix[kGMix];
+        guide_decode(o, mix);
+        const vec3 d = v3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
+        float f[kGLobes], dt[kGLobes], q = 0.0f;
+        for (int j = 0; j < kGLobes; j++) {
+            const vec3 mu = v3(mix[5 * j + 2], mix[5 * j + 3], mix[5 * j + 4]);
+            dt[j] = dot(mu, d);
+            f[j] = vmf_pdf(mix[5 * j + 1], mu, d);
+            q += mix[5 * j] * f[j];
+        }
+        const float ci = c[i];
+        const float qs = fmaxf(q, 1e-30f);
+        lacc += -ci * logf(qs) * invC;
+        const float gq = -ci / qs * invC;
+        float* gi = g + (size_t)i * kGOut;
+        for (int j = 0; j < kGLobes; j++) {
+            const float w = mix[5 * j], k = mix[5 * j + 1];
+            const float wf = w * f[j];
+            gi[5 * j] = gq * (wf - w * q);
+            const float kr = o[5 * j + 1];
+            const bool inside = kr > kGKappaLo && kr < kGKappaHi;
+            const float em = -expm1f(-2.0f * k);
+            const float dlogf = 1.0f / k - 2.0f * expf(-2.0f * k) / em + (dt[j] - 1.0f);
+            gi[5 * j + 1] = inside ? gq * wf * dlogf * k : 0.0f;
+            const float mx = o[5 * j + 2], my = o[5 * j + 3], mz = o[5 * j + 4];
+            const float nm = fmaxf(sqrtf(mx * mx + my * my + mz * mz), 1e-6f);
+            const float s = gq * wf * k / nm;
+            gi[5 * j + 2] = s * (d.x - dt[j] * mix[5 * j + 2]);
+            gi[5 * j + 3] = s * (d.y - dt[j] * mix[5 * j + 3]);
+            gi[5 * j + 4] = s * (d.z - dt[j] * mix[5 * j + 4]);
+        }
+        for (int j = kGMix; j < kGOut; j++) gi[j] = 0.0f;
+    }
+    for (int off = 32; off > 0; off >>= 1) lacc += __shfl_down(lacc, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, lacc);
+}
+
+// dW[l] += g^T act: wave (tile o, tile i, chunk of kChunk rows).
+constexpr int kChunk = 1024;
+__global__ __launch_bounds__(64) void k_dw(const float* __restrict__ g, int gw, const uint16_t* __restrict__ act,
+                                           int K, uint32_t n, float* __restrict__ gradW) {
+    const int lane = threadIdx.x;
+    const int ot = blockIdx.x, it = blockIdx.y;
+    const uint32_t s0 = blockIdx.z * kChunk;
+    const int o = ot * 32 + (lane & 31), i = it * 32 + (lane & 31);
+    f32x16 acc = {0};
+    for (uint32_t kb = s0; kb < s0 + kChunk && kb < n; kb += 16) {
+        float av[8];
+        bf16x8 b;
+        for (int j = 0; j < 8; j++) {
+            const uint32_t srow = kb + (lane >> 5) * 8 + j;
+            const bool ok = srow < n;
+            av[j] = ok && o < gw ? g[(size_t)srow * gw + o] : 0.0f;
+            b[j] = ok ? (short)act[(size_t)srow * K + i] : (short)0;
+        }
+        acc = mfma(pack8(av), b, acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int orow = ot * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (orow < gw) atomicAdd(&gradW[orow * K + i], acc[r]);
+    }
+}
+
+// db[l] += sum over rows of g.
+__global__ void k_db(const float* __restrict__ g, int gw, uint32_t n, float* __restrict__ gradb) {
+    const int o = threadIdx.x;
+    if (o >= gw) return;
+    float acc = 0.0f;
+    for (uint32_t s = blockIdx.x; s < n; s += gridDim.x) acc += g[(size_t)s * gw + o];
+    atomicAdd(&gradb[o], acc);
+}
+
+// g_prev = (g W) * (act > 0): rows of 64 per wave; K = gw (64 or 48), out
+// width 64 (layer input).
+__global__ __launch_bounds__(kWaves * 64) void k_dact(Net net, int l, const float* __restrict__ g, int gw,
+                                                      const uint16_t* __restrict__ act, uint32_t n,
+                                                      float* __restrict__ gprev) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint16_t* WT = net.wt + l * kWSlot;  // [in = 64][64]
+    for (uint32_t tile = blockIdx.x * kWaves + wave; (size_t)tile * kRowsPerWave < n; tile += gridDim.x * kWaves) {
+        const uint32_t row0 = tile * kRowsPerWave;
+        for (int rb = 0; rb < 2; rb++) {
+            f32x16 acc0 = {0}, acc1 = {0};
+            const uint32_t arow = row0 + rb * 32 + (lane & 31);
+            for (int kt = 0; kt < gw / 16; kt++) {
+                float av[8];
+                for (int j = 0; j < 8; j++)
+                    av[j] = arow < n ? g[(size_t)arow * gw + kt * 16 + (lane >> 5) * 8 + j] : 0.0f;
+                const bf16x8 a = pack8(av);
+                const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(WT + (lane & 31) * 64 + kt * 16 + (lane >> 5) * 8);
+                const bf16x8 b1 =
+                    *reinterpret_cast<const bf16x8*>(WT + (32 + (lane & 31)) * 64 + kt * 16 + (lane >> 5) * 8);
+                acc0 = mfma(a, b0, acc0);
+                acc1 = mfma(a, b1, acc1);
+            }
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const uint32_t row = row0 + rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (row >= n) continue;
+                const int c0 = lane & 31, c1 = 32 + (lane & 31);
+                const bool m0 = act[(size_t)row * 64 + c0] != 0 && !(act[(size_t)row * 64 + c0] & 0x8000);
+                const bool m1 = act[(size_t)row * 64 + c1] != 0 && !(act[(size_t)row * 64 + c1] & 0x8000);
+                gprev[(size_t)row * 64 + c0] = m0 ? acc0[r] : 0.0f;
+                gprev[(size_t)row * 64 + c1] = m1 ? acc1[r] : 0.0f;
+            }
+        }
+    }
+}
+
+// Adam on the float32 masters; refreshes the bfloat16 W, W^T and biases.
+__global__ void k_adam(float* __restrict__ theta, float* __restrict__ m, float* __restrict__ v,
+                       float* __restrict__ grad, float lr, float c1, float c2, int apply, uint16_t* __restrict__ w,
+                       uint16_t* __restrict__ wt, float* __restrict__ bias) {
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < kGParams; idx += gridDim.x * blockDim.x) {
+        float th = theta[idx];
+        if (apply) {
+            const float gr = grad[idx];
+            const float mm = 0.9f * m[idx] + 0.1f * gr;
+            const float vv = 0.999f * v[idx] + 0.001f * gr * gr;
+            m[idx] = mm;
+            v[idx] = vv;
+            th = th - lr * (mm / c1) / (sqrtf(vv / c2) + 1e-8f);
+            theta[idx] = th;
+            grad[idx] = 0.0f;
+        }
+        int l = 4;
+        while (l > 0 && idx < glayer_w(l)) l--;
+        const int K = glayer_in(l), off = idx - glayer_w(l);
+        if (off < glayer_out(l) * K) {
+            const int o = off / K, i = off % K;
+            w[l * kWSlot + o * K + i] = f2bf(th);
+            wt[l * kWSlot + i * 64 + o] = f2bf(th);
+        } else {
+            bias[l * 64 + (off - glayer_out(l) * K)] = th;
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct GuideState {
+    float *theta = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *bias = nullptr;
+    uint16_t *w = nullptr, *wt = nullptr;
+    int step = 0;
+    float lr = 1e-2f;
+    // training buffers
+    uint32_t cap = 0;
+    uint16_t* act[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    float *out = nullptr, *g0 = nullptr, *g1 = nullptr, *scalars = nullptr;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+template <typename T>
+dl_status galloc(Ctx* c, GuideState* g, T** p, size_t count) {
+    void* q = nullptr;
+    DL_HIP_CHECK(c, dmalloc(c, &q, std::max<size_t>(1, count) * sizeof(T)));
+    g->allocs.push_back(q);
+    *p = (T*)q;
+    return DL_OK;
+}
+
+#define DL_TRY(expr)                \
+    do {                            \
+        dl_status s_ = (expr);      \
+        if (s_ != DL_OK) return s_; \
+    } while (0)
+
+Net net_of(const GuideState* g) { return Net{g->w, g->wt, g->bias}; }
+
+// Deterministic initial weights: He-uniform (last layer x0.1), zero biases,
+// uniforms from splitmix64 (Steele et al. 2014).
+void init_weights(uint64_t seed, std::vector<float>& th) {
+    th.assign(kGParams, 0.0f);
+    uint64_t x = seed;
+    auto next = [&x]() {
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (float)((z >> 40) * (1.0 / 16777216.0));
+    };
+    for (int l = 0; l < 5; l++) {
+        const float a = std::sqrt(6.0f / glayer_in(l)) * (l == 4 ? 0.1f : 1.0f);
+        for (int k = 0; k < glayer_out(l) * glayer_in(l); k++) th[glayer_w(l) + k] = (2.0f * next() - 1.0f) * a;
+    }
+}
+
+dl_status ensure_train(Ctx* c, GuideState* g, uint32_t n) {
+    if (g->cap >= n) return DL_OK;
+    for (int l = 0; l < 5; l++)
+        if (g->act[l]) {
+            dfree(c, g->act[l]);
+            g->allocs.erase(std::find(g->allocs.begin(), g->allocs.end(), (void*)g->act[l]));
+        }
+    for (float** p : {&g->out, &g->g0, &g->g1})
+        if (*p) {
+            dfree(c, *p);
+            g->allocs.erase(std::find(g->allocs.begin(), g->allocs.end(), (void*)*p));
+        }
+    g->cap = n;
+    for (int l = 0; l < 5; l++) DL_TRY(galloc(c, g, &g->act[l], (size_t)n * glayer_in(l)));
+    DL_TRY(galloc(c, g, &g->out, (size_t)n * kGOut));
+    DL_TRY(galloc(c, g, &g->g0, (size_t)n * 64));
+    DL_TRY(galloc(c, g, &g->g1, (size_t)n * 64));
+    return DL_OK;
+}
+
+}  // namespace
+
+void free_guide(Ctx* c) {
+    GuideState* g = (GuideState*)c->guide;
+    if (!g) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : g->allocs) (void)dfree(c, p);
+    delete g;
+    c->guide = nullptr;
+}
+
+GuideState* guide_state(Ctx* c) { return (GuideState*)c->guide; }
+const float* guide_bias(GuideState* g) { return g->bias; }
+
+dl_status guide_init(Ctx* c, uint64_t seed, float lr, const float* params) {
+    free_guide(c);
+    GuideState* g = new GuideState();
+    c->guide = g;
+    g->lr = lr;
+    DL_TRY(galloc(c, g, &g->theta, kGParams));
+    DL_TRY(galloc(c, g, &g->m, kGParams));
+    DL_TRY(galloc(c, g, &g->v, kGParams));
+    DL_TRY(galloc(c, g, &g->grad, kGParams));
+    DL_TRY(galloc(c, g, &g->bias, 5 * 64));
+    DL_TRY(galloc(c, g, &g->w, 5 * kWSlot));
+    DL_TRY(galloc(c, g, &g->wt, 5 * kWSlot));
+    DL_TRY(galloc(c, g, &g->scalars, 4));
+    std::vector<float> th;
+    if (params)
+        th.assign(params, params + kGParams);
+    else
+        init_weights(seed, th);
+    DL_HIP_CHECK(c, hipMemcpyAsync(g->theta, th.data(), kGParams * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    DL_HIP_CHECK(c, hipMemsetAsync(g->m, 0, kGParams * sizeof(float), c->stream));
+    DL_HIP_CHECK(c, hipMemsetAsync(g->v, 0, kGParams * sizeof(float), c->stream));
+    DL_HIP_CHECK(c, hipMemsetAsync(g->grad, 0, kGParams * sizeof(float), c->stream));
+    DL_HIP_CHECK(c, hipMemsetAsync(g->w, 0, 5 * kWSlot * sizeof(uint16_t), c->stream));
+    DL_HIP_CHECK(c, hipMemsetAsync(g->wt, 0, 5 * kWSlot * sizeof(uint16_t), c->stream));
+    DL_HIP_CHECK(c, hipMemsetAsync(g->bias, 0, 5 * 64 * sizeof(float), c->stream));
+    k_adam<<<(kGParams + 255) / 256, 256, 0, c->stream>>>(g->theta, g->m, g->v, g->grad, 0.f, 1.f, 1.f, 0, g->w,
+                                                          g->wt, g->bias);
+    DL_HIP_CHECK(c, hipGetLastError());
+    return DL_OK;
+}
+
+// Forward pass over n float32 input rows (device), raw outputs to out.
+dl_status guide_forward(Ctx* c, const float* x, uint32_t n, float* out, bool keep) {
+    GuideState* g = (GuideState*)c->guide;
+    if (n == 0) return DL_OK;
+    const uint32_t tiles = (n + kRowsPerWave - 1) / kRowsPerWave;
+    const int grid = (int)std::max<uint32_t>(1, std::min<uint32_t>((tiles + kWaves - 1) / kWaves, (uint32_t)c->num_cus * 4));
+    k_mlp_forward<<<grid, kWaves * 64, 0, c->stream>>>(net_of(g), x, n, out, keep ? 1 : 0, g->act[0], g->act[1],
+                                                       g->act[2], g->act[3], g->act[4]);
+    DL_HIP_CHECK(c, hipGetLastError());
+    return DL_OK;
+}
+
+// One training step on n records (device arrays): gradient, then Adam.
+dl_status guide_train(Ctx* c, const float* x, const float* dirs, const float* cw, uint32_t n, bool apply) {
+    GuideState* g = (GuideState*)c->guide;
+    DL_TRY(ensure_train(c, g, n));
+    DL_HIP_CHECK(c, hipMemsetAsync(g->scalars, 0, 4 * sizeof(float), c->stream));
+    DL_TRY(guide_forward(c, x, n, g->out, true));
+    const int eg = std::max(1, std::min<int>((int)((n + 255) / 256), c->num_cus * 8));
+    k_sum<<<eg, 256, 0, c->stream>>>(cw, n, g->scalars);
+    k_loss_grad<<<eg, 256, 0, c->stream>>>(g->out, dirs, cw, n, g->scalars, g->g0, g->scalars + 1);
+    float* gcur = g->g0;
+    float* gnext = g->g1;
+    int gw = kGOut;
+    const uint32_t chunks = (n + kChunk - 1) / kChunk;
+    for (int l = 4; l >= 0; l--) {
+        const int K = glayer_in(l);
+        k_dw<<<dim3((gw + 31) / 32, K / 32, chunks), 64, 0, c->stream>>>(gcur, gw, g->act[l], K, n,
+                                                                          g->grad + glayer_w(l));
+        k_db<<<std::min<uint32_t>(n, 1024u), 64, 0, c->stream>>>(gcur, gw, n, g->grad + glayer_b(l));
+        if (l > 0) {
+            const uint32_t tiles = (n + kRowsPerWave - 1) / kRowsPerWave;
+            const int grid = (int)std::max<uint32_t>(
+                1, std::min<uint32_t>((tiles + kWaves - 1) / kWaves, (uint32_t)c->num_cus * 4));
+            k_dact<<<grid, kWaves * 64, 0, c->stream>>>(net_of(g), l, gcur, gw, g->act[l], n, gnext);
+            std::swap(gcur, gnext);
+            gw = 64;
+        }
+    }
+    DL_HIP_CHECK(c, hipGetLastError());
+    if (apply) {
+        g->step++;
+        const float c1 = 1.0f - std::pow(0.9f, (float)g->step), c2 = 1.0f - std::pow(0.999f, (float)g->step);
+        k_adam<<<(kGParams + 255) / 256, 256, 0, c->stream>>>(g->theta, g->m, g->v, g->grad, g->lr, c1, c2, 1, g->w,
+                                                              g->wt, g->bias);
+        DL_HIP_CHECK(c, hipGetLastError());
+    }
+    return DL_OK;
+}
+
+dl_status guide_decode_rows(Ctx* c, const float* out, uint32_t n, float* mix) {
+    if (n == 0) return DL_OK;
+    k_mlp_decode<<<std::max(1, std::min<int>((int)((n + 255) / 256), c->num_cus * 8)), 256, 0, c->stream>>>(out, n,
+                                                                                                        mix);
+    DL_HIP_CHECK(c, hipGetLastError());
+    return DL_OK;
+}
+
+}  // namespace dl
+
+using namespace dl;
+
+extern "C" {
+
+dl_status dl_hip_guide_init(void* h, uint64_t seed, float lr, const float* params) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    return guide_init(c, seed, lr, params);
+}
+
+dl_status dl_hip_guide_params(void* h, float* params, float* grads) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    GuideState* g = (GuideState*)c->guide;
+    if (!g) return set_error(c, DL_ERR_STATE, "guide not initialised");
+    (void)hipSetDevice(c->device);
+    if (params)
+        DL_HIP_CHECK(c, hipMemcpyAsync(params, g->theta, kGParams * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (grads)
+        DL_HIP_CHECK(c, hipMemcpyAsync(grads, g->grad, kGParams * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return DL_OK;
+}
+
+dl_status dl_hip_guide_eval(void* h, const float* x, uint32_t n, float* out, float* mix) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !x || !out) return DL_ERR_INVALID;
+    if (!c->guide) return set_error(c, DL_ERR_STATE, "guide not initialised");
+    (void)hipSetDevice(c->device);
+    DL_TRY(guide_forward(c, x, n, out, false));
+    if (mix) DL_TRY(guide_decode_rows(c, out, n, mix));
+    DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return DL_OK;
+}
+
+dl_status dl_hip_guide_train_step(void* h, const float* x, const float* dirs, const float* weights, uint32_t n,
+                                  int apply, float* loss) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !x || !dirs || !weights) return DL_ERR_INVALID;
+    GuideState* g = (GuideState*)c->guide;
+    if (!g) return set_error(c, DL_ERR_STATE, "guide not initialised");
+    if (n == 0) return set_error(c, DL_ERR_INVALID, "empty batch");
+    (void)hipSetDevice(c->device);
+    DL_TRY(guide_train(c, x, dirs, weights, n, apply != 0));
+    if (loss) DL_HIP_CHECK(c, hipMemcpyAsync(loss, g->scalars + 1, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return DL_OK;
+}
+
+}  // extern "C"

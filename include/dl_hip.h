@@ -237,6 +237,27 @@ dl_status dl_hip_set_camera(void* ctx, const dl_camera_desc* camera);
    pressure.) */
 dl_status dl_hip_trim(void* ctx);
 
+/* ---- learned directional proposal ----------------------------------------
+ * A mixture of 8 von Mises-Fisher lobes per shading point predicted by a
+ * 96-64-64-64-64-48 ReLU MLP (bfloat16 weights and activations, float32
+ * accumulation on the matrix cores), trained online (KL divergence from the
+ * incident radiance, Adam).  Rendering with DL_RENDER_GUIDED samples
+ * directions from it in one-sample MIS with BSDF sampling (50/50).  Inputs
+ * and outputs below are device arrays (dl_hip_malloc): x is n x 96 features
+ * (layout of oracle/guide.py), out n x 48 raw outputs, mix n x 40 (w, kappa,
+ * mu per lobe), dirs n x 3 unit directions, weights n training weights.
+ * (No reference counterpart: the reference has no path guiding.) */
+/* Resets the network: params (21,808 floats, host) or seeded initial weights. */
+dl_status dl_hip_guide_init(void* ctx, uint64_t seed, float learning_rate, const float* params);
+/* Copies the float32 parameters and the accumulated gradient (host arrays of
+   21,808 floats; either may be NULL). */
+dl_status dl_hip_guide_params(void* ctx, float* params, float* grads);
+dl_status dl_hip_guide_eval(void* ctx, const float* x, uint32_t n, float* out, float* mix);
+/* One step: loss = sum(-w log q(d)) / sum(w); apply = 0 accumulates the
+   gradient without updating (for inspection). */
+dl_status dl_hip_guide_train_step(void* ctx, const float* x, const float* dirs, const float* weights, uint32_t n,
+                                  int apply, float* loss);
+
 /* ---- render service ----------------------------------------------------
  * Many render jobs on one device: scenes are registered once (their arrays
  * copied), jobs name a scene and render parameters and run on `lanes`
