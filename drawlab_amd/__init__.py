@@ -90,7 +90,7 @@ class dl_render_params(ctypes.Structure):
 class dl_render_stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
         "paths", "closest_rays", "shadow_rays", "invalid_samples", "ms_total", "ms_closest", "ms_shadow", "ms_shade",
-        "ms_film", "ms_other", "launches_closest", "launches_shadow", "nodes_closest", "tris_closest")]
+        "ms_film", "ms_other", "launches_closest", "launches_shadow", "nodes_closest", "tris_closest", "ms_guide")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -375,12 +375,13 @@ class Backend:
         return p
 
     def render(self, spp=None, seed=0, max_depth=None, rr_depth=None, spp_per_pass=0, want_image=True,
-               count_traversal=False, first_sample=0, accumulate=False):
+               count_traversal=False, first_sample=0, accumulate=False, guided=False, guide_frozen=False):
         d = self.desc
         spp = d.spp if spp is None else spp
         max_depth = d.max_depth if max_depth is None else max_depth
         rr_depth = d.rr_depth if rr_depth is None else rr_depth
-        flags = (1 if count_traversal else 0) | (2 if accumulate else 0)
+        flags = (1 if count_traversal else 0) | (2 if accumulate else 0) | (4 if guided else 0) | \
+            (8 if guide_frozen else 0)
         p = self._params(spp, seed, max_depth, rr_depth, spp_per_pass, first_sample, flags)
         img = np.zeros((d.camera.height, d.camera.width, 3), np.float32) if want_image else None
         st = dl_render_stats()

@@ -191,6 +191,19 @@ dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
         vo += md.num_vertices;
         fo += md.num_triangles;
     }
+    // scene bounds (network inputs use positions normalised to them)
+    {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (size_t i = 0; i < c->tri_verts.size(); i += 3)
+            for (int k = 0; k < 3; k++) {
+                lo[k] = std::min(lo[k], c->tri_verts[i + k]);
+                hi[k] = std::max(hi[k], c->tri_verts[i + k]);
+            }
+        for (int k = 0; k < 3; k++) {
+            c->bbox_lo[k] = lo[k];
+            c->bbox_inv[k] = hi[k] > lo[k] ? 1.0f / (hi[k] - lo[k]) : 1.0f;
+        }
+    }
     c->mats.assign(sd->materials, sd->materials + sd->num_materials);
     c->light_recs.assign(sd->num_lights, LightRec{});
     c->env_light = -1;

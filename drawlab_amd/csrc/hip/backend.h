@@ -90,6 +90,7 @@ struct Ctx {
     dl_camera_desc cam{};
     dl_film_desc film{};
     float bg[3] = {0, 0, 0};
+    float bbox_lo[3] = {0, 0, 0}, bbox_inv[3] = {1, 1, 1};  // scene bounds: lo, 1 / extent per axis
     DevScene ds;
     dl_accel_stats accel{};
     std::vector<void*> scene_allocs;
@@ -148,8 +149,13 @@ void invalidate_film(Ctx* c);  // the next render clears the film even with DL_R
 
 // guide.hip
 struct GuideState;
+struct Net;
 void free_guide(Ctx* c);
 dl_status guide_init(Ctx* c, uint64_t seed, float lr, const float* params);
+// bf16 weights / biases of the current network (guide must be initialised)
+void guide_net_ptrs(Ctx* c, const uint16_t** w, const uint16_t** wt, const float** bias);
+// one training step on R records; rad: final path radiance (float4 per path)
+dl_status guide_train_records(Ctx* c, const float* records, uint32_t R, const float4* rad);
 
 // bvh_gpu.hip
 dl_status build_bvh_gpu(Ctx* c, dl_accel_stats* stats, int method);

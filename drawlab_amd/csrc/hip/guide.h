@@ -49,28 +49,33 @@ __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint
 enum { kRecPos = 0, kRecWo = 3, kRecN = 6, kRecRough = 9, kRecDir = 10, kRecPdf = 13, kRecThr = 14, kRecRad = 17,
        kRecPath = 20, kRecValid = 21 };
 
-// Network input of one shading point.
-__device__ __forceinline__ void guide_features(const vec3& pn, const vec3& wo, const vec3& n, float rough,
-                                               float* f /* 96 */) {
-    const float pc[3] = {pn.x, pn.y, pn.z};
-    for (int a = 0; a < 3; a++) {
-        float scale = 3.14159265358979323846f;
-        for (int k = 0; k < kGFreq; k++) {
-            float s, c;
-            sincosf(scale * pc[a], &s, &c);
-            f[24 * a + 2 * k] = s;
-            f[24 * a + 2 * k + 1] = c;
-            scale *= 2.0f;
+// Network inputs 8q..8q+7 of one shading point (q = 0..11): for q < 9 the
+// pairs sin(2^k pi x_a), cos(2^k pi x_a) with a = q / 3 and k = 4 (q % 3)
+// .. +3; q = 9 holds wo, the shading normal and the roughness; 10 and 11
+// are zero padding.
+__device__ __forceinline__ void guide_feature_chunk(const vec3& pn, const vec3& wo, const vec3& n, float rough, int q,
+                                                    float* f /* 8 */) {
+    if (q < 9) {
+        const int a = q / 3, k0 = 4 * (q % 3);
+        const float x = a == 0 ? pn.x : a == 1 ? pn.y : pn.z;
+        for (int j = 0; j < 4; j++) {
+            float sn, cs;
+            sincosf(ldexpf(3.14159265358979323846f, k0 + j) * x, &sn, &cs);
+            f[2 * j] = sn;
+            f[2 * j + 1] = cs;
         }
+    } else if (q == 9) {
+        f[0] = wo.x;
+        f[1] = wo.y;
+        f[2] = wo.z;
+        f[3] = n.x;
+        f[4] = n.y;
+        f[5] = n.z;
+        f[6] = rough;
+        f[7] = 0.0f;
+    } else {
+        for (int j = 0; j < 8; j++) f[j] = 0.0f;
     }
-    f[72] = wo.x;
-    f[73] = wo.y;
-    f[74] = wo.z;
-    f[75] = n.x;
-    f[76] = n.y;
-    f[77] = n.z;
-    f[78] = rough;
-    for (int i = 79; i < kGIn; i++) f[i] = 0.0f;
 }
 
 // Roughness input per material: 1 for diffuse, alpha for the microfacet
@@ -156,5 +161,93 @@ __device__ __forceinline__ vec3 guide_sample(const float* mix, float u0, float u
     onb(mu, t, b);
     return normalized(add(add(scale(t, s * cp), scale(b, s * sp)), scale(mu, c)));
 }
+
+// ---- network evaluation on the matrix cores --------------------------------
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kGWaves = 4;            // waves per block of the network kernels
+constexpr int kGRowsPerWave = 64;
+constexpr int kGWSlot = 64 * 96;      // bf16 elements reserved per layer (W or W^T)
+
+__device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 pack8(const float* v) {
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 8; j++) r[j] = (short)f2bf(v[j]);
+    return r;
+}
+
+struct Net {
+    const uint16_t* w;   // W[l]: [64][in(l)] bf16 (layer 4 rows 48..63 zero), slot kGWSlot
+    const uint16_t* wt;  // W^T[l]: [in(l)][64] bf16
+    const float* bias;   // [5][64] (layer 4 entries 48..63 zero)
+};
+
+// One layer for the wave's 64 LDS rows (stride kGRowStride): out = act W^T
+// + b.  Hidden layers write ReLU(out) back in place as bfloat16; the last
+// layer calls emit(row, col, value) for col < 48.
+template <typename Emit>
+__device__ __forceinline__ void layer(const Net& net, int l, uint16_t* s, Emit emit) {
+    const int lane = threadIdx.x & 63;
+    const int K = glayer_in(l);
+    const uint16_t* W = net.w + l * kGWSlot;
+    const float* bias = net.bias + l * 64;
+#pragma unroll 1
+    for (int rb = 0; rb < 2; rb++) {
+        f32x16 acc0 = {0}, acc1 = {0};
+        const uint16_t* arow = s + (rb * 32 + (lane & 31)) * kGRowStride + (lane >> 5) * 8;
+        const uint16_t* b0 = W + (lane & 31) * K + (lane >> 5) * 8;
+        const uint16_t* b1 = W + (32 + (lane & 31)) * K + (lane >> 5) * 8;
+        for (int kt = 0; kt < K / 16; kt++) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow + kt * 16);
+            acc0 = mfma(a, *reinterpret_cast<const bf16x8*>(b0 + kt * 16), acc0);
+            acc1 = mfma(a, *reinterpret_cast<const bf16x8*>(b1 + kt * 16), acc1);
+        }
+        const int c0 = lane & 31, c1 = 32 + (lane & 31);
+        const float bb0 = bias[c0], bb1 = bias[c1];
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int row = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const float v0 = acc0[r] + bb0, v1 = acc1[r] + bb1;
+            if (l < 4) {
+                s[row * kGRowStride + c0] = f2bf(fmaxf(v0, 0.0f));
+                s[row * kGRowStride + c1] = f2bf(fmaxf(v1, 0.0f));
+            } else {
+                emit(row, c0, v0);
+                if (c1 < kGOut) emit(row, c1, v1);
+            }
+        }
+    }
+    // the next layer reads rows written by other lanes of this wave (LDS
+    // operations of one wave complete in order; this keeps the compiler
+    // from reordering them)
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Forward pass of rows [row0, row0 + 64) whose bfloat16 inputs are already
+// in the wave's LDS rows.  keep: copy each layer's input rows to act[l].
+template <typename Emit>
+__device__ __forceinline__ void forward_rows(const Net& net, uint16_t* s, uint32_t row0, uint32_t n, bool keep,
+                                             uint16_t* const* act, Emit emit) {
+    const int lane = threadIdx.x & 63;
+    __builtin_amdgcn_wave_barrier();  // input rows were written by their own lanes
+    for (int l = 0; l < 5; l++) {
+        if (keep) {
+            const int K = glayer_in(l);
+            const uint32_t row = row0 + lane;
+            if (row < n) {
+                const uint4* src = reinterpret_cast<const uint4*>(s + lane * kGRowStride);
+                uint4* dst = reinterpret_cast<uint4*>(act[l] + (size_t)row * K);
+                for (int q = 0; q < K / 8; q++) dst[q] = src[q];
+            }
+        }
+        layer(net, l, s, emit);
+    }
+}
+
 
 }  // namespace dl

@@ -26,100 +26,19 @@
 
 namespace dl {
 
-typedef short bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
 namespace {
 
-constexpr int kWaves = 4;             // waves per block of the network kernels
-constexpr int kRowsPerWave = 64;
-constexpr int kWSlot = 64 * 96;       // bf16 elements reserved per layer (W or W^T)
-
-__device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
-}
-
-__device__ __forceinline__ bf16x8 pack8(const float* v) {
-    bf16x8 r;
-#pragma unroll
-    for (int j = 0; j < 8; j++) r[j] = (short)f2bf(v[j]);
-    return r;
-}
-
-struct Net {
-    const uint16_t* w;   // W[l]: [64][in(l)] bf16 (layer 4 rows 48..63 zero), slot kWSlot
-    const uint16_t* wt;  // W^T[l]: [in(l)][64] bf16
-    const float* bias;   // [5][64] (layer 4 entries 48..63 zero)
-};
-
-// One layer for the wave's 64 LDS rows (stride kGRowStride): out = act W^T
-// + b.  Hidden layers write ReLU(out) back in place as bfloat16; the last
-// layer calls emit(row, col, value) for col < 48.
-template <typename Emit>
-__device__ __forceinline__ void layer(const Net& net, int l, uint16_t* s, Emit emit) {
-    const int lane = threadIdx.x & 63;
-    const int K = glayer_in(l);
-    const uint16_t* W = net.w + l * kWSlot;
-    const float* bias = net.bias + l * 64;
-#pragma unroll 1
-    for (int rb = 0; rb < 2; rb++) {
-        f32x16 acc0 = {0}, acc1 = {0};
-        const uint16_t* arow = s + (rb * 32 + (lane & 31)) * kGRowStride + (lane >> 5) * 8;
-        const uint16_t* b0 = W + (lane & 31) * K + (lane >> 5) * 8;
-        const uint16_t* b1 = W + (32 + (lane & 31)) * K + (lane >> 5) * 8;
-        for (int kt = 0; kt < K / 16; kt++) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow + kt * 16);
-            acc0 = mfma(a, *reinterpret_cast<const bf16x8*>(b0 + kt * 16), acc0);
-            acc1 = mfma(a, *reinterpret_cast<const bf16x8*>(b1 + kt * 16), acc1);
-        }
-        const int c0 = lane & 31, c1 = 32 + (lane & 31);
-        const float bb0 = bias[c0], bb1 = bias[c1];
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int row = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const float v0 = acc0[r] + bb0, v1 = acc1[r] + bb1;
-            if (l < 4) {
-                s[row * kGRowStride + c0] = f2bf(fmaxf(v0, 0.0f));
-                s[row * kGRowStride + c1] = f2bf(fmaxf(v1, 0.0f));
-            } else {
-                emit(row, c0, v0);
-                if (c1 < kGOut) emit(row, c1, v1);
-            }
-        }
-    }
-}
-
-// Forward pass of rows [row0, row0 + 64) whose bfloat16 inputs are already
-// in the wave's LDS rows.  keep: copy each layer's input rows to act[l].
-template <typename Emit>
-__device__ __forceinline__ void forward_rows(const Net& net, uint16_t* s, uint32_t row0, uint32_t n, bool keep,
-                                             uint16_t* const* act, Emit emit) {
-    const int lane = threadIdx.x & 63;
-    for (int l = 0; l < 5; l++) {
-        if (keep) {
-            const int K = glayer_in(l);
-            const uint32_t row = row0 + lane;
-            if (row < n) {
-                const uint4* src = reinterpret_cast<const uint4*>(s + lane * kGRowStride);
-                uint4* dst = reinterpret_cast<uint4*>(act[l] + (size_t)row * K);
-                for (int q = 0; q < K / 8; q++) dst[q] = src[q];
-            }
-        }
-        layer(net, l, s, emit);
-    }
-}
-
 // Network evaluation / training forward on float32 inputs x (n x 96).
-__global__ __launch_bounds__(kWaves * 64) void k_mlp_forward(Net net, const float* __restrict__ x, uint32_t n,
+__global__ __launch_bounds__(kGWaves * 64) void k_mlp_forward(Net net, const float* __restrict__ x, uint32_t n,
                                                              float* __restrict__ out, int keep, uint16_t* act0,
                                                              uint16_t* act1, uint16_t* act2, uint16_t* act3,
                                                              uint16_t* act4) {
-    __shared__ uint16_t s_act[kWaves][kRowsPerWave * kGRowStride];
+    __shared__ uint16_t s_act[kGWaves][kGRowsPerWave * kGRowStride];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint16_t* s = s_act[wave];
     uint16_t* const act[5] = {act0, act1, act2, act3, act4};
-    for (uint32_t tile = blockIdx.x * kWaves + wave; (size_t)tile * kRowsPerWave < n; tile += gridDim.x * kWaves) {
-        const uint32_t row0 = tile * kRowsPerWave;
+    for (uint32_t tile = blockIdx.x * kGWaves + wave; (size_t)tile * kGRowsPerWave < n; tile += gridDim.x * kGWaves) {
+        const uint32_t row0 = tile * kGRowsPerWave;
         const uint32_t row = row0 + lane;
         float f[8];
         for (int q = 0; q < kGIn / 8; q++) {
@@ -233,13 +152,13 @@ __global__ void k_db(const float* __restrict__ g, int gw, uint32_t n, float* __r
 
 // g_prev = (g W) * (act > 0): rows of 64 per wave; K = gw (64 or 48), out
 // width 64 (layer input).
-__global__ __launch_bounds__(kWaves * 64) void k_dact(Net net, int l, const float* __restrict__ g, int gw,
+__global__ __launch_bounds__(kGWaves * 64) void k_dact(Net net, int l, const float* __restrict__ g, int gw,
                                                       const uint16_t* __restrict__ act, uint32_t n,
                                                       float* __restrict__ gprev) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint16_t* WT = net.wt + l * kWSlot;  // [in = 64][64]
-    for (uint32_t tile = blockIdx.x * kWaves + wave; (size_t)tile * kRowsPerWave < n; tile += gridDim.x * kWaves) {
-        const uint32_t row0 = tile * kRowsPerWave;
+    const uint16_t* WT = net.wt + l * kGWSlot;  // [in = 64][64]
+    for (uint32_t tile = blockIdx.x * kGWaves + wave; (size_t)tile * kGRowsPerWave < n; tile += gridDim.x * kGWaves) {
+        const uint32_t row0 = tile * kGRowsPerWave;
         for (int rb = 0; rb < 2; rb++) {
             f32x16 acc0 = {0}, acc1 = {0};
             const uint32_t arow = row0 + rb * 32 + (lane & 31);
@@ -289,11 +208,60 @@ __global__ void k_adam(float* __restrict__ theta, float* __restrict__ m, float* 
         const int K = glayer_in(l), off = idx - glayer_w(l);
         if (off < glayer_out(l) * K) {
             const int o = off / K, i = off % K;
-            w[l * kWSlot + o * K + i] = f2bf(th);
-            wt[l * kWSlot + i * 64 + o] = f2bf(th);
+            w[l * kGWSlot + o * K + i] = f2bf(th);
+            wt[l * kGWSlot + i * 64 + o] = f2bf(th);
         } else {
             bias[l * 64 + (off - glayer_out(l) * K)] = th;
         }
+    }
+}
+
+// Training batch from render records: features, direction and the weight
+// c = L_i(d) / p(d), with the incident radiance estimated from the path's
+// final radiance as (L_final - L_before) / T (per channel, averaged).
+__global__ void k_records_batch(const float* __restrict__ rec, uint32_t R, const float4* __restrict__ rad,
+                                float* __restrict__ x, float* __restrict__ d, float* __restrict__ c) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < R; r += gridDim.x * blockDim.x) {
+        const float* q = rec + (size_t)r * kGRecFloats;
+        float w = 0.0f;
+        vec3 dir = v3(0.0f, 0.0f, 1.0f);
+        if (q[kRecValid] != 0.0f) {
+            const uint32_t p = __float_as_uint(q[kRecPath]);
+            const float4 rf = rad[p];
+            const float fin[3] = {rf.x, rf.y, rf.z};
+            float sum = 0.0f;
+            int cnt = 0;
+            for (int k = 0; k < 3; k++) {
+                const float t = q[kRecThr + k];
+                if (t > 0.0f) {
+                    sum += (fin[k] - q[kRecRad + k]) / t;
+                    cnt++;
+                }
+            }
+            const float pdf = q[kRecPdf];
+            w = (cnt > 0 && pdf > 0.0f) ? (sum / (float)cnt) / pdf : 0.0f;
+            if (!(w >= 0.0f) || !isfinite(w)) w = 0.0f;
+            dir = v3(q[kRecDir], q[kRecDir + 1], q[kRecDir + 2]);
+            if (!isfinite(dir.x) || !isfinite(dir.y) || !isfinite(dir.z)) {
+                w = 0.0f;
+                dir = v3(0.0f, 0.0f, 1.0f);
+            }
+        }
+        const vec3 pn = v3(q[kRecPos], q[kRecPos + 1], q[kRecPos + 2]);
+        const vec3 wo = v3(q[kRecWo], q[kRecWo + 1], q[kRecWo + 2]);
+        const vec3 nn = v3(q[kRecN], q[kRecN + 1], q[kRecN + 2]);
+        for (int ch = 0; ch < kGIn / 8; ch++) {
+            float f[8];
+            if (w > 0.0f)
+                guide_feature_chunk(pn, wo, nn, q[kRecRough], ch, f);
+            else
+                for (int j = 0; j < 8; j++) f[j] = 0.0f;
+            for (int j = 0; j < 8; j++) x[(size_t)r * kGIn + ch * 8 + j] = f[j];
+        }
+        d[3 * r] = dir.x;
+        d[3 * r + 1] = dir.y;
+        d[3 * r + 2] = dir.z;
+        c[r] = w;
     }
 }
 
@@ -303,6 +271,9 @@ __global__ void k_adam(float* __restrict__ theta, float* __restrict__ m, float* 
 // host side
 // ---------------------------------------------------------------------------
 struct GuideState {
+    // training batch built from render records
+    uint32_t batch_cap = 0;
+    float *bx = nullptr, *bd = nullptr, *bc = nullptr;
     float *theta = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *bias = nullptr;
     uint16_t *w = nullptr, *wt = nullptr;
     int step = 0;
@@ -395,8 +366,8 @@ dl_status guide_init(Ctx* c, uint64_t seed, float lr, const float* params) {
     DL_TRY(galloc(c, g, &g->v, kGParams));
     DL_TRY(galloc(c, g, &g->grad, kGParams));
     DL_TRY(galloc(c, g, &g->bias, 5 * 64));
-    DL_TRY(galloc(c, g, &g->w, 5 * kWSlot));
-    DL_TRY(galloc(c, g, &g->wt, 5 * kWSlot));
+    DL_TRY(galloc(c, g, &g->w, 5 * kGWSlot));
+    DL_TRY(galloc(c, g, &g->wt, 5 * kGWSlot));
     DL_TRY(galloc(c, g, &g->scalars, 4));
     std::vector<float> th;
     if (params)
@@ -407,8 +378,8 @@ dl_status guide_init(Ctx* c, uint64_t seed, float lr, const float* params) {
     DL_HIP_CHECK(c, hipMemsetAsync(g->m, 0, kGParams * sizeof(float), c->stream));
     DL_HIP_CHECK(c, hipMemsetAsync(g->v, 0, kGParams * sizeof(float), c->stream));
     DL_HIP_CHECK(c, hipMemsetAsync(g->grad, 0, kGParams * sizeof(float), c->stream));
-    DL_HIP_CHECK(c, hipMemsetAsync(g->w, 0, 5 * kWSlot * sizeof(uint16_t), c->stream));
-    DL_HIP_CHECK(c, hipMemsetAsync(g->wt, 0, 5 * kWSlot * sizeof(uint16_t), c->stream));
+    DL_HIP_CHECK(c, hipMemsetAsync(g->w, 0, 5 * kGWSlot * sizeof(uint16_t), c->stream));
+    DL_HIP_CHECK(c, hipMemsetAsync(g->wt, 0, 5 * kGWSlot * sizeof(uint16_t), c->stream));
     DL_HIP_CHECK(c, hipMemsetAsync(g->bias, 0, 5 * 64 * sizeof(float), c->stream));
     k_adam<<<(kGParams + 255) / 256, 256, 0, c->stream>>>(g->theta, g->m, g->v, g->grad, 0.f, 1.f, 1.f, 0, g->w,
                                                           g->wt, g->bias);
@@ -420,9 +391,9 @@ dl_status guide_init(Ctx* c, uint64_t seed, float lr, const float* params) {
 dl_status guide_forward(Ctx* c, const float* x, uint32_t n, float* out, bool keep) {
     GuideState* g = (GuideState*)c->guide;
     if (n == 0) return DL_OK;
-    const uint32_t tiles = (n + kRowsPerWave - 1) / kRowsPerWave;
-    const int grid = (int)std::max<uint32_t>(1, std::min<uint32_t>((tiles + kWaves - 1) / kWaves, (uint32_t)c->num_cus * 4));
-    k_mlp_forward<<<grid, kWaves * 64, 0, c->stream>>>(net_of(g), x, n, out, keep ? 1 : 0, g->act[0], g->act[1],
+    const uint32_t tiles = (n + kGRowsPerWave - 1) / kGRowsPerWave;
+    const int grid = (int)std::max<uint32_t>(1, std::min<uint32_t>((tiles + kGWaves - 1) / kGWaves, (uint32_t)c->num_cus * 4));
+    k_mlp_forward<<<grid, kGWaves * 64, 0, c->stream>>>(net_of(g), x, n, out, keep ? 1 : 0, g->act[0], g->act[1],
                                                        g->act[2], g->act[3], g->act[4]);
     DL_HIP_CHECK(c, hipGetLastError());
     return DL_OK;
@@ -447,10 +418,10 @@ dl_status guide_train(Ctx* c, const float* x, const float* dirs, const float* cw
                                                                           g->grad + glayer_w(l));
         k_db<<<std::min<uint32_t>(n, 1024u), 64, 0, c->stream>>>(gcur, gw, n, g->grad + glayer_b(l));
         if (l > 0) {
-            const uint32_t tiles = (n + kRowsPerWave - 1) / kRowsPerWave;
+            const uint32_t tiles = (n + kGRowsPerWave - 1) / kGRowsPerWave;
             const int grid = (int)std::max<uint32_t>(
-                1, std::min<uint32_t>((tiles + kWaves - 1) / kWaves, (uint32_t)c->num_cus * 4));
-            k_dact<<<grid, kWaves * 64, 0, c->stream>>>(net_of(g), l, gcur, gw, g->act[l], n, gnext);
+                1, std::min<uint32_t>((tiles + kGWaves - 1) / kGWaves, (uint32_t)c->num_cus * 4));
+            k_dact<<<grid, kGWaves * 64, 0, c->stream>>>(net_of(g), l, gcur, gw, g->act[l], n, gnext);
             std::swap(gcur, gnext);
             gw = 64;
         }
@@ -464,6 +435,32 @@ dl_status guide_train(Ctx* c, const float* x, const float* dirs, const float* cw
         DL_HIP_CHECK(c, hipGetLastError());
     }
     return DL_OK;
+}
+
+void guide_net_ptrs(Ctx* c, const uint16_t** w, const uint16_t** wt, const float** bias) {
+    GuideState* g = (GuideState*)c->guide;
+    *w = g->w;
+    *wt = g->wt;
+    *bias = g->bias;
+}
+
+dl_status guide_train_records(Ctx* c, const float* records, uint32_t R, const float4* rad) {
+    GuideState* g = (GuideState*)c->guide;
+    if (g->batch_cap < R) {
+        for (float** p : {&g->bx, &g->bd, &g->bc})
+            if (*p) {
+                dfree(c, *p);
+                g->allocs.erase(std::find(g->allocs.begin(), g->allocs.end(), (void*)*p));
+            }
+        g->batch_cap = R;
+        DL_TRY(galloc(c, g, &g->bx, (size_t)R * kGIn));
+        DL_TRY(galloc(c, g, &g->bd, (size_t)R * 3));
+        DL_TRY(galloc(c, g, &g->bc, R));
+    }
+    k_records_batch<<<std::max(1, std::min<int>((int)((R + 255) / 256), c->num_cus * 8)), 256, 0, c->stream>>>(
+        records, R, rad, g->bx, g->bd, g->bc);
+    DL_HIP_CHECK(c, hipGetLastError());
+    return guide_train(c, g->bx, g->bd, g->bc, R, true);
 }
 
 dl_status guide_decode_rows(Ctx* c, const float* out, uint32_t n, float* mix) {

@@ -27,6 +27,7 @@
 #include <cstring>
 
 #include "backend.h"
+#include "guide.h"
 #include "shading.h"
 #include "trace.h"
 
@@ -64,6 +65,8 @@ struct RenderBufs {
     Counters* cnt;
     float4* film;
     float* rgb;
+    float* gmix;  // guided rendering: mixture (40 floats) per entry of the queue being shaded
+    float* grec;  // guided rendering: training records of the pass
 };
 
 struct RenderState {
@@ -72,6 +75,7 @@ struct RenderState {
     RenderBufs b{};
     bool film_valid = false;       // film holds the accumulation of earlier calls
     bool types_present[8] = {false, false, false, false, false, false, false, false};
+    uint32_t gmix_cap = 0, grec_cap = 0;  // guided rendering buffers (entries)
     Counters* cnt_host = nullptr;  // pinned
     std::vector<void*> allocs;
     std::vector<hipEvent_t> evs;
@@ -164,6 +168,16 @@ __device__ __forceinline__ uint32_t queue_total(const uint32_t* counts) {
     for (int t = 0; t < kTypes; t++) s += counts[t];
     return s;
 }
+
+// Learned proposal during a pass: mixtures of the queue being shaded,
+// training records (nullptr: not training) and the scene bounds.
+struct GuideView {
+    const float* mix;
+    float* rec;
+    uint32_t rec_stride;  // paths p with p % rec_stride == 0 are recorded
+    uint32_t rec_cap;     // records
+    float lo[3], inv[3];
+};
 
 struct PassParams {
     uint32_t P;           // paths in this pass
@@ -370,8 +384,9 @@ __global__ __launch_bounds__(kStageBlock) void k_post_camera(PassParams pp, Scen
 // (path.cpp:32-69).
 // One kernel instantiation per BSDF type (the material-sorted queue of that
 // type), so each carries only its own BSDF's code and registers.
-template <int TYPE>
-__global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView s, RenderBufs rs, int cur) {
+template <int TYPE, bool GUIDED>
+__global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView s, RenderBufs rs, int cur,
+                                                       GuideView gv) {
     const uint32_t n = rs.cnt->mat[cur][TYPE];
     const uint32_t* __restrict__ queue = rs.matq[cur] + (size_t)TYPE * pp.cap;
     const uint32_t WH = pp.W * pp.H;
@@ -404,6 +419,7 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                 make_its(s, prim, h.z, h.w, type == DL_BSDF_ANISO_GGX, its);
                 const int mat = s.meshes[its.mesh].material;
                 const float* prm = s.mats[mat].params;
+                const float* gm = GUIDED ? gv.mix + (size_t)i * kGMix : nullptr;
                 uint32_t flags = 0;
                 // ---- emitter sampling (Scene::sampleEmitterDirection, scene.cpp:139-170)
                 float lx = rng.next_float();
@@ -432,7 +448,10 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                     q.eta = 1.f;
                     q.measure = kMeasureSolidAngle;
                     const vec3 bv = bsdf_eval(type, prm, q);
-                    const float bp = bsdf_pdf(type, prm, q);
+                    float bp = bsdf_pdf(type, prm, q);
+                    // with the proposal, directions come from the 50/50 mixture
+                    // of BSDF and proposal sampling: its pdf enters the MIS
+                    if (GUIDED) bp = (1.0f - kGuideFraction) * bp + kGuideFraction * guide_pdf(gm, ds.d);
                     const float mis = ds.delta ? 1.0f : mis_weight(ds.pdf, bp);
                     const vec3 contrib = mul(mul(scale(thr, mis), bv), spectrum);
                     if (csum(spectrum) > 0) {
@@ -451,18 +470,78 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                 br.wo = v3(0, 0, 0);
                 br.eta = 1.f;
                 br.measure = kMeasureUnknown;
-                const float bx = rng.next_float();
-                const float by = rng.next_float();
-                const vec3 fr = bsdf_sample(type, prm, br, bx, by);
+                vec3 fr;
+                float prev_pdf;
+                vec3 wd = v3(0.0f, 0.0f, 0.0f);
+                if (GUIDED) {
+                    // one-sample MIS: proposal or BSDF with probability 1/2
+                    // each; the weight divides by the mixture density
+                    const float us = rng.next_float();
+                    const float bx = rng.next_float();
+                    const float by = rng.next_float();
+                    if (us < kGuideFraction) {
+                        wd = guide_sample(gm, bx, by);
+                        br.wo = to_local(its.sh, wd);
+                        br.measure = kMeasureSolidAngle;
+                        br.eta = 1.0f;
+                        const float pb = bsdf_pdf(type, prm, br);
+                        const float pm = (1.0f - kGuideFraction) * pb + kGuideFraction * guide_pdf(gm, wd);
+                        fr = (br.wo.z > 0.0f && pm > 0.0f) ? scale(bsdf_eval(type, prm, br), br.wo.z / pm)
+                                                           : v3(0.0f, 0.0f, 0.0f);
+                        prev_pdf = pm;
+                    } else {
+                        const vec3 f0 = bsdf_sample(type, prm, br, bx, by);
+                        wd = to_world(its.sh, br.wo);
+                        const float pb = bsdf_pdf(type, prm, br);
+                        const float pm = (1.0f - kGuideFraction) * pb + kGuideFraction * guide_pdf(gm, wd);
+                        fr = pm > 0.0f ? scale(f0, pb / pm) : v3(0.0f, 0.0f, 0.0f);
+                        prev_pdf = pm;
+                    }
+                } else {
+                    const float bx = rng.next_float();
+                    const float by = rng.next_float();
+                    fr = bsdf_sample(type, prm, br, bx, by);
+                    prev_pdf = bsdf_pdf(type, prm, br);
+                }
                 thr = mul(thr, fr);
                 eta *= br.eta;
-                const float prev_pdf = bsdf_pdf(type, prm, br);
                 if (bsdf_is_diffuse(type)) flags |= kFlagDiffuse;
                 if (!is_zero(thr)) {
                     rs.ray_o[p] = f4(its.p, kEpsilon);
                     rs.ray_d[p] = f4(to_world(its.sh, br.wo), INFINITY);
                     flags |= kFlagCont;
                     want_cont = true;
+                }
+                aux.w = 0;
+                if (GUIDED && gv.rec && want_cont) {
+                    // training record of this vertex (its radiance before the
+                    // continuation is completed by k_post)
+                    const uint32_t depth = (uint32_t)aux.z;
+                    const uint32_t slot = (p / gv.rec_stride) * kGRecDepth + depth;
+                    if (depth < (uint32_t)kGRecDepth && p % gv.rec_stride == 0 && slot < gv.rec_cap) {
+                        float* r = gv.rec + (size_t)slot * kGRecFloats;
+                        const vec3 wo = neg(normalized(d));
+                        r[kRecPos] = (its.p.x - gv.lo[0]) * gv.inv[0];
+                        r[kRecPos + 1] = (its.p.y - gv.lo[1]) * gv.inv[1];
+                        r[kRecPos + 2] = (its.p.z - gv.lo[2]) * gv.inv[2];
+                        r[kRecWo] = wo.x;
+                        r[kRecWo + 1] = wo.y;
+                        r[kRecWo + 2] = wo.z;
+                        r[kRecN] = its.sh.n.x;
+                        r[kRecN + 1] = its.sh.n.y;
+                        r[kRecN + 2] = its.sh.n.z;
+                        r[kRecRough] = guide_roughness(type, prm);
+                        r[kRecDir] = wd.x;
+                        r[kRecDir + 1] = wd.y;
+                        r[kRecDir + 2] = wd.z;
+                        r[kRecPdf] = prev_pdf;
+                        r[kRecThr] = thr.x;
+                        r[kRecThr + 1] = thr.y;
+                        r[kRecThr + 2] = thr.z;
+                        r[kRecPath] = __uint_as_float(p);
+                        r[kRecValid] = 1.0f;
+                        aux.w = (int)slot + 1;
+                    }
                 }
                 rs.thr[p] = f4(thr, eta);
                 rs.rad[p] = f4(rad, 0.f);
@@ -492,7 +571,8 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
 
 // NEE resolution, emitter hit / miss with MIS, Russian roulette
 // (path.cpp:46-56, 71-97; miss.cu:31-42).
-__global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s, RenderBufs rs, int cur) {
+__global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s, RenderBufs rs, int cur,
+                                                      float* __restrict__ grec) {
     const uint32_t* counts = rs.cnt->mat[cur];
     const uint32_t n = queue_total(counts);
     const int next = cur ^ 1;
@@ -516,6 +596,13 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
                 const uint32_t flags = (uint32_t)aux.y;
                 vec3 rad = f3(rs.rad[p]);
                 if ((flags & kFlagNee) && rs.occ[p] == 0) rad = add(rad, f3(rs.nee[p]));
+            if (grec && aux.w > 0) {
+                // radiance before the continuation of a recorded vertex
+                float* r = grec + (size_t)(aux.w - 1) * kGRecFloats;
+                r[kRecRad] = rad.x;
+                r[kRecRad + 1] = rad.y;
+                r[kRecRad + 2] = rad.z;
+            }
                 if (flags & kFlagCont) {
                     const float4 th4 = rs.thr[p];
                     vec3 thr = f3(th4);
@@ -587,6 +674,66 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
             mq[(size_t)t * cap + slot] = pv[u];
         });
     }
+}
+
+// Proposal mixtures for the queue of material TYPE: features of each queued
+// shading point (position in the scene box, wo, shading normal, roughness),
+// the network on the matrix cores (64 points per wave), decoded mixtures
+// written per queue entry.
+template <int TYPE>
+__global__ __launch_bounds__(kGWaves * 64) void k_guide_infer(PassParams pp, SceneView s, RenderBufs rs, int cur,
+                                                              GuideView gv, Net net) {
+    __shared__ uint16_t s_act[kGWaves][kGRowsPerWave * kGRowStride];
+    const uint32_t n = rs.cnt->mat[cur][TYPE];
+    const uint32_t* __restrict__ queue = rs.matq[cur] + (size_t)TYPE * pp.cap;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint16_t* sm = s_act[wave];
+    float* smf = reinterpret_cast<float*>(sm);
+    for (uint32_t tile = blockIdx.x * kGWaves + wave; (size_t)tile * kGRowsPerWave < n; tile += gridDim.x * kGWaves) {
+        const uint32_t row0 = tile * kGRowsPerWave, i = row0 + lane;
+        vec3 pn = v3(0.f, 0.f, 0.f), wo = v3(0.f, 0.f, 0.f), nn = v3(0.f, 0.f, 0.f);
+        float rough = 0.f;
+        if (i < n) {
+            const uint32_t p = queue[i];
+            const float4 h = rs.hit[p];
+            Its its;
+            make_its(s, __float_as_uint(h.x), h.z, h.w, TYPE == DL_BSDF_ANISO_GGX, its);
+            const float* prm = s.mats[s.meshes[its.mesh].material].params;
+            pn = v3((its.p.x - gv.lo[0]) * gv.inv[0], (its.p.y - gv.lo[1]) * gv.inv[1], (its.p.z - gv.lo[2]) * gv.inv[2]);
+            wo = neg(normalized(f3(rs.ray_d[p])));
+            nn = its.sh.n;
+            rough = guide_roughness(TYPE, prm);
+        }
+        for (int q = 0; q < kGIn / 8; q++) {
+            float f[8];
+            if (i < n)
+                guide_feature_chunk(pn, wo, nn, rough, q, f);
+            else
+                for (int j = 0; j < 8; j++) f[j] = 0.0f;
+            *reinterpret_cast<bf16x8*>(sm + lane * kGRowStride + q * 8) = pack8(f);
+        }
+        // raw outputs overwrite the activation rows of their own 32-row block
+        // after that block's last MFMA (byte ranges checked: 192 B per output
+        // row vs 208 B per activation row)
+        forward_rows(net, sm, row0, n, false, nullptr, [&](int r, int c, float v) { smf[r * kGOut + c] = v; });
+        if (i < n) {
+            float o[kGOut], mix[kGMix];
+            for (int j = 0; j < kGOut; j++) o[j] = smf[lane * kGOut + j];
+            guide_decode(o, mix);
+            float4* dst = reinterpret_cast<float4*>(rs.gmix + (size_t)i * kGMix);
+            for (int j = 0; j < kGMix / 4; j++) dst[j] = make_float4(mix[4 * j], mix[4 * j + 1], mix[4 * j + 2], mix[4 * j + 3]);
+        }
+        __builtin_amdgcn_wave_barrier();  // the next tile overwrites these LDS rows
+    }
+}
+
+__global__ void k_clear_records(float* rec, uint32_t R) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < R; r += gridDim.x * blockDim.x)
+        rec[(size_t)r * kGRecFloats + kRecValid] = 0.0f;
+}
+
+__host__ __device__ inline bool is_guide_type(int t) {
+    return t == DL_BSDF_DIFFUSE || t == DL_BSDF_MICROFACET || t == DL_BSDF_CONDUCTOR || t == DL_BSDF_ANISO_GGX;
 }
 
 __device__ __forceinline__ bool sample_valid(const float4& c) {  // Color3f::isValid  color.h:50-58
@@ -772,6 +919,20 @@ dl_status ensure_state(Ctx* c, uint32_t P, uint32_t pixels) {
     return DL_OK;
 }
 
+constexpr uint32_t kGuideRecords = 1u << 18;  // training records per pass
+
+dl_status ensure_guide_bufs(Ctx* c, RenderState* rs, uint32_t cap) {
+    if (rs->gmix_cap < cap) {
+        DL_TRY(alloc(c, rs, &rs->b.gmix, (size_t)cap * kGMix));
+        rs->gmix_cap = cap;
+    }
+    if (rs->grec_cap < kGuideRecords) {
+        DL_TRY(alloc(c, rs, &rs->b.grec, (size_t)kGuideRecords * kGRecFloats));
+        rs->grec_cap = kGuideRecords;
+    }
+    return DL_OK;
+}
+
 SceneView scene_view(Ctx* c) {
     SceneView s;
     const DevScene& d = c->ds;
@@ -816,7 +977,7 @@ struct Timer {
     }
 };
 
-enum { kTagClosest = 0, kTagShadow = 1, kTagShade = 2, kTagFilm = 3, kTagOther = 4 };
+enum { kTagClosest = 0, kTagShadow = 1, kTagShade = 2, kTagFilm = 3, kTagOther = 4, kTagGuide = 5 };
 
 }  // namespace
 
@@ -838,6 +999,23 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
     for (const auto& m : c->mats) rs->types_present[m.type] = true;
     const BvhView bvh{c->ds.nodes, c->ds.tris};
     const bool count = stats && (prm->flags & DL_RENDER_COUNT_TRAVERSAL);
+    const bool guided = (prm->flags & DL_RENDER_GUIDED) != 0;
+    const bool train = guided && !(prm->flags & DL_RENDER_GUIDE_FROZEN) && paths_sample < 0;
+    GuideView gv{};
+    Net net{};
+    if (guided) {
+        if (!c->guide) DL_TRY(guide_init(c, 0x5EEDull + prm->seed, 1e-2f, nullptr));
+        DL_TRY(ensure_guide_bufs(c, rs, cap));
+        guide_net_ptrs(c, &net.w, &net.wt, &net.bias);
+        gv.mix = rs->b.gmix;
+        gv.rec = train ? rs->b.grec : nullptr;
+        gv.rec_cap = kGuideRecords;
+        for (int k = 0; k < 3; k++) {
+            gv.lo[k] = c->bbox_lo[k];
+            gv.inv[k] = c->bbox_inv[k];
+        }
+    }
+    const int guide_grid = std::max(1, c->num_cus) * 2;
     const int stage_grid = std::max(1, c->num_cus) * 8;
     const int trace_grid = (int)c->trace_blocks;
     std::vector<int> tags;
@@ -864,6 +1042,12 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
         paths += pp.P;
         closest += pp.P;
         int cur = 0;
+        if (train) {
+            // one record slot per (recorded path, depth < kGRecDepth)
+            gv.rec_stride = std::max<uint32_t>(1u, (uint32_t)(((uint64_t)pp.P * kGRecDepth + kGuideRecords - 1) /
+                                                              kGuideRecords));
+            k_clear_records<<<std::max(1, c->num_cus) * 4, 256, 0, c->stream>>>(rs->b.grec, kGuideRecords);
+        }
         k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, 0);
         k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, 1);
         DL_TRY(tm.start(kTagOther));
@@ -893,13 +1077,32 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
             DL_TRY(tm.start(kTagShade));
             for (int t = 0; t < DL_BSDF_COUNT; t++) {
                 if (!rs->types_present[t] || rs->cnt_host->mat[cur][t] == 0) continue;
+                if (guided && is_guide_type(t)) {
+                    DL_TRY(tm.stop());
+                    DL_TRY(tm.start(kTagGuide));
+                    switch (t) {
+                        case DL_BSDF_DIFFUSE: k_guide_infer<DL_BSDF_DIFFUSE><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
+                        case DL_BSDF_MICROFACET: k_guide_infer<DL_BSDF_MICROFACET><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
+                        case DL_BSDF_ANISO_GGX: k_guide_infer<DL_BSDF_ANISO_GGX><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
+                        case DL_BSDF_CONDUCTOR: k_guide_infer<DL_BSDF_CONDUCTOR><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
+                    }
+                    DL_TRY(tm.stop());
+                    DL_TRY(tm.start(kTagShade));
+                    switch (t) {
+                        case DL_BSDF_DIFFUSE: k_shade<DL_BSDF_DIFFUSE, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                        case DL_BSDF_MICROFACET: k_shade<DL_BSDF_MICROFACET, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                        case DL_BSDF_ANISO_GGX: k_shade<DL_BSDF_ANISO_GGX, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                        case DL_BSDF_CONDUCTOR: k_shade<DL_BSDF_CONDUCTOR, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                    }
+                    continue;
+                }
                 switch (t) {
-                    case DL_BSDF_DIFFUSE: k_shade<DL_BSDF_DIFFUSE><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur); break;
-                    case DL_BSDF_MIRROR: k_shade<DL_BSDF_MIRROR><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur); break;
-                    case DL_BSDF_DIELECTRIC: k_shade<DL_BSDF_DIELECTRIC><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur); break;
-                    case DL_BSDF_MICROFACET: k_shade<DL_BSDF_MICROFACET><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur); break;
-                    case DL_BSDF_ANISO_GGX: k_shade<DL_BSDF_ANISO_GGX><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur); break;
-                    case DL_BSDF_CONDUCTOR: k_shade<DL_BSDF_CONDUCTOR><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur); break;
+                    case DL_BSDF_DIFFUSE: k_shade<DL_BSDF_DIFFUSE, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                    case DL_BSDF_MIRROR: k_shade<DL_BSDF_MIRROR, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                    case DL_BSDF_DIELECTRIC: k_shade<DL_BSDF_DIELECTRIC, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                    case DL_BSDF_MICROFACET: k_shade<DL_BSDF_MICROFACET, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                    case DL_BSDF_ANISO_GGX: k_shade<DL_BSDF_ANISO_GGX, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                    case DL_BSDF_CONDUCTOR: k_shade<DL_BSDF_CONDUCTOR, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
                 }
             }
             DL_TRY(tm.stop());
@@ -913,12 +1116,17 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
             k_shadow_queue<<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
             DL_TRY(tm.stop());
             DL_TRY(tm.start(kTagShade));
-            k_post<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
+            k_post<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv.rec);
             DL_TRY(tm.stop());
             DL_HIP_CHECK(c, hipGetLastError());
             cur ^= 1;
         }
         if (paths_sample >= 0) break;
+        if (train) {
+            DL_TRY(tm.start(kTagGuide));
+            DL_TRY(guide_train_records(c, rs->b.grec, kGuideRecords, rs->b.rad));
+            DL_TRY(tm.stop());
+        }
         DL_TRY(tm.start(kTagFilm));
         if ((int)ceilf(c->film.radius) + 1 <= kFilmHalo) {
             const dim3 fgrid((W + kFilmTile - 1) / kFilmTile, (H + kFilmTile - 1) / kFilmTile);
@@ -960,6 +1168,7 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
                 case kTagShadow: stats->ms_shadow += e; stats->launches_shadow += 1; break;
                 case kTagShade: stats->ms_shade += e; break;
                 case kTagFilm: stats->ms_film += e; break;
+                case kTagGuide: stats->ms_guide += e; break;
                 default: stats->ms_other += e; break;
             }
         }

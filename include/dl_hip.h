@@ -155,8 +155,11 @@ typedef struct dl_render_params {
 
 enum {
     DL_RENDER_COUNT_TRAVERSAL = 1, /* count BVH node visits / triangle tests of closest-hit rays */
-    DL_RENDER_ACCUMULATE = 2       /* add to the film of the previous call instead of clearing it
+    DL_RENDER_ACCUMULATE = 2,      /* add to the film of the previous call instead of clearing it
                                       (progressive rendering, src/optix/cuda/integrator/path.cu:86-90) */
+    DL_RENDER_GUIDED = 4,          /* sample directions from the learned proposal too (one-sample MIS
+                                      with BSDF sampling) and train it online after every pass */
+    DL_RENDER_GUIDE_FROZEN = 8     /* with DL_RENDER_GUIDED: use the proposal without training it */
 };
 
 typedef struct dl_render_stats {
@@ -174,6 +177,7 @@ typedef struct dl_render_stats {
     double launches_shadow;
     double nodes_closest;      /* with DL_RENDER_COUNT_TRAVERSAL: node visits of closest-hit rays */
     double tris_closest;       /* with DL_RENDER_COUNT_TRAVERSAL: triangle tests of closest-hit rays */
+    double ms_guide;           /* with DL_RENDER_GUIDED: proposal inference and training kernels */
 } dl_render_stats;
 
 /* ---- context ---------------------------------------------------------- */

@@ -118,3 +118,37 @@ def paths_parity(backend, desc, samples, seed, builder=0, camera_matrices=None):
     print("bit-exact paths: %d / %d (%.5f)" % (same_all, tot, frac), flush=True)
     assert frac >= 0.999
     return frac
+
+
+def tile_means(img, t=16):
+    """Per t x t tile and channel means of an (H, W, 3) image."""
+    H, W, _ = img.shape
+    return img[:H - H % t, :W - W % t].reshape(H // t, t, W // t, t, 3).mean(axis=(1, 3))
+
+
+def tile_z(batches_a, batches_b, t=16):
+    """z = (mean_a - mean_b) / sqrt(var_a / Ka + var_b / Kb) per tile and
+    channel, with the variances taken across independent batches; returns
+    (z, degrees of freedom of the smaller side)."""
+    A = np.stack([tile_means(b, t) for b in batches_a])
+    B = np.stack([tile_means(b, t) for b in batches_b])
+    va = A.var(axis=0, ddof=1) / len(A)
+    vb = B.var(axis=0, ddof=1) / len(B)
+    return (A.mean(axis=0) - B.mean(axis=0)) / np.sqrt(va + vb + 1e-30), min(len(A), len(B)) - 1
+
+
+def assert_unbiased(zd, alpha=0.01):
+    """z from batch means follows a Student t with the batches' degrees of
+    freedom under the null.  Bonferroni over all tiles and channels for the
+    largest |z|, and a binomial tail test for the count beyond 3."""
+    from scipy import stats
+    z, dof = zd
+    m = z.size
+    zmax = stats.t.isf(alpha / (2 * m), dof)
+    p3 = 2 * stats.t.sf(3.0, dof)
+    k3 = int(np.sum(np.abs(z) > 3))
+    tail = stats.binom.sf(k3 - 1, m, p3)
+    print("tiles %d  dof %d  max |z| %.2f (limit %.2f)  beyond 3: %d (P = %.3g)" % (m, dof, np.abs(z).max(), zmax, k3,
+                                                                                    tail), flush=True)
+    assert np.abs(z).max() < zmax
+    assert tail > alpha
