@@ -238,6 +238,118 @@ def service_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def _relmse(x, y):
+    """mean over pixels and channels of (x - y)^2 / (y^2 + 1e-2)."""
+    return float(np.mean((x.astype(np.float64) - y) ** 2 / (y.astype(np.float64) ** 2 + 1e-2)))
+
+
+def _time_to(target, times, errs):
+    """Wall time at which relMSE first reaches `target`, interpolated
+    log-log between the two ladder renders that bracket it (None if the
+    ladder never reaches it)."""
+    for i, e in enumerate(errs):
+        if e <= target:
+            if i == 0:
+                return times[0]
+            t0, t1, e0, e1 = times[i - 1], times[i], errs[i - 1], errs[i]
+            if e0 <= e1:
+                return t1
+            a = (np.log(e0) - np.log(target)) / (np.log(e0) - np.log(e1))
+            return float(np.exp(np.log(t0) + a * (np.log(t1) - np.log(t0))))
+    return None
+
+
+def _err_at(t, times, errs):
+    """relMSE at wall time t, interpolated log-log on the ladder."""
+    if t <= times[0]:
+        return errs[0]
+    for i in range(1, len(times)):
+        if t <= times[i]:
+            a = (np.log(t) - np.log(times[i - 1])) / (np.log(times[i]) - np.log(times[i - 1]))
+            return float(np.exp(np.log(errs[i - 1]) + a * (np.log(errs[i]) - np.log(errs[i - 1]))))
+    return None
+
+
+def guided_bench(args):
+    """Guided against plain rendering at equal wall time.  The reference
+    is the plain HIP integrator at --ref-spp samples per pixel (its own
+    per-path parity with the CPU oracle is tested separately); each method
+    renders a ladder of sample counts from scratch -- the guided one with a
+    freshly initialised network trained online inside the timed render --
+    and the wall time of every render (including the image download) is
+    paired with its relMSE against the reference, both averaged over
+    --seeds independent renders (relMSE of one render is dominated by a few
+    rare paths and varies several-fold between seeds)."""
+    import drawlab_amd
+    from drawlab_amd import scenes
+
+    W, H = (int(v) for v in args.guided_res.lower().split("x"))
+    desc = scenes.displaced_sphere(width=W, height=H)
+    be = drawlab_amd.Backend(0)
+    be.upload_scene(desc)
+    be.build_accel(drawlab_amd.BUILD_GPU)
+    ref_seed, test_seed, net_seed = 0x5EED + 1, 7, 0x5EED
+    t0 = time.perf_counter()
+    ref, _ = be.render(spp=args.ref_spp, seed=ref_seed)
+    t_ref = time.perf_counter() - t0
+    print("reference: %dx%d x %d spp in %.1f s" % (W, H, args.ref_spp, t_ref), file=sys.stderr, flush=True)
+    ladder = [16 << i for i in range(64) if (16 << i) <= args.max_spp]
+
+    def one(spp, guided, seed):
+        be.sync()
+        t = time.perf_counter()
+        if guided:
+            be.guide_init(seed=net_seed + seed, lr=1e-2)
+        img, st = be.render(spp=spp, seed=seed, guided=guided, guide_steps=args.guide_steps)
+        be.sync()
+        return time.perf_counter() - t, img, st
+
+    one(16, False, 1)
+    one(16, True, 1)
+    seeds = [test_seed + k for k in range(args.seeds)]
+    curves = {}
+    for name, g in (("plain", False), ("guided", True)):
+        ts, es, gms, trs = [], [], [], []
+        for spp in ladder:
+            runs = [one(spp, g, s) for s in seeds]
+            ts.append(float(np.mean([r[0] for r in runs])))
+            es.append(float(np.mean([_relmse(r[1], ref) for r in runs])))
+            gms.append(float(np.mean([r[2]["ms_guide"] for r in runs])))
+            trs.append(float(np.mean([r[2]["ms_train"] for r in runs])))
+            print("%-6s %6d spp  %8.3f s  relMSE %.4e  (inference %.1f ms, training %.1f ms)" % (
+                name, spp, ts[-1], es[-1], gms[-1], trs[-1]), file=sys.stderr, flush=True)
+        curves[name] = {"spp": ladder, "seconds": [round(v, 4) for v in ts], "relmse": [float("%.5g" % v) for v in es],
+                        "ms_inference": [round(v, 2) for v in gms], "ms_training": [round(v, 2) for v in trs]}
+    target = args.target_relmse
+    tp = _time_to(target, curves["plain"]["seconds"], curves["plain"]["relmse"])
+    tg = _time_to(target, curves["guided"]["seconds"], curves["guided"]["relmse"])
+    eq = _err_at(tp, curves["guided"]["seconds"], curves["guided"]["relmse"]) if tp is not None else None
+    out = {
+        "metric": METRIC,
+        "value": eq,
+        "unit": "relMSE of the guided render at the plain render's time to relMSE %g" % target,
+        "n_gpus": 1,
+        "steps": len(ladder),
+        "warmup": 2,
+        "ms_per_step": None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (network bf16 with f32 accumulation)",
+        "data": "synthetic (seeded procedural scene generated in code)",
+        "config": {"workload": "guided_vs_plain_displaced_sphere_2M", "width": W, "height": H,
+                   "reference_spp": args.ref_spp, "reference_seconds": round(t_ref, 2), "target_relmse": target,
+                   "seeds_per_point": args.seeds,
+                   "guide_steps_per_pass": args.guide_steps or "16 for the first 256 steps, then 4"},
+        "time_to_target_s": {"plain": tp, "guided": tg,
+                             "ratio_guided_over_plain": (tg / tp) if (tp and tg) else None},
+        "curves": curves,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -252,7 +364,14 @@ def main():
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"))
     ap.add_argument("--command", default="", help="command line recorded in a summary")
     ap.add_argument("--copy-mib", type=int, default=1024, help="calibration copy size used by the PMC runs")
-    ap.add_argument("--workload", choices=("displaced_sphere", "service"), default="displaced_sphere")
+    ap.add_argument("--workload", choices=("displaced_sphere", "service", "guided"), default="displaced_sphere")
+    ap.add_argument("--guided-res", default="480x270", help="guided workload: image size WxH")
+    ap.add_argument("--ref-spp", type=int, default=65536, help="guided workload: reference samples per pixel")
+    ap.add_argument("--max-spp", type=int, default=4096, help="guided workload: largest ladder render")
+    ap.add_argument("--target-relmse", type=float, default=1e-3)
+    ap.add_argument("--seeds", type=int, default=3, help="guided workload: renders averaged per ladder point")
+    ap.add_argument("--guide-steps", type=int, default=0,
+                    help="guided workload: Adam steps per pass (0: 16 for the first 256 steps, then 4)")
     ap.add_argument("--jobs", type=int, default=64, help="service workload: number of jobs")
     ap.add_argument("--lanes", type=int, default=4, help="service workload: worker lanes")
     ap.add_argument("--policy", choices=("fifo", "sjf"), default="fifo")
@@ -269,6 +388,9 @@ def main():
         sys.exit(2)
     if args.workload == "service":
         service_bench(args)
+        return
+    if args.workload == "guided":
+        guided_bench(args)
         return
 
     import drawlab_amd

@@ -84,13 +84,14 @@ class dl_accel_stats(ctypes.Structure):
 class dl_render_params(ctypes.Structure):
     _fields_ = [("spp", ctypes.c_int32), ("seed", ctypes.c_uint32), ("max_depth", ctypes.c_int32),
                 ("rr_depth", ctypes.c_int32), ("spp_per_pass", ctypes.c_int32), ("first_sample", ctypes.c_int32),
-                ("flags", ctypes.c_int32)]
+                ("flags", ctypes.c_int32), ("guide_steps", ctypes.c_int32)]
 
 
 class dl_render_stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
         "paths", "closest_rays", "shadow_rays", "invalid_samples", "ms_total", "ms_closest", "ms_shadow", "ms_shade",
-        "ms_film", "ms_other", "launches_closest", "launches_shadow", "nodes_closest", "tris_closest", "ms_guide")]
+        "ms_film", "ms_other", "launches_closest", "launches_shadow", "nodes_closest", "tris_closest", "ms_guide",
+        "ms_train", "train_steps")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -375,7 +376,8 @@ class Backend:
         return p
 
     def render(self, spp=None, seed=0, max_depth=None, rr_depth=None, spp_per_pass=0, want_image=True,
-               count_traversal=False, first_sample=0, accumulate=False, guided=False, guide_frozen=False):
+               count_traversal=False, first_sample=0, accumulate=False, guided=False, guide_frozen=False,
+               guide_steps=0):
         d = self.desc
         spp = d.spp if spp is None else spp
         max_depth = d.max_depth if max_depth is None else max_depth
@@ -383,6 +385,7 @@ class Backend:
         flags = (1 if count_traversal else 0) | (2 if accumulate else 0) | (4 if guided else 0) | \
             (8 if guide_frozen else 0)
         p = self._params(spp, seed, max_depth, rr_depth, spp_per_pass, first_sample, flags)
+        p.guide_steps = int(guide_steps)
         img = np.zeros((d.camera.height, d.camera.width, 3), np.float32) if want_image else None
         st = dl_render_stats()
         self._check(self.L.dl_hip_render(self.h, ctypes.byref(p), _p(img), ctypes.byref(st)))

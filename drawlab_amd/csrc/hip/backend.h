@@ -155,7 +155,8 @@ dl_status guide_init(Ctx* c, uint64_t seed, float lr, const float* params);
 // bf16 weights / biases of the current network (guide must be initialised)
 void guide_net_ptrs(Ctx* c, const uint16_t** w, const uint16_t** wt, const float** bias);
 // one training step on R records; rad: final path radiance (float4 per path)
-dl_status guide_train_records(Ctx* c, const float* records, uint32_t R, const float4* rad);
+dl_status guide_train_records(Ctx* c, const float* records, uint32_t R, const float4* rad, int steps);
+int guide_step_count(Ctx* c);  // Adam steps taken since guide_init
 
 // bvh_gpu.hip
 dl_status build_bvh_gpu(Ctx* c, dl_accel_stats* stats, int method);

@@ -18,8 +18,13 @@ constexpr int kGMix = kGLobes * 5;        // w, kappa, mu.xyz per lobe
 constexpr int kGRowStride = kGIn + 8;     // bf16 activations per LDS row (16-byte pad)
 constexpr int kGRecFloats = 24;           // training record
 constexpr int kGRecDepth = 4;             // path vertices recorded per recorded path
-constexpr float kGKappaLo = -3.0f, kGKappaHi = 7.0f;
+constexpr float kGKappaLo = -3.0f, kGKappaHi = 7.0f;  // log kappa = lo + (hi - lo) sigmoid(raw)
 constexpr float kGuideFraction = 0.5f;    // one-sample MIS: guide vs BSDF sampling
+constexpr uint32_t kGuideBatch = 1u << 18;  // records per Adam step
+// Adam steps after each pass of a training render: kGuideWarmStepsPerPass
+// until the network has taken kGuideWarmSteps steps, kGuideStepsPerPass after
+constexpr int kGuideStepsPerPass = 4, kGuideWarmStepsPerPass = 16, kGuideWarmSteps = 256;
+constexpr float kGuideMaxShare = 1.0f / 1024.0f;  // largest share of a batch one record may take
 
 // Offsets of each layer's W[out][in] and b[out] in the parameter vector.
 __host__ __device__ constexpr int glayer_in(int l) { return l == 0 ? 96 : 64; }
@@ -90,9 +95,20 @@ __device__ __forceinline__ float guide_roughness(int type, const float* p) {
     }
 }
 
-__device__ __forceinline__ bool guide_type(int type) {
-    return type == DL_BSDF_DIFFUSE || type == DL_BSDF_MICROFACET || type == DL_BSDF_CONDUCTOR ||
-           type == DL_BSDF_ANISO_GGX;
+// Materials sampled with the proposal: those with a diffuse part or a GGX
+// roughness of at least kGuideMinRoughness.  For glossier lobes, BSDF
+// sampling combined with emitter sampling already follows the integrand
+// closely, and the proposal's half of the samples would mostly be wasted.
+constexpr float kGuideMinRoughness = 0.3f;
+__host__ __device__ inline bool guide_material(int type, const float* p) {
+    switch (type) {
+        case DL_BSDF_DIFFUSE: return true;
+        case DL_BSDF_MICROFACET: return p[3] > 0.0f || p[4] > 0.0f || p[5] > 0.0f || p[0] >= kGuideMinRoughness;
+        case DL_BSDF_CONDUCTOR: return p[0] >= kGuideMinRoughness;
+        case DL_BSDF_ANISO_GGX:
+            return p[0] > 0.0f || p[1] > 0.0f || p[2] > 0.0f || sqrtf(p[6] * p[7]) >= kGuideMinRoughness;
+        default: return false;
+    }
 }
 
 // Raw network outputs (48) -> mixture (w, kappa, mu) of 8 lobes.
@@ -105,11 +121,11 @@ __device__ __forceinline__ void guide_decode(const float* o, float* mix /* 40 */
         sum += e[j];
     }
     for (int j = 0; j < kGLobes; j++) {
-        const float kr = fminf(fmaxf(o[5 * j + 1], kGKappaLo), kGKappaHi);
+        const float sg = 1.0f / (1.0f + expf(-o[5 * j + 1]));
         const float mx_ = o[5 * j + 2], my = o[5 * j + 3], mz = o[5 * j + 4];
         const float nm = fmaxf(sqrtf(mx_ * mx_ + my * my + mz * mz), 1e-6f);
         mix[5 * j] = e[j] / sum;
-        mix[5 * j + 1] = expf(kr);
+        mix[5 * j + 1] = expf(kGKappaLo + (kGKappaHi - kGKappaLo) * sg);
         mix[5 * j + 2] = mx_ / nm;
         mix[5 * j + 3] = my / nm;
         mix[5 * j + 4] = mz / nm;

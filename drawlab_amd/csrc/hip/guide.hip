@@ -61,15 +61,23 @@ __global__ void k_mlp_decode(const float* __restrict__ out, uint32_t n, float* _
     }
 }
 
-__global__ void k_sum(const float* __restrict__ c, uint32_t n, float* __restrict__ total) {
+// Sum of a 256-thread block's values added to *dst with one atomic.
+__device__ __forceinline__ void block_sum_add(float v, float* dst) {
+    __shared__ float s_part[4];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(dst, s_part[0] + s_part[1] + s_part[2] + s_part[3]);
+}
+
+__global__ __launch_bounds__(256) void k_sum(const float* __restrict__ c, uint32_t n, float* __restrict__ total) {
     float acc = 0.0f;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += c[i];
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(total, acc);
+    block_sum_add(acc, total);
 }
 
 // dLoss/dout for loss = sum(-c log q(d)) / sum(c); loss accumulated in *loss.
-__global__ void k_loss_grad(const float* __restrict__ out, const float* __restrict__ dirs,
+__global__ __launch_bounds__(256) void k_loss_grad(const float* __restrict__ out, const float* __restrict__ dirs,
                             const float* __restrict__ c, uint32_t n, const float* __restrict__ total,
                             float* __restrict__ g, float* __restrict__ loss) {
     const float invC = 1.0f / fmaxf(*total, 1e-30f);
@@ -96,11 +104,10 @@ __global__ void k_loss_grad(const float* __restrict__ out, const float* __restri
             const float w = mix[5 * j], k = mix[5 * j + 1];
             const float wf = w * f[j];
             gi[5 * j] = gq * (wf - w * q);
-            const float kr = o[5 * j + 1];
-            const bool inside = kr > kGKappaLo && kr < kGKappaHi;
+            const float sg = 1.0f / (1.0f + expf(-o[5 * j + 1]));
             const float em = -expm1f(-2.0f * k);
             const float dlogf = 1.0f / k - 2.0f * expf(-2.0f * k) / em + (dt[j] - 1.0f);
-            gi[5 * j + 1] = inside ? gq * wf * dlogf * k : 0.0f;
+            gi[5 * j + 1] = gq * wf * dlogf * k * (kGKappaHi - kGKappaLo) * sg * (1.0f - sg);
             const float mx = o[5 * j + 2], my = o[5 * j + 3], mz = o[5 * j + 4];
             const float nm = fmaxf(sqrtf(mx * mx + my * my + mz * mz), 1e-6f);
             const float s = gq * wf * k / nm;
@@ -110,44 +117,78 @@ __global__ void k_loss_grad(const float* __restrict__ out, const float* __restri
         }
         for (int j = kGMix; j < kGOut; j++) gi[j] = 0.0f;
     }
-    for (int off = 32; off > 0; off >>= 1) lacc += __shfl_down(lacc, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, lacc);
+    block_sum_add(lacc, loss);
 }
 
-// dW[l] += g^T act: wave (tile o, tile i, chunk of kChunk rows).
-constexpr int kChunk = 1024;
-__global__ __launch_bounds__(64) void k_dw(const float* __restrict__ g, int gw, const uint16_t* __restrict__ act,
-                                           int K, uint32_t n, float* __restrict__ gradW) {
-    const int lane = threadIdx.x;
+// Weight and bias gradients of one layer without atomics: block (tile o,
+// tile i, chunk of kChunk rows) of kDwWaves waves, each wave accumulating
+// g^T act over its share of the chunk on the matrix cores; the waves' tiles
+// are summed in LDS and the block writes its chunk's partial dW tile (and,
+// for i-tile 0, the partial bias sums of its outputs) to part[chunk]; then
+// k_dw_reduce adds the chunks to the gradient in a fixed order, so a
+// training step is deterministic.
+constexpr int kChunk = 1024, kDwWaves = 4;
+__global__ __launch_bounds__(kDwWaves * 64) void k_dw(const float* __restrict__ g, int gw,
+                                                      const uint16_t* __restrict__ act, int K, uint32_t n,
+                                                      float* __restrict__ part) {
+    __shared__ float s_acc[kDwWaves][16][64];
+    __shared__ float s_b[kDwWaves][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ot = blockIdx.x, it = blockIdx.y;
-    const uint32_t s0 = blockIdx.z * kChunk;
+    const uint32_t chunk = blockIdx.z;
+    const uint32_t r0 = chunk * kChunk + wave * (kChunk / kDwWaves);
+    const uint32_t r1 = min(r0 + kChunk / kDwWaves, n);
     const int o = ot * 32 + (lane & 31), i = it * 32 + (lane & 31);
     f32x16 acc = {0};
-    for (uint32_t kb = s0; kb < s0 + kChunk && kb < n; kb += 16) {
+    float bsum = 0.0f;
+    for (uint32_t kb = r0; kb < r1; kb += 16) {
         float av[8];
         bf16x8 b;
         for (int j = 0; j < 8; j++) {
             const uint32_t srow = kb + (lane >> 5) * 8 + j;
-            const bool ok = srow < n;
+            const bool ok = srow < r1;
             av[j] = ok && o < gw ? g[(size_t)srow * gw + o] : 0.0f;
             b[j] = ok ? (short)act[(size_t)srow * K + i] : (short)0;
+            bsum += av[j];
         }
         acc = mfma(pack8(av), b, acc);
     }
 #pragma unroll
+    for (int r = 0; r < 16; r++) s_acc[wave][r][lane] = acc[r];
+    s_b[wave][lane] = bsum;
+    __syncthreads();
+    if (wave != 0) return;
+    const size_t stride = (size_t)gw * K + gw;
+    float* dst = part + (size_t)chunk * stride;
+#pragma unroll
     for (int r = 0; r < 16; r++) {
+        float v = 0.0f;
+        for (int w = 0; w < kDwWaves; w++) v += s_acc[w][r][lane];
         const int orow = ot * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        if (orow < gw) atomicAdd(&gradW[orow * K + i], acc[r]);
+        if (orow < gw) dst[(size_t)orow * K + i] = v;
+    }
+    if (it == 0 && lane < 32 && o < gw) {
+        float v = 0.0f;
+        for (int w = 0; w < kDwWaves; w++) v += s_b[w][lane] + s_b[w][lane + 32];
+        dst[(size_t)gw * K + o] = v;
     }
 }
 
-// db[l] += sum over rows of g.
-__global__ void k_db(const float* __restrict__ g, int gw, uint32_t n, float* __restrict__ gradb) {
-    const int o = threadIdx.x;
-    if (o >= gw) return;
-    float acc = 0.0f;
-    for (uint32_t s = blockIdx.x; s < n; s += gridDim.x) acc += g[(size_t)s * gw + o];
-    atomicAdd(&gradb[o], acc);
+// grad[e] += sum over chunks of part[chunk][e], e over the layer's weights
+// then biases (their order in the parameter vector).  Block: 64 elements x 4
+// groups of chunks (group q sums chunks q, q + 4, ...); the groups are added
+// in a fixed order.
+__global__ __launch_bounds__(256) void k_dw_reduce(const float* __restrict__ part, uint32_t chunks, uint32_t cnt,
+                                                   float* __restrict__ grad) {
+    __shared__ float s_sum[4][64];
+    const uint32_t e = blockIdx.x * 64 + (threadIdx.x & 63);
+    const uint32_t q = threadIdx.x >> 6;
+    float v = 0.0f;
+    if (e < cnt)
+        for (uint32_t c = q; c < chunks; c += 4) v += part[(size_t)c * cnt + e];
+    s_sum[q][threadIdx.x & 63] = v;
+    __syncthreads();
+    if (q == 0 && e < cnt) grad[e] += (s_sum[0][threadIdx.x] + s_sum[1][threadIdx.x]) + (s_sum[2][threadIdx.x] + s_sum[3][threadIdx.x]);
 }
 
 // g_prev = (g W) * (act > 0): rows of 64 per wave; K = gw (64 or 48), out
@@ -216,52 +257,154 @@ __global__ void k_adam(float* __restrict__ theta, float* __restrict__ m, float* 
     }
 }
 
-// Training batch from render records: features, direction and the weight
-// c = L_i(d) / p(d), with the incident radiance estimated from the path's
-// final radiance as (L_final - L_before) / T (per channel, averaged).
-__global__ void k_records_batch(const float* __restrict__ rec, uint32_t R, const float4* __restrict__ rad,
-                                float* __restrict__ x, float* __restrict__ d, float* __restrict__ c) {
-    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < R; r += gridDim.x * blockDim.x) {
-        const float* q = rec + (size_t)r * kGRecFloats;
-        float w = 0.0f;
-        vec3 dir = v3(0.0f, 0.0f, 1.0f);
-        if (q[kRecValid] != 0.0f) {
-            const uint32_t p = __float_as_uint(q[kRecPath]);
-            const float4 rf = rad[p];
-            const float fin[3] = {rf.x, rf.y, rf.z};
-            float sum = 0.0f;
-            int cnt = 0;
-            for (int k = 0; k < 3; k++) {
-                const float t = q[kRecThr + k];
-                if (t > 0.0f) {
-                    sum += (fin[k] - q[kRecRad + k]) / t;
-                    cnt++;
-                }
-            }
-            const float pdf = q[kRecPdf];
-            w = (cnt > 0 && pdf > 0.0f) ? (sum / (float)cnt) / pdf : 0.0f;
-            if (!(w >= 0.0f) || !isfinite(w)) w = 0.0f;
-            dir = v3(q[kRecDir], q[kRecDir + 1], q[kRecDir + 2]);
-            if (!isfinite(dir.x) || !isfinite(dir.y) || !isfinite(dir.z)) {
-                w = 0.0f;
-                dir = v3(0.0f, 0.0f, 1.0f);
-            }
+// Training weight of one render record: c = L_i(d) / p(d), the incident
+// radiance estimated from the path's final radiance as (L_final - L_before)
+// / T (T: throughput after the vertex, per channel, averaged over the
+// channels with T > 0), divided by the density d was sampled from.  Zero
+// for an empty slot, a non-finite estimate or a non-finite direction.
+__device__ __forceinline__ float record_weight(const float* __restrict__ q, const float4* __restrict__ rad) {
+    if (q[kRecValid] == 0.0f) return 0.0f;
+    const uint32_t p = __float_as_uint(q[kRecPath]);
+    const float4 rf = rad[p];
+    const float fin[3] = {rf.x, rf.y, rf.z};
+    float sum = 0.0f;
+    int cnt = 0;
+    for (int k = 0; k < 3; k++) {
+        const float t = q[kRecThr + k];
+        if (t > 0.0f) {
+            sum += (fin[k] - q[kRecRad + k]) / t;
+            cnt++;
         }
-        const vec3 pn = v3(q[kRecPos], q[kRecPos + 1], q[kRecPos + 2]);
-        const vec3 wo = v3(q[kRecWo], q[kRecWo + 1], q[kRecWo + 2]);
-        const vec3 nn = v3(q[kRecN], q[kRecN + 1], q[kRecN + 2]);
-        for (int ch = 0; ch < kGIn / 8; ch++) {
-            float f[8];
-            if (w > 0.0f)
-                guide_feature_chunk(pn, wo, nn, q[kRecRough], ch, f);
+    }
+    const float pdf = q[kRecPdf];
+    float w = (cnt > 0 && pdf > 0.0f) ? (sum / (float)cnt) / pdf : 0.0f;
+    if (!(w >= 0.0f) || !isfinite(w)) w = 0.0f;
+    if (!isfinite(q[kRecDir]) || !isfinite(q[kRecDir + 1]) || !isfinite(q[kRecDir + 2])) w = 0.0f;
+    return w;
+}
+
+// Batch row r from record q with weight w (features zeroed when w == 0).
+__device__ __forceinline__ void write_batch_row(const float* __restrict__ q, float w, uint32_t r, float* __restrict__ x,
+                                                float* __restrict__ d, float* __restrict__ c) {
+    const vec3 pn = v3(q[kRecPos], q[kRecPos + 1], q[kRecPos + 2]);
+    const vec3 wo = v3(q[kRecWo], q[kRecWo + 1], q[kRecWo + 2]);
+    const vec3 nn = v3(q[kRecN], q[kRecN + 1], q[kRecN + 2]);
+    for (int ch = 0; ch < kGIn / 8; ch++) {
+        float f[8];
+        if (w > 0.0f)
+            guide_feature_chunk(pn, wo, nn, q[kRecRough], ch, f);
+        else
+            for (int j = 0; j < 8; j++) f[j] = 0.0f;
+        for (int j = 0; j < 8; j++) x[(size_t)r * kGIn + ch * 8 + j] = f[j];
+    }
+    const bool ok = w > 0.0f;
+    d[3 * r] = ok ? q[kRecDir] : 0.0f;
+    d[3 * r + 1] = ok ? q[kRecDir + 1] : 0.0f;
+    d[3 * r + 2] = ok ? q[kRecDir + 2] : 1.0f;
+    c[r] = w;
+}
+
+// ---- resampling of records in proportion to their weights ----------------
+// The batch gradient sum_r c_r grad(-log q(d_r)) / sum_r c_r equals in
+// expectation the mean of grad(-log q(d)) over records drawn with
+// probability c_r / sum c; drawing them (systematic resampling over the
+// weights' prefix sums) gives every batch row weight 1, so one batch spends
+// its rows on the records that carry radiance.
+constexpr int kScanThreads = 256, kScanItems = 4, kScanTile = kScanThreads * kScanItems;
+
+// Record weights, each capped at `share` of *total when total is given (a
+// rare path of enormous weight would otherwise fill whole batches with
+// copies of itself).
+__global__ void k_record_weights(const float* __restrict__ rec, uint32_t R, const float4* __restrict__ rad,
+                                 const float* __restrict__ total, float share, float* __restrict__ w) {
+    const float cap = total ? share * *total : INFINITY;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < R; r += gridDim.x * blockDim.x)
+        w[r] = fminf(record_weight(rec + (size_t)r * kGRecFloats, rad), cap);
+}
+
+// In-place inclusive prefix sums of each kScanTile-element tile of w; the
+// tile totals to tile_sum.  Fixed summation order (deterministic).
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(float* __restrict__ w, uint32_t R,
+                                                             float* __restrict__ tile_sum) {
+    __shared__ float s_wave[kScanThreads / 64];
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float v[kScanItems];
+    float run = 0.0f;
+    for (int k = 0; k < kScanItems; k++) {
+        run += base + k < R ? w[base + k] : 0.0f;
+        v[k] = run;
+    }
+    float x = run;
+    for (int off = 1; off < 64; off <<= 1) {
+        const float y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    float woff = 0.0f;
+    for (int j = 0; j < wave; j++) woff += s_wave[j];
+    const float excl = woff + (x - run);
+    for (int k = 0; k < kScanItems; k++)
+        if (base + k < R) w[base + k] = excl + v[k];
+    if (threadIdx.x == kScanThreads - 1) tile_sum[blockIdx.x] = woff + x;
+}
+
+// Exclusive prefix of the tile totals: tile_off[b] = sum of tiles < b,
+// tile_off[nt] = the grand total.  One block; each thread sums a run of
+// tiles serially, then one thread prefixes the runs.
+__global__ __launch_bounds__(kScanThreads) void k_scan_top(const float* __restrict__ tile_sum, uint32_t nt,
+                                                           float* __restrict__ tile_off) {
+    __shared__ float s_run[kScanThreads];
+    const uint32_t per = (nt + kScanThreads - 1) / kScanThreads;
+    const uint32_t b0 = threadIdx.x * per;
+    float acc = 0.0f;
+    for (uint32_t b = b0; b < b0 + per && b < nt; b++) acc += tile_sum[b];
+    s_run[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int j = 0; j < kScanThreads; j++) {
+            const float a = s_run[j];
+            s_run[j] = t;
+            t += a;
+        }
+        tile_off[nt] = t;
+    }
+    __syncthreads();
+    float t = s_run[threadIdx.x];
+    for (uint32_t b = b0; b < b0 + per && b < nt; b++) {
+        tile_off[b] = t;
+        t += tile_sum[b];
+    }
+}
+
+// Batch row j <- the record whose cumulative-weight interval holds
+// (j + u) / B of the total (first r with G(r) > t, G the inclusive prefix
+// sum, G(r) = tile_off[r / tile] + w_scanned[r]).
+__global__ void k_resample(const float* __restrict__ rec, uint32_t R, const float* __restrict__ cdf,
+                           const float* __restrict__ tile_off, uint32_t nt, uint32_t B, float u,
+                           float* __restrict__ x, float* __restrict__ d, float* __restrict__ c) {
+    const float total = tile_off[nt];
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < B; j += gridDim.x * blockDim.x) {
+        if (!(total > 0.0f) || !isfinite(total)) {
+            write_batch_row(rec, 0.0f, j, x, d, c);
+            continue;
+        }
+        const float t = ((float)j + u) * (total / (float)B);
+        uint32_t lo = 0, hi = R;  // first r in [lo, hi) with G(r) > t, or R
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            const float g = tile_off[mid / kScanTile] + cdf[mid];
+            if (g > t)
+                hi = mid;
             else
-                for (int j = 0; j < 8; j++) f[j] = 0.0f;
-            for (int j = 0; j < 8; j++) x[(size_t)r * kGIn + ch * 8 + j] = f[j];
+                lo = mid + 1;
         }
-        d[3 * r] = dir.x;
-        d[3 * r + 1] = dir.y;
-        d[3 * r + 2] = dir.z;
-        c[r] = w;
+        const uint32_t r = lo < R ? lo : R - 1;
+        const float g_prev = r == 0 ? 0.0f : tile_off[(r - 1) / kScanTile] + cdf[r - 1];
+        const float g_r = tile_off[r / kScanTile] + cdf[r];
+        write_batch_row(rec + (size_t)r * kGRecFloats, g_r > g_prev ? 1.0f : 0.0f, j, x, d, c);
     }
 }
 
@@ -274,6 +417,9 @@ struct GuideState {
     // training batch built from render records
     uint32_t batch_cap = 0;
     float *bx = nullptr, *bd = nullptr, *bc = nullptr;
+    // record weights (prefix-summed in place) and their tile offsets
+    uint32_t rw_cap = 0;
+    float *rw = nullptr, *tile_sum = nullptr, *tile_off = nullptr;
     float *theta = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *bias = nullptr;
     uint16_t *w = nullptr, *wt = nullptr;
     int step = 0;
@@ -282,6 +428,7 @@ struct GuideState {
     uint32_t cap = 0;
     uint16_t* act[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     float *out = nullptr, *g0 = nullptr, *g1 = nullptr, *scalars = nullptr;
+    float* part = nullptr;  // per-chunk partial weight/bias gradients of one layer
     std::vector<void*> allocs;
 };
 
@@ -329,7 +476,7 @@ dl_status ensure_train(Ctx* c, GuideState* g, uint32_t n) {
             dfree(c, g->act[l]);
             g->allocs.erase(std::find(g->allocs.begin(), g->allocs.end(), (void*)g->act[l]));
         }
-    for (float** p : {&g->out, &g->g0, &g->g1})
+    for (float** p : {&g->out, &g->g0, &g->g1, &g->part})
         if (*p) {
             dfree(c, *p);
             g->allocs.erase(std::find(g->allocs.begin(), g->allocs.end(), (void*)*p));
@@ -339,6 +486,8 @@ dl_status ensure_train(Ctx* c, GuideState* g, uint32_t n) {
     DL_TRY(galloc(c, g, &g->out, (size_t)n * kGOut));
     DL_TRY(galloc(c, g, &g->g0, (size_t)n * 64));
     DL_TRY(galloc(c, g, &g->g1, (size_t)n * 64));
+    // largest layer: 64 x 96 weights + 64 biases per chunk
+    DL_TRY(galloc(c, g, &g->part, (size_t)((n + kChunk - 1) / kChunk) * (64 * 96 + 64)));
     return DL_OK;
 }
 
@@ -406,7 +555,7 @@ dl_status guide_train(Ctx* c, const float* x, const float* dirs, const float* cw
     DL_HIP_CHECK(c, hipMemsetAsync(g->scalars, 0, 4 * sizeof(float), c->stream));
     DL_TRY(guide_forward(c, x, n, g->out, true));
     const int eg = std::max(1, std::min<int>((int)((n + 255) / 256), c->num_cus * 8));
-    k_sum<<<eg, 256, 0, c->stream>>>(cw, n, g->scalars);
+    k_sum<<<std::min(eg, 64), 256, 0, c->stream>>>(cw, n, g->scalars);
     k_loss_grad<<<eg, 256, 0, c->stream>>>(g->out, dirs, cw, n, g->scalars, g->g0, g->scalars + 1);
     float* gcur = g->g0;
     float* gnext = g->g1;
@@ -414,9 +563,10 @@ dl_status guide_train(Ctx* c, const float* x, const float* dirs, const float* cw
     const uint32_t chunks = (n + kChunk - 1) / kChunk;
     for (int l = 4; l >= 0; l--) {
         const int K = glayer_in(l);
-        k_dw<<<dim3((gw + 31) / 32, K / 32, chunks), 64, 0, c->stream>>>(gcur, gw, g->act[l], K, n,
-                                                                          g->grad + glayer_w(l));
-        k_db<<<std::min<uint32_t>(n, 1024u), 64, 0, c->stream>>>(gcur, gw, n, g->grad + glayer_b(l));
+        const uint32_t cnt = (uint32_t)(gw * K + gw);
+        k_dw<<<dim3((gw + 31) / 32, K / 32, chunks), kDwWaves * 64, 0, c->stream>>>(gcur, gw, g->act[l], K, n,
+                                                                                    g->part);
+        k_dw_reduce<<<(cnt + 63) / 64, 256, 0, c->stream>>>(g->part, chunks, cnt, g->grad + glayer_w(l));
         if (l > 0) {
             const uint32_t tiles = (n + kGRowsPerWave - 1) / kGRowsPerWave;
             const int grid = (int)std::max<uint32_t>(
@@ -437,6 +587,8 @@ dl_status guide_train(Ctx* c, const float* x, const float* dirs, const float* cw
     return DL_OK;
 }
 
+int guide_step_count(Ctx* c) { return c->guide ? ((GuideState*)c->guide)->step : 0; }
+
 void guide_net_ptrs(Ctx* c, const uint16_t** w, const uint16_t** wt, const float** bias) {
     GuideState* g = (GuideState*)c->guide;
     *w = g->w;
@@ -444,23 +596,53 @@ void guide_net_ptrs(Ctx* c, const uint16_t** w, const uint16_t** wt, const float
     *bias = g->bias;
 }
 
-dl_status guide_train_records(Ctx* c, const float* records, uint32_t R, const float4* rad) {
+// `steps` Adam steps on batches of kGuideBatch records resampled from the R
+// render records in proportion to their weights; returns the steps taken.
+dl_status guide_train_records(Ctx* c, const float* records, uint32_t R, const float4* rad, int steps) {
     GuideState* g = (GuideState*)c->guide;
-    if (g->batch_cap < R) {
+    if (R == 0 || steps <= 0) return DL_OK;
+    const uint32_t B = kGuideBatch;
+    if (g->batch_cap < B) {
         for (float** p : {&g->bx, &g->bd, &g->bc})
             if (*p) {
                 dfree(c, *p);
                 g->allocs.erase(std::find(g->allocs.begin(), g->allocs.end(), (void*)*p));
             }
-        g->batch_cap = R;
-        DL_TRY(galloc(c, g, &g->bx, (size_t)R * kGIn));
-        DL_TRY(galloc(c, g, &g->bd, (size_t)R * 3));
-        DL_TRY(galloc(c, g, &g->bc, R));
+        g->batch_cap = B;
+        DL_TRY(galloc(c, g, &g->bx, (size_t)B * kGIn));
+        DL_TRY(galloc(c, g, &g->bd, (size_t)B * 3));
+        DL_TRY(galloc(c, g, &g->bc, B));
     }
-    k_records_batch<<<std::max(1, std::min<int>((int)((R + 255) / 256), c->num_cus * 8)), 256, 0, c->stream>>>(
-        records, R, rad, g->bx, g->bd, g->bc);
+    const uint32_t nt = (R + kScanTile - 1) / kScanTile;
+    if (g->rw_cap < R) {
+        for (float** p : {&g->rw, &g->tile_sum, &g->tile_off})
+            if (*p) {
+                dfree(c, *p);
+                g->allocs.erase(std::find(g->allocs.begin(), g->allocs.end(), (void*)*p));
+            }
+        g->rw_cap = R;
+        DL_TRY(galloc(c, g, &g->rw, R));
+        DL_TRY(galloc(c, g, &g->tile_sum, nt));
+        DL_TRY(galloc(c, g, &g->tile_off, nt + 1));
+    }
+    const int eg = std::max(1, std::min<int>((int)((R + 255) / 256), c->num_cus * 8));
+    // weights, their total, then the capped weights and their prefix sums
+    k_record_weights<<<eg, 256, 0, c->stream>>>(records, R, rad, nullptr, 0.0f, g->rw);
+    k_scan_tiles<<<nt, kScanThreads, 0, c->stream>>>(g->rw, R, g->tile_sum);
+    k_scan_top<<<1, kScanThreads, 0, c->stream>>>(g->tile_sum, nt, g->tile_off);
+    k_record_weights<<<eg, 256, 0, c->stream>>>(records, R, rad, g->tile_off + nt, kGuideMaxShare, g->rw);
+    k_scan_tiles<<<nt, kScanThreads, 0, c->stream>>>(g->rw, R, g->tile_sum);
+    k_scan_top<<<1, kScanThreads, 0, c->stream>>>(g->tile_sum, nt, g->tile_off);
     DL_HIP_CHECK(c, hipGetLastError());
-    return guide_train(c, g->bx, g->bd, g->bc, R, true);
+    const int bg = std::max(1, std::min<int>((int)((B + 255) / 256), c->num_cus * 8));
+    for (int s = 0; s < steps; s++) {
+        // stratum offset of this step: golden-ratio sequence over the steps
+        const double u = std::fmod(0.5 + 0.6180339887498949 * (double)(g->step + 1), 1.0);
+        k_resample<<<bg, 256, 0, c->stream>>>(records, R, g->rw, g->tile_off, nt, B, (float)u, g->bx, g->bd, g->bc);
+        DL_HIP_CHECK(c, hipGetLastError());
+        DL_TRY(guide_train(c, g->bx, g->bd, g->bc, B, true));
+    }
+    return DL_OK;
 }
 
 dl_status guide_decode_rows(Ctx* c, const float* out, uint32_t n, float* mix) {

@@ -173,6 +173,7 @@ __device__ __forceinline__ uint32_t queue_total(const uint32_t* counts) {
 // training records (nullptr: not training) and the scene bounds.
 struct GuideView {
     const float* mix;
+    float frac;           // probability of sampling the proposal (0: BSDF only)
     float* rec;
     uint32_t rec_stride;  // paths p with p % rec_stride == 0 are recorded
     uint32_t rec_cap;     // records
@@ -451,7 +452,8 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                     float bp = bsdf_pdf(type, prm, q);
                     // with the proposal, directions come from the 50/50 mixture
                     // of BSDF and proposal sampling: its pdf enters the MIS
-                    if (GUIDED) bp = (1.0f - kGuideFraction) * bp + kGuideFraction * guide_pdf(gm, ds.d);
+                    if (GUIDED && gv.frac > 0.0f && guide_material(type, prm))
+                        bp = (1.0f - gv.frac) * bp + gv.frac * guide_pdf(gm, ds.d);
                     const float mis = ds.delta ? 1.0f : mis_weight(ds.pdf, bp);
                     const vec3 contrib = mul(mul(scale(thr, mis), bv), spectrum);
                     if (csum(spectrum) > 0) {
@@ -474,18 +476,19 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                 float prev_pdf;
                 vec3 wd = v3(0.0f, 0.0f, 0.0f);
                 if (GUIDED) {
-                    // one-sample MIS: proposal or BSDF with probability 1/2
-                    // each; the weight divides by the mixture density
+                    // one-sample MIS: proposal with probability frac, else
+                    // the BSDF; the weight divides by the mixture density
+                    const float fg = guide_material(type, prm) ? gv.frac : 0.0f;
                     const float us = rng.next_float();
                     const float bx = rng.next_float();
                     const float by = rng.next_float();
-                    if (us < kGuideFraction) {
+                    if (us < fg) {
                         wd = guide_sample(gm, bx, by);
                         br.wo = to_local(its.sh, wd);
                         br.measure = kMeasureSolidAngle;
                         br.eta = 1.0f;
                         const float pb = bsdf_pdf(type, prm, br);
-                        const float pm = (1.0f - kGuideFraction) * pb + kGuideFraction * guide_pdf(gm, wd);
+                        const float pm = (1.0f - fg) * pb + fg * guide_pdf(gm, wd);
                         fr = (br.wo.z > 0.0f && pm > 0.0f) ? scale(bsdf_eval(type, prm, br), br.wo.z / pm)
                                                            : v3(0.0f, 0.0f, 0.0f);
                         prev_pdf = pm;
@@ -493,7 +496,7 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                         const vec3 f0 = bsdf_sample(type, prm, br, bx, by);
                         wd = to_world(its.sh, br.wo);
                         const float pb = bsdf_pdf(type, prm, br);
-                        const float pm = (1.0f - kGuideFraction) * pb + kGuideFraction * guide_pdf(gm, wd);
+                        const float pm = fg > 0.0f ? (1.0f - fg) * pb + fg * guide_pdf(gm, wd) : pb;
                         fr = pm > 0.0f ? scale(f0, pb / pm) : v3(0.0f, 0.0f, 0.0f);
                         prev_pdf = pm;
                     }
@@ -513,7 +516,7 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                     want_cont = true;
                 }
                 aux.w = 0;
-                if (GUIDED && gv.rec && want_cont) {
+                if (GUIDED && gv.rec && want_cont && guide_material(type, prm)) {
                     // training record of this vertex (its radiance before the
                     // continuation is completed by k_post)
                     const uint32_t depth = (uint32_t)aux.z;
@@ -596,13 +599,13 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
                 const uint32_t flags = (uint32_t)aux.y;
                 vec3 rad = f3(rs.rad[p]);
                 if ((flags & kFlagNee) && rs.occ[p] == 0) rad = add(rad, f3(rs.nee[p]));
-            if (grec && aux.w > 0) {
-                // radiance before the continuation of a recorded vertex
-                float* r = grec + (size_t)(aux.w - 1) * kGRecFloats;
-                r[kRecRad] = rad.x;
-                r[kRecRad + 1] = rad.y;
-                r[kRecRad + 2] = rad.z;
-            }
+                if (grec && aux.w > 0) {
+                    // radiance before the continuation of a recorded vertex
+                    float* r = grec + (size_t)(aux.w - 1) * kGRecFloats;
+                    r[kRecRad] = rad.x;
+                    r[kRecRad + 1] = rad.y;
+                    r[kRecRad + 2] = rad.z;
+                }
                 if (flags & kFlagCont) {
                     const float4 th4 = rs.thr[p];
                     vec3 thr = f3(th4);
@@ -732,9 +735,6 @@ __global__ void k_clear_records(float* rec, uint32_t R) {
         rec[(size_t)r * kGRecFloats + kRecValid] = 0.0f;
 }
 
-__host__ __device__ inline bool is_guide_type(int t) {
-    return t == DL_BSDF_DIFFUSE || t == DL_BSDF_MICROFACET || t == DL_BSDF_CONDUCTOR || t == DL_BSDF_ANISO_GGX;
-}
 
 __device__ __forceinline__ bool sample_valid(const float4& c) {  // Color3f::isValid  color.h:50-58
     return !(c.x < 0 || !isfinite(c.x) || c.y < 0 || !isfinite(c.y) || c.z < 0 || !isfinite(c.z));
@@ -919,7 +919,7 @@ dl_status ensure_state(Ctx* c, uint32_t P, uint32_t pixels) {
     return DL_OK;
 }
 
-constexpr uint32_t kGuideRecords = 1u << 18;  // training records per pass
+constexpr uint32_t kGuideRecords = 1u << 23;  // training record slots per pass (96 B each)
 
 dl_status ensure_guide_bufs(Ctx* c, RenderState* rs, uint32_t cap) {
     if (rs->gmix_cap < cap) {
@@ -977,13 +977,15 @@ struct Timer {
     }
 };
 
-enum { kTagClosest = 0, kTagShadow = 1, kTagShade = 2, kTagFilm = 3, kTagOther = 4, kTagGuide = 5 };
+enum { kTagClosest = 0, kTagShadow = 1, kTagShade = 2, kTagFilm = 3, kTagOther = 4, kTagGuide = 5, kTagTrain = 6 };
 
 }  // namespace
 
 dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render_stats* stats, int paths_sample,
                  float* paths_out) {
     if (prm->spp <= 0) return set_error(c, DL_ERR_INVALID, "spp must be positive");
+    if (prm->guide_steps < 0 || prm->guide_steps > 4096)
+        return set_error(c, DL_ERR_INVALID, "guide_steps must be in [0, 4096]");
     if (c->light_recs.empty()) return set_error(c, DL_ERR_INVALID, "There is not emitter in the scene.");
     (void)hipSetDevice(c->device);
     const uint32_t W = (uint32_t)c->cam.width, H = (uint32_t)c->cam.height, WH = W * H;
@@ -996,7 +998,11 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
     RenderState* rs = (RenderState*)c->rs;
     const SceneView sv = scene_view(c);
     for (int t = 0; t < 8; t++) rs->types_present[t] = false;
-    for (const auto& m : c->mats) rs->types_present[m.type] = true;
+    bool guide_types[8] = {false, false, false, false, false, false, false, false};
+    for (const auto& m : c->mats) {
+        rs->types_present[m.type] = true;
+        if (guide_material(m.type, m.params)) guide_types[m.type] = true;
+    }
     const BvhView bvh{c->ds.nodes, c->ds.tris};
     const bool count = stats && (prm->flags & DL_RENDER_COUNT_TRAVERSAL);
     const bool guided = (prm->flags & DL_RENDER_GUIDED) != 0;
@@ -1008,6 +1014,7 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
         DL_TRY(ensure_guide_bufs(c, rs, cap));
         guide_net_ptrs(c, &net.w, &net.wt, &net.bias);
         gv.mix = rs->b.gmix;
+        gv.frac = kGuideFraction;
         gv.rec = train ? rs->b.grec : nullptr;
         gv.rec_cap = kGuideRecords;
         for (int k = 0; k < 3; k++) {
@@ -1028,8 +1035,18 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
         k_clear_film<<<stage_grid, kStageBlock, 0, c->stream>>>(rs->b.film, WH);
     rs->film_valid = paths_sample < 0;
     const uint32_t total_spp = paths_sample >= 0 ? 1u : (uint32_t)prm->spp;
-    for (uint32_t done = 0; done < total_spp;) {
-        const uint32_t kk = std::min(k, total_spp - done);
+    // A training render without a fixed pass size starts with short passes
+    // (1, 1, 2, 4, ... samples per pixel up to k) so the network, trained
+    // after every pass, improves early; a pass of a network that has never
+    // been trained samples the BSDF only (its records start the training).
+    const bool schedule = train && prm->spp_per_pass <= 0;
+
+    double train_steps = 0;
+    uint32_t pass = 0;
+    for (uint32_t done = 0; done < total_spp; ++pass) {
+        uint32_t kk = std::min(k, total_spp - done);
+        if (schedule) kk = std::min(kk, pass == 0 ? 1u : 1u << std::min<uint32_t>(pass - 1, 30));
+        if (guided) gv.frac = (train && guide_step_count(c) == 0) ? 0.0f : kGuideFraction;
         PassParams pp;
         pp.P = WH * kk;
         pp.W = W;
@@ -1077,10 +1094,10 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
             DL_TRY(tm.start(kTagShade));
             for (int t = 0; t < DL_BSDF_COUNT; t++) {
                 if (!rs->types_present[t] || rs->cnt_host->mat[cur][t] == 0) continue;
-                if (guided && is_guide_type(t)) {
+                if (guided && guide_types[t]) {
                     DL_TRY(tm.stop());
                     DL_TRY(tm.start(kTagGuide));
-                    switch (t) {
+                    if (gv.frac > 0.0f) switch (t) {
                         case DL_BSDF_DIFFUSE: k_guide_infer<DL_BSDF_DIFFUSE><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
                         case DL_BSDF_MICROFACET: k_guide_infer<DL_BSDF_MICROFACET><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
                         case DL_BSDF_ANISO_GGX: k_guide_infer<DL_BSDF_ANISO_GGX><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
@@ -1123,9 +1140,13 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
         }
         if (paths_sample >= 0) break;
         if (train) {
-            DL_TRY(tm.start(kTagGuide));
-            DL_TRY(guide_train_records(c, rs->b.grec, kGuideRecords, rs->b.rad));
+            DL_TRY(tm.start(kTagTrain));
+            const int steps_per_pass = prm->guide_steps > 0                          ? prm->guide_steps
+                                       : guide_step_count(c) < kGuideWarmSteps ? kGuideWarmStepsPerPass
+                                                                                : kGuideStepsPerPass;
+            DL_TRY(guide_train_records(c, rs->b.grec, kGuideRecords, rs->b.rad, steps_per_pass));
             DL_TRY(tm.stop());
+            train_steps += steps_per_pass;
         }
         DL_TRY(tm.start(kTagFilm));
         if ((int)ceilf(c->film.radius) + 1 <= kFilmHalo) {
@@ -1169,6 +1190,7 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
                 case kTagShade: stats->ms_shade += e; break;
                 case kTagFilm: stats->ms_film += e; break;
                 case kTagGuide: stats->ms_guide += e; break;
+                case kTagTrain: stats->ms_train += e; break;
                 default: stats->ms_other += e; break;
             }
         }
@@ -1179,6 +1201,7 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
         stats->invalid_samples = (double)rs->cnt_host->invalid;
         stats->nodes_closest = (double)rs->cnt_host->nodes;
         stats->tris_closest = (double)rs->cnt_host->tris;
+        stats->train_steps = train_steps;
     }
     return DL_OK;
 }

@@ -151,6 +151,8 @@ typedef struct dl_render_params {
     int32_t spp_per_pass;      /* samples per pixel per wavefront pass (0 = auto) */
     int32_t first_sample;      /* index of the first sample (progressive rendering) */
     int32_t flags;             /* DL_RENDER_* */
+    int32_t guide_steps;       /* DL_RENDER_GUIDED training: Adam steps after each pass (0 = 16 until
+                                  the network has taken 256 steps, then 4) */
 } dl_render_params;
 
 enum {
@@ -177,7 +179,9 @@ typedef struct dl_render_stats {
     double launches_shadow;
     double nodes_closest;      /* with DL_RENDER_COUNT_TRAVERSAL: node visits of closest-hit rays */
     double tris_closest;       /* with DL_RENDER_COUNT_TRAVERSAL: triangle tests of closest-hit rays */
-    double ms_guide;           /* with DL_RENDER_GUIDED: proposal inference and training kernels */
+    double ms_guide;           /* with DL_RENDER_GUIDED: proposal inference kernels */
+    double ms_train;           /* with DL_RENDER_GUIDED: online training (batches, gradients, Adam) */
+    double train_steps;        /* with DL_RENDER_GUIDED: Adam steps taken during the render */
 } dl_render_stats;
 
 /* ---- context ---------------------------------------------------------- */

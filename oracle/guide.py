@@ -17,8 +17,9 @@ Layout (shared with the device):
     W5[48][64] b5[48] -- 21,808 values.
   outputs (48): lobe j uses 5j (weight logit), 5j+1 (log kappa), 5j+2..5j+4
     (mean direction, unnormalised); 40..47 are unused.
-  mixture: w = softmax(logits), kappa = exp(clamp(raw, -3, 7)),
-    mu = m / max(|m|, 1e-6).
+  mixture: w = softmax(logits), kappa = exp(-3 + 10 sigmoid(raw)) (a
+    smooth map onto [e^-3, e^7] whose gradient never vanishes, so no lobe
+    is frozen at a bound), mu = m / max(|m|, 1e-6).
   vMF density: kappa / (2 pi (1 - exp(-2 kappa))) * exp(kappa (mu.w - 1)).
   sampling (Jakob 2012, "Numerically stable sampling of the von
     Mises-Fisher distribution on S^2"): cos = 1 + log(u1 + (1 - u1)
@@ -113,7 +114,8 @@ def forward(theta, x, keep=False, exact=False):
 
 
 def decode(out):
-    """(w, kappa, mu, norms, kraw_clamped) of raw outputs (N, 48)."""
+    """(w, kappa, mu, norms, s) of raw outputs (N, 48); s = sigmoid of the
+    raw concentration output."""
     out = np.asarray(out, np.float64)
     N = out.shape[0]
     o = out[:, :40].reshape(N, N_LOBES, 5)
@@ -121,13 +123,12 @@ def decode(out):
     z = logits - logits.max(axis=1, keepdims=True)
     e = np.exp(z)
     w = e / e.sum(axis=1, keepdims=True)
-    kr = o[:, :, 1]
-    kc = np.clip(kr, KAPPA_LO, KAPPA_HI)
-    kappa = np.exp(kc)
+    s = 1.0 / (1.0 + np.exp(-o[:, :, 1]))
+    kappa = np.exp(KAPPA_LO + (KAPPA_HI - KAPPA_LO) * s)
     m = o[:, :, 2:5]
     nm = np.maximum(np.linalg.norm(m, axis=2), 1e-6)
     mu = m / nm[:, :, None]
-    return w, kappa, mu, nm, kr
+    return w, kappa, mu, nm, s
 
 
 def vmf_pdf(kappa, mu, d):
@@ -176,7 +177,7 @@ def loss_and_grad(theta, x, d, c, exact=False):
     theta = np.asarray(theta, np.float64 if exact else np.float32)
     layers = split(theta)
     out, acts, pre = forward(theta, x, keep=True, exact=exact)
-    w, kappa, mu, nm, kr = decode(out)
+    w, kappa, mu, nm, sg = decode(out)
     d = np.asarray(d, np.float64)
     c = np.asarray(c, np.float64)
     N = out.shape[0]
@@ -189,8 +190,7 @@ def loss_and_grad(theta, x, d, c, exact=False):
     g_logit = g_q[:, None] * (wf - w * q[:, None])
     dot = np.sum(mu * d[:, None, :], axis=2)
     dlogf_dk = 1.0 / kappa - 2.0 * np.exp(-2 * kappa) / (-np.expm1(-2 * kappa)) + (dot - 1.0)
-    inside = (kr > KAPPA_LO) & (kr < KAPPA_HI)
-    g_kraw = g_q[:, None] * wf * dlogf_dk * kappa * inside
+    g_kraw = g_q[:, None] * wf * dlogf_dk * kappa * (KAPPA_HI - KAPPA_LO) * sg * (1.0 - sg)
     proj = d[:, None, :] - dot[:, :, None] * mu
     g_m = (g_q[:, None] * wf * kappa)[:, :, None] * proj / nm[:, :, None]
     g_out = np.zeros((N, N_OUT))
