@@ -20,6 +20,14 @@ spp = floor(16 * 64^U); Poisson arrivals at 0.9x the one-at-a-time service
 rate.  Prints p50/p99 job latency of the multi-lane service and of
 one-at-a-time rendering, and both saturated throughputs (jobs/s).
 
+`--workload guided`: the learned proposal against plain rendering at equal
+wall time on the same scene at 480x270 -- a 65,536-spp plain reference, then
+ladders of 16..4096 spp of both methods (the guided network trained from
+scratch inside every timed render), each point averaged over three seeds.
+Prints relMSE against wall time for both, the time each takes to reach
+relMSE 1e-3 (when the ladder gets there) and the guided relMSE at the
+plain render's time.
+
 Profile helpers (they read rocprofv3 CSV output and write the summaries that
 `roofline.traffic` and the kernel-time tables come from):
   bench.py --summarize-stats DIR OUT.md       kernel time table
@@ -90,10 +98,11 @@ def short_name(name):
     return n.split("(")[0]
 
 
-def summarize_stats(d, out_md, command):
+def summarize_stats(d, out_md, command, title=""):
     path = sorted(glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True))[0]
     rows = list(csv.DictReader(open(path)))
-    lines = ["# Kernel time: %s bench (1920x1080, 8 spp per pass, max depth 16)" % WORKLOAD, "",
+    title = title or "%s bench (1920x1080, 8 spp per pass, max depth 16)" % WORKLOAD
+    lines = ["# Kernel time: %s" % title, "",
              "Command (repository root, MI355X):", "", "```", command, "```", "",
              "| kernel | calls | total ms | avg us | min us | max us | % |", "|---|---:|---:|---:|---:|---:|---:|"]
     for r in rows:
@@ -363,6 +372,7 @@ def main():
     ap.add_argument("--summarize-stats", nargs=2, metavar=("DIR", "OUT_MD"))
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"))
     ap.add_argument("--command", default="", help="command line recorded in a summary")
+    ap.add_argument("--title", default="", help="title of a kernel-time summary")
     ap.add_argument("--copy-mib", type=int, default=1024, help="calibration copy size used by the PMC runs")
     ap.add_argument("--workload", choices=("displaced_sphere", "service", "guided"), default="displaced_sphere")
     ap.add_argument("--guided-res", default="480x270", help="guided workload: image size WxH")
@@ -377,7 +387,7 @@ def main():
     ap.add_argument("--policy", choices=("fifo", "sjf"), default="fifo")
     args = ap.parse_args()
     if args.summarize_stats:
-        summarize_stats(args.summarize_stats[0], args.summarize_stats[1], args.command)
+        summarize_stats(args.summarize_stats[0], args.summarize_stats[1], args.command, args.title)
         return
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, copy_bytes=args.copy_mib << 20, commands=args.command)
