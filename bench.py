@@ -207,11 +207,45 @@ def service_bench(args):
     latn, _, spann, stn = run(args.lanes, policy, zero)
     print("service saturated: %.2f s, %.2f jobs/s" % (spann, len(jobs) / spann), file=sys.stderr, flush=True)
     # Poisson arrivals at 0.9x the one-at-a-time rate
-    rate = 0.9 * len(jobs) / span1
+    cap1 = len(jobs) / span1
+    rate = 0.9 * cap1
     rng = np.random.default_rng(7)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(jobs)))
+    unit = np.cumsum(rng.exponential(1.0, len(jobs)))  # arrivals at rate 1, scaled per rate
+    arrivals = unit / rate
     lat1p, _, _, _ = run(1, drawlab_amd.POLICY_FIFO, arrivals)
     latnp, _, _, stp = run(args.lanes, policy, arrivals)
+    # p99 latency against the arrival rate (same arrival pattern, rescaled):
+    # the rate each mode sustains at the one-at-a-time p99 of the 0.9x run
+    curve1 = [(0.9 * cap1, float(np.percentile(lat1p, 99)))]
+    curven = [(0.9 * cap1, float(np.percentile(latnp, 99)))]
+    for f in args.rate_sweep:
+        r = f * cap1
+        if f < 1.0:
+            l1, _, _, _ = run(1, drawlab_amd.POLICY_FIFO, unit / r)
+            curve1.append((r, float(np.percentile(l1, 99))))
+        ln, _, _, _ = run(args.lanes, policy, unit / r)
+        curven.append((r, float(np.percentile(ln, 99))))
+        print("rate %.2f jobs/s: p99 one-at-a-time %s, service %.3f s" % (
+            r, "%.3f s" % curve1[-1][1] if f < 1.0 else "-", curven[-1][1]), file=sys.stderr, flush=True)
+    curve1.sort()
+    curven.sort()
+    target = float(np.percentile(lat1p, 99))
+
+    def rate_at(curve, p99):
+        # highest rate whose p99 stays within p99 (linear between points)
+        best = None
+        for (r0, l0), (r1, l1) in zip([(0.0, 0.0)] + curve[:-1], curve):
+            if l1 <= p99:
+                best = r1
+            elif l0 <= p99:
+                best = r0 + (r1 - r0) * (p99 - l0) / (l1 - l0)
+                break
+            else:
+                break
+        return best
+
+    r1 = rate_at(curve1, target)
+    rn = rate_at(curven, target)
 
     def q(a):
         return {"p50": round(float(np.percentile(a, 50)), 4), "p99": round(float(np.percentile(a, 99)), 4),
@@ -234,6 +268,11 @@ def service_bench(args):
                    "arrival_rate_hz": round(rate, 4), "scenes": [d.name for d in pool],
                    "gpaths": round(paths / 1e9, 3)},
         "latency_s": {"service": q(latnp), "one_at_a_time": q(lat1p)},
+        "p99_vs_rate": {"one_at_a_time": [[round(r, 3), round(l, 3)] for r, l in curve1],
+                        "service": [[round(r, 3), round(l, 3)] for r, l in curven]},
+        "equal_p99": {"p99_s": round(target, 3), "rate_one_at_a_time": r1, "rate_service": rn,
+                      "ratio": (rn / r1) if (rn and r1) else None,
+                      "note": "service rate is a lower bound when it stays within the p99 at every swept rate"},
         "saturated_jobs_per_s": {"service": round(len(jobs) / spann, 4),
                                  "one_at_a_time": round(len(jobs) / span1, 4)},
         "saturated_mpaths_per_s": {"service": round(paths / spann / 1e6, 1),
@@ -385,6 +424,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=64, help="service workload: number of jobs")
     ap.add_argument("--lanes", type=int, default=4, help="service workload: worker lanes")
     ap.add_argument("--policy", choices=("fifo", "sjf"), default="fifo")
+    ap.add_argument("--rate-sweep", type=float, nargs="*", default=[0.6, 0.8, 1.0, 1.2, 1.4],
+                    help="service workload: arrival rates (x one-at-a-time capacity) of the p99 sweep")
     args = ap.parse_args()
     if args.summarize_stats:
         summarize_stats(args.summarize_stats[0], args.summarize_stats[1], args.command, args.title)

@@ -28,6 +28,7 @@ EXPORTED_SYMBOLS = (
     "dl_hip_render", "dl_hip_render_paths", "dl_hip_malloc", "dl_hip_free", "dl_hip_memcpy_h2d",
     "dl_hip_memcpy_d2h", "dl_hip_memcpy_d2d", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_camera_perspective", "dl_film_filter",
     "dl_mesh_tangents", "dl_hip_memory_usage", "dl_hip_trim", "dl_hip_set_camera",
+    "dl_hip_scene_keep", "dl_hip_scene_select", "dl_hip_scene_drop",
     "dl_hip_service_start", "dl_hip_service_add_scene", "dl_hip_service_submit", "dl_hip_service_poll",
     "dl_hip_service_wait", "dl_hip_service_result", "dl_hip_service_cancel", "dl_hip_service_stats",
     "dl_hip_service_stop", "dl_hip_service_last_error",
@@ -135,6 +136,11 @@ def lib():
             "dl_hip_guide_params": (i32, [P, P, P]),
             "dl_hip_guide_eval": (i32, [P, P, u32, P, P]),
             "dl_hip_guide_train_step": (i32, [P, P, P, P, u32, i32, c.POINTER(c.c_float)]),
+            "dl_hip_memory_usage": (i32, [P, c.POINTER(c.c_uint64), c.POINTER(c.c_uint64)]),
+            "dl_hip_trim": (i32, [P]),
+            "dl_hip_scene_keep": (i32, [P, c.c_uint64]),
+            "dl_hip_scene_select": (i32, [P, c.c_uint64]),
+            "dl_hip_scene_drop": (i32, [P, c.c_uint64]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -337,6 +343,35 @@ class Backend:
     def sync(self):
         self._check(self.L.dl_hip_sync(self.h))
 
+    # ---- device memory and resident scenes --------------------------------
+    def memory_usage(self):
+        """(current, peak since the previous query) device bytes held."""
+        cur, peak = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self.L.dl_hip_memory_usage(self.h, ctypes.byref(cur), ctypes.byref(peak)))
+        return cur.value, peak.value
+
+    def trim(self):
+        self._check(self.L.dl_hip_trim(self.h))
+        self.desc = None
+        self._kept = {}
+
+    def scene_keep(self, key):
+        self._check(self.L.dl_hip_scene_keep(self.h, int(key)))
+        self._kept = getattr(self, "_kept", {})
+        self._kept[int(key)] = self.desc
+
+    def scene_select(self, key):
+        """Make the scene kept under `key` current; False if none is kept."""
+        st = self.L.dl_hip_scene_select(self.h, int(key))
+        if st == 4:  # DL_ERR_STATE: no such resident scene
+            return False
+        self._check(st)
+        self.desc = self._kept[int(key)]
+        return True
+
+    def scene_drop(self, key):
+        self._check(self.L.dl_hip_scene_drop(self.h, int(key)))
+
     def last_kernel_ms(self):
         ms = ctypes.c_double()
         self._check(self.L.dl_hip_last_kernel_ms(self.h, ctypes.byref(ms)))
@@ -468,7 +503,7 @@ class dl_service_stats(ctypes.Structure):
     _fields_ = [("jobs_done", ctypes.c_uint64), ("jobs_failed", ctypes.c_uint64),
                 ("jobs_cancelled", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
                 ("peak_reserved_bytes", ctypes.c_uint64), ("estimate_misses", ctypes.c_uint64),
-                ("scene_uploads", ctypes.c_uint64), ("paths", ctypes.c_double)]
+                ("scene_uploads", ctypes.c_uint64), ("scene_selects", ctypes.c_uint64), ("paths", ctypes.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

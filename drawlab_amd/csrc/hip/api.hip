@@ -38,6 +38,27 @@ dl_status upload(Ctx* c, T** dst, const T* src, size_t count) {
     return DL_OK;
 }
 
+// Exchanges the context's current scene with s.
+void swap_scene(Ctx* c, SceneState& s) {
+    std::swap(c->has_scene, s.has_scene);
+    std::swap(c->has_accel, s.has_accel);
+    std::swap(c->nprims, s.nprims);
+    std::swap(c->nverts, s.nverts);
+    std::swap(c->tri_verts, s.tri_verts);
+    std::swap(c->mesh_recs, s.mesh_recs);
+    std::swap(c->mats, s.mats);
+    std::swap(c->light_recs, s.light_recs);
+    std::swap(c->env_light, s.env_light);
+    std::swap(c->cam, s.cam);
+    std::swap(c->film, s.film);
+    std::swap(c->bg, s.bg);
+    std::swap(c->bbox_lo, s.bbox_lo);
+    std::swap(c->bbox_inv, s.bbox_inv);
+    std::swap(c->ds, s.ds);
+    std::swap(c->accel, s.accel);
+    std::swap(c->scene_allocs, s.scene_allocs);
+}
+
 void free_scene(Ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     // the acceleration structure is allocated by the builders, outside
@@ -45,10 +66,38 @@ void free_scene(Ctx* c) {
     if (c->ds.nodes) dfree(c, c->ds.nodes);
     if (c->ds.tris) dfree(c, c->ds.tris);
     for (void* p : c->scene_allocs) dfree(c, p);
-    c->scene_allocs.clear();
-    c->ds = DevScene();
-    c->has_scene = false;
-    c->has_accel = false;
+    SceneState empty;
+    swap_scene(c, empty);  // resets every scene field
+    c->scene_key = 0;
+}
+
+// Frees one resident (stashed) scene.
+void free_stashed(Ctx* c, SceneState& s) {
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (s.ds.nodes) dfree(c, s.ds.nodes);
+    if (s.ds.tris) dfree(c, s.ds.tris);
+    for (void* p : s.scene_allocs) dfree(c, p);
+    s = SceneState();
+}
+
+// Releases the current scene before another becomes current: kept resident
+// under its key when it has one, freed otherwise.
+void release_current(Ctx* c) {
+    invalidate_film(c);
+    if (c->scene_key != 0 && c->has_scene) {
+        SceneState s;
+        swap_scene(c, s);
+        c->stash[c->scene_key] = std::move(s);
+        c->scene_key = 0;
+    } else {
+        free_scene(c);
+    }
+}
+
+void free_all_scenes(Ctx* c) {
+    free_scene(c);
+    for (auto& kv : c->stash) free_stashed(c, kv.second);
+    c->stash.clear();
 }
 
 #define DL_TRY(expr)                     \
@@ -112,7 +161,7 @@ dl_status dl_hip_destroy(void* h) {
     hipSetDevice(c->device);
     free_render_state(c);
     free_guide(c);
-    free_scene(c);
+    free_all_scenes(c);
     if (c->stack_ovf) dfree(c, c->stack_ovf);
     if (c->work) dfree(c, c->work);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -129,8 +178,7 @@ dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
     Ctx* c = (Ctx*)h;
     if (!c || !sd) return DL_ERR_INVALID;
     hipSetDevice(c->device);
-    free_render_state(c);
-    free_scene(c);
+    release_current(c);
     if (sd->num_meshes == 0) return set_error(c, DL_ERR_INVALID, "scene has no meshes");
     if (sd->camera.width <= 0 || sd->camera.height <= 0) return set_error(c, DL_ERR_INVALID, "bad camera size");
     for (uint32_t i = 0; i < sd->num_materials; i++)
@@ -420,7 +468,50 @@ dl_status dl_hip_trim(void* h) {
     (void)hipSetDevice(c->device);
     free_render_state(c);
     free_guide(c);
-    free_scene(c);
+    free_all_scenes(c);
+    return DL_OK;
+}
+dl_status dl_hip_scene_keep(void* h, uint64_t key) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    if (key == 0) return set_error(c, DL_ERR_INVALID, "scene key 0 is reserved");
+    if (!c->has_accel) return set_error(c, DL_ERR_STATE, "scene_keep before build_accel");
+    auto it = c->stash.find(key);
+    if (it != c->stash.end()) {  // an older scene under the same key is replaced
+        free_stashed(c, it->second);
+        c->stash.erase(it);
+    }
+    c->scene_key = key;
+    return DL_OK;
+}
+dl_status dl_hip_scene_select(void* h, uint64_t key) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    if (key == 0) return set_error(c, DL_ERR_INVALID, "scene key 0 is reserved");
+    (void)hipSetDevice(c->device);
+    if (c->scene_key == key && c->has_scene) return DL_OK;
+    auto it = c->stash.find(key);
+    if (it == c->stash.end()) return set_error(c, DL_ERR_STATE, "no resident scene with this key");
+    SceneState s = std::move(it->second);
+    c->stash.erase(it);
+    release_current(c);
+    swap_scene(c, s);
+    c->scene_key = key;
+    return DL_OK;
+}
+dl_status dl_hip_scene_drop(void* h, uint64_t key) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    for (auto it = c->stash.begin(); it != c->stash.end();) {
+        if (key == 0 || it->first == key) {
+            free_stashed(c, it->second);
+            it = c->stash.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    if (key != 0 && c->scene_key == key) free_scene(c);
     return DL_OK;
 }
 dl_status dl_hip_sync(void* h) {

@@ -4,6 +4,7 @@
 
 #include <mutex>
 #include <string>
+#include <map>
 #include <unordered_map>
 #include <vector>
 
@@ -64,6 +65,28 @@ struct LightRec {
     int32_t env_visual;
 };
 
+// Everything a context holds for one scene (geometry, BVH, materials,
+// lights, environment, camera, film).  The current scene lives in the Ctx
+// fields of the same names; scenes kept resident for a later
+// dl_hip_scene_select live in Ctx::stash as SceneStates.
+struct SceneState {
+    bool has_scene = false;
+    bool has_accel = false;
+    uint32_t nprims = 0, nverts = 0;
+    std::vector<float> tri_verts;
+    std::vector<MeshRec> mesh_recs;
+    std::vector<dl_material_desc> mats;
+    std::vector<LightRec> light_recs;
+    int env_light = -1;
+    dl_camera_desc cam{};
+    dl_film_desc film{};
+    float bg[3] = {0, 0, 0};
+    float bbox_lo[3] = {0, 0, 0}, bbox_inv[3] = {1, 1, 1};
+    DevScene ds;
+    dl_accel_stats accel{};
+    std::vector<void*> scene_allocs;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -94,6 +117,8 @@ struct Ctx {
     DevScene ds;
     dl_accel_stats accel{};
     std::vector<void*> scene_allocs;
+    uint64_t scene_key = 0;                 // nonzero: keep the current scene resident when replaced
+    std::map<uint64_t, SceneState> stash;   // resident scenes by key
 
     // render scratch (reused across calls, grown on demand)
     std::vector<void*> render_allocs;

@@ -3,16 +3,18 @@
 // Lanes are worker threads, each owning one backend context (its own HIP
 // stream, scene, BVH and render state); kernels of jobs on different lanes
 // overlap on the device's hardware queues, which is what packs small jobs
-// into the GPU.  Each lane keeps its last scene resident, and a lane prefers
-// a queued job of its resident scene among the next `lanes` queue entries
-// (bounded bypass, so no job waits behind more than `lanes` later arrivals
-// for that reason).
+// into the GPU.  A lane keeps every scene it has uploaded resident
+// (dl_hip_scene_keep / dl_hip_scene_select: switching scenes costs no upload
+// and no BVH build), and prefers a queued job of one of its resident scenes
+// among the next `lanes` queue entries (bounded bypass, so no job waits
+// behind more than `lanes` later arrivals for that reason).
 //
 // Admission: every lane holds a reservation of device bytes -- while it runs
 // a job, the larger of what its context already holds and a conservative
 // estimate of what it can reach during the job (fixed context buffers +
-// scene + BVH build scratch + render state); when idle, the bytes its
-// context actually holds.  A job starts on a lane only if the
+// its other resident scenes, as measured + the job's scene and BVH build
+// scratch + render state); when idle, the bytes its context actually
+// holds.  Under budget pressure idle lanes free everything they hold.  A job starts on a lane only if the
 // sum of all reservations stays within the budget, so the sum of the lanes'
 // device memory never exceeds it.  After each job the lane's measured peak
 // is compared with its reservation (estimate_misses counts violations).
@@ -76,7 +78,8 @@ struct Lane {
     int index = 0;
     void* ctx = nullptr;
     uint64_t fixed_bytes = 0;     // context buffers that exist without a scene
-    uint64_t resident_scene = 0;  // 0: none
+    uint64_t resident_scene = 0;  // the context's current scene (0: none)
+    std::map<uint64_t, uint64_t> kept;  // resident scenes -> device bytes they hold (measured at upload)
     uint64_t state_paths = 0, state_pixels = 0;  // render state the context holds
     uint64_t reservation = 0;
     std::thread th;
@@ -119,7 +122,12 @@ uint64_t reservation_for(const Lane& lane, const Job& job, const SceneEntry& s) 
     const uint64_t paths = px * pass_spp(job.params, px);
     const uint64_t state = std::max(paths, lane.state_paths) * kStateBytesPerPath +
                            std::max(px, lane.state_pixels) * kFilmBytesPerPixel + kSlackBytes;
-    return lane.fixed_bytes + s.bytes_estimate() + state;
+    uint64_t others = 0;
+    for (const auto& kv : lane.kept)
+        if (kv.first != job.scene) others += kv.second;
+    const auto it = lane.kept.find(job.scene);
+    const uint64_t scene = it != lane.kept.end() ? it->second : s.bytes_estimate();
+    return lane.fixed_bytes + others + scene + state;
 }
 
 // Pick the next job for `lane` (caller holds the lock): policy order, with a
@@ -143,6 +151,8 @@ std::deque<uint64_t>::iterator pick(Service& S, Lane& lane) {
     const size_t window = std::min<size_t>(q.size(), (size_t)S.cfg.lanes);
     for (size_t i = 0; i < window; i++)
         if (S.jobs[q[i]]->scene == lane.resident_scene) return q.begin() + (long)i;
+    for (size_t i = 0; i < window; i++)
+        if (lane.kept.count(S.jobs[q[i]]->scene)) return q.begin() + (long)i;
     return best;
 }
 
@@ -156,12 +166,23 @@ void finish(Service& S, Job& job, int state, const std::string& error) {
 }
 
 // Render one job on the lane's context (no lock held).  Returns "" or an error.
-std::string run_job(Service& S, Lane& lane, Job& job, const SceneEntry& scene, bool upload, dl_render_stats& total) {
+// switch: 0 the scene is current, 1 select a resident scene, 2 upload and
+// build (then keep it resident; *kept_bytes = the device bytes it added).
+std::string run_job(Service& S, Lane& lane, Job& job, const SceneEntry& scene, int switch_, uint64_t* kept_bytes,
+                    dl_render_stats& total) {
     void* c = lane.ctx;
     auto fail = [&]() { return std::string(dl_hip_last_error(c)); };
-    if (upload) {
+    if (switch_ == 1 && dl_hip_scene_select(c, job.scene) != DL_OK) return fail();
+    if (switch_ == 2) {
+        uint64_t before = 0, after = 0;
+        // the current scene stays resident (it was kept), so the new one adds
+        // its own bytes only
+        dl_hip_memory_usage(c, &before, nullptr);
         if (dl_hip_upload_scene(c, &scene.desc) != DL_OK) return fail();
         if (dl_hip_build_accel(c, S.cfg.builder, nullptr) != DL_OK) return fail();
+        if (dl_hip_scene_keep(c, job.scene) != DL_OK) return fail();
+        dl_hip_memory_usage(c, &after, nullptr);
+        *kept_bytes = after > before ? after - before : 0;
     }
     if (dl_hip_set_camera(c, &job.camera) != DL_OK) return fail();
     const uint64_t px = job_pixels(job);
@@ -205,6 +226,7 @@ void lane_main(Service* Sp, Lane* lp) {
             dl_hip_memory_usage(lane.ctx, &cur, nullptr);
             lk.lock();
             lane.resident_scene = 0;
+            lane.kept.clear();
             lane.state_paths = lane.state_pixels = 0;
             lane.reservation = cur;
             S.cv.notify_all();
@@ -244,8 +266,9 @@ void lane_main(Service* Sp, Lane* lp) {
         job.info.lane = lane.index;
         job.info.t_start = S.now();
         job.info.reserved_bytes = need;
-        const bool upload = lane.resident_scene != job.scene;
-        if (upload) S.stats.scene_uploads++;
+        const int switch_ = lane.resident_scene == job.scene ? 0 : lane.kept.count(job.scene) ? 1 : 2;
+        if (switch_ == 1) S.stats.scene_selects++;
+        if (switch_ == 2) S.stats.scene_uploads++;
         Job* jp = &job;
         lk.unlock();
 
@@ -253,12 +276,24 @@ void lane_main(Service* Sp, Lane* lp) {
         // query gives this job's peak
         dl_hip_memory_usage(lane.ctx, nullptr, nullptr);
         dl_render_stats total{};
-        std::string e = run_job(S, lane, *jp, scene, upload, total);
+        uint64_t kept_bytes = 0;
+        std::string e = run_job(S, lane, *jp, scene, switch_, &kept_bytes, total);
         uint64_t cur = 0, peak = 0;
         dl_hip_memory_usage(lane.ctx, &cur, &peak);
 
         lk.lock();
-        lane.resident_scene = e.empty() || e == "cancelled" ? jp->scene : 0;
+        const bool ok = e.empty() || e == "cancelled";
+        if (switch_ == 2 && ok) lane.kept[jp->scene] = kept_bytes;
+        if (!ok) {
+            // the context may be left between scenes: start over from nothing
+            lk.unlock();
+            dl_hip_trim(lane.ctx);
+            dl_hip_memory_usage(lane.ctx, &cur, nullptr);
+            lk.lock();
+            lane.kept.clear();
+            lane.state_paths = lane.state_pixels = 0;
+        }
+        lane.resident_scene = ok ? jp->scene : 0;
         jp->info.paths = total.paths;
         jp->info.ms_gpu = total.ms_total;
         jp->info.peak_bytes = peak;

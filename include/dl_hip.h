@@ -239,11 +239,25 @@ dl_status dl_hip_memory_usage(void* ctx, uint64_t* current_bytes, uint64_t* peak
    (src/optix/host/optix_scene.cpp:43-52). */
 dl_status dl_hip_set_camera(void* ctx, const dl_camera_desc* camera);
 
-/* Frees the context's scene, acceleration structure and render state (the
-   context keeps only its fixed buffers; upload a scene again to render).
-   (No reference counterpart; used by the render service under memory
-   pressure.) */
+/* Frees the context's scenes (current and resident), acceleration
+   structures and render state (the context keeps only its fixed buffers;
+   upload a scene again to render).  (No reference counterpart; used by the
+   render service under memory pressure.) */
 dl_status dl_hip_trim(void* ctx);
+
+/* Resident scenes (no reference counterpart; for callers that render
+   several scenes in turn, such as the render service).
+   dl_hip_scene_keep tags the current scene (uploaded and built) with a
+   nonzero key: when another scene is uploaded or selected, it stays in
+   device memory instead of being freed.  dl_hip_scene_select makes the
+   scene kept under `key` current again -- geometry, BVH, materials,
+   lights, environment, camera and film as they were -- without upload or
+   build (DL_ERR_STATE when no scene is kept under that key).
+   dl_hip_scene_drop frees the scene kept under `key` (0: every kept scene
+   that is not current; the current scene too when it is the one named). */
+dl_status dl_hip_scene_keep(void* ctx, uint64_t key);
+dl_status dl_hip_scene_select(void* ctx, uint64_t key);
+dl_status dl_hip_scene_drop(void* ctx, uint64_t key);
 
 /* ---- learned directional proposal ----------------------------------------
  * A mixture of 8 von Mises-Fisher lobes per shading point predicted by a
@@ -305,6 +319,7 @@ typedef struct dl_service_stats {
     uint64_t peak_reserved_bytes;  /* largest sum of lane reservations */
     uint64_t estimate_misses;      /* jobs whose lane peak exceeded its reservation */
     uint64_t scene_uploads;        /* uploads + BVH builds performed */
+    uint64_t scene_selects;        /* jobs that switched a lane to a scene it kept resident */
     double paths;
 } dl_service_stats;
 

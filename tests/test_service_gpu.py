@@ -40,6 +40,8 @@ def test_jobs_match_standalone_renders(backend):
         print("stats", st, flush=True)
         assert st["jobs_done"] == len(jobs) and st["estimate_misses"] == 0
         assert st["peak_reserved_bytes"] <= st["budget_bytes"]
+        # each lane uploads each scene at most once (then keeps it resident)
+        assert st["scene_uploads"] <= 2 * 3
     finally:
         svc.stop()
     for (d, spp, seed, k), img, info in zip(jobs, imgs, infos):
@@ -49,6 +51,38 @@ def test_jobs_match_standalone_renders(backend):
         assert info["paths"] == rst["paths"]
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), (spp, seed, k)
         assert info["t_submit"] <= info["t_start"] <= info["t_finish"]
+
+
+def test_resident_scenes(backend):
+    """Scenes kept resident (dl_hip_scene_keep) and selected again render
+    bit-identically to fresh uploads; dropping frees their memory."""
+    a = scenes.cornell_box(64, 64, spp=8, max_depth=8, rr_depth=3)
+    b = scenes.mis_plates(64, 36, max_depth=6)
+    refs = []
+    for d in (a, b):
+        backend.upload_scene(d)
+        backend.build_accel(drawlab_amd.BUILD_GPU)
+        refs.append(backend.render(spp=8, seed=3)[0])
+    backend.trim()
+    base, _ = backend.memory_usage()
+    for key, d in ((11, a), (12, b)):
+        backend.upload_scene(d)
+        backend.build_accel(drawlab_amd.BUILD_GPU)
+        backend.scene_keep(key)
+    for key, ref in ((11, refs[0]), (12, refs[1]), (11, refs[0])):
+        assert backend.scene_select(key)
+        img, _ = backend.render(spp=8, seed=3)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), key
+    both, _ = backend.memory_usage()
+    backend.scene_drop(12)
+    assert not backend.scene_select(12)
+    one, _ = backend.memory_usage()
+    assert one < both
+    img, _ = backend.render(spp=8, seed=3)          # scene 11 is still current
+    assert np.array_equal(img.view(np.uint32), refs[0].view(np.uint32))
+    backend.trim()
+    assert backend.memory_usage()[0] <= base
+    assert not backend.scene_select(11)
 
 
 def test_job_camera_override(backend):
