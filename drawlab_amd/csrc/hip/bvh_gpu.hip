@@ -104,6 +104,42 @@ __global__ __launch_bounds__(kB) void k_keys(const float4* __restrict__ blo, con
     }
 }
 
+// 21-bit integer -> 63-bit word with two zero bits after every bit (the
+// interleave of a 3-D Morton code), by shift-and-mask halving steps.
+__device__ __forceinline__ unsigned long long spread3(uint32_t a) {
+    unsigned long long x = a & 0x1fffffull;
+    x = (x | (x << 32)) & 0x001f00000000ffffull;
+    x = (x | (x << 16)) & 0x001f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+// 63-bit Morton codes (21 bits per axis) of the centroids, with the
+// primitive ids as sort values (PLOC: finer than the 30-bit keys, so
+// dense meshes keep a spatial order inside a cell of the coarse grid).
+__global__ __launch_bounds__(kB) void k_keys63(const float4* __restrict__ blo, const float4* __restrict__ bhi,
+                                               uint32_t n, const uint32_t* __restrict__ bounds,
+                                               unsigned long long* __restrict__ keys, uint32_t* __restrict__ ids) {
+    float lo[3], ext[3];
+    for (int a = 0; a < 3; a++) {
+        lo[a] = unordered(bounds[a]);
+        ext[a] = unordered(bounds[3 + a]) - lo[a];
+    }
+    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+        const float4 l = blo[i], h = bhi[i];
+        const float c[3] = {0.5f * (l.x + h.x), 0.5f * (l.y + h.y), 0.5f * (l.z + h.z)};
+        uint32_t q[3];
+        for (int a = 0; a < 3; a++) {
+            const float t = ext[a] > 0.f ? (c[a] - lo[a]) / ext[a] : 0.5f;
+            q[a] = (uint32_t)fminf(fmaxf(t * 2097152.0f, 0.0f), 2097151.0f);
+        }
+        keys[i] = (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
+        ids[i] = i;
+    }
+}
+
 __device__ __forceinline__ int delta(const unsigned long long* __restrict__ k, int n, int i, int j) {
     if (j < 0 || j >= n) return -1;
     return __clzll(k[i] ^ k[j]);
@@ -399,13 +435,16 @@ dl_status build_bvh_lbvh(Ctx* c, dl_accel_stats* stats) {
 // Morton-ordered primitives, every cluster finds its nearest neighbour
 // (smallest merged surface area) within kPlocRadius positions, mutual
 // nearest neighbours merge, and the survivors are compacted in order, until
-// one cluster remains.  Subtree primitive counts are kept at merge time;
-// the triangles are laid out in depth-first leaf order so that every subtree
-// of at most kMaxLeafSize primitives becomes one leaf.
+// one cluster remains.  Each merge also computes the node's SAH cost
+// bottom-up (traversal kSahNode per node, kSahTri per triangle test):
+// a subtree of at most kMaxLeafSize primitives becomes one leaf when that
+// is cheaper than keeping it as internal nodes.  The triangles are laid out
+// in depth-first leaf order, so every leaf's triangles are contiguous.
 // ===========================================================================
 namespace {
 
-constexpr int kPlocRadius = 16;
+constexpr int kPlocRadius = 24;
+constexpr float kSahNode = 1.0f, kSahTri = 1.0f;
 
 __device__ __forceinline__ float half_area(const float4& lo, const float4& hi) {
     const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
@@ -465,7 +504,8 @@ __global__ __launch_bounds__(kB) void k_ploc_merge(int n, int top, const int* __
                                                    float4* __restrict__ chi, int2* __restrict__ child,
                                                    float4* __restrict__ nlo, float4* __restrict__ nhi,
                                                    int* __restrict__ ncount, int* __restrict__ ecount,
-                                                   int* __restrict__ parent_int, int* __restrict__ parent_leaf) {
+                                                   int* __restrict__ parent_int, int* __restrict__ parent_leaf,
+                                                   float* __restrict__ ncost, int* __restrict__ nleaf) {
     for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
         if (!merge[i]) continue;
         const int j = nn[i];
@@ -479,8 +519,17 @@ __global__ __launch_bounds__(kB) void k_ploc_merge(int n, int top, const int* __
         nhi[node] = hi;
         const int cnt = (ra < 0 ? 1 : ncount[ra]) + (rb < 0 ? 1 : ncount[rb]);
         ncount[node] = cnt;
-        // nodes that survive leaf collapsing in this subtree
-        ecount[node] = (cnt > kMaxLeafSize ? 1 : 0) + (ra < 0 ? 0 : ecount[ra]) + (rb < 0 ? 0 : ecount[rb]);
+        // SAH: internal cost from the children's (already final) costs;
+        // a small enough subtree becomes one leaf when that is not dearer
+        const float area = half_area(lo, hi);
+        const float ca = ra < 0 ? kSahTri : ncost[ra], cb = rb < 0 ? kSahTri : ncost[rb];
+        const float cint = area > 0.0f ? kSahNode + (half_area(a0, a1) * ca + half_area(b0, b1) * cb) / area
+                                       : kSahNode + ca + cb;
+        const bool leaf = cnt <= kMaxLeafSize && (float)cnt * kSahTri <= cint;
+        ncost[node] = leaf ? (float)cnt * kSahTri : cint;
+        nleaf[node] = leaf ? 1 : 0;
+        // nodes emitted in this subtree (none below a leaf)
+        ecount[node] = leaf ? 0 : 1 + (ra < 0 ? 0 : ecount[ra]) + (rb < 0 ? 0 : ecount[rb]);
         if (ra < 0) parent_leaf[~ra] = node; else parent_int[ra] = node;
         if (rb < 0) parent_leaf[~rb] = node; else parent_int[rb] = node;
         cref[i] = node;
@@ -552,16 +601,29 @@ __global__ __launch_bounds__(kB) void k_ploc_offsets(int n, const int2* __restri
     atomicMax(max_depth, dmax);
 }
 
+// Node i lies inside a leaf: it or an ancestor was made a leaf (only
+// subtrees of at most kMaxLeafSize primitives can be, so the walk is short).
+__device__ __forceinline__ bool in_leaf(int i, const int* __restrict__ ncount, const int* __restrict__ nleaf,
+                                        const int* __restrict__ parent_int) {
+    for (;;) {
+        if (nleaf[i]) return true;
+        if (i == 0) return false;
+        i = parent_int[i];
+        if (ncount[i] > kMaxLeafSize) return false;
+    }
+}
+
 __global__ __launch_bounds__(kB) void k_ploc_emit(int n, const int2* __restrict__ child,
                                                   const float4* __restrict__ nlo, const float4* __restrict__ nhi,
-                                                  const int* __restrict__ ncount, const int* __restrict__ leaf_pos,
+                                                  const int* __restrict__ ncount, const int* __restrict__ nleaf,
+                                                  const int* __restrict__ parent_int, const int* __restrict__ leaf_pos,
                                                   const int* __restrict__ node_pos, const int* __restrict__ node_rank,
                                                   const uint32_t* __restrict__ sorted, const float4* __restrict__ blo,
                                                   const float4* __restrict__ bhi, float4* __restrict__ nodes,
                                                   uint32_t* __restrict__ leaves) {
     uint32_t nl = 0;
     for (int i = blockIdx.x * kB + threadIdx.x; i < n - 1; i += gridDim.x * kB) {
-        if (ncount[i] <= kMaxLeafSize) continue;  // collapsed into a leaf of its parent
+        if (in_leaf(i, ncount, nleaf, parent_int)) continue;  // part of a leaf of an ancestor
         const int2 ch = child[i];
         const int refs[2] = {ch.x, ch.y};
         float lo[2][3], hi[2][3];
@@ -570,20 +632,23 @@ __global__ __launch_bounds__(kB) void k_ploc_emit(int n, const int2* __restrict_
             const int r = refs[c];
             float4 a, b;
             int count, pos;
+            bool leaf;
             if (r < 0) {
                 const uint32_t p = sorted[~r];
                 a = blo[p];
                 b = bhi[p];
                 count = 1;
                 pos = leaf_pos[~r];
+                leaf = true;
             } else {
                 a = nlo[r];
                 b = nhi[r];
                 count = ncount[r];
                 pos = node_pos[r];
+                leaf = nleaf[r] != 0;
             }
             pad(a, b, lo[c], hi[c]);
-            if (count <= kMaxLeafSize) {
+            if (leaf) {
                 ref[c] = ~pos;
                 cnt[c] = count;
                 nl++;
@@ -628,7 +693,8 @@ dl_status build_bvh_ploc(Ctx* c, dl_accel_stats* stats) {
     unsigned long long *keys, *keys_sorted;
     uint32_t *bounds, *sorted;
     int *cref[2], *nn, *merge, *keep, *midx, *kidx, *ncount, *ecount, *parent_int, *parent_leaf, *leaf_pos, *node_pos,
-        *node_rank;
+        *node_rank, *nleaf;
+    float* ncost;
     int2* child;
     DL_TRY(sc.get(&blo, n));
     DL_TRY(sc.get(&bhi, n));
@@ -656,8 +722,13 @@ dl_status build_bvh_ploc(Ctx* c, dl_accel_stats* stats) {
     DL_TRY(sc.get(&node_pos, n));
     DL_TRY(sc.get(&node_rank, n));
     DL_TRY(sc.get(&child, n));
+    DL_TRY(sc.get(&nleaf, n));
+    DL_TRY(sc.get(&ncost, n));
     size_t sort_bytes = 0, scan_bytes = 0;
-    DL_HIP_CHECK(c, hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, keys, keys_sorted, n, 0, 64, c->stream));
+    uint32_t* ids = nullptr;
+    DL_TRY(sc.get(&ids, n));
+    DL_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys, keys_sorted, ids, sorted, n, 0, 63,
+                                                       c->stream));
     DL_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, merge, midx, n + 1, c->stream));
     void* temp = nullptr;
     DL_TRY(sc.get((char**)&temp, std::max(sort_bytes, scan_bytes)));
@@ -677,9 +748,10 @@ dl_status build_bvh_ploc(Ctx* c, dl_accel_stats* stats) {
     DL_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
     k_init_bounds<<<1, 64, 0, c->stream>>>(bounds);
     k_prim_boxes<<<grid, kB, 0, c->stream>>>(c->ds.pos, c->ds.idx, (uint32_t)n, blo, bhi, bounds);
-    k_keys<<<grid, kB, 0, c->stream>>>(blo, bhi, (uint32_t)n, bounds, keys);
-    DL_HIP_CHECK(c, hipcub::DeviceRadixSort::SortKeys(temp, sort_bytes, keys, keys_sorted, n, 0, 64, c->stream));
-    k_low_ids<<<grid, kB, 0, c->stream>>>(keys_sorted, n, sorted);
+    k_keys63<<<grid, kB, 0, c->stream>>>(blo, bhi, (uint32_t)n, bounds, keys, ids);
+    // stable: equal codes keep increasing primitive ids
+    DL_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs(temp, sort_bytes, keys, keys_sorted, ids, sorted, n, 0, 63,
+                                                       c->stream));
     k_ploc_init<<<grid, kB, 0, c->stream>>>(n, sorted, blo, bhi, cref[0], clo[0], chi[0]);
     int cur = 0, count = n, top = n - 2, iters = 0;
     while (count > 1) {
@@ -691,7 +763,7 @@ dl_status build_bvh_ploc(Ctx* c, dl_accel_stats* stats) {
         DL_HIP_CHECK(c, hipMemcpyAsync(&host_cnt[0], midx + count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         DL_HIP_CHECK(c, hipMemcpyAsync(&host_cnt[1], kidx + count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         k_ploc_merge<<<grid, kB, 0, c->stream>>>(count, top, nn, merge, midx, cref[cur], clo[cur], chi[cur], child, nlo,
-                                                 nhi, ncount, ecount, parent_int, parent_leaf);
+                                                 nhi, ncount, ecount, parent_int, parent_leaf, ncost, nleaf);
         k_ploc_compact<<<grid, kB, 0, c->stream>>>(count, keep, kidx, cref[cur], clo[cur], chi[cur], cref[cur ^ 1],
                                                    clo[cur ^ 1], chi[cur ^ 1]);
         DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -704,8 +776,8 @@ dl_status build_bvh_ploc(Ctx* c, dl_accel_stats* stats) {
     if (top != -1) return set_error(c, DL_ERR_STATE, "PLOC node count mismatch");
     k_ploc_offsets<<<grid, kB, 0, c->stream>>>(n, child, ncount, ecount, parent_int, parent_leaf, leaf_pos, node_pos,
                                                node_rank, bounds + 6);
-    k_ploc_emit<<<grid, kB, 0, c->stream>>>(n, child, nlo, nhi, ncount, leaf_pos, node_pos, node_rank, sorted, blo, bhi,
-                                            (float4*)pn, bounds + 7);
+    k_ploc_emit<<<grid, kB, 0, c->stream>>>(n, child, nlo, nhi, ncount, nleaf, parent_int, leaf_pos, node_pos,
+                                            node_rank, sorted, blo, bhi, (float4*)pn, bounds + 7);
     k_ploc_gather<<<grid, kB, 0, c->stream>>>(n, sorted, leaf_pos, c->ds.pos, c->ds.idx, (float4*)pt);
     DL_HIP_CHECK(c, hipGetLastError());
     DL_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
