@@ -1,8 +1,10 @@
 // HBM layout of the acceleration structure used by the gfx950 traversal
 // kernels.  Shared by the host builder and the device code.
 //
-// Node: a binary "child-pair" node, 64 bytes = 4 x 16-byte records, so one
-// node visit is four dwordx4 loads from one 64-byte-aligned line:
+// The builders produce binary "child-pair" nodes; a device pass then
+// collapses them into the 4-wide nodes the traversal kernels read.
+//
+// Binary node, 64 bytes = 4 x 16-byte records:
 //   r0 = (c0.lo.x, c0.hi.x, c1.lo.x, c1.hi.x)
 //   r1 = (c0.lo.y, c0.hi.y, c1.lo.y, c1.hi.y)
 //   r2 = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
@@ -12,6 +14,14 @@
 // outward (relative 2^-16 of their magnitude), so the slab test never
 // rejects a box that holds a triangle Moller-Trumbore accepts, including
 // rays with zero direction components (no 0 * inf).
+//
+// 4-wide node (traversal), 128 bytes = 8 x 16-byte records, one 128-byte
+// line, child boxes as structure of arrays:
+//   q0 = lo.x of children 0..3, q1 = hi.x, q2 = lo.y, q3 = hi.y, q4 = lo.z,
+//   q5 = hi.z, q6 = ref of children 0..3, q7 = count of children 0..3
+// with ref / count as in the binary node; an unused slot is an empty leaf
+// (ref = ~0, count = 0).
+// Node 0 is the root; boxes are the binary nodes' padded boxes.
 //
 // Triangle: 48 bytes = 3 x 16-byte records, pre-gathered in leaf order:
 //   t0 = (p0.x, p0.y, p0.z, e1.x)
@@ -24,26 +34,36 @@
 
 namespace dl {
 
-constexpr int kNodeBytes = 64;
+constexpr int kNodeBytes = 64;    // binary node (builders)
+constexpr int kNode4Bytes = 128;  // 4-wide node (traversal)
 constexpr int kTriBytes = 48;
 constexpr int kMaxLeafSize = 4;
 // Traversal stack: kLdsStack entries in LDS per lane (8 KB per 128-lane
 // block, so LDS does not limit occupancy below the 8 waves/SIMD the VGPR
-// budget allows), the rest in a global overflow buffer; no tree is deeper
-// than kMaxStack internal levels (the host builder caps depth at kMaxDepth,
-// an LBVH over 64-bit keys has at most 64 levels).
+// budget allows), the rest in a global overflow buffer.  No binary tree is
+// deeper than kMaxBinaryDepth internal levels (the host builder caps depth
+// at kMaxDepth, an LBVH over 64-bit keys has at most 64 levels; PLOC is
+// checked), so the 4-wide tree has at most as many levels, and a 4-wide
+// visit pushes at most 3 entries: kMaxStack = 3 x kMaxBinaryDepth entries
+// always suffice.
 constexpr int kLdsStack = 16;
-constexpr int kMaxStack = 64;
+constexpr int kMaxBinaryDepth = 64;
+constexpr int kMaxStack = 3 * kMaxBinaryDepth;
 constexpr int kMaxDepth = 48;
 
 struct BvhNodeH {
     float r0[4], r1[4], r2[4];
     int32_t r3[4];
 };
+struct Bvh4NodeH {
+    float q[6][4];
+    int32_t ref[4], cnt[4];
+};
 struct BvhTriH {
     float t0[4], t1[4], t2[4];
 };
 static_assert(sizeof(BvhNodeH) == kNodeBytes, "node layout");
+static_assert(sizeof(Bvh4NodeH) == kNode4Bytes, "4-wide node layout");
 static_assert(sizeof(BvhTriH) == kTriBytes, "tri layout");
 
 }  // namespace dl

@@ -352,8 +352,13 @@ dl_status dl_hip_build_accel(void* h, int builder, dl_accel_stats* stats) {
     } else {
         return set_error(c, DL_ERR_INVALID, "unknown builder");
     }
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        DL_TRY(collapse_bvh4(c, st.num_nodes, &st));
+        st.build_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
     st.num_prims = c->nprims;
-    st.node_bytes = (uint64_t)st.num_nodes * kNodeBytes;
+    st.node_bytes = (uint64_t)st.num_nodes * kNode4Bytes;
     st.tri_bytes = (uint64_t)c->nprims * kTriBytes;
     c->accel = st;
     c->has_accel = true;

@@ -185,5 +185,7 @@ int guide_step_count(Ctx* c);  // Adam steps taken since guide_init
 
 // bvh_gpu.hip
 dl_status build_bvh_gpu(Ctx* c, dl_accel_stats* stats, int method);
+// Binary nodes of any builder -> the 4-wide nodes the traversal reads.
+dl_status collapse_bvh4(Ctx* c, uint32_t num_binary, dl_accel_stats* stats);
 
 }  // namespace dl

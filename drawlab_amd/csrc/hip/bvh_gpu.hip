@@ -13,7 +13,7 @@
 //
 // Each level of a Karras tree strictly lengthens the common key prefix, so
 // over 64-bit keys no internal node is deeper than 64: the traversal stack
-// (kMaxStack) always suffices.
+// (kMaxBinaryDepth) always suffices.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kB) void k_boxes(int n, const int2* __restrict__ ra
         int node = parent_leaf[k];
         // A well-formed tree reaches the root within 64 levels; the bound
         // only guarantees termination.
-        for (int level = 0; level < 2 * kMaxStack; ++level) {
+        for (int level = 0; level < 2 * kMaxBinaryDepth; ++level) {
             __threadfence();
             if (atomicAdd(&arrive[node], 1) == 0) break;
             __threadfence();
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kB) void k_depth(int n, const int* __restrict__ par
     for (int k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB) {
         int node = parent_leaf[k];
         uint32_t d = 1;
-        while (node != 0 && d < 2 * kMaxStack) {
+        while (node != 0 && d < 2 * kMaxBinaryDepth) {
             node = parent_int[node];
             ++d;
         }
@@ -424,7 +424,7 @@ dl_status build_bvh_lbvh(Ctx* c, dl_accel_stats* stats) {
     stats->num_leaves = hb[7];
     stats->max_depth = hb[6];
     stats->build_ms = ms;
-    if (hb[6] > (uint32_t)kMaxStack) return set_error(c, DL_ERR_STATE, "LBVH deeper than the traversal stack");
+    if (hb[6] > (uint32_t)kMaxBinaryDepth) return set_error(c, DL_ERR_STATE, "LBVH deeper than the traversal stack");
     return DL_OK;
 }
 
@@ -585,7 +585,7 @@ __global__ __launch_bounds__(kB) void k_ploc_offsets(int n, const int2* __restri
             if (ch.y == ref && ch.x >= 0) rank += ecount[ch.x];
             if (ch.y == ref) off += ch.x < 0 ? 1 : ncount[ch.x];
             rank += 1;  // the ancestor itself precedes its subtree
-            if (node == 0 || d >= 4 * kMaxStack) break;
+            if (node == 0 || d >= 4 * kMaxBinaryDepth) break;
             ref = node;
             node = parent_int[node];
             ++d;
@@ -792,7 +792,171 @@ dl_status build_bvh_ploc(Ctx* c, dl_accel_stats* stats) {
     stats->num_leaves = hb[7];
     stats->max_depth = hb[6];
     stats->build_ms = ms;
-    if (hb[6] > (uint32_t)kMaxStack) return set_error(c, DL_ERR_STATE, "PLOC tree deeper than the traversal stack");
+    if (hb[6] > (uint32_t)kMaxBinaryDepth) return set_error(c, DL_ERR_STATE, "PLOC tree deeper than the traversal stack");
+    return DL_OK;
+}
+
+// ===========================================================================
+// Collapse of a binary BVH (any builder) into 4-wide nodes: every 4-wide
+// node takes a binary node's two children and opens the internal child of
+// largest surface area (lowest slot on ties) until it has four children or
+// only leaves.  Top-down, one level per step: the frontier holds binary
+// nodes that become 4-wide nodes; their internal children get consecutive
+// indices (prefix sum), so the 4-wide nodes are in breadth-first order.
+// ===========================================================================
+namespace {
+
+struct Cand {
+    float lo[3], hi[3];
+    int ref, cnt;
+};
+
+__device__ __forceinline__ void bin_child(const float4* __restrict__ bn, int b, int k, Cand& c) {
+    const float4* np = bn + 4 * (size_t)b;
+    const float4 r0 = np[0], r1 = np[1], r2 = np[2];
+    const int4 r3 = reinterpret_cast<const int4*>(np)[3];
+    c.lo[0] = k ? r0.z : r0.x;
+    c.hi[0] = k ? r0.w : r0.y;
+    c.lo[1] = k ? r1.z : r1.x;
+    c.hi[1] = k ? r1.w : r1.y;
+    c.lo[2] = k ? r2.z : r2.x;
+    c.hi[2] = k ? r2.w : r2.y;
+    c.ref = k ? r3.y : r3.x;
+    c.cnt = k ? r3.w : r3.z;
+}
+
+__device__ __forceinline__ float cand_area(const Cand& c) {
+    const float dx = c.hi[0] - c.lo[0], dy = c.hi[1] - c.lo[1], dz = c.hi[2] - c.lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// Children of the 4-wide node made from binary node b; returns how many.
+__device__ __forceinline__ int gather4(const float4* __restrict__ bn, int b, Cand (&c)[4]) {
+    bin_child(bn, b, 0, c[0]);
+    bin_child(bn, b, 1, c[1]);
+    int m = 2;
+    while (m < 4) {
+        int best = -1;
+        float ba = -1.0f;
+        for (int k = 0; k < m; ++k)
+            if (c[k].ref >= 0) {
+                const float a = cand_area(c[k]);
+                if (a > ba) {
+                    ba = a;
+                    best = k;
+                }
+            }
+        if (best < 0) break;
+        const int r = c[best].ref;
+        bin_child(bn, r, 1, c[m]);
+        bin_child(bn, r, 0, c[best]);
+        ++m;
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(kB) void k_c4_count(const float4* __restrict__ bn, const int* __restrict__ fbin, int nf,
+                                                 int* __restrict__ nint) {
+    for (int i = blockIdx.x * kB + threadIdx.x; i < nf; i += gridDim.x * kB) {
+        Cand c[4];
+        const int m = gather4(bn, fbin[i], c);
+        int k = 0;
+        for (int j = 0; j < m; ++j) k += c[j].ref >= 0 ? 1 : 0;
+        nint[i] = k;
+        if (i == 0) nint[nf] = 0;
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_c4_emit(const float4* __restrict__ bn, const int* __restrict__ fbin,
+                                                const int* __restrict__ fout, int nf, const int* __restrict__ off,
+                                                int base, float4* __restrict__ n4, int* __restrict__ nbin,
+                                                int* __restrict__ nout) {
+    for (int i = blockIdx.x * kB + threadIdx.x; i < nf; i += gridDim.x * kB) {
+        Cand c[4];
+        const int m = gather4(bn, fbin[i], c);
+        float q[6][4];
+        int ref[4], cnt[4];
+        int k = off[i];
+        for (int j = 0; j < 4; ++j) {
+            if (j < m) {
+                for (int a = 0; a < 3; ++a) {
+                    q[2 * a][j] = c[j].lo[a];
+                    q[2 * a + 1][j] = c[j].hi[a];
+                }
+                if (c[j].ref >= 0) {
+                    nbin[k] = c[j].ref;
+                    nout[k] = base + k;
+                    ref[j] = base + k;
+                    cnt[j] = 0;
+                    ++k;
+                } else {
+                    ref[j] = c[j].ref;
+                    cnt[j] = c[j].cnt;
+                }
+            } else {  // unused slot: an empty leaf
+                for (int a = 0; a < 6; ++a) q[a][j] = 0.0f;
+                ref[j] = ~0;
+                cnt[j] = 0;
+            }
+        }
+        float4* np = n4 + 8 * (size_t)fout[i];
+        for (int a = 0; a < 6; ++a) np[a] = make_float4(q[a][0], q[a][1], q[a][2], q[a][3]);
+        reinterpret_cast<int4*>(np)[6] = make_int4(ref[0], ref[1], ref[2], ref[3]);
+        reinterpret_cast<int4*>(np)[7] = make_int4(cnt[0], cnt[1], cnt[2], cnt[3]);
+    }
+}
+
+}  // namespace
+
+// Replaces the context's binary nodes (num_binary internal nodes, root 0)
+// with 4-wide nodes; stats get the 4-wide node count and depth.
+dl_status collapse_bvh4(Ctx* c, uint32_t num_binary, dl_accel_stats* stats) {
+    const int cap = (int)std::max<uint32_t>(1, num_binary);
+    Scratch sc{c};
+    int *fbin[2], *fout[2], *nint, *off;
+    for (int b = 0; b < 2; b++) {
+        DL_TRY(sc.get(&fbin[b], cap));
+        DL_TRY(sc.get(&fout[b], cap));
+    }
+    DL_TRY(sc.get(&nint, cap + 1));
+    DL_TRY(sc.get(&off, cap + 1));
+    size_t scan_bytes = 0;
+    DL_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, nint, off, cap + 1, c->stream));
+    void* temp = nullptr;
+    DL_TRY(sc.get((char**)&temp, std::max<size_t>(1, scan_bytes)));
+    void* p4 = nullptr;
+    DL_HIP_CHECK(c, dmalloc(c, &p4, (size_t)cap * kNode4Bytes));
+    int* host_cnt = nullptr;
+    DL_HIP_CHECK(c, hipHostMalloc((void**)&host_cnt, sizeof(int), hipHostMallocDefault));
+    struct HostFree {
+        int* p;
+        ~HostFree() { (void)hipHostFree(p); }
+    } hf{host_cnt};
+    const int zero = 0;
+    DL_HIP_CHECK(c, hipMemcpyAsync(fbin[0], &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    DL_HIP_CHECK(c, hipMemcpyAsync(fout[0], &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const int grid = std::max(1, c->num_cus) * 8;
+    int nf = 1, next = 1, cur = 0, levels = 0;
+    while (nf > 0) {
+        if (++levels > kMaxBinaryDepth || next > cap) {
+            dfree(c, p4);
+            return set_error(c, DL_ERR_STATE, "4-wide BVH collapse exceeded its bounds");
+        }
+        k_c4_count<<<grid, kB, 0, c->stream>>>(c->ds.nodes, fbin[cur], nf, nint);
+        DL_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(temp, scan_bytes, nint, off, nf + 1, c->stream));
+        k_c4_emit<<<grid, kB, 0, c->stream>>>(c->ds.nodes, fbin[cur], fout[cur], nf, off, next, (float4*)p4,
+                                              fbin[cur ^ 1], fout[cur ^ 1]);
+        DL_HIP_CHECK(c, hipMemcpyAsync(host_cnt, off + nf, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        DL_HIP_CHECK(c, hipGetLastError());
+        next += host_cnt[0];
+        nf = host_cnt[0];
+        cur ^= 1;
+    }
+    dfree(c, c->ds.nodes);
+    c->ds.nodes = (float4*)p4;
+    stats->num_nodes = (uint32_t)next;
+    stats->max_depth = (uint32_t)levels;
     return DL_OK;
 }
 

@@ -1,5 +1,7 @@
-// BVH traversal for gfx950: one ray per lane, child-pair nodes (bvh_layout.h),
-// persistent lanes with per-lane ray refill,
+// BVH traversal for gfx950: one ray per lane, 4-wide nodes (bvh_layout.h:
+// one 128-byte line per visit, four slab tests, the nearest hit internal
+// child next and the others stacked farthest first), persistent lanes with
+// per-lane ray refill,
 // per-lane traversal stack staged in LDS with a lane-interleaved layout
 // (entry k of lane l at stack[k * blockDim + l]: every wave touches 64
 // consecutive dwords whatever the stack depths, so the accesses are
@@ -57,21 +59,27 @@ struct LaneStack {
     }
 };
 
-__device__ __forceinline__ void slab2(const float4& r0, const float4& r1, const float4& r2, const vec3& o,
-                                      const vec3& rcp, float tmin, float tmax, bool& h0, float& n0, bool& h1,
-                                      float& n1) {
-    float a0 = (r0.x - o.x) * rcp.x, b0 = (r0.y - o.x) * rcp.x;
-    float a1 = (r1.x - o.y) * rcp.y, b1 = (r1.y - o.y) * rcp.y;
-    float a2 = (r2.x - o.z) * rcp.z, b2 = (r2.y - o.z) * rcp.z;
-    n0 = fmaxf(fmaxf(fminf(a0, b0), fminf(a1, b1)), fmaxf(fminf(a2, b2), tmin));
-    float f0 = fminf(fminf(fmaxf(a0, b0), fmaxf(a1, b1)), fminf(fmaxf(a2, b2), tmax));
-    h0 = n0 <= f0 * kFarSlack;
-    float c0 = (r0.z - o.x) * rcp.x, d0 = (r0.w - o.x) * rcp.x;
-    float c1 = (r1.z - o.y) * rcp.y, d1 = (r1.w - o.y) * rcp.y;
-    float c2 = (r2.z - o.z) * rcp.z, d2 = (r2.w - o.z) * rcp.z;
-    n1 = fmaxf(fmaxf(fminf(c0, d0), fminf(c1, d1)), fmaxf(fminf(c2, d2), tmin));
-    float f1 = fminf(fminf(fmaxf(c0, d0), fmaxf(c1, d1)), fminf(fmaxf(c2, d2), tmax));
-    h1 = n1 <= f1 * kFarSlack;
+// Slab test of one child box (lo, hi per axis): entry distance n and
+// whether [n, far] meets [tmin, tmax] (far widened by kFarSlack).
+__device__ __forceinline__ bool slab1(float lx, float hx, float ly, float hy, float lz, float hz, const vec3& o,
+                                      const vec3& rcp, float tmin, float tmax, float& n) {
+    const float a0 = (lx - o.x) * rcp.x, b0 = (hx - o.x) * rcp.x;
+    const float a1 = (ly - o.y) * rcp.y, b1 = (hy - o.y) * rcp.y;
+    const float a2 = (lz - o.z) * rcp.z, b2 = (hz - o.z) * rcp.z;
+    n = fmaxf(fmaxf(fminf(a0, b0), fminf(a1, b1)), fmaxf(fminf(a2, b2), tmin));
+    const float f = fminf(fminf(fmaxf(a0, b0), fmaxf(a1, b1)), fminf(fmaxf(a2, b2), tmax));
+    return n <= f * kFarSlack;
+}
+
+// Compare-exchange of (key, ref) pairs: smaller key first.
+__device__ __forceinline__ void cswap(float& ka, int& ra, float& kb, int& rb) {
+    const bool s = kb < ka;
+    const float k = s ? kb : ka;
+    kb = s ? ka : kb;
+    ka = k;
+    const int r = s ? rb : ra;
+    rb = s ? ra : rb;
+    ra = r;
 }
 
 // Moller-Trumbore on a pre-gathered triangle; returns false on a miss.
@@ -168,23 +176,26 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
         if (!__ballot(active)) break;
         if (!active) continue;
 
-        const float4* np = bvh.nodes + 4 * cur;
-        const float4 r0 = np[0], r1 = np[1], r2 = np[2];
-        const int4 r3 = reinterpret_cast<const int4*>(np)[3];
+        const float4* np = bvh.nodes + 8 * (size_t)cur;
+        const float4 qx = np[0], qX = np[1], qy = np[2], qY = np[3], qz = np[4], qZ = np[5];
+        const int4 ref = reinterpret_cast<const int4*>(np)[6];
+        const int4 cnt4 = reinterpret_cast<const int4*>(np)[7];
         if (COUNT) cnt.nodes++;
-        bool h0, h1;
-        float n0, n1;
-        slab2(r0, r1, r2, o, rcp, tmin, best_t, h0, n0, h1, n1);
+        float n0, n1, n2, n3;
+        const bool h0 = slab1(qx.x, qX.x, qy.x, qY.x, qz.x, qZ.x, o, rcp, tmin, best_t, n0);
+        const bool h1 = slab1(qx.y, qX.y, qy.y, qY.y, qz.y, qZ.y, o, rcp, tmin, best_t, n1);
+        const bool h2 = slab1(qx.z, qX.z, qy.z, qY.z, qz.z, qZ.z, o, rcp, tmin, best_t, n2);
+        const bool h3 = slab1(qx.w, qX.w, qy.w, qY.w, qz.w, qZ.w, o, rcp, tmin, best_t, n3);
         bool hit_any = false;
-        // Triangles of the hit leaf children, both sides in one loop (the
-        // closest-hit result is the minimum over (t, id) and so does not
-        // depend on the order).
-        const bool l0 = h0 && r3.x < 0, l1 = h1 && r3.y < 0;
-        const int c0 = l0 ? r3.z : 0;
-        const int ntri = c0 + (l1 ? r3.w : 0);
-        const int f0 = ~r3.x, f1 = ~r3.y;
+        // Triangles of all hit leaf children in one loop (the closest-hit
+        // result is the minimum over (t, id) and so does not depend on the
+        // order).
+        const int s0 = (h0 && ref.x < 0) ? cnt4.x : 0;
+        const int s1 = s0 + ((h1 && ref.y < 0) ? cnt4.y : 0);
+        const int s2 = s1 + ((h2 && ref.z < 0) ? cnt4.z : 0);
+        const int ntri = s2 + ((h3 && ref.w < 0) ? cnt4.w : 0);
         for (int j = 0; j < ntri; ++j) {
-            const int k = j < c0 ? f0 + j : f1 + (j - c0);
+            const int k = j < s0 ? ~ref.x + j : j < s1 ? ~ref.y + (j - s0) : j < s2 ? ~ref.z + (j - s1) : ~ref.w + (j - s2);
             const float4* tp = bvh.tris + 3 * k;
             const float4 ta = tp[0], tb = tp[1], tc = tp[2];
             if (COUNT) cnt.tris++;
@@ -203,20 +214,25 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
                 }
             }
         }
-        if (h0 && r3.x < 0) h0 = false;
-        if (h1 && r3.y < 0) h1 = false;
+        // hit internal children: keys = entry distances (+inf: not taken),
+        // sorted by a 5-exchange network; the nearest is visited next and
+        // the others are stacked farthest first
+        float k0 = (h0 && ref.x >= 0) ? n0 : INFINITY, k1 = (h1 && ref.y >= 0) ? n1 : INFINITY;
+        float k2 = (h2 && ref.z >= 0) ? n2 : INFINITY, k3 = (h3 && ref.w >= 0) ? n3 : INFINITY;
+        int c0 = ref.x, c1 = ref.y, c2 = ref.z, c3 = ref.w;
+        cswap(k0, c0, k1, c1);
+        cswap(k2, c2, k3, c3);
+        cswap(k0, c0, k2, c2);
+        cswap(k1, c1, k3, c3);
+        cswap(k1, c1, k2, c2);
         bool done = false;
         if (ANY && hit_any) {
             done = true;
-        } else if (h0 && h1) {
-            const bool first0 = ANY ? true : n0 <= n1;
-            const int nearc = first0 ? r3.x : r3.y, farc = first0 ? r3.y : r3.x;
-            stk.push(sp, farc);
-            cur = nearc;
-        } else if (h0) {
-            cur = r3.x;
-        } else if (h1) {
-            cur = r3.y;
+        } else if (k0 != INFINITY) {
+            if (k3 != INFINITY) stk.push(sp, c3);
+            if (k2 != INFINITY) stk.push(sp, c2);
+            if (k1 != INFINITY) stk.push(sp, c1);
+            cur = c0;
         } else if (sp == 0) {
             done = true;
         } else {
