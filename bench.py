@@ -33,6 +33,7 @@ Profile helpers (they read rocprofv3 CSV output and write the summaries that
 `roofline.traffic` and the kernel-time tables come from):
   bench.py --summarize-stats DIR OUT.md       kernel time table
   bench.py --summarize-pmc FETCH_DIR WRITE_DIR OUT.json
+  bench.py --summarize-counters DIR [DIR ...] OUT.md   occupancy, VALU, L2 table
 """
 import argparse
 import csv
@@ -151,6 +152,52 @@ def summarize_pmc(fetch_dir, write_dir, out_json, copy_bytes, commands):
     with open(out_json, "w") as fo:
         json.dump(out, fo, indent=1)
     print(json.dumps({k: out[k] for k in list(out)[:12]}, indent=1))
+
+
+def summarize_counters(dirs, out_md, command, kernels):
+    """Per-kernel table of rocprofv3 counters (one column per counter, the
+    dispatch-time-weighted mean over the kernel's dispatches) from several
+    PMC runs' counter_collection.csv files."""
+    acc = {}
+    names = []
+    for d in dirs:
+        path = sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True))[0]
+        for r in csv.DictReader(open(path)):
+            k = short_name(r["Kernel_Name"])
+            if k not in kernels:
+                continue
+            c = r["Counter_Name"]
+            if c not in names:
+                names.append(c)
+            w = max(1.0, float(r["End_Timestamp"]) - float(r["Start_Timestamp"]))
+            s = acc.setdefault((k, c), [0.0, 0.0, 0])
+            s[0] += w * float(r["Counter_Value"])
+            s[1] += w
+            s[2] += 1
+    lines = ["# Hardware counters: %s bench (1920x1080, 32 spp per pass)" % WORKLOAD, "",
+             "Commands (repository root, MI355X):", "", "```", command, "```", "",
+             "Dispatch-time-weighted means over each kernel's dispatches.  VALUBusy: % of GPU time the vector ALUs "
+             "issue; VALUUtilization: % of active lanes per VALU instruction; OccupancyPercent: resident waves as "
+             "% of the maximum; L2 hit: TCC_HIT / (TCC_HIT + TCC_MISS); MemUnitStalled: % of GPU time the memory "
+             "unit is stalled.", "",
+             "| kernel | dispatches | " + " | ".join(names) + " | L2 hit % |",
+             "|---|---:|" + "---:|" * (len(names) + 1)]
+    for k in kernels:
+        row = []
+        n = 0
+        vals = {}
+        for c in names:
+            s = acc.get((k, c))
+            v = s[0] / s[1] if s else float("nan")
+            vals[c] = v
+            n = max(n, s[2] if s else 0)
+            row.append("%.3g" % v)
+        h, m = vals.get("TCC_HIT_sum"), vals.get("TCC_MISS_sum")
+        l2 = 100.0 * h / (h + m) if h is not None and m is not None and h + m > 0 else float("nan")
+        lines.append("| %s | %d | %s | %.1f |" % (k, n, " | ".join(row), l2))
+    with open(out_md, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
 
 
 def _run_jobs(svc, sids, pool, jobs, arrivals):
@@ -411,6 +458,8 @@ def main():
                     help="after the timed steps, run the plain copy kernel twice over this many MiB")
     ap.add_argument("--summarize-stats", nargs=2, metavar=("DIR", "OUT_MD"))
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"))
+    ap.add_argument("--summarize-counters", nargs="+", metavar="DIR_OR_OUT",
+                    help="PMC run directories, then the output .md")
     ap.add_argument("--command", default="", help="command line recorded in a summary")
     ap.add_argument("--title", default="", help="title of a kernel-time summary")
     ap.add_argument("--copy-mib", type=int, default=1024, help="calibration copy size used by the PMC runs")
@@ -430,6 +479,11 @@ def main():
     args = ap.parse_args()
     if args.summarize_stats:
         summarize_stats(args.summarize_stats[0], args.summarize_stats[1], args.command, args.title)
+        return
+    if args.summarize_counters:
+        summarize_counters(args.summarize_counters[:-1], args.summarize_counters[-1], args.command,
+                           ["k_trace_queue<false>", "k_shadow_queue", "k_post", "k_shade<0, false>",
+                            "k_shade<5, false>", "k_film_tiled"])
         return
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, copy_bytes=args.copy_mib << 20, commands=args.command)
