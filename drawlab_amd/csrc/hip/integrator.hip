@@ -794,17 +794,28 @@ __global__ __launch_bounds__(kStageBlock) void k_film(PassParams pp, uint32_t k,
 
 // Tiled gather for filter radii up to kFilmHalo - 1: one 16x16 pixel tile per
 // block; each sample layer's tile plus halo is staged in LDS once instead of
-// being fetched by each of the (2R+1)^2 pixels it can reach.
+// being fetched by each of the (2R+1)^2 pixels it can reach.  Staging also
+// computes what film_add derives from the sample alone -- its 32x32 block
+// origin, position relative to it and the range of film pixels its
+// footprint covers (empty for an invalid sample) -- so a (pixel, sample)
+// pair costs a range test, two table lookups and the accumulation, with
+// film_add's arithmetic and order.
 constexpr int kFilmTile = 16;
-constexpr int kFilmHalo = 8;
+constexpr int kFilmHalo = 5;
 constexpr int kFilmSpan = kFilmTile + 2 * kFilmHalo;
 __global__ __launch_bounds__(kFilmTile* kFilmTile) void k_film_tiled(PassParams pp, uint32_t k, dl_film_desc film,
                                                                      RenderBufs rs) {
     __shared__ float4 s_rad[kFilmSpan * kFilmSpan];
-    __shared__ float2 s_pos[kFilmSpan * kFilmSpan];
+    __shared__ float4 s_rel[kFilmSpan * kFilmSpan];  // plx, ply, block origin x, y (int bits)
+    __shared__ int4 s_cov[kFilmSpan * kFilmSpan];    // covered film pixels: x lo, x hi, y lo, y hi
     const uint32_t W = pp.W, H = pp.H, WH = W * H;
-    const int R = (int)ceilf(film.radius) + 1;
+    // A sample at P in [px, px + 1] reaches the film pixels with
+    // ceil(P - 1/2 - r) <= q <= floor(P - 1/2 + r), i.e. within
+    // floor(r + 1/2) of its own pixel px.
+    const int R = (int)floorf(film.radius + 0.5f);
     const int S = kFilmTile + 2 * R;  // staged span
+    const int b = film.border;
+    const float r = film.radius;
     const int tx0 = (int)blockIdx.x * kFilmTile, ty0 = (int)blockIdx.y * kFilmTile;
     const int qx = tx0 + (int)(threadIdx.x % kFilmTile), qy = ty0 + (int)(threadIdx.x / kFilmTile);
     const bool inside = qx < (int)W && qy < (int)H;
@@ -816,8 +827,20 @@ __global__ __launch_bounds__(kFilmTile* kFilmTile) void k_film_tiled(PassParams 
             const int gx = tx0 - R + e % S, gy = ty0 - R + e / S;
             if (gx >= 0 && gx < (int)W && gy >= 0 && gy < (int)H) {
                 const uint32_t sp = j * WH + (uint32_t)gy * W + (uint32_t)gx;
-                s_rad[e] = rs.rad[sp];
-                s_pos[e] = rs.pos[sp];
+                const float4 L = rs.rad[sp];
+                const float2 P = rs.pos[sp];
+                // film_add's sample-only terms (block.cpp:68-128)
+                const int oy = (gy / 32) * 32, sy = min(32, (int)H - oy);
+                const int ox = (gx / 32) * 32, sx = min(32, (int)W - ox);
+                const float plx = P.x - 0.5f - (float)(ox - b);
+                const float ply = P.y - 0.5f - (float)(oy - b);
+                const int x0 = max((int)ceilf(plx - r), 0), x1 = min((int)floorf(plx + r), sx + 2 * b - 1);
+                const int y0 = max((int)ceilf(ply - r), 0), y1 = min((int)floorf(ply + r), sy + 2 * b - 1);
+                const bool ok = sample_valid(L);
+                if (!ok && gx >= tx0 && gx < tx0 + kFilmTile && gy >= ty0 && gy < ty0 + kFilmTile) invalid++;
+                s_rad[e] = L;
+                s_rel[e] = make_float4(plx, ply, __int_as_float(ox - b), __int_as_float(oy - b));
+                s_cov[e] = ok ? make_int4(x0 + ox - b, x1 + ox - b, y0 + oy - b, y1 + oy - b) : make_int4(1, 0, 1, 0);
             }
         }
         __syncthreads();
@@ -825,12 +848,17 @@ __global__ __launch_bounds__(kFilmTile* kFilmTile) void k_film_tiled(PassParams 
             for (int py = max(0, qy - R); py <= min((int)H - 1, qy + R); ++py)
                 for (int px = max(0, qx - R); px <= min((int)W - 1, qx + R); ++px) {
                     const int e = (py - (ty0 - R)) * S + (px - (tx0 - R));
+                    const int4 cv = s_cov[e];
+                    if (qx < cv.x || qx > cv.y || qy < cv.z || qy > cv.w) continue;
                     const float4 L = s_rad[e];
-                    if (!sample_valid(L)) {
-                        if (px == qx && py == qy) invalid++;
-                        continue;
-                    }
-                    film_add(film, W, H, qx, qy, px, py, L, s_pos[e], acc);
+                    const float4 rel = s_rel[e];
+                    const int xl = qx - __float_as_int(rel.z), yl = qy - __float_as_int(rel.w);
+                    const float wx = film.table[(int)(fabsf((float)xl - rel.x) * film.lookup)];
+                    const float wy = film.table[(int)(fabsf((float)yl - rel.y) * film.lookup)];
+                    acc.x += (L.x * wx) * wy;
+                    acc.y += (L.y * wx) * wy;
+                    acc.z += (L.z * wx) * wy;
+                    acc.w += (1.0f * wx) * wy;
                 }
         }
         __syncthreads();
