@@ -28,7 +28,7 @@ EXPORTED_SYMBOLS = (
     "dl_hip_render", "dl_hip_render_paths", "dl_hip_malloc", "dl_hip_free", "dl_hip_memcpy_h2d",
     "dl_hip_memcpy_d2h", "dl_hip_memcpy_d2d", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_camera_perspective", "dl_film_filter",
     "dl_mesh_tangents", "dl_hip_memory_usage", "dl_hip_trim", "dl_hip_set_camera",
-    "dl_hip_scene_keep", "dl_hip_scene_select", "dl_hip_scene_drop",
+    "dl_hip_scene_keep", "dl_hip_scene_select", "dl_hip_scene_drop", "dl_hip_accel_export",
     "dl_hip_service_start", "dl_hip_service_add_scene", "dl_hip_service_submit", "dl_hip_service_poll",
     "dl_hip_service_wait", "dl_hip_service_result", "dl_hip_service_cancel", "dl_hip_service_stats",
     "dl_hip_service_stop", "dl_hip_service_last_error",
@@ -141,6 +141,7 @@ def lib():
             "dl_hip_scene_keep": (i32, [P, c.c_uint64]),
             "dl_hip_scene_select": (i32, [P, c.c_uint64]),
             "dl_hip_scene_drop": (i32, [P, c.c_uint64]),
+            "dl_hip_accel_export": (i32, [P, P, P]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -342,6 +343,15 @@ class Backend:
 
     def sync(self):
         self._check(self.L.dl_hip_sync(self.h))
+
+    def accel_export(self):
+        """(nodes, triangles) the traversal reads, as float32 arrays of 32
+        and 12 words per record (layouts in csrc/bvh_layout.h)."""
+        st = self.accel_stats
+        nodes = np.zeros(st["node_bytes"] // 4, np.float32)
+        tris = np.zeros(st["tri_bytes"] // 4, np.float32)
+        self._check(self.L.dl_hip_accel_export(self.h, _p(nodes), _p(tris)))
+        return nodes, tris
 
     # ---- device memory and resident scenes --------------------------------
     def memory_usage(self):

@@ -476,6 +476,16 @@ dl_status dl_hip_trim(void* h) {
     free_all_scenes(c);
     return DL_OK;
 }
+dl_status dl_hip_accel_export(void* h, void* nodes, void* tris) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    if (!c->has_accel) return set_error(c, DL_ERR_STATE, "accel_export before build_accel");
+    (void)hipSetDevice(c->device);
+    DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (nodes) DL_HIP_CHECK(c, hipMemcpy(nodes, c->ds.nodes, c->accel.node_bytes, hipMemcpyDeviceToHost));
+    if (tris) DL_HIP_CHECK(c, hipMemcpy(tris, c->ds.tris, c->accel.tri_bytes, hipMemcpyDeviceToHost));
+    return DL_OK;
+}
 dl_status dl_hip_scene_keep(void* h, uint64_t key) {
     Ctx* c = (Ctx*)h;
     if (!c) return DL_ERR_INVALID;

@@ -245,6 +245,13 @@ dl_status dl_hip_set_camera(void* ctx, const dl_camera_desc* camera);
    render service under memory pressure.) */
 dl_status dl_hip_trim(void* ctx);
 
+/* Copies the acceleration structure the traversal kernels read -- 4-wide
+   nodes and pre-gathered triangles, layouts in drawlab_amd/csrc/bvh_layout.h
+   -- to host buffers of stats.node_bytes and stats.tri_bytes bytes (either
+   may be NULL).  (No reference counterpart: for CPU replays of the
+   traversal that recompute its node visits and triangle tests.) */
+dl_status dl_hip_accel_export(void* ctx, void* nodes, void* tris);
+
 /* Resident scenes (no reference counterpart; for callers that render
    several scenes in turn, such as the render service).
    dl_hip_scene_keep tags the current scene (uploaded and built) with a

@@ -65,6 +65,7 @@ def lib():
             "dlo_bsdf_query": (None, [_P, i32, _f32p, u32, _f32p]),
             "dlo_env_tables": (None, [_P, _f32p, _u32p, _f32p]),
             "dlo_spiral_blocks": (u32, [i32, i32, _P]),
+            "dlo_bvh4_replay": (None, [_f32p, _f32p, _f32p, u32, i32, _P, _P, _P, _P, _P, _P, _P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -305,6 +306,27 @@ def from_desc(desc, build=True, camera_matrices=None):
     if build:
         s.build()
     return s
+
+
+def bvh4_replay(nodes, tris, rays, any_hit=False):
+    """CPU replay of the HIP backend's traversal over its exported 4-wide
+    nodes and triangles (oracle/replay.cpp).  Returns (prim, t, u, v,
+    node_visits, tri_tests) for closest hits, (occ, node_visits, tri_tests)
+    for any hits."""
+    rays = np.ascontiguousarray(np.asarray(rays, np.float32).reshape(8, -1))
+    n = rays.shape[1]
+    nodes = np.ascontiguousarray(nodes, np.float32)
+    tris = np.ascontiguousarray(tris, np.float32)
+    cn = np.zeros(n, np.uint32)
+    ct = np.zeros(n, np.uint32)
+    if any_hit:
+        occ = np.zeros(n, np.uint8)
+        lib().dlo_bvh4_replay(nodes, tris, rays, n, 1, None, None, None, None, _ptr(occ), _ptr(cn), _ptr(ct))
+        return occ, cn, ct
+    prim = np.zeros(n, np.uint32)
+    t, u, v = (np.zeros(n, np.float32) for _ in range(3))
+    lib().dlo_bvh4_replay(nodes, tris, rays, n, 0, _ptr(prim), _ptr(t), _ptr(u), _ptr(v), None, _ptr(cn), _ptr(ct))
+    return prim, t, u, v, cn, ct
 
 
 def pcg32(initstate, initseq, n):

@@ -65,3 +65,40 @@ def test_displaced_sphere_full_size_hits(backend, builder):
             continue
         nbad, idx = compare_hits(backend.trace_closest(rays), osc.trace_closest(rays, oracle.ACCEL_OCTREE_ROBUST))
         assert nbad == 0, "%s: %d differ" % (name, nbad)
+
+
+@pytest.mark.parametrize("builder", [drawlab_amd.BUILD_GPU, drawlab_amd.BUILD_HOST_SAH, drawlab_amd.BUILD_GPU_LBVH],
+                         ids=["gpu", "host_sah", "lbvh"])
+def test_traversal_counts_match_cpu_replay(backend, builder):
+    """The node visits and triangle tests of every ray -- the counts behind
+    the roofline's algorithmic bytes -- and the hits themselves equal a CPU
+    replay of the traversal over the exported 4-wide BVH (oracle/replay.cpp),
+    including rays with NaN components, which finish with no visit."""
+    desc = scenes.displaced_sphere(width=64, height=36, grid=128, glass_level=3, env_size=(64, 32))
+    backend.upload_scene(desc)
+    st = backend.build_accel(builder)
+    nodes, tris = backend.accel_export()
+    assert nodes.size == st["num_nodes"] * 32 and tris.size == st["num_prims"] * 12
+    osc = oracle.from_desc(desc)
+    batches = ray_batches(desc, osc, 3000, seed=7)
+    nan_rays = np.array(batches["random"][:, :64])
+    nan_rays[np.arange(64) % 8, np.arange(64)] = np.nan
+    batches["nan"] = nan_rays
+    total = 0
+    for name, rays in batches.items():
+        rays = np.ascontiguousarray(rays, np.float32)
+        if name == "shadow":
+            occ, _, _ = oracle.bvh4_replay(nodes, tris, rays, any_hit=True)
+            assert np.array_equal(backend.trace_shadow(rays), occ), name
+            continue
+        g = backend.trace_closest(rays, counted=True)
+        r = oracle.bvh4_replay(nodes, tris, rays)
+        for a, b, what in zip(g, r, ("prim", "t", "u", "v", "node visits", "triangle tests")):
+            a, b = np.asarray(a), np.asarray(b)
+            bad = np.flatnonzero(a.view(np.uint32) != b.view(np.uint32))
+            assert bad.size == 0, "%s %s: %d rays differ (first %s)" % (name, what, bad.size, bad[:8])
+        total += int(r[4].sum())
+        if name == "nan":
+            assert not r[4].any()
+    print("replayed node visits", total, flush=True)
+    assert total > 0
