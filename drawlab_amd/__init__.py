@@ -47,7 +47,12 @@ class dl_mesh_desc(ctypes.Structure):
 
 
 class dl_material_desc(ctypes.Structure):
-    _fields_ = [("type", ctypes.c_int32), ("params", ctypes.c_float * 16)]
+    _fields_ = [("type", ctypes.c_int32), ("params", ctypes.c_float * 16), ("tex", ctypes.c_int32 * 5),
+                ("maps_object_space", ctypes.c_int32)]
+
+
+class dl_texture_desc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("rgb", ctypes.c_void_p)]
 
 
 class dl_light_desc(ctypes.Structure):
@@ -73,7 +78,8 @@ class dl_scene_desc(ctypes.Structure):
     _fields_ = [("meshes", ctypes.POINTER(dl_mesh_desc)), ("num_meshes", ctypes.c_uint32),
                 ("materials", ctypes.POINTER(dl_material_desc)), ("num_materials", ctypes.c_uint32),
                 ("lights", ctypes.POINTER(dl_light_desc)), ("num_lights", ctypes.c_uint32),
-                ("camera", dl_camera_desc), ("film", dl_film_desc), ("bg_color", ctypes.c_float * 3)]
+                ("camera", dl_camera_desc), ("film", dl_film_desc), ("bg_color", ctypes.c_float * 3),
+                ("textures", ctypes.POINTER(dl_texture_desc)), ("num_textures", ctypes.c_uint32)]
 
 
 class dl_accel_stats(ctypes.Structure):
@@ -263,6 +269,15 @@ def scene_descriptor(desc: SceneDesc):
         mats[i].type = b.type
         for k, v in enumerate(b.params):
             mats[i].params[k] = v
+        for slot, t in (b.textures or {}).items():
+            mats[i].tex[slot] = int(t) + 1
+        mats[i].maps_object_space = int(bool(b.maps_object_space))
+    texs = (dl_texture_desc * max(1, len(desc.textures)))()
+    for i, rgb in enumerate(desc.textures):
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        keep.append(rgb)
+        texs[i].height, texs[i].width = rgb.shape[0], rgb.shape[1]
+        texs[i].rgb = _p(rgb)
     sd = dl_scene_desc()
     sd.meshes = meshes
     sd.num_meshes = len(desc.meshes)
@@ -277,7 +292,9 @@ def scene_descriptor(desc: SceneDesc):
     fd = getattr(desc, "film", None)
     sd.film = fd if fd is not None else film_desc(desc.rfilter)
     sd.bg_color[:] = desc.bg_color
-    return sd, (keep, meshes, mats, lights)
+    sd.textures = texs
+    sd.num_textures = len(desc.textures)
+    return sd, (keep, meshes, mats, lights, texs)
 
 
 class Backend:

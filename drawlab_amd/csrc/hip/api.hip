@@ -181,9 +181,21 @@ dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
     release_current(c);
     if (sd->num_meshes == 0) return set_error(c, DL_ERR_INVALID, "scene has no meshes");
     if (sd->camera.width <= 0 || sd->camera.height <= 0) return set_error(c, DL_ERR_INVALID, "bad camera size");
-    for (uint32_t i = 0; i < sd->num_materials; i++)
+    for (uint32_t i = 0; i < sd->num_materials; i++) {
         if (sd->materials[i].type < 0 || sd->materials[i].type >= DL_BSDF_COUNT)
             return set_error(c, DL_ERR_INVALID, "unknown material type");
+        for (int k = 0; k < DL_TEX_SLOTS; k++)
+            if (sd->materials[i].tex[k] < 0 || (uint32_t)sd->materials[i].tex[k] > sd->num_textures)
+                return set_error(c, DL_ERR_INVALID, "material texture index out of range");
+    }
+    uint64_t texels = 0;
+    for (uint32_t t = 0; t < sd->num_textures; t++) {
+        const dl_texture_desc& td = sd->textures[t];
+        if (!td.rgb || td.width <= 0 || td.height <= 0 || (uint64_t)td.width * (uint64_t)td.height > (1ull << 28))
+            return set_error(c, DL_ERR_INVALID, "invalid texture");
+        texels += (uint64_t)td.width * (uint64_t)td.height;
+    }
+    if (texels >= 0x7fffffffull) return set_error(c, DL_ERR_INVALID, "textures too large");
     uint64_t nv = 0, nf = 0;
     for (uint32_t m = 0; m < sd->num_meshes; m++) {
         const dl_mesh_desc& md = sd->meshes[m];
@@ -200,7 +212,7 @@ dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
     c->nprims = (uint32_t)nf;
     c->nverts = (uint32_t)nv;
 
-    std::vector<float> pos(3 * nv), nrm(3 * nv, 0.0f), tan(3 * nv, 0.0f);
+    std::vector<float> pos(3 * nv), nrm(3 * nv, 0.0f), tan(3 * nv, 0.0f), uvs(2 * nv, 0.0f);
     std::vector<uint32_t> idx(3 * nf), prim_mesh(nf);
     c->tri_verts.assign(9 * nf, 0.0f);
     c->mesh_recs.assign(sd->num_meshes, MeshRec{});
@@ -218,6 +230,7 @@ dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
         r.has_tangents = md.tangents != nullptr || md.texcoords != nullptr;
         std::memcpy(&pos[3 * vo], md.positions, sizeof(float) * 3 * md.num_vertices);
         if (md.normals) std::memcpy(&nrm[3 * vo], md.normals, sizeof(float) * 3 * md.num_vertices);
+        if (md.texcoords) std::memcpy(&uvs[2 * vo], md.texcoords, sizeof(float) * 2 * md.num_vertices);
         if (md.tangents)
             std::memcpy(&tan[3 * vo], md.tangents, sizeof(float) * 3 * md.num_vertices);
         else if (md.texcoords)
@@ -312,6 +325,22 @@ dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
     DL_TRY(upload(c, &ds.env_q, env_q.data(), env_q.size()));
     DL_TRY(upload(c, &ds.env_alias, env_alias.data(), env_alias.size()));
     DL_TRY(upload(c, &ds.env_pdf, env_pdf.data(), env_pdf.size()));
+    DL_TRY(upload(c, &ds.uv, uvs.data(), uvs.size()));
+    {
+        std::vector<float4> tx((size_t)texels);
+        std::vector<int4> info(sd->num_textures);
+        size_t o = 0;
+        for (uint32_t t = 0; t < sd->num_textures; t++) {
+            const dl_texture_desc& td = sd->textures[t];
+            const size_t n = (size_t)td.width * (size_t)td.height;
+            for (size_t i = 0; i < n; i++) tx[o + i] = make_float4(td.rgb[3 * i], td.rgb[3 * i + 1], td.rgb[3 * i + 2], 0.f);
+            info[t] = make_int4((int)o, td.width, td.height, 0);
+            o += n;
+        }
+        DL_TRY(upload(c, &ds.texels, tx.data(), tx.size()));
+        DL_TRY(upload(c, &ds.tex_info, info.data(), info.size()));
+        DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));  // tx / info leave scope
+    }
     DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     c->has_scene = true;
     return DL_OK;

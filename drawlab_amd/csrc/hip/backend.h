@@ -25,6 +25,11 @@ struct DevScene {
     float* tan = nullptr;        // 3 * nv   (zeros for meshes without tangents)
     uint32_t* idx = nullptr;     // 3 * nprims (global vertex indices)
     uint32_t* prim_mesh = nullptr;  // nprims -> mesh index
+    float* uv = nullptr;         // 2 * nv   (texture coordinates; zeros for meshes without)
+    // bitmap textures: texels (rgb, 0) of all textures, per texture
+    // (first texel, width, height, 0)
+    float4* texels = nullptr;
+    int4* tex_info = nullptr;
     // per mesh records (MeshRec)
     void* meshes = nullptr;
     // materials (dl_material_desc) and lights (LightRec)

@@ -417,9 +417,9 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                 rng.state = rs.rng[p];
                 rng.inc = ((uint64_t)(pp.sample_base + p / WH) << 1u) | 1u;
                 Its its;
-                make_its(s, prim, h.z, h.w, type == DL_BSDF_ANISO_GGX, its);
-                const int mat = s.meshes[its.mesh].material;
-                const float* prm = s.mats[mat].params;
+                int mat;
+                float prm[8];
+                shading_point(s, prim, h.z, h.w, type, its, mat, prm);
                 const float* gm = GUIDED ? gv.mix + (size_t)i * kGMix : nullptr;
                 uint32_t flags = 0;
                 // ---- emitter sampling (Scene::sampleEmitterDirection, scene.cpp:139-170)
@@ -700,8 +700,9 @@ __global__ __launch_bounds__(kGWaves * 64) void k_guide_infer(PassParams pp, Sce
             const uint32_t p = queue[i];
             const float4 h = rs.hit[p];
             Its its;
-            make_its(s, __float_as_uint(h.x), h.z, h.w, TYPE == DL_BSDF_ANISO_GGX, its);
-            const float* prm = s.mats[s.meshes[its.mesh].material].params;
+            int mat;
+            float prm[8];
+            shading_point(s, __float_as_uint(h.x), h.z, h.w, TYPE, its, mat, prm);
             pn = v3((its.p.x - gv.lo[0]) * gv.inv[0], (its.p.y - gv.lo[1]) * gv.inv[1], (its.p.z - gv.lo[2]) * gv.inv[2]);
             wo = neg(normalized(f3(rs.ray_d[p])));
             nn = its.sh.n;
@@ -977,6 +978,9 @@ SceneView scene_view(Ctx* c) {
     s.env_q = d.env_q;
     s.env_alias = d.env_alias;
     s.env_pdf = d.env_pdf;
+    s.uv = d.uv;
+    s.texels = d.texels;
+    s.tex_info = d.tex_info;
     s.num_lights = (int)c->light_recs.size();
     s.env_light = c->env_light;
     for (int k = 0; k < 3; k++) s.bg[k] = c->bg[k];

@@ -21,6 +21,9 @@ struct SceneView {
     const float* __restrict__ env_q;
     const uint32_t* __restrict__ env_alias;
     const float* __restrict__ env_pdf;
+    const float* __restrict__ uv;        // texture coordinates, 2 per vertex
+    const float4* __restrict__ texels;   // bitmap textures
+    const int4* __restrict__ tex_info;   // per texture: first texel, width, height
     int num_lights;
     int env_light;
     float bg[3];
@@ -60,6 +63,169 @@ __device__ __forceinline__ void make_its(const SceneView& s, uint32_t prim, floa
             its.sh.b = normalized(cross(its.sh.n, st));
         }
     }
+}
+
+// ---- bitmap textures (reference GPU path) ---------------------------------
+// Texture coordinates of a hit as the reference's GPU path interpolates them
+// (src/optix/cuda/shader_common.h:36-45): zero for a mesh without them.
+__device__ __forceinline__ float2 hit_texcoord(const SceneView& s, uint32_t prim, float u, float v) {
+    const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
+    if (!mr.has_texcoords) return make_float2(0.0f, 0.0f);
+    const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
+    const float b0 = (1.0f - u) - v;
+    return make_float2((b0 * s.uv[2 * i0] + u * s.uv[2 * i1]) + v * s.uv[2 * i2],
+                       (b0 * s.uv[2 * i0 + 1] + u * s.uv[2 * i1 + 1]) + v * s.uv[2 * i2 + 1]);
+}
+
+// Integer texel coordinate f (a float holding an integer) wrapped into
+// [0, n) -- cudaAddressModeWrap; fmod is exact.
+__device__ __forceinline__ int wrap_texel(float f, int n) {
+    float r = fmodf(f, (float)n);
+    if (r < 0.0f) r += (float)n;
+    const int i = (int)r;
+    return i < n ? i : n - 1;
+}
+
+// Bilinear lookup of texture t (0-based) at normalized coordinates (u, v):
+// texel centres at ((i + 1/2) / w, (j + 1/2) / h), wrap addressing
+// (src/optix/host/cuda_texture.cpp:57-82), binary32 weights.
+__device__ __forceinline__ vec3 tex_sample(const SceneView& s, int t, float u, float v) {
+    const int4 ti = s.tex_info[t];
+    const int w = ti.y, h = ti.z;
+    if (!isfinite(u) || !isfinite(v)) return v3(0.0f, 0.0f, 0.0f);
+    const float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float ax = x - fx, ay = y - fy;
+    const int x0 = wrap_texel(fx, w), y0 = wrap_texel(fy, h);
+    const int x1 = x0 + 1 < w ? x0 + 1 : 0, y1 = y0 + 1 < h ? y0 + 1 : 0;
+    const float4* T = s.texels + ti.x;
+    const float4 t00 = T[y0 * w + x0], t10 = T[y0 * w + x1], t01 = T[y1 * w + x0], t11 = T[y1 * w + x1];
+    const float bx = 1.0f - ax, by = 1.0f - ay;
+    return v3((t00.x * bx + t10.x * ax) * by + (t01.x * bx + t11.x * ax) * ay,
+              (t00.y * bx + t10.y * ax) * by + (t01.y * bx + t11.y * ax) * ay,
+              (t00.z * bx + t10.z * ax) * by + (t01.z * bx + t11.z * ax) * ay);
+}
+
+__device__ __forceinline__ bool has_textures(const dl_material_desc& m) {
+    return (m.tex[0] | m.tex[1] | m.tex[2] | m.tex[3] | m.tex[4]) != 0;
+}
+
+// The first 8 material parameters at a surface point: textured slots
+// replaced by their texture values (diffuse.cu:16-19 albedo;
+// aniso_ggx.cu:17-36 pd, ps, axay).
+__device__ __forceinline__ void resolve_params(const SceneView& s, const dl_material_desc& m, float2 tc, float* p) {
+    for (int k = 0; k < 8; k++) p[k] = m.params[k];
+    if (m.type == DL_BSDF_DIFFUSE && m.tex[DL_TEX_ALBEDO]) {
+        const vec3 a = tex_sample(s, m.tex[DL_TEX_ALBEDO] - 1, tc.x, tc.y);
+        p[0] = a.x;
+        p[1] = a.y;
+        p[2] = a.z;
+    }
+    if (m.type == DL_BSDF_ANISO_GGX) {
+        if (m.tex[DL_TEX_PD]) {
+            const vec3 a = tex_sample(s, m.tex[DL_TEX_PD] - 1, tc.x, tc.y);
+            p[0] = a.x;
+            p[1] = a.y;
+            p[2] = a.z;
+        }
+        if (m.tex[DL_TEX_PS]) {
+            const vec3 a = tex_sample(s, m.tex[DL_TEX_PS] - 1, tc.x, tc.y);
+            p[3] = a.x;
+            p[4] = a.y;
+            p[5] = a.z;
+        }
+        if (m.tex[DL_TEX_AXAY]) {
+            const vec3 a = tex_sample(s, m.tex[DL_TEX_AXAY] - 1, tc.x, tc.y);
+            p[6] = a.x;
+            p[7] = a.y;
+        }
+    }
+}
+
+// Onb(normal, tangent) of the reference GPU path (include/optix/math/onb.h:11-33).
+__device__ __forceinline__ Frame onb_frame(vec3 n, vec3 t) {
+    Frame f;
+    f.n = n;
+    if (t.x == 0.0f && t.y == 0.0f && t.z == 0.0f) {
+        vec3 b = fabsf(n.x) > fabsf(n.z) ? v3(-n.y, n.x, 0.0f) : v3(0.0f, -n.z, n.y);
+        f.b = normalized(b);
+        f.t = cross(f.b, n);
+    } else {
+        f.t = t;
+        f.b = normalized(cross(n, t));
+    }
+    return f;
+}
+
+__device__ __forceinline__ vec3 onb_to_world(const Frame& f, vec3 p) {
+    return add(add(scale(f.t, p.x), scale(f.b, p.y)), scale(f.n, p.z));
+}
+
+// Shading frame of an aniso_ggx point with a normal and/or tangent map
+// (src/optix/cuda/shader_common.h:46-147): the map value * 2 - 1, in the
+// frame of the interpolated normal and tangent (tangent space, the default)
+// or as is (object space); without a normal map the interpolated vertex
+// normal, without either the geometric normal.
+__device__ __forceinline__ Frame map_frame(const SceneView& s, uint32_t prim, float u, float v,
+                                           const dl_material_desc& m, float2 tc) {
+    const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
+    const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
+    const float b0 = (1.0f - u) - v;
+    const vec3 p0 = ld3(s.pos, i0), p1 = ld3(s.pos, i1), p2 = ld3(s.pos, i2);
+    const vec3 gn = normalized(cross(sub(p1, p0), sub(p2, p0)));
+    const bool frame_ok = mr.has_tangents && mr.has_normals;
+    Frame tbn{};
+    if (frame_ok) {
+        const vec3 n = normalized(add(add(scale(ld3(s.nrm, i0), b0), scale(ld3(s.nrm, i1), u)), scale(ld3(s.nrm, i2), v)));
+        const vec3 t = normalized(add(add(scale(ld3(s.tan, i0), b0), scale(ld3(s.tan, i1), u)), scale(ld3(s.tan, i2), v)));
+        tbn = onb_frame(n, t);
+    }
+    vec3 sn = gn;
+    if (m.tex[DL_TEX_NORMAL]) {
+        const vec3 c = tex_sample(s, m.tex[DL_TEX_NORMAL] - 1, tc.x, tc.y);
+        const vec3 l = v3(c.x * 2.0f - 1.0f, c.y * 2.0f - 1.0f, c.z * 2.0f - 1.0f);
+        if (m.maps_object_space)
+            sn = l;
+        else if (frame_ok)
+            sn = onb_to_world(tbn, l);
+    } else if (mr.has_normals) {
+        sn = add(add(scale(ld3(s.nrm, i0), b0), scale(ld3(s.nrm, i1), u)), scale(ld3(s.nrm, i2), v));
+    }
+    sn = normalized(sn);
+    vec3 st = v3(0.0f, 0.0f, 0.0f);
+    if (m.tex[DL_TEX_TANGENT]) {
+        const vec3 c = tex_sample(s, m.tex[DL_TEX_TANGENT] - 1, tc.x, tc.y);
+        const vec3 l = v3(c.x * 2.0f - 1.0f, c.y * 2.0f - 1.0f, c.z * 2.0f - 1.0f);
+        if (m.maps_object_space)
+            st = l;
+        else if (frame_ok)
+            st = onb_to_world(tbn, l);
+        // a tangent-space map on a mesh without tangents leaves no tangent
+        // (the reference normalises the zero vector to NaN)
+        if (st.x != 0.0f || st.y != 0.0f || st.z != 0.0f) st = normalized(st);
+    } else if (mr.has_tangents) {
+        const vec3 t0 = ld3(s.tan, i0);
+        if (!(isnan(t0.x) || isnan(t0.y) || isnan(t0.z)))
+            st = normalized(add(add(scale(t0, b0), scale(ld3(s.tan, i1), u)), scale(ld3(s.tan, i2), v)));
+    }
+    return onb_frame(sn, st);
+}
+
+// Hit record and the first 8 material parameters of a shading point: the
+// reference CPU record (make_its); for a textured material the texture
+// values, and for aniso_ggx with normal / tangent maps the mapped frame.
+__device__ __forceinline__ void shading_point(const SceneView& s, uint32_t prim, float u, float v, int type, Its& its,
+                                              int& mat, float (&p)[8]) {
+    make_its(s, prim, u, v, type == DL_BSDF_ANISO_GGX, its);
+    mat = s.meshes[its.mesh].material;
+    const dl_material_desc& md = s.mats[mat];
+    float2 tc = make_float2(0.0f, 0.0f);
+    if (has_textures(md)) {
+        tc = hit_texcoord(s, prim, u, v);
+        if (type == DL_BSDF_ANISO_GGX && (md.tex[DL_TEX_NORMAL] | md.tex[DL_TEX_TANGENT]) != 0)
+            its.sh = map_frame(s, prim, u, v, md, tc);
+    }
+    resolve_params(s, md, tc, p);
 }
 
 // Shading normal only (emitter hits need its.p and shFrame.n).

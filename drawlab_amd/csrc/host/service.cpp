@@ -57,6 +57,7 @@ struct SceneEntry {
     std::vector<dl_mesh_desc> meshes;
     std::vector<dl_material_desc> materials;
     std::vector<dl_light_desc> lights;
+    std::vector<dl_texture_desc> textures;
     dl_scene_desc desc{};
     uint64_t verts = 0, prims = 0, texels = 0;
     uint64_t bytes_estimate() const {
@@ -363,7 +364,7 @@ dl_status dl_hip_service_add_scene(void* h, const dl_scene_desc* d, uint64_t* sc
     if (d->camera.width <= 0 || d->camera.height <= 0) return err(S, DL_ERR_INVALID, "invalid camera size");
     std::unique_ptr<SceneEntry> e(new SceneEntry);
     e->desc = *d;
-    e->fbuf.reserve(4 * d->num_meshes + d->num_lights);
+    e->fbuf.reserve(4 * d->num_meshes + d->num_lights + d->num_textures);
     e->ubuf.reserve(d->num_meshes);
     for (uint32_t i = 0; i < d->num_meshes; i++) {
         dl_mesh_desc m = d->meshes[i];
@@ -394,9 +395,19 @@ dl_status dl_hip_service_add_scene(void* h, const dl_scene_desc* d, uint64_t* sc
         }
         e->lights.push_back(l);
     }
+    for (uint32_t i = 0; i < d->num_textures; i++) {
+        dl_texture_desc t = d->textures[i];
+        if (!t.rgb || t.width <= 0 || t.height <= 0) return err(S, DL_ERR_INVALID, "invalid texture");
+        const size_t n = (size_t)t.width * t.height;
+        e->fbuf.emplace_back(t.rgb, t.rgb + 3 * n);
+        t.rgb = e->fbuf.back().data();
+        e->texels += n;
+        e->textures.push_back(t);
+    }
     e->desc.meshes = e->meshes.data();
     e->desc.materials = e->materials.data();
     e->desc.lights = e->lights.data();
+    e->desc.textures = e->textures.data();
     std::lock_guard<std::mutex> lk(S->m);
     *scene_id = S->next_scene++;
     S->scenes[*scene_id] = std::move(e);

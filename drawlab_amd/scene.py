@@ -15,10 +15,17 @@ BSDF_DIFFUSE, BSDF_MIRROR, BSDF_DIELECTRIC, BSDF_MICROFACET, BSDF_ANISO_GGX, BSD
 FILTER_GAUSSIAN, FILTER_BOX, FILTER_TENT, FILTER_MITCHELL = range(4)
 
 
+TEX_ALBEDO, TEX_PD, TEX_PS, TEX_AXAY, TEX_NORMAL, TEX_TANGENT = 0, 0, 1, 2, 3, 4
+
+
 @dataclass
 class Bsdf:
     type: int
     params: Tuple[float, ...] = ()
+    # texture slot -> index into SceneDesc.textures (diffuse: TEX_ALBEDO;
+    # aniso_ggx: TEX_PD, TEX_PS, TEX_AXAY, TEX_NORMAL, TEX_TANGENT)
+    textures: Optional[dict] = None
+    maps_object_space: bool = False  # aniso_ggx normal / tangent maps (default: tangent space)
 
     @staticmethod
     def diffuse(albedo=(0.5, 0.5, 0.5)):
@@ -107,6 +114,12 @@ class SceneDesc:
     # default (area meshes, then point lights, then the environment map), e.g.
     # for an XML file that declares the environment map first.
     emitters: Optional[List[Tuple[str, int]]] = None
+    # bitmap textures: (height, width, 3) float32, row 0 at v = 0
+    textures: List[np.ndarray] = field(default_factory=list)
+
+    def add_texture(self, rgb) -> int:
+        self.textures.append(np.ascontiguousarray(rgb, dtype=np.float32).reshape(np.shape(rgb)[0], -1, 3))
+        return len(self.textures) - 1
 
     def add_bsdf(self, b: Bsdf) -> int:
         self.bsdfs.append(b)

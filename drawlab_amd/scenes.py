@@ -16,7 +16,8 @@ import math
 
 import numpy as np
 
-from .scene import Bsdf, Camera, EnvMap, RFilter, SceneDesc, FILTER_GAUSSIAN
+from .scene import (BSDF_ANISO_GGX, BSDF_DIFFUSE, FILTER_GAUSSIAN, TEX_ALBEDO, TEX_AXAY, TEX_NORMAL, TEX_PD, TEX_TANGENT,
+                    Bsdf, Camera, EnvMap, RFilter, SceneDesc)
 
 
 # ---------------------------------------------------------------------------
@@ -251,6 +252,81 @@ def displaced_sphere(width=1920, height=1080, spp=256, max_depth=16, grid=1024, 
                       height=height)
     s.rfilter = RFilter(FILTER_GAUSSIAN, 2.0, 0.5)
     s.spp, s.max_depth, s.rr_depth = spp, max_depth, 4
+    return s
+
+
+def checker_texture(w, h, seed, n=8):
+    """(h, w, 3) float texture: an n x n checkerboard of seeded colours times
+    a smooth ramp (exercises bilinear weights and wrap addressing)."""
+    rng = np.random.default_rng(seed)
+    cols = rng.uniform(0.1, 0.9, size=(n, n, 3)).astype(np.float32)
+    y, x = np.mgrid[0:h, 0:w]
+    c = cols[(y * n) // h, (x * n) // w]
+    ramp = (0.6 + 0.4 * np.sin(2 * np.pi * x / w) * np.cos(2 * np.pi * y / h)).astype(np.float32)
+    return (c * ramp[..., None]).astype(np.float32)
+
+
+def normal_map(w, h, seed, amp=0.35):
+    """(h, w, 3) tangent-space normal map (stored as n * 0.5 + 0.5) of
+    seeded bumps."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:h, 0:w] / np.array([h, w])[:, None, None]
+    gx = np.zeros((h, w))
+    gy = np.zeros((h, w))
+    for _ in range(6):
+        fx, fy, ph = rng.uniform(1, 4), rng.uniform(1, 4), rng.uniform(0, 2 * np.pi)
+        gx += amp * np.cos(2 * np.pi * (fx * x + fy * y) + ph) * fx / 4
+        gy += amp * np.cos(2 * np.pi * (fx * x + fy * y) + ph) * fy / 4
+    n = np.stack([-gx, -gy, np.ones_like(gx)], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    return (n * 0.5 + 0.5).astype(np.float32)
+
+
+def textured_box(width=64, height=64, spp=16, max_depth=6, rr_depth=3):
+    """Cornell-like box whose surfaces use every texture input: a floor with
+    an albedo texture (texture coordinates 0..3, wrapped), a back wall of
+    anisotropic GGX with pd / axay textures and a tangent-space normal map,
+    a left wall of anisotropic GGX with object-space normal and tangent maps."""
+    s = SceneDesc(name="textured_box")
+    t_alb = s.add_texture(checker_texture(48, 40, 1))
+    t_pd = s.add_texture(checker_texture(32, 32, 2, n=4))
+    axay = np.stack(np.meshgrid(np.linspace(0.05, 0.6, 16), np.linspace(0.6, 0.05, 16)), -1)
+    t_axay = s.add_texture(np.concatenate([axay, np.zeros((16, 16, 1))], -1))
+    t_nrm = s.add_texture(normal_map(64, 64, 3))
+    on = np.array([0.3, 0.2, 1.0]) / np.linalg.norm([0.3, 0.2, 1.0])
+    ot = np.cross(on, [0.0, 1.0, 0.0])
+    ot /= np.linalg.norm(ot)
+    t_on = s.add_texture(np.tile((on * 0.5 + 0.5).astype(np.float32), (8, 8, 1)))
+    t_ot = s.add_texture(np.tile((ot * 0.5 + 0.5).astype(np.float32), (8, 8, 1)))
+    white = s.add_bsdf(Bsdf.diffuse((0.73, 0.73, 0.73)))
+    red = s.add_bsdf(Bsdf.diffuse((0.63, 0.06, 0.05)))
+    floor = s.add_bsdf(Bsdf(BSDF_DIFFUSE, (0.5, 0.5, 0.5), textures={TEX_ALBEDO: t_alb}))
+    back = s.add_bsdf(Bsdf(BSDF_ANISO_GGX, (0.4, 0.4, 0.4, 0.5, 0.5, 0.5, 0.3, 0.1),
+                           textures={TEX_PD: t_pd, TEX_AXAY: t_axay, TEX_NORMAL: t_nrm}))
+    left = s.add_bsdf(Bsdf(BSDF_ANISO_GGX, (0.3, 0.5, 0.2, 0.4, 0.4, 0.4, 0.2, 0.05),
+                           textures={TEX_NORMAL: t_on, TEX_TANGENT: t_ot}, maps_object_space=True))
+    lamp = s.add_bsdf(Bsdf.diffuse((0.0, 0.0, 0.0)))
+
+    def textured_quad(p, e0, e1, bsdf, uv_scale=1.0):
+        V, F, UV = quad(p, e0, e1, with_uv=True)
+        n = np.cross(np.asarray(e0, np.float64), np.asarray(e1, np.float64))
+        N = np.tile((n / np.linalg.norm(n)).astype(np.float32), (4, 1))
+        s.add_mesh(V, F, bsdf, N=N, UV=UV * np.float32(uv_scale))
+
+    textured_quad((-1, -1, -1), (0, 0, 2), (2, 0, 0), floor, uv_scale=3.0)
+    textured_quad((-1, -1, -1), (2, 0, 0), (0, 2, 0), back)
+    textured_quad((-1, -1, 1), (0, 0, -2), (0, 2, 0), left)
+    V, F, _ = quad((1, -1, -1), (0, 0, 2), (0, 2, 0))
+    s.add_mesh(V, F, red)
+    V, F, _ = quad((-1, 1, -1), (2, 0, 0), (0, 0, 2))
+    s.add_mesh(V, F, white)
+    V, F, _ = quad((-0.25, 0.999, -0.25), (0.5, 0, 0), (0, 0, 0.5))
+    s.add_mesh(V, F, lamp, radiance=(12.0, 10.0, 8.0))
+    V, F = box((0.1, -1.0, -0.3), (0.6, -0.2, 0.2))
+    s.add_mesh(V, F, white)
+    s.camera = Camera(eye=(0.0, 0.0, 3.8), lookat=(0.0, 0.0, 0.0), up=(0, 1, 0), fov=39.3, width=width, height=height)
+    s.rfilter = RFilter(FILTER_GAUSSIAN, 2.0, 0.5)
+    s.spp, s.max_depth, s.rr_depth = spp, max_depth, rr_depth
     return s
 
 

@@ -76,10 +76,39 @@ typedef struct dl_mesh_desc {
     int32_t light;             /* index into dl_scene_desc.lights, -1 if not an emitter */
 } dl_mesh_desc;
 
+/* Texture slots of a material (texture children of the reference plugins):
+   diffuse: DL_TEX_ALBEDO ("albedo", diffuse.cpp:69-75);
+   aniso_ggx: DL_TEX_PD, DL_TEX_PS, DL_TEX_AXAY, DL_TEX_NORMAL, DL_TEX_TANGENT
+   ("pd", "ps", "axay", "normal", "tangent", aniso_ggx.cpp:43-58). */
+enum {
+    DL_TEX_ALBEDO = 0,
+    DL_TEX_PD = 0,
+    DL_TEX_PS = 1,
+    DL_TEX_AXAY = 2,
+    DL_TEX_NORMAL = 3,
+    DL_TEX_TANGENT = 4,
+    DL_TEX_SLOTS = 5
+};
+
 typedef struct dl_material_desc {
     int32_t type;              /* DL_BSDF_* */
     float params[16];
+    int32_t tex[DL_TEX_SLOTS]; /* 1 + index into dl_scene_desc.textures; 0 = the constant in params */
+    int32_t maps_object_space; /* aniso_ggx normal / tangent maps: 0 = tangent space (the reference
+                                  default is_tangent_space = true), 1 = object space */
 } dl_material_desc;
+
+/* Bitmap texture (BitmapTexture, src/tracer/texture/bitmap.cpp), sampled as
+   the reference's GPU path samples it (src/optix/host/cuda_texture.cpp:
+   normalized coordinates, bilinear filter, wrap addressing; the texel
+   centres at ((i + 1/2) / width, (j + 1/2) / height)), with binary32
+   interpolation weights.  rgb: height x width x 3 floats, row 0 at v = 0 --
+   BitmapTexture::createCUDATexture uploads float images (EXR, HDR) flipped
+   upside down and 8-bit images as stored, read as value / 255. */
+typedef struct dl_texture_desc {
+    int32_t width, height;
+    const float* rgb;
+} dl_texture_desc;
 
 typedef struct dl_light_desc {
     int32_t type;              /* DL_LIGHT_* */
@@ -126,6 +155,8 @@ typedef struct dl_scene_desc {
     dl_camera_desc camera;
     dl_film_desc film;
     float bg_color[3];
+    const dl_texture_desc* textures;  /* referenced by dl_material_desc.tex */
+    uint32_t num_textures;
 } dl_scene_desc;
 
 typedef struct dl_accel_stats {

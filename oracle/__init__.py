@@ -40,6 +40,9 @@ def lib():
             "dlo_scene_new": (_P, []),
             "dlo_scene_free": (None, [_P]),
             "dlo_scene_add_bsdf": (i32, [_P, i32, _f32p, i32]),
+            "dlo_scene_add_texture": (i32, [_P, _f32p, i32, i32]),
+            "dlo_scene_set_bsdf_textures": (None, [_P, i32, _i32p, i32]),
+            "dlo_texture_sample": (None, [_f32p, i32, i32, _f32p, u32, _f32p]),
             "dlo_scene_add_mesh": (i32, [_P, _f32p, u32, _u32p, u32, _P, _P, i32, _P]),
             "dlo_scene_add_point_light": (i32, [_P, _f32p, _f32p]),
             "dlo_scene_add_envmap": (i32, [_P, _f32p, i32, i32, f32, i32]),
@@ -116,6 +119,16 @@ class Scene:
     def add_bsdf(self, btype, params=()):
         p = _f(list(params) + [0.0] * (16 - len(params)))
         return self.L.dlo_scene_add_bsdf(self.h, int(btype), p, len(params))
+
+    def add_texture(self, rgb):
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        return self.L.dlo_scene_add_texture(self.h, rgb.reshape(-1), int(rgb.shape[1]), int(rgb.shape[0]))
+
+    def set_bsdf_textures(self, bsdf, slots, object_space=False):
+        t = np.zeros(5, np.int32)
+        for slot, idx in slots.items():
+            t[slot] = int(idx) + 1
+        self.L.dlo_scene_set_bsdf_textures(self.h, int(bsdf), t, int(bool(object_space)))
 
     def add_mesh(self, V, F, bsdf, N=None, UV=None, radiance=None):
         V = _f(V)
@@ -250,8 +263,12 @@ def from_desc(desc, build=True, camera_matrices=None):
     used instead of perspective_matrices() (e.g. the values the product's
     set-up computed, so both sides trace identical rays)."""
     s = Scene()
+    for rgb in getattr(desc, "textures", []):
+        s.add_texture(rgb)
     for b in desc.bsdfs:
-        s.add_bsdf(b.type, b.params)
+        bi = s.add_bsdf(b.type, b.params)
+        if b.textures:
+            s.set_bsdf_textures(bi, b.textures, b.maps_object_space)
     # Emitters are registered in the scene's emitter order: an area light
     # when its mesh is added (meshes keep their order), point lights and the
     # environment map where the order places them.
@@ -306,6 +323,17 @@ def from_desc(desc, build=True, camera_matrices=None):
     if build:
         s.build()
     return s
+
+
+def texture_sample(rgb, uv):
+    """Bilinear, wrap-addressed lookups of an (h, w, 3) texture at (n, 2)
+    normalized coordinates (the sampler of textured materials)."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    out = np.zeros((uv.shape[0], 3), np.float32)
+    lib().dlo_texture_sample(rgb.reshape(-1), int(rgb.shape[1]), int(rgb.shape[0]), uv.reshape(-1), uv.shape[0],
+                             out.reshape(-1))
+    return out
 
 
 def bvh4_replay(nodes, tris, rays, any_hit=False):

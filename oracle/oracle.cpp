@@ -81,6 +81,18 @@ struct Bsdf {
     int type = DLO_BSDF_DIFFUSE;
     float p[16] = {0};
     float ks = 0;  // microfacet: 1 - max(kd)   microfacet.cpp:32
+    // texture slots (1 + texture index, 0 = the constant): diffuse albedo;
+    // aniso_ggx pd, ps, axay, normal, tangent (diffuse.cpp:69-75,
+    // aniso_ggx.cpp:43-58); normal / tangent maps in object space if set
+    int tex[5] = {0, 0, 0, 0, 0};
+    int object_space = 0;
+};
+
+// Bitmap texture as the reference's GPU path samples it (cuda_texture.cpp:
+// normalized coordinates, bilinear, wrap); rgb row 0 at v = 0.
+struct Tex {
+    int w = 0, h = 0;
+    std::vector<float> rgb;
 };
 
 struct Emitter {
@@ -123,6 +135,7 @@ struct OcNode {
 struct Scene {
     std::vector<Mesh> meshes;
     std::vector<Bsdf> bsdfs;
+    std::vector<Tex> textures;
     std::vector<Emitter> emitters;
     int env = -1;
     Camera cam;
@@ -426,11 +439,121 @@ struct Its {
     V3 p;
     float t;
     V2 uv;
+    V2 tc;  // texture coordinates as the GPU path interpolates them (0 without)
     Frame sh, geo;
     int mesh;
     int tri;
     V3 bary;
 };
+
+// ---------------------------------------------------------------------------
+// Bitmap textures, sampled as on the reference GPU path
+// (src/optix/host/cuda_texture.cpp:57-82, src/optix/cuda/shader_common.h)
+// ---------------------------------------------------------------------------
+static int wrap_texel(float f, int n) {
+    float r = std::fmod(f, (float)n);
+    if (r < 0.0f) r += (float)n;
+    int i = (int)r;
+    return i < n ? i : n - 1;
+}
+
+static V3 tex_sample(const Tex& t, float u, float v) {
+    if (!std::isfinite(u) || !std::isfinite(v)) return mk(0, 0, 0);
+    const int w = t.w, h = t.h;
+    const float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+    const float fx = std::floor(x), fy = std::floor(y);
+    const float ax = x - fx, ay = y - fy;
+    const int x0 = wrap_texel(fx, w), y0 = wrap_texel(fy, h);
+    const int x1 = x0 + 1 < w ? x0 + 1 : 0, y1 = y0 + 1 < h ? y0 + 1 : 0;
+    const float* T = t.rgb.data();
+    const float bx = 1.0f - ax, by = 1.0f - ay;
+    float out[3];
+    for (int c = 0; c < 3; c++) {
+        const float t00 = T[3 * ((size_t)y0 * w + x0) + c], t10 = T[3 * ((size_t)y0 * w + x1) + c];
+        const float t01 = T[3 * ((size_t)y1 * w + x0) + c], t11 = T[3 * ((size_t)y1 * w + x1) + c];
+        out[c] = (t00 * bx + t10 * ax) * by + (t01 * bx + t11 * ax) * ay;
+    }
+    return mk(out[0], out[1], out[2]);
+}
+
+static bool textured(const Bsdf& b) { return (b.tex[0] | b.tex[1] | b.tex[2] | b.tex[3] | b.tex[4]) != 0; }
+
+// BSDF with its textured parameters replaced by their values at texture
+// coordinates tc (diffuse.cu:16-19; aniso_ggx.cu:17-36).
+static Bsdf resolve_bsdf(const Scene& s, const Bsdf& b, V2 tc) {
+    Bsdf r = b;
+    auto put = [&](int slot, int first, int n) {
+        if (!b.tex[slot]) return;
+        const V3 a = tex_sample(s.textures[b.tex[slot] - 1], tc.x, tc.y);
+        const float v[3] = {a.x, a.y, a.z};
+        for (int k = 0; k < n; k++) r.p[first + k] = v[k];
+    };
+    if (b.type == DLO_BSDF_DIFFUSE) put(0, 0, 3);
+    if (b.type == DLO_BSDF_ANISO_GGX) {
+        put(0, 0, 3);
+        put(1, 3, 3);
+        put(2, 6, 2);
+    }
+    return r;
+}
+
+// Onb(normal, tangent)  include/optix/math/onb.h:11-33
+static Frame onb_frame(V3 n, V3 t) {
+    Frame f;
+    f.n = n;
+    if (t.x == 0.0f && t.y == 0.0f && t.z == 0.0f) {
+        V3 b = std::fabs(n.x) > std::fabs(n.z) ? mk(-n.y, n.x, 0.0f) : mk(0.0f, -n.z, n.y);
+        f.b = normalized(b);
+        f.t = cross(f.b, n);
+    } else {
+        f.t = t;
+        f.b = normalized(cross(n, t));
+    }
+    return f;
+}
+
+static V3 onb_to_world(const Frame& f, V3 p) { return (f.t * p.x + f.b * p.y) + f.n * p.z; }
+
+// aniso_ggx shading frame with normal / tangent maps (shader_common.h:46-147)
+static Frame map_frame(const Scene& s, const Mesh& m, uint32_t f, V3 bary, const Bsdf& b, V2 tc) {
+    const float b0 = bary.x, u = bary.y, v = bary.z;
+    const V3 p0 = m.pos(f, 0), p1 = m.pos(f, 1), p2 = m.pos(f, 2);
+    const V3 gn = normalized(cross(p1 - p0, p2 - p0));
+    const bool frame_ok = !m.T.empty() && !m.N.empty();
+    Frame tbn{};
+    if (frame_ok) {
+        const V3 n = normalized((m.nrm(f, 0) * b0 + m.nrm(f, 1) * u) + m.nrm(f, 2) * v);
+        const V3 t = normalized((m.tan(f, 0) * b0 + m.tan(f, 1) * u) + m.tan(f, 2) * v);
+        tbn = onb_frame(n, t);
+    }
+    V3 sn = gn;
+    if (b.tex[3]) {
+        const V3 c = tex_sample(s.textures[b.tex[3] - 1], tc.x, tc.y);
+        const V3 l = mk(c.x * 2.0f - 1.0f, c.y * 2.0f - 1.0f, c.z * 2.0f - 1.0f);
+        if (b.object_space)
+            sn = l;
+        else if (frame_ok)
+            sn = onb_to_world(tbn, l);
+    } else if (!m.N.empty()) {
+        sn = (m.nrm(f, 0) * b0 + m.nrm(f, 1) * u) + m.nrm(f, 2) * v;
+    }
+    sn = normalized(sn);
+    V3 st = mk(0, 0, 0);
+    if (b.tex[4]) {
+        const V3 c = tex_sample(s.textures[b.tex[4] - 1], tc.x, tc.y);
+        const V3 l = mk(c.x * 2.0f - 1.0f, c.y * 2.0f - 1.0f, c.z * 2.0f - 1.0f);
+        if (b.object_space)
+            st = l;
+        else if (frame_ok)
+            st = onb_to_world(tbn, l);
+        if (st.x != 0.0f || st.y != 0.0f || st.z != 0.0f) st = normalized(st);
+    } else if (!m.T.empty()) {
+        const V3 t0 = m.tan(f, 0);
+        if (!(std::isnan(t0.x) || std::isnan(t0.y) || std::isnan(t0.z)))
+            st = normalized((t0 * b0 + m.tan(f, 1) * u) + m.tan(f, 2) * v);
+    }
+    return onb_frame(sn, st);
+}
 
 static void make_its(const Scene& s, const HitRec& h, Its& its) {
     const Mesh& m = s.meshes[h.mesh];
@@ -466,6 +589,14 @@ static void make_its(const Scene& s, const HitRec& h, Its& its) {
             its.sh.t = st;
             its.sh.b = normalized(cross(n, st));
         }
+    }
+    its.tc = V2{0.0f, 0.0f};
+    if (textured(b)) {
+        if (!m.UV.empty()) {
+            V2 t0 = m.tex(f, 0), t1 = m.tex(f, 1), t2 = m.tex(f, 2);
+            its.tc = V2{(bary.x * t0.x + bary.y * t1.x) + bary.z * t2.x, (bary.x * t0.y + bary.y * t1.y) + bary.z * t2.y};
+        }
+        if (b.type == DLO_BSDF_ANISO_GGX && (b.tex[3] | b.tex[4]) != 0) its.sh = map_frame(s, m, f, bary, b, its.tc);
     }
 }
 
@@ -986,7 +1117,7 @@ static V3 Li(const Scene& s, const RenderOpts& o, Sampler& sampler, Ray ray, uin
     }
     for (int depth = 0;; ++depth) {
         if (o.max_depth >= 0 && depth + 2 > o.max_depth) break;
-        const Bsdf& bsdf = s.bsdfs[s.meshes[its.mesh].bsdf];
+        const Bsdf bsdf = resolve_bsdf(s, s.bsdfs[s.meshes[its.mesh].bsdf], its.tc);
         // Emitter sampling  path.cpp:32-56
         DirSample ds;
         V2 lsmp = sampler.next2d();
@@ -1250,6 +1381,37 @@ int dlo_scene_add_bsdf(void* h, int type, const float* params, int n) {
     if (type == DLO_BSDF_MICROFACET) b.ks = 1 - cmax(col(b.p + 3));
     s.bsdfs.push_back(b);
     return (int)s.bsdfs.size() - 1;
+}
+
+int dlo_scene_add_texture(void* h, const float* rgb, int w, int hh) {
+    Scene& s = *(Scene*)h;
+    Tex t;
+    t.w = w;
+    t.h = hh;
+    t.rgb.assign(rgb, rgb + 3 * (size_t)w * (size_t)hh);
+    s.textures.push_back(std::move(t));
+    return (int)s.textures.size() - 1;
+}
+
+// Texture lookups at n (u, v) pairs (tests of the sampler).
+void dlo_texture_sample(const float* rgb, int w, int hh, const float* uv, uint32_t n, float* out) {
+    Tex t;
+    t.w = w;
+    t.h = hh;
+    t.rgb.assign(rgb, rgb + 3 * (size_t)w * (size_t)hh);
+    for (uint32_t i = 0; i < n; i++) {
+        const V3 c = tex_sample(t, uv[2 * i], uv[2 * i + 1]);
+        out[3 * i] = c.x;
+        out[3 * i + 1] = c.y;
+        out[3 * i + 2] = c.z;
+    }
+}
+
+void dlo_scene_set_bsdf_textures(void* h, int bsdf, const int* tex5, int object_space) {
+    Scene& s = *(Scene*)h;
+    Bsdf& b = s.bsdfs[bsdf];
+    for (int k = 0; k < 5; k++) b.tex[k] = tex5[k];
+    b.object_space = object_space;
 }
 
 // Mesh::activate  src/tracer/mesh/mesh.cpp:15-45 (area CDF, tangents)

@@ -57,6 +57,9 @@ enum { DLO_ACCEL_BRUTE = 0, DLO_ACCEL_OCTREE = 1, DLO_ACCEL_OCTREE_LOWEST = 2, D
 void* dlo_scene_new(void);
 void dlo_scene_free(void* scene);
 int dlo_scene_add_bsdf(void* scene, int type, const float* params, int n);
+int dlo_scene_add_texture(void* scene, const float* rgb, int w, int h);
+void dlo_scene_set_bsdf_textures(void* scene, int bsdf, const int* tex5, int object_space);
+void dlo_texture_sample(const float* rgb, int w, int h, const float* uv, uint32_t n, float* out);
 int dlo_scene_add_mesh(void* scene, const float* V, uint32_t nv, const uint32_t* F, uint32_t nf, const float* N,
                        const float* UV, int bsdf, const float* radiance);
 int dlo_scene_add_point_light(void* scene, const float* position, const float* intensity);
