@@ -7,6 +7,7 @@
 // computes radiance.
 #pragma once
 #include <cstdint>
+#include <cstdio>
 #include <functional>
 #include <map>
 #include <sstream>
@@ -49,6 +50,12 @@ std::string format(const char* f, const Args&... args) {
     std::ostringstream os;
     detail::format_to(os, f, args...);
     return os.str();
+}
+
+// A warning on stderr (the reference logs these with spdlog::warn).
+template <typename... Args>
+void warn(const char* f, const Args&... args) {
+    std::fprintf(stderr, "[warning] %s\n", format(f, args...).c_str());
 }
 
 class Exception : public std::runtime_error {

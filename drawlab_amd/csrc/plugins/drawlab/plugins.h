@@ -5,6 +5,8 @@
 // GPU only.
 #pragma once
 #include <memory>
+#include <utility>
+#include <vector>
 
 #include "dl_hip.h"
 #include "drawlab/core.h"
@@ -18,15 +20,26 @@ class Integrator;
 class Sampler;
 class ReconstructionFilter;
 
+// ---- Texture (include/tracer/texture.h, texture/bitmap.cpp) ----------------
+// A bitmap texture, loaded when constructed: rgb rows as uploaded (row 0 at
+// v = 0).
+class Texture : public Object {
+public:
+    EClassType getClassType() const override { return ETexture; }
+    int width = 0, height = 0;
+    std::vector<float> rgb;
+};
+
 // ---- BSDF (include/tracer/bsdf.h:46-111) -----------------------------------
 class BSDF : public Object {
 public:
     EClassType getClassType() const override { return EBSDF; }
-    // Material descriptor of the C ABI (type + parameter block).
+    // Material descriptor of the C ABI (type + parameter block); texture
+    // slots are filled in by exportScene from `textures`.
     virtual dl_material_desc descriptor() const = 0;
-    // Set when a <texture> child was attached (textured inputs are not
-    // supported by the HIP backend).
-    std::string texture_error;
+    // (DL_TEX_* slot, texture) of the textured inputs
+    std::vector<std::pair<int, const Texture*>> textures;
+    bool maps_object_space = false;
 };
 
 // ---- Mesh (include/tracer/mesh.h) ------------------------------------------
@@ -150,6 +163,7 @@ private:
 // A dl_scene_desc that points into the Scene's arrays (valid while the Scene
 // lives), with the render parameters of its integrator and sampler.
 struct ExportedScene {
+    std::vector<dl_texture_desc> textures;
     std::vector<dl_mesh_desc> meshes;
     std::vector<dl_material_desc> materials;
     std::vector<dl_light_desc> lights;
@@ -166,6 +180,9 @@ struct Image {
 };
 // Radiance .hdr (RGBE, run-length encoded or flat), .pfm, uncompressed .exr.
 Image loadImage(const std::string& filename);
+// Texture rows as the reference uploads them (row 0 at v = 0, 8-bit
+// samples / 255): .png / .exr / .pfm rows bottom up, .hdr top down.
+Image loadTextureImage(const std::string& filename);
 void savePFM(const std::string& filename, const Image& img);
 void saveEXR(const std::string& filename, const Image& img);
 

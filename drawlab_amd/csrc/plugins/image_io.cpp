@@ -5,8 +5,14 @@
 //   PFM: "PF\n W H\n scale\n", little-endian when scale < 0, rows bottom-up.
 //   OpenEXR: scanline images with NO_COMPRESSION and HALF or FLOAT R, G, B
 //     channels (read); written as FLOAT B, G, R, one scanline per chunk.
+//   PNG (textures): non-interlaced, 8- or 16-bit grey, grey + alpha, RGB,
+//     RGBA or 8-bit palette; zlib stream inflated by the system zlib, rows
+//     unfiltered (None, Sub, Up, Average, Paeth), 16-bit samples reduced to
+//     their high byte (as stb_image returns them).
 // The reference reads images through stb_image / OpenEXR
 // (src/core/bitmap/bitmap.cpp:33-214), neither of which is vendored.
+#include <zlib.h>
+
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -239,7 +245,137 @@ Image load_exr(const std::string& fn) {
     return img;
 }
 
+uint32_t be32(const unsigned char* p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
+
+// 8-bit samples of a PNG, `channels` per pixel, rows top first.
+std::vector<unsigned char> load_png(const std::string& fn, int& w, int& h, int& channels) {
+    const std::vector<unsigned char> b = read_file(fn);
+    static const unsigned char sig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
+    if (b.size() < 8 || std::memcmp(b.data(), sig, 8) != 0) throw Exception("\"%s\" is not a PNG file", fn);
+    size_t pos = 8;
+    int depth = 0, ctype = -1, interlace = 0;
+    std::vector<unsigned char> idat, plte;
+    w = h = 0;
+    while (pos + 8 <= b.size()) {
+        const uint32_t len = be32(&b[pos]);
+        const std::string type(reinterpret_cast<const char*>(&b[pos + 4]), 4);
+        pos += 8;
+        if (pos + (size_t)len + 4 > b.size()) throw Exception("\"%s\": truncated PNG chunk", fn);
+        const unsigned char* d = &b[pos];
+        if (type == "IHDR") {
+            if (len < 13) throw Exception("\"%s\": bad IHDR", fn);
+            w = (int)be32(d);
+            h = (int)be32(d + 4);
+            depth = d[8];
+            ctype = d[9];
+            interlace = d[12];
+        } else if (type == "PLTE") {
+            plte.assign(d, d + len);
+        } else if (type == "IDAT") {
+            idat.insert(idat.end(), d, d + len);
+        } else if (type == "IEND") {
+            break;
+        }
+        pos += (size_t)len + 4;  // data + CRC
+    }
+    if (w <= 0 || h <= 0 || (uint64_t)w * (uint64_t)h > (1ull << 28)) throw Exception("\"%s\": bad PNG size", fn);
+    if (interlace != 0) throw Exception("\"%s\": interlaced PNG is not supported", fn);
+    int spp;  // samples per pixel in the file
+    switch (ctype) {
+        case 0: spp = 1; break;
+        case 2: spp = 3; break;
+        case 3: spp = 1; break;
+        case 4: spp = 2; break;
+        case 6: spp = 4; break;
+        default: throw Exception("\"%s\": unknown PNG colour type %d", fn, ctype);
+    }
+    if (!((depth == 8) || (depth == 16 && ctype != 3)))
+        throw Exception("\"%s\": PNG bit depth %d is not supported (8, or 16 without a palette)", fn, depth);
+    const size_t bpp = (size_t)spp * (depth / 8);  // bytes per pixel
+    const size_t stride = (size_t)w * bpp;
+    std::vector<unsigned char> raw(((size_t)stride + 1) * (size_t)h);
+    uLongf out_len = (uLongf)raw.size();
+    if (uncompress(raw.data(), &out_len, idat.data(), (uLong)idat.size()) != Z_OK || out_len != raw.size())
+        throw Exception("\"%s\": corrupt PNG image data", fn);
+    std::vector<unsigned char> px(stride * (size_t)h), prev(stride, 0);
+    for (int y = 0; y < h; y++) {
+        const unsigned char f = raw[(size_t)y * (stride + 1)];
+        const unsigned char* in = &raw[(size_t)y * (stride + 1) + 1];
+        unsigned char* o = &px[(size_t)y * stride];
+        for (size_t i = 0; i < stride; i++) {
+            const int a = i >= bpp ? o[i - bpp] : 0, up = prev[i], c = i >= bpp ? prev[i - bpp] : 0;
+            int v = in[i];
+            switch (f) {
+                case 0: break;
+                case 1: v += a; break;
+                case 2: v += up; break;
+                case 3: v += (a + up) / 2; break;
+                case 4: {
+                    const int p = a + up - c, pa = std::abs(p - a), pb = std::abs(p - up), pc = std::abs(p - c);
+                    v += (pa <= pb && pa <= pc) ? a : (pb <= pc ? up : c);
+                    break;
+                }
+                default: throw Exception("\"%s\": bad PNG filter %d", fn, (int)f);
+            }
+            o[i] = (unsigned char)v;
+        }
+        std::memcpy(prev.data(), o, stride);
+    }
+    // 8-bit samples; a palette expands to RGB
+    channels = ctype == 3 ? 3 : spp;
+    std::vector<unsigned char> out((size_t)w * h * channels);
+    for (size_t i = 0; i < (size_t)w * h; i++) {
+        if (ctype == 3) {
+            const size_t e = px[i];
+            if (3 * e + 2 >= plte.size()) throw Exception("\"%s\": palette index out of range", fn);
+            for (int k = 0; k < 3; k++) out[3 * i + k] = plte[3 * e + k];
+        } else {
+            for (int k = 0; k < spp; k++) out[(size_t)spp * i + k] = px[bpp * i + (size_t)k * (depth / 8)];
+        }
+    }
+    return out;
+}
+
 }  // namespace
+
+Image loadTextureImage(const std::string& filename) {
+    const std::string l = toLower(filename);
+    Image img;
+    if (endsWith(l, ".png")) {
+        int w, h, c;
+        std::vector<unsigned char> px = load_png(filename, w, h, c);
+        // Bitmap::loadLDR (bitmap.cpp:131-157): stb loads the rows bottom up,
+        // then the first 3 w h samples become the RGB triples (grey
+        // replicated; with 4 channels the reference reads the interleaved
+        // samples as if they were RGB, and so does this).  Grey + alpha has
+        // only 2 w h samples, which the reference reads past: rejected.
+        if (c == 2) throw Exception("\"%s\": grey + alpha PNG textures are not supported", filename);
+        std::vector<unsigned char> flip(px.size());
+        const size_t row = (size_t)w * c;
+        for (int y = 0; y < h; y++) std::memcpy(&flip[(size_t)y * row], &px[(size_t)(h - 1 - y) * row], row);
+        img.width = w;
+        img.height = h;
+        img.rgb.resize((size_t)w * h * 3);
+        for (size_t i = 0; i < img.rgb.size(); i++) {
+            const unsigned char v = c == 1 ? flip[i / 3] : flip[i];
+            img.rgb[i] = (float)v / 255.0f;  // cudaReadModeNormalizedFloat of an 8-bit texel
+        }
+        return img;  // uploaded as loaded (bitmap.cpp:74-81)
+    }
+    if (endsWith(l, ".exr") || endsWith(l, ".pfm")) {
+        // loaded top row first, flipped on upload (bitmap.cpp:86)
+        img = loadImage(filename);
+        const size_t row = (size_t)img.width * 3;
+        for (int y = 0; y < img.height / 2; y++)
+            for (size_t i = 0; i < row; i++) std::swap(img.rgb[(size_t)y * row + i], img.rgb[(size_t)(img.height - 1 - y) * row + i]);
+        return img;
+    }
+    if (endsWith(l, ".hdr") || endsWith(l, ".pic")) {
+        // flipped on load (bitmap.cpp:109) and again on upload: top row first
+        return loadImage(filename);
+    }
+    throw Exception("Unsupported texture format \"%s\" (expected .png, .exr, .pfm or .hdr)", filename);
+}
 
 Image loadImage(const std::string& filename) {
     const std::string l = toLower(filename);

@@ -166,14 +166,27 @@ public:
 DRAWLAB_REGISTER_CLASS(Rectangle, "rectangle");
 
 // ============================================================================
-// Textures (src/tracer/texture/bitmap.cpp): parsed, but textured inputs are
-// not supported by the HIP backend; the owning BSDF reports it on export.
+// Textures (src/tracer/texture/bitmap.cpp, texture.cpp): "filename" loaded
+// when constructed.  The reference's GPU path samples the bitmap at the hit's
+// texture coordinates and does not apply the uoffset / voffset / uscale /
+// vscale properties of Texture2D (texture.cpp:15-28, used by the CPU eval
+// only, which throws); they are parsed and likewise unused.
 // ============================================================================
-class BitmapTexture : public Object {
+class BitmapTexture : public Texture {
 public:
-    explicit BitmapTexture(const PropertyList& props) : m_filename(props.getString("filename")) {}
-    EClassType getClassType() const override { return ETexture; }
-    std::string toString() const override { return format("BitmapTexture[%s]", m_filename); }
+    explicit BitmapTexture(const PropertyList& props) : m_filename(props.getString("filename")) {
+        if (props.getString("coordinates", "uv") != "uv") throw Exception("Only UV coordinates are supported at the moment!");
+        (void)props.getFloat("uoffset", 0.0f);
+        (void)props.getFloat("voffset", 0.0f);
+        const float s = props.getFloat("uvscale", 1.0f);
+        (void)props.getFloat("uscale", s);
+        (void)props.getFloat("vscale", s);
+        Image img = loadTextureImage(resolvePath(m_filename));
+        width = img.width;
+        height = img.height;
+        rgb = std::move(img.rgb);
+    }
+    std::string toString() const override { return format("BitmapTexture[%s, %ix%i]", m_filename, width, height); }
 
 private:
     std::string m_filename;
@@ -189,19 +202,32 @@ static void put3(float* p, const Color3f& c) {
     p[2] = c[2];
 }
 
+// Texture children by name -> DL_TEX_* slot (a later child of the same
+// name replaces an earlier one, as the reference's member assignment does).
 class TexturableBSDF : public BSDF {
-public:
-    void addChild(Object* obj) override {
-        if (obj->getClassType() != ETexture)
-            throw Exception("BSDF::addChild(<%s>) is not supported!", classTypeName(obj->getClassType()));
-        texture_error = "textured BSDF inputs (" + obj->toString() + ") are not supported by the hip backend";
+protected:
+    void bind(int slot, const Object* obj) {
+        const Texture* t = static_cast<const Texture*>(obj);
+        for (auto& s : textures)
+            if (s.first == slot) {
+                s.second = t;
+                return;
+            }
+        textures.emplace_back(slot, t);
     }
 };
 
-// Diffuse (src/tracer/bsdf/diffuse.cpp:14-24): albedo, default 0.5.
+// Diffuse (src/tracer/bsdf/diffuse.cpp:14-24, 69-76): albedo, default 0.5;
+// a texture child named "albedo" replaces it.
 class Diffuse : public TexturableBSDF {
 public:
     explicit Diffuse(const PropertyList& p) : m_albedo(p.getColor("albedo", Color3f(0.5f))) {}
+    void addChild(Object* obj) override {
+        if (obj->getClassType() == ETexture && obj->getObjectName() == "albedo")
+            bind(DL_TEX_ALBEDO, obj);
+        else
+            warn("The child(%s) of diffuse is not used!", obj->getObjectName());
+    }
     dl_material_desc descriptor() const override {
         dl_material_desc d{};
         d.type = DL_BSDF_DIFFUSE;
@@ -273,15 +299,28 @@ private:
 };
 DRAWLAB_REGISTER_CLASS(Microfacet, "microfacet");
 
-// AnisoGGX (src/tracer/bsdf/aniso_ggx.cpp:10-27): pd, ps, axay (x, y used),
-// default 0.5 each.
+// AnisoGGX (src/tracer/bsdf/aniso_ggx.cpp:10-58): pd, ps, axay (x, y used),
+// default 0.5 each; texture children "pd", "ps", "axay", "normal",
+// "tangent"; is_tangent_space (default true) for the normal / tangent maps.
 class AnisoGGX : public TexturableBSDF {
 public:
     explicit AnisoGGX(const PropertyList& p)
         : m_pd(p.getColor("pd", Color3f(0.5f))),
           m_ps(p.getColor("ps", Color3f(0.5f))),
           m_axay(p.getColor("axay", Color3f(0.5f))) {
-        (void)p.getBoolean("is_tangent_space", true);
+        maps_object_space = !p.getBoolean("is_tangent_space", true);
+    }
+    void addChild(Object* obj) override {
+        if (obj->getClassType() != ETexture) {
+            warn("The child(%s) of aniso_ggx is not used!", obj->getObjectName());
+            return;
+        }
+        const std::string& n = obj->getObjectName();
+        if (n == "pd") bind(DL_TEX_PD, obj);
+        else if (n == "ps") bind(DL_TEX_PS, obj);
+        else if (n == "axay") bind(DL_TEX_AXAY, obj);
+        else if (n == "normal") bind(DL_TEX_NORMAL, obj);
+        else if (n == "tangent") bind(DL_TEX_TANGENT, obj);
     }
     dl_material_desc descriptor() const override {
         dl_material_desc d{};
@@ -696,9 +735,27 @@ void exportScene(const Scene& scene, ExportedScene& out) {
     out.meshes.clear();
     out.materials.clear();
     out.lights.clear();
+    out.textures.clear();
+    std::vector<const Texture*> tex_objs;
     for (const BSDF* b : scene.getBSDFs()) {
-        if (!b->texture_error.empty()) throw Exception("%s", b->texture_error);
-        out.materials.push_back(b->descriptor());
+        dl_material_desc d = b->descriptor();
+        // each distinct texture once, in order of first use
+        // (BSDF::createOptixMaterial, e.g. aniso_ggx.cpp:79-138)
+        for (const auto& s : b->textures) {
+            size_t k = 0;
+            while (k < tex_objs.size() && tex_objs[k] != s.second) k++;
+            if (k == tex_objs.size()) {
+                tex_objs.push_back(s.second);
+                dl_texture_desc td{};
+                td.width = s.second->width;
+                td.height = s.second->height;
+                td.rgb = s.second->rgb.data();
+                out.textures.push_back(td);
+            }
+            d.tex[s.first] = (int32_t)k + 1;
+        }
+        d.maps_object_space = b->maps_object_space ? 1 : 0;
+        out.materials.push_back(d);
     }
     const auto& meshes = scene.getMeshes();
     for (size_t i = 0; i < meshes.size(); i++) {
@@ -729,6 +786,8 @@ void exportScene(const Scene& scene, ExportedScene& out) {
     s.num_materials = (uint32_t)out.materials.size();
     s.lights = out.lights.data();
     s.num_lights = (uint32_t)out.lights.size();
+    s.textures = out.textures.data();
+    s.num_textures = (uint32_t)out.textures.size();
     s.camera = scene.getCamera()->descriptor();
     s.film = scene.getCamera()->getReconstructionFilter()->descriptor();
     put3(s.bg_color, scene.getBackgroundColor());

@@ -74,9 +74,17 @@ def _arr(ptr, count, dtype):
 
 def _to_scene_desc(sd, params):
     d = SceneDesc()
+    for i in range(sd.num_textures):
+        t = sd.textures[i]
+        d.add_texture(_arr(t.rgb, 3 * t.width * t.height, np.float32).reshape(t.height, t.width, 3).copy())
     for i in range(sd.num_materials):
         m = sd.materials[i]
-        d.bsdfs.append(Bsdf(int(m.type), tuple(float(x) for x in m.params[:_NPARAMS[int(m.type)]])))
+        b = Bsdf(int(m.type), tuple(float(x) for x in m.params[:_NPARAMS[int(m.type)]]))
+        slots = {s: int(m.tex[s]) - 1 for s in range(len(m.tex)) if m.tex[s] > 0}  # tex[] holds 1 + index
+        if slots:
+            b.textures = slots
+            b.maps_object_space = bool(m.maps_object_space)
+        d.bsdfs.append(b)
     lights = [sd.lights[i] for i in range(sd.num_lights)]
     for i in range(sd.num_meshes):
         md = sd.meshes[i]

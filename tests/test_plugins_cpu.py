@@ -273,9 +273,9 @@ def test_export_errors(tmp_path):
     tpl = MINIMAL.replace('<integrator type="path"/>', '<integrator type="normals"/>')
     with pytest.raises(host.XmlError, match="not supported by the hip backend"):
         load(tmp_path, LIGHT, template=tpl)
-    with pytest.raises(host.XmlError, match="textured BSDF inputs"):
+    with pytest.raises(host.XmlError, match="Unsupported texture format"):
         load(tmp_path, LIGHT + """<mesh type="rectangle"><vector name="pos" value="0,0,0"/><vector name="v0" value="1,0,0"/>
-          <vector name="v1" value="0,1,0"/><bsdf type="diffuse"><texture type="bitmap">
+          <vector name="v1" value="0,1,0"/><bsdf type="diffuse"><texture type="bitmap" name="albedo">
           <string name="filename" value="wood.jpg"/></texture></bsdf></mesh>""")
 
 
