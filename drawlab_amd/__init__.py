@@ -20,13 +20,15 @@ LIB_PATH = os.path.join(_HERE, "lib", "libdrawlab_hip.so")
 REPO_ROOT = os.path.dirname(_HERE)
 
 BUILD_GPU, BUILD_HOST_SAH, BUILD_GPU_LBVH = 0, 1, 2
+RENDER_NORMALS = 16
 LIGHT_AREA, LIGHT_POINT, LIGHT_ENV = 0, 1, 2
 
 EXPORTED_SYMBOLS = (
     "dl_hip_create", "dl_hip_destroy", "dl_hip_last_error", "dl_hip_device_name", "dl_hip_upload_scene",
     "dl_hip_build_accel", "dl_hip_trace_closest", "dl_hip_trace_shadow", "dl_hip_trace_closest_counted",
     "dl_hip_render", "dl_hip_render_paths", "dl_hip_malloc", "dl_hip_free", "dl_hip_memcpy_h2d",
-    "dl_hip_memcpy_d2h", "dl_hip_memcpy_d2d", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_camera_perspective", "dl_film_filter",
+    "dl_hip_memcpy_d2h", "dl_hip_memcpy_d2d", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_camera_perspective",
+    "dl_camera_opencv", "dl_film_filter", "dl_hip_set_view_options",
     "dl_mesh_tangents", "dl_hip_memory_usage", "dl_hip_trim", "dl_hip_set_camera",
     "dl_hip_scene_keep", "dl_hip_scene_select", "dl_hip_scene_drop", "dl_hip_accel_export",
     "dl_hip_service_start", "dl_hip_service_add_scene", "dl_hip_service_submit", "dl_hip_service_poll",
@@ -66,7 +68,11 @@ class dl_camera_desc(ctypes.Structure):
                 ("eye", ctypes.c_float * 3), ("U", ctypes.c_float * 3), ("V", ctypes.c_float * 3),
                 ("W", ctypes.c_float * 3), ("sample_to_camera", ctypes.c_float * 16),
                 ("camera_to_world", ctypes.c_float * 16), ("inv_size", ctypes.c_float * 2),
-                ("near_clip", ctypes.c_float), ("far_clip", ctypes.c_float)]
+                ("near_clip", ctypes.c_float), ("far_clip", ctypes.c_float), ("intrinsics", ctypes.c_float * 4)]
+
+
+class dl_view_options(ctypes.Structure):
+    _fields_ = [("bg_color", ctypes.c_float * 3), ("epsilon", ctypes.c_float), ("hide_lights", ctypes.c_int32)]
 
 
 class dl_film_desc(ctypes.Structure):
@@ -136,6 +142,9 @@ def lib():
             "dl_hip_last_kernel_ms": (i32, [P, c.POINTER(c.c_double)]),
             "dl_camera_virtual": (None, [P, P, P, c.c_float, i32, i32, c.POINTER(dl_camera_desc)]),
             "dl_camera_perspective": (i32, [P, c.c_float, c.c_float, c.c_float, i32, i32, c.POINTER(dl_camera_desc)]),
+            "dl_camera_opencv": (i32, [P, c.c_float, c.c_float, c.c_float, c.c_float, i32, i32,
+                                       c.POINTER(dl_camera_desc)]),
+            "dl_hip_set_view_options": (i32, [P, c.POINTER(dl_view_options)]),
             "dl_film_filter": (None, [i32, c.c_float, c.c_float, c.c_float, c.POINTER(dl_film_desc)]),
             "dl_mesh_tangents": (None, [P, u32, P, u32, P, P]),
             "dl_hip_guide_init": (i32, [P, c.c_uint64, c.c_float, P]),
@@ -170,6 +179,13 @@ def camera_desc(cam: Camera) -> dl_camera_desc:
                                          int(cam.height), ctypes.byref(out))
         if st != 0:
             raise ValueError("perspective camera: projection is not invertible")
+        return out
+    if cam.type == "opencv":
+        ext = np.ascontiguousarray(cam.extrinsic, dtype=np.float32).reshape(16)
+        st = lib().dl_camera_opencv(_p(ext), float(cam.cx), float(cam.cy), float(cam.fx), float(cam.fy),
+                                    int(cam.width), int(cam.height), ctypes.byref(out))
+        if st != 0:
+            raise ValueError("opencv camera: extrinsic matrix not invertible or zero focal length")
         return out
     e = np.asarray(cam.eye, np.float32)
     la = np.asarray(cam.lookat, np.float32)
@@ -439,19 +455,33 @@ class Backend:
 
     def render(self, spp=None, seed=0, max_depth=None, rr_depth=None, spp_per_pass=0, want_image=True,
                count_traversal=False, first_sample=0, accumulate=False, guided=False, guide_frozen=False,
-               guide_steps=0):
+               guide_steps=0, normals=False):
         d = self.desc
         spp = d.spp if spp is None else spp
         max_depth = d.max_depth if max_depth is None else max_depth
         rr_depth = d.rr_depth if rr_depth is None else rr_depth
         flags = (1 if count_traversal else 0) | (2 if accumulate else 0) | (4 if guided else 0) | \
-            (8 if guide_frozen else 0)
+            (8 if guide_frozen else 0) | (RENDER_NORMALS if normals else 0)
         p = self._params(spp, seed, max_depth, rr_depth, spp_per_pass, first_sample, flags)
         p.guide_steps = int(guide_steps)
         img = np.zeros((d.camera.height, d.camera.width, 3), np.float32) if want_image else None
         st = dl_render_stats()
         self._check(self.L.dl_hip_render(self.h, ctypes.byref(p), _p(img), ctypes.byref(st)))
         return img, st.as_dict()
+
+    def set_view_options(self, bg_color=None, epsilon=0.0, hide_lights=False):
+        """Viewer settings of the current scene (dl_hip_set_view_options):
+        background colour (None: the scene's), ray offset (0: 1e-4) and
+        hidden light surfaces; reset_view_options() restores the defaults."""
+        o = dl_view_options()
+        bg = self.desc.bg_color if bg_color is None else bg_color
+        for k in range(3):
+            o.bg_color[k] = float(bg[k])
+        o.epsilon, o.hide_lights = float(epsilon), int(bool(hide_lights))
+        self._check(self.L.dl_hip_set_view_options(self.h, ctypes.byref(o)))
+
+    def reset_view_options(self):
+        self._check(self.L.dl_hip_set_view_options(self.h, None))
 
     # ---- learned directional proposal (dl_hip_guide_*) ------------------
     GUIDE_PARAMS = 21808
@@ -491,11 +521,11 @@ class Backend:
                                                    ctypes.byref(loss)))
         return float(loss.value)
 
-    def render_paths(self, sample, seed=0, max_depth=None, rr_depth=None):
+    def render_paths(self, sample, seed=0, max_depth=None, rr_depth=None, normals=False):
         d = self.desc
         max_depth = d.max_depth if max_depth is None else max_depth
         rr_depth = d.rr_depth if rr_depth is None else rr_depth
-        p = self._params(sample + 1, seed, max_depth, rr_depth)
+        p = self._params(sample + 1, seed, max_depth, rr_depth, flags=RENDER_NORMALS if normals else 0)
         out = np.zeros((d.camera.height * d.camera.width, 5), np.float32)
         self._check(self.L.dl_hip_render_paths(self.h, ctypes.byref(p), int(sample), _p(out)))
         return out

@@ -52,6 +52,8 @@ void swap_scene(Ctx* c, SceneState& s) {
     std::swap(c->cam, s.cam);
     std::swap(c->film, s.film);
     std::swap(c->bg, s.bg);
+    std::swap(c->view_set, s.view_set);
+    std::swap(c->view, s.view);
     std::swap(c->bbox_lo, s.bbox_lo);
     std::swap(c->bbox_inv, s.bbox_inv);
     std::swap(c->ds, s.ds);
@@ -78,6 +80,20 @@ void free_stashed(Ctx* c, SceneState& s) {
     if (s.ds.tris) dfree(c, s.ds.tris);
     for (void* p : s.scene_allocs) dfree(c, p);
     s = SceneState();
+}
+
+// Camera descriptor checks shared by upload and set_camera.
+dl_status check_camera(Ctx* c, const dl_camera_desc& cam) {
+    if (cam.width <= 0 || cam.height <= 0) return set_error(c, DL_ERR_INVALID, "bad camera size");
+    if (cam.type != DL_CAMERA_VIRTUAL && cam.type != DL_CAMERA_PERSPECTIVE && cam.type != DL_CAMERA_OPENCV)
+        return set_error(c, DL_ERR_INVALID, "unknown camera type");
+    if (cam.type == DL_CAMERA_OPENCV) {
+        for (int k = 0; k < 4; k++)
+            if (!std::isfinite(cam.intrinsics[k])) return set_error(c, DL_ERR_INVALID, "opencv intrinsics must be finite");
+        if (cam.intrinsics[2] == 0.0f || cam.intrinsics[3] == 0.0f)
+            return set_error(c, DL_ERR_INVALID, "opencv focal lengths must be nonzero");
+    }
+    return DL_OK;
 }
 
 // Releases the current scene before another becomes current: kept resident
@@ -180,7 +196,7 @@ dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
     hipSetDevice(c->device);
     release_current(c);
     if (sd->num_meshes == 0) return set_error(c, DL_ERR_INVALID, "scene has no meshes");
-    if (sd->camera.width <= 0 || sd->camera.height <= 0) return set_error(c, DL_ERR_INVALID, "bad camera size");
+    DL_TRY(check_camera(c, sd->camera));
     for (uint32_t i = 0; i < sd->num_materials; i++) {
         if (sd->materials[i].type < 0 || sd->materials[i].type >= DL_BSDF_COUNT)
             return set_error(c, DL_ERR_INVALID, "unknown material type");
@@ -489,10 +505,27 @@ dl_status dl_hip_set_camera(void* h, const dl_camera_desc* cam) {
     Ctx* c = (Ctx*)h;
     if (!c || !cam) return DL_ERR_INVALID;
     if (!c->has_scene) return set_error(c, DL_ERR_STATE, "set_camera before upload_scene");
-    if (cam->width <= 0 || cam->height <= 0) return set_error(c, DL_ERR_INVALID, "bad camera size");
-    if (cam->type != DL_CAMERA_VIRTUAL && cam->type != DL_CAMERA_PERSPECTIVE)
-        return set_error(c, DL_ERR_INVALID, "unknown camera type");
+    DL_TRY(check_camera(c, *cam));
     c->cam = *cam;
+    invalidate_film(c);
+    return DL_OK;
+}
+dl_status dl_hip_set_view_options(void* h, const dl_view_options* o) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return DL_ERR_INVALID;
+    if (!c->has_scene) return set_error(c, DL_ERR_STATE, "set_view_options before upload_scene");
+    if (!o) {
+        c->view_set = false;
+        c->view = dl_view_options{};
+        invalidate_film(c);
+        return DL_OK;
+    }
+    if (!(o->epsilon >= 0.0f && o->epsilon < 1.0f))
+        return set_error(c, DL_ERR_INVALID, "epsilon must be in [0, 1)");
+    for (int k = 0; k < 3; k++)
+        if (!std::isfinite(o->bg_color[k])) return set_error(c, DL_ERR_INVALID, "bg_color must be finite");
+    c->view_set = true;
+    c->view = *o;
     invalidate_film(c);
     return DL_OK;
 }

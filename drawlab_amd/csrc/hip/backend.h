@@ -86,6 +86,8 @@ struct SceneState {
     dl_camera_desc cam{};
     dl_film_desc film{};
     float bg[3] = {0, 0, 0};
+    bool view_set = false;  // view options given (dl_hip_set_view_options)
+    dl_view_options view{};
     float bbox_lo[3] = {0, 0, 0}, bbox_inv[3] = {1, 1, 1};
     DevScene ds;
     dl_accel_stats accel{};
@@ -118,6 +120,8 @@ struct Ctx {
     dl_camera_desc cam{};
     dl_film_desc film{};
     float bg[3] = {0, 0, 0};
+    bool view_set = false;  // view options given (dl_hip_set_view_options); they belong to the scene
+    dl_view_options view{};
     float bbox_lo[3] = {0, 0, 0}, bbox_inv[3] = {1, 1, 1};  // scene bounds: lo, 1 / extent per axis
     DevScene ds;
     dl_accel_stats accel{};

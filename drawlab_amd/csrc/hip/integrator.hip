@@ -36,7 +36,13 @@ namespace dl {
 constexpr int kStageBlock = 256;
 constexpr int kTypes = 8;  // material queue segments (DL_BSDF_COUNT rounded up)
 
-enum : uint32_t { kFlagDiffuse = 1u, kFlagNee = 2u, kFlagCont = 4u };
+// kFlagCamera: the continuation is still the camera ray (it passed through
+// a hidden light), so its next hit or miss is treated as the first one.
+enum : uint32_t { kFlagDiffuse = 1u, kFlagNee = 2u, kFlagCont = 4u, kFlagCamera = 8u };
+// Material-queue segment of paths that pass straight through a hidden
+// light's surface (after the BSDF types).
+constexpr int kPassQ = DL_BSDF_COUNT;
+static_assert(kPassQ < kTypes, "pass-through queue needs a segment");
 
 struct Counters {
     uint32_t mat[2][kTypes];
@@ -188,6 +194,7 @@ struct PassParams {
     int max_depth;
     int rr_depth;
     uint32_t cap;         // queue segment capacity
+    float eps;            // ray offset (tmin; shadow rays end this short of the light)
 };
 
 // ---------------------------------------------------------------------------
@@ -238,7 +245,7 @@ __global__ __launch_bounds__(kStageBlock) void k_raygen(PassParams pp, dl_camera
         const float ax = rng.next_float();
         const float ay = rng.next_float();
         const float px = (float)x + ax, py = (float)y + ay;
-        rng.next_float();  // aperture sample (unused by both cameras)
+        rng.next_float();  // aperture sample (unused by every camera)
         rng.next_float();
         if (cam.type == DL_CAMERA_PERSPECTIVE) {
             // PerspectiveCamera::sampleRay  perspective.cpp:75-95
@@ -253,12 +260,25 @@ __global__ __launch_bounds__(kStageBlock) void k_raygen(PassParams pp, dl_camera
             mat4_apply(cam.camera_to_world, d_in, d4);
             rs.ray_o[p] = make_float4(o4[0] / o4[3], o4[1] / o4[3], o4[2] / o4[3], cam.near_clip * inv_z);
             rs.ray_d[p] = make_float4(d4[0], d4[1], d4[2], cam.far_clip * inv_z);
+        } else if (cam.type == DL_CAMERA_OPENCV) {
+            // OpencvCamera (include/optix/camera/opencv.h:28-46): the film
+            // position relative to the principal point over the focal
+            // lengths, at z = 1 (w = 1) in camera space, mapped to the world
+            // by the inverse extrinsic matrix; the ray leaves the eye toward
+            // that point
+            const float in[4] = {(px - cam.intrinsics[0]) / cam.intrinsics[2],
+                                 (py - cam.intrinsics[1]) / cam.intrinsics[3], 1.0f, 1.0f};
+            float q[4];
+            mat4_apply(cam.camera_to_world, in, q);
+            const vec3 d = normalized(sub(v3(q[0], q[1], q[2]), pcol(cam.eye)));
+            rs.ray_o[p] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], pp.eps);
+            rs.ray_d[p] = f4(d, INFINITY);
         } else {
             // VirtualCamera::sampleRay  virtual.cpp:52-65
             const float lx = px, ly = ((float)cam.height - py) - 1;
             const float dx = lx * invx * 2 - 1, dy = ly * invy * 2 - 1;
             const vec3 d = normalized(add(add(scale(pcol(cam.U), dx), scale(pcol(cam.V), dy)), pcol(cam.W)));
-            rs.ray_o[p] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], kEpsilon);
+            rs.ray_o[p] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], pp.eps);
             rs.ray_d[p] = f4(d, INFINITY);
         }
         rs.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
@@ -325,6 +345,10 @@ __device__ __forceinline__ int material_of(const SceneView& s, uint32_t prim, in
 }
 
 // First intersection (path.cpp:21-29) and a camera-ray miss (miss.cu:23-29).
+// HIDE: light surfaces are hidden (hitgroup_pgs.cu:55-64): a camera ray that
+// hits one adds no emission and passes through it (queue kPassQ), still as
+// the camera ray.
+template <bool HIDE>
 __global__ __launch_bounds__(kStageBlock) void k_post_camera(PassParams pp, SceneView s, RenderBufs rs, int next) {
     const uint32_t n = rs.cnt->ray;
     __shared__ uint32_t s_p[kStageItems][kStageBlock];
@@ -350,6 +374,10 @@ __global__ __launch_bounds__(kStageBlock) void k_post_camera(PassParams pp, Scen
                         const LightRec& e = s.lights[s.env_light];
                         rad = e.env_visual ? env_eval(s, e, d) : v3(s.bg[0], s.bg[1], s.bg[2]);
                     }
+                } else if (HIDE && s.meshes[s.prim_mesh[prim]].light >= 0) {
+                    rs.aux[p] = make_int4(0, (int)(kFlagCont | kFlagCamera), 0, 0);
+                    alive = true;
+                    type = kPassQ;
                 } else {
                     const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
                     if (mr.light >= 0) {
@@ -375,9 +403,66 @@ __global__ __launch_bounds__(kStageBlock) void k_post_camera(PassParams pp, Scen
         }
         uint32_t* const mq = rs.matq[next];
         const uint32_t cap = pp.cap;
-        block_append<DL_BSDF_COUNT, kStageItems>(rs.cnt->mat[next], bv, [&](int t, int u, uint32_t slot) {
+        block_append<HIDE ? kPassQ + 1 : DL_BSDF_COUNT, kStageItems>(rs.cnt->mat[next], bv,
+                                                                     [&](int t, int u, uint32_t slot) {
             mq[(size_t)t * cap + slot] = pv[u];
         });
+    }
+}
+
+// Paths whose ray hit a hidden light surface continue from the hit point in
+// the same direction (hitgroup_pgs.cu:55-64: fr = 1, eta = 1, p = its.p,
+// wo = ray direction): no random numbers, no bounce, the previous vertex's
+// BSDF pdf and diffuse flag kept for the MIS of the next emitter hit.
+__global__ __launch_bounds__(kStageBlock) void k_pass(PassParams pp, SceneView s, RenderBufs rs, int cur) {
+    const uint32_t n = rs.cnt->mat[cur][kPassQ];
+    const uint32_t* __restrict__ queue = rs.matq[cur] + (size_t)kPassQ * pp.cap;
+    __shared__ uint32_t s_p[kStageItems][kStageBlock];
+    const uint32_t items = stage_items(n);
+    for (uint32_t i0 = blockIdx.x * items * kStageBlock; i0 < n; i0 += gridDim.x * items * kStageBlock) {
+#pragma unroll 1
+        for (int u = 0; u < kStageItems; u++) {
+            const uint32_t i = (uint32_t)u < items ? i0 + (uint32_t)u * kStageBlock + threadIdx.x : n;
+            uint32_t p = 0xffffffffu;
+            if (i < n) {
+                p = queue[i];
+                const float4 h = rs.hit[p];
+                Its its;
+                make_its(s, __float_as_uint(h.x), h.z, h.w, false, its);
+                int4 aux = rs.aux[p];
+                aux.y = (int)(((uint32_t)aux.y & (kFlagDiffuse | kFlagCamera)) | kFlagCont);
+                rs.aux[p] = aux;
+                rs.ray_o[p] = f4(its.p, pp.eps);
+                rs.ray_d[p] = f4(f3(rs.ray_d[p]), INFINITY);
+            }
+            s_p[u][threadIdx.x] = p;
+        }
+        uint32_t pv[kStageItems], bv[kStageItems];
+#pragma unroll
+        for (int u = 0; u < kStageItems; u++) {
+            pv[u] = s_p[u][threadIdx.x];
+            bv[u] = pv[u] != 0xffffffffu ? 1u : 0u;
+        }
+        block_append<1, kStageItems>(&rs.cnt->ray, bv, [&](int, int u, uint32_t slot) { rs.rayq[slot] = pv[u]; });
+    }
+}
+
+// The "normals" integrator (src/tracer/integrator/normal.cpp:9-19): the
+// component-wise absolute value of the shading normal of the first hit
+// (the record of OCTree::rayIntersect), zero on a miss.
+__global__ __launch_bounds__(kStageBlock) void k_normals(SceneView s, RenderBufs rs) {
+    const uint32_t n = rs.cnt->ray;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t p = rs.rayq[i];
+        const float4 h = rs.hit[p];
+        const uint32_t prim = __float_as_uint(h.x);
+        vec3 rad = v3(0.0f, 0.0f, 0.0f);
+        if (prim != 0xffffffffu) {
+            Its its;
+            make_its(s, prim, h.z, h.w, false, its);
+            rad = v3(fabsf(its.sh.n.x), fabsf(its.sh.n.y), fabsf(its.sh.n.z));
+        }
+        rs.rad[p] = f4(rad, 0.f);
     }
 }
 
@@ -458,8 +543,8 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                     const vec3 contrib = mul(mul(scale(thr, mis), bv), spectrum);
                     if (csum(spectrum) > 0) {
                         rs.nee[p] = f4(contrib, 0.f);
-                        rs.sh_o[p] = f4(its.p, kEpsilon);
-                        rs.sh_d[p] = f4(ds.d, ds.dist - kEpsilon);
+                        rs.sh_o[p] = f4(its.p, pp.eps);
+                        rs.sh_d[p] = f4(ds.d, ds.dist - pp.eps);
                         flags |= kFlagNee;
                         want_shadow = true;
                     } else {
@@ -510,7 +595,7 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                 eta *= br.eta;
                 if (bsdf_is_diffuse(type)) flags |= kFlagDiffuse;
                 if (!is_zero(thr)) {
-                    rs.ray_o[p] = f4(its.p, kEpsilon);
+                    rs.ray_o[p] = f4(its.p, pp.eps);
                     rs.ray_d[p] = f4(to_world(its.sh, br.wo), INFINITY);
                     flags |= kFlagCont;
                     want_cont = true;
@@ -574,6 +659,11 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
 
 // NEE resolution, emitter hit / miss with MIS, Russian roulette
 // (path.cpp:46-56, 71-97; miss.cu:31-42).
+// HIDE: light surfaces are hidden (hitgroup_pgs.cu:55-64): a path ray that
+// hits one collects its emission as usual, unless it is still the camera
+// ray, and passes through (queue kPassQ) without a bounce; a camera ray that
+// passed through is treated as the camera ray at its next hit or miss.
+template <bool HIDE>
 __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s, RenderBufs rs, int cur,
                                                       float* __restrict__ grec) {
     const uint32_t* counts = rs.cnt->mat[cur];
@@ -616,13 +706,41 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
                     const uint32_t prim = __float_as_uint(h.x);
                     const vec3 o = f3(rs.ray_o[p]);
                     const vec3 d = f3(rs.ray_d[p]);
+                    const bool cam = HIDE && (flags & kFlagCamera) != 0;
                     if (prim == 0xffffffffu) {
                         if (s.env_light >= 0) {
                             const LightRec& e = s.lights[s.env_light];
-                            float epdf;
-                            const vec3 ev = env_eval_pdf(s, e, d, epdf);
-                            const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
-                            rad = add(rad, mul(scale(thr, mis), ev));
+                            if (cam) {
+                                rad = add(rad, mul(thr, e.env_visual ? env_eval(s, e, d) : v3(s.bg[0], s.bg[1], s.bg[2])));
+                            } else {
+                                float epdf;
+                                const vec3 ev = env_eval_pdf(s, e, d, epdf);
+                                const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
+                                rad = add(rad, mul(scale(thr, mis), ev));
+                            }
+                        }
+                    } else if (HIDE && (cam || s.meshes[s.prim_mesh[prim]].light >= 0)) {
+                        const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
+                        if (mr.light >= 0) {
+                            if (!cam) {
+                                vec3 hp, hn;
+                                hit_point_normal(s, prim, h.z, h.w, hp, hn);
+                                const vec3 ev =
+                                    dot(hn, neg(d)) > 0.f ? pcol(s.lights[mr.light].radiance) : v3(0, 0, 0);
+                                const DirSample hs = dir_sample(o, hp, hn);
+                                const float epdf = pdf_emitter_direction(s, mr.light, hs);
+                                const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
+                                rad = add(rad, mul(scale(thr, mis), ev));
+                            }
+                            aux.w = 0;  // the record's radiance before its continuation is already kept
+                            rs.aux[p] = aux;
+                            alive = true;
+                            ntype = kPassQ;
+                        } else {
+                            // the camera ray's first visible surface (as k_post_camera)
+                            alive = !(pp.max_depth >= 0 && 0 + 2 > pp.max_depth);
+                            int mat;
+                            ntype = material_of(s, prim, mat);
                         }
                     } else {
                         const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
@@ -673,7 +791,8 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
         }
         uint32_t* const mq = rs.matq[next];
         const uint32_t cap = pp.cap;
-        block_append<DL_BSDF_COUNT, kStageItems>(rs.cnt->mat[next], bv, [&](int t, int u, uint32_t slot) {
+        block_append<HIDE ? kPassQ + 1 : DL_BSDF_COUNT, kStageItems>(rs.cnt->mat[next], bv,
+                                                                     [&](int t, int u, uint32_t slot) {
             mq[(size_t)t * cap + slot] = pv[u];
         });
     }
@@ -983,7 +1102,7 @@ SceneView scene_view(Ctx* c) {
     s.tex_info = d.tex_info;
     s.num_lights = (int)c->light_recs.size();
     s.env_light = c->env_light;
-    for (int k = 0; k < 3; k++) s.bg[k] = c->bg[k];
+    for (int k = 0; k < 3; k++) s.bg[k] = c->view_set ? c->view.bg_color[k] : c->bg[k];
     return s;
 }
 
@@ -1019,6 +1138,11 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
     if (prm->guide_steps < 0 || prm->guide_steps > 4096)
         return set_error(c, DL_ERR_INVALID, "guide_steps must be in [0, 4096]");
     if (c->light_recs.empty()) return set_error(c, DL_ERR_INVALID, "There is not emitter in the scene.");
+    const bool normals = (prm->flags & DL_RENDER_NORMALS) != 0;
+    if (normals && (prm->flags & DL_RENDER_GUIDED))
+        return set_error(c, DL_ERR_INVALID, "DL_RENDER_NORMALS and DL_RENDER_GUIDED exclude each other");
+    const bool hide = c->view_set && c->view.hide_lights != 0;
+    const float eps = c->view_set && c->view.epsilon > 0.0f ? c->view.epsilon : kEpsilon;
     (void)hipSetDevice(c->device);
     const uint32_t W = (uint32_t)c->cam.width, H = (uint32_t)c->cam.height, WH = W * H;
     // Default pass: about 2^26 paths (32 spp at 1920x1080, ~16 GB of path
@@ -1093,6 +1217,7 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
         pp.max_depth = prm->max_depth;
         pp.rr_depth = prm->rr_depth;
         pp.cap = cap;
+        pp.eps = eps;
         paths += pp.P;
         closest += pp.P;
         int cur = 0;
@@ -1114,7 +1239,12 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
             k_trace_queue<false><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
         DL_TRY(tm.stop());
         DL_TRY(tm.start(kTagShade));
-        k_post_camera<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
+        if (normals)
+            k_normals<<<stage_grid, kStageBlock, 0, c->stream>>>(sv, rs->b);
+        else if (hide)
+            k_post_camera<true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
+        else
+            k_post_camera<false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
         DL_TRY(tm.stop());
         DL_HIP_CHECK(c, hipGetLastError());
         for (int iter = 0;; ++iter) {
@@ -1129,6 +1259,8 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
             if (live == 0) break;
             k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, cur ^ 1);
             DL_TRY(tm.start(kTagShade));
+            if (rs->cnt_host->mat[cur][kPassQ] > 0)
+                k_pass<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
             for (int t = 0; t < DL_BSDF_COUNT; t++) {
                 if (!rs->types_present[t] || rs->cnt_host->mat[cur][t] == 0) continue;
                 if (guided && guide_types[t]) {
@@ -1170,7 +1302,10 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
             k_shadow_queue<<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
             DL_TRY(tm.stop());
             DL_TRY(tm.start(kTagShade));
-            k_post<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv.rec);
+            if (hide)
+                k_post<true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv.rec);
+            else
+                k_post<false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv.rec);
             DL_TRY(tm.stop());
             DL_HIP_CHECK(c, hipGetLastError());
             cur ^= 1;

@@ -174,6 +174,30 @@ dl_status dl_camera_perspective(const float to_world[16], float fov, float near_
     return DL_OK;
 }
 
+// OpencvCamera (opencv.cpp:9-24): the inverse of the extrinsic (world to
+// camera) matrix, the eye at its translation (inverse * (0, 0, 0, 1)) and
+// the intrinsics as given.
+dl_status dl_camera_opencv(const float extrinsic[16], float cx, float cy, float fx, float fy, int width, int height,
+                           dl_camera_desc* out) {
+    std::memset(out, 0, sizeof(*out));
+    if (!extrinsic || width <= 0 || height <= 0 || fx == 0.0f || fy == 0.0f) return DL_ERR_INVALID;
+    float ext[4][4], inv[4][4];
+    for (int j = 0; j < 4; j++)
+        for (int i = 0; i < 4; i++) ext[j][i] = extrinsic[4 * j + i];
+    if (!dlm::mat4_inverse(ext, inv)) return DL_ERR_INVALID;
+    out->type = DL_CAMERA_OPENCV;
+    out->width = width;
+    out->height = height;
+    for (int j = 0; j < 4; j++)
+        for (int i = 0; i < 4; i++) out->camera_to_world[4 * j + i] = inv[j][i];
+    for (int j = 0; j < 3; j++) out->eye[j] = inv[j][3];
+    out->intrinsics[0] = cx;
+    out->intrinsics[1] = cy;
+    out->intrinsics[2] = fx;
+    out->intrinsics[3] = fy;
+    return DL_OK;
+}
+
 void dl_film_filter(int type, float radius, float p0, float p1, dl_film_desc* out) {
     std::memset(out, 0, sizeof(*out));
     if (type == 1) radius = 0.5f;  // BoxFilter

@@ -112,6 +112,9 @@ protected:
 class Integrator : public Object {
 public:
     EClassType getClassType() const override { return EIntegrator; }
+    // DL_RENDER_* flags of the integrator on the HIP backend, or -1 when
+    // it has no HIP counterpart.
+    virtual int hipFlags() const { return -1; }
     // Path-tracing parameters: maxDepth (-1 = unlimited) and rrDepth.
     virtual bool isPathTracer() const { return false; }
     virtual int maxDepth() const { return -1; }

@@ -561,18 +561,37 @@ private:
 };
 DRAWLAB_REGISTER_CLASS(PerspectiveCamera, "perspective");
 
-// OpencvCamera (src/tracer/camera/opencv.cpp): parsed; not rendered by the
-// hip backend.
+// OpencvCamera (src/tracer/camera/opencv.cpp:9-24): cx, cy, fx, fy
+// (pixels) and the extrinsic (world-to-camera) transform, all required;
+// width 1280, height 720.  Rays as the reference's GPU path casts them
+// (include/optix/camera/opencv.h:24-47; its CPU sampleRay throws).
 class OpencvCamera : public Camera {
 public:
     explicit OpencvCamera(const PropertyList& p) {
+        m_cx = p.getFloat("cx");
+        m_cy = p.getFloat("cy");
+        m_fx = p.getFloat("fx");
+        m_fy = p.getFloat("fy");
+        m_ext = p.getTransform("extrinsic");
         m_w = p.getInteger("width", 1280);
         m_h = p.getInteger("height", 720);
     }
     dl_camera_desc descriptor() const override {
-        throw Exception("camera 'opencv' is not supported by the hip backend");
+        dl_camera_desc d;
+        float ext[16];
+        for (int j = 0; j < 4; j++)
+            for (int i = 0; i < 4; i++) ext[4 * j + i] = m_ext.matrix().m[j][i];
+        if (dl_camera_opencv(ext, m_cx, m_cy, m_fx, m_fy, m_w, m_h, &d) != DL_OK)
+            throw Exception("OpencvCamera: the extrinsic matrix is not invertible or a focal length is zero");
+        return d;
     }
-    std::string toString() const override { return "OpencvCamera[]"; }
+    std::string toString() const override {
+        return format("OpencvCamera[cxcy = %f %f, fxfy = %f %f, size = %ix%i]", m_cx, m_cy, m_fx, m_fy, m_w, m_h);
+    }
+
+private:
+    float m_cx, m_cy, m_fx, m_fy;
+    Transform m_ext;
 };
 DRAWLAB_REGISTER_CLASS(OpencvCamera, "opencv");
 
@@ -587,6 +606,7 @@ class PathIntegrator : public Integrator {
 public:
     explicit PathIntegrator(const PropertyList& p)
         : m_max_depth(p.getInteger("maxDepth", -1)), m_rr_depth(p.getInteger("rrDepth", 4)) {}
+    int hipFlags() const override { return 0; }
     bool isPathTracer() const override { return true; }
     int maxDepth() const override { return m_max_depth; }
     int rrDepth() const override { return m_rr_depth; }
@@ -606,8 +626,11 @@ public:
 private:
     std::string m_name;
 };
+// NormalIntegrator (src/tracer/integrator/normal.cpp:9-19): |shading
+// normal| of the first hit, rendered by DL_RENDER_NORMALS.
 struct NormalIntegrator : OtherIntegrator {
-    explicit NormalIntegrator(const PropertyList& p) : OtherIntegrator(p, "normals") {}
+    explicit NormalIntegrator(const PropertyList& p) : OtherIntegrator(p, "NormalIntegrator[]") {}
+    int hipFlags() const override { return DL_RENDER_NORMALS; }
 };
 struct WhittedIntegrator : OtherIntegrator {
     explicit WhittedIntegrator(const PropertyList& p) : OtherIntegrator(p, "whitted") {}
@@ -729,8 +752,9 @@ std::string Scene::toString() const {
 // ============================================================================
 void exportScene(const Scene& scene, ExportedScene& out) {
     const Integrator* integ = scene.getIntegrator();
-    if (!integ->isPathTracer())
-        throw Exception("integrator '%s' is not supported by the hip backend (only 'path')", integ->toString());
+    if (integ->hipFlags() < 0)
+        throw Exception("integrator '%s' is not supported by the hip backend (only 'path' and 'normals')",
+                        integ->toString());
     if (scene.getEmitters().empty()) throw Exception("There is not emitter in the scene.");
     out.meshes.clear();
     out.materials.clear();
@@ -797,6 +821,7 @@ void exportScene(const Scene& scene, ExportedScene& out) {
     p.seed = scene.getSampler()->getSeed();
     p.max_depth = integ->maxDepth();
     p.rr_depth = integ->rrDepth();
+    p.flags = integ->hipFlags();
 }
 
 }  // namespace drawlab

@@ -73,9 +73,9 @@ class EnvMap:
 
 @dataclass
 class Camera:
-    """"virtual" (VirtualCamera: eye/lookat/up, vertical fov) or
+    """"virtual" (VirtualCamera: eye/lookat/up, vertical fov),
     "perspective" (PerspectiveCamera: toWorld 4x4, horizontal fov, clip
-    distances)."""
+    distances) or "opencv" (OpencvCamera: extrinsic 4x4, cx, cy, fx, fy)."""
     eye: Tuple[float, float, float] = (0.0, 0.0, 5.0)
     lookat: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     up: Tuple[float, float, float] = (0.0, 1.0, 0.0)
@@ -86,6 +86,13 @@ class Camera:
     to_world: Optional[np.ndarray] = None  # (4, 4) float32, row-major
     near: float = 1e-4
     far: float = 1e4
+    # "opencv" (OpencvCamera): world-to-camera extrinsic (4, 4) row-major,
+    # principal point and focal lengths in pixels
+    extrinsic: Optional[np.ndarray] = None
+    cx: float = 0.0
+    cy: float = 0.0
+    fx: float = 1.0
+    fy: float = 1.0
 
 
 @dataclass

@@ -213,6 +213,26 @@ def cornell_box(width=512, height=512, spp=64, max_depth=8, rr_depth=3):
     return s
 
 
+def opencv_camera(eye, lookat, up, fov, width, height):
+    """An "opencv" camera (OpencvCamera) looking from eye at lookat: the
+    world-to-camera extrinsic with OpenCV axes (x right, y down, z forward)
+    and square pixels whose vertical field of view is fov degrees, the
+    principal point at the image centre."""
+    eye, lookat, up = (np.asarray(a, np.float64) for a in (eye, lookat, up))
+    z = lookat - eye
+    z /= np.linalg.norm(z)
+    x = np.cross(z, up)
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    R = np.stack([x, y, z])
+    ext = np.eye(4)
+    ext[:3, :3] = R
+    ext[:3, 3] = -R @ eye
+    f = 0.5 * height / np.tan(np.radians(fov) / 2)
+    return Camera(type="opencv", width=width, height=height, extrinsic=ext.astype(np.float32), cx=width / 2,
+                  cy=height / 2, fx=float(f), fy=float(f))
+
+
 def sky_envmap(width=1024, height=512, sun_dir=(0.5, 0.7071, 0.5), sun_peak=2e4, sun_sigma_deg=0.5):
     """Lat-long sky: gradient 0.1 (horizon and below) to 1.0 (zenith) plus a
     Gaussian sun lobe (peak sun_peak, sigma sun_sigma_deg)."""

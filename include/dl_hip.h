@@ -62,7 +62,7 @@ enum {
 };
 
 enum { DL_LIGHT_AREA = 0, DL_LIGHT_POINT = 1, DL_LIGHT_ENV = 2 };
-enum { DL_CAMERA_VIRTUAL = 0, DL_CAMERA_PERSPECTIVE = 1 };
+enum { DL_CAMERA_VIRTUAL = 0, DL_CAMERA_PERSPECTIVE = 1, DL_CAMERA_OPENCV = 2 };
 
 typedef struct dl_mesh_desc {
     const float* positions;    /* 3 * num_vertices, world space */
@@ -125,7 +125,11 @@ typedef struct dl_light_desc {
    after its constructor -- eye, U, V, W as the reference computes them.
    DL_CAMERA_PERSPECTIVE: PerspectiveCamera (src/tracer/camera/perspective.cpp:
    21-95) after activate() -- the sample-to-camera and camera-to-world
-   matrices (row-major), 1/size and the clip distances. */
+   matrices (row-major), 1/size and the clip distances.
+   DL_CAMERA_OPENCV: OpencvCamera (src/tracer/camera/opencv.cpp:9-24, rays as
+   include/optix/camera/opencv.h:24-47 casts them) -- camera_to_world = the
+   inverse of the extrinsic matrix (row-major), eye = its translation,
+   intrinsics = (cx, cy, fx, fy) in pixels; film row 0 is image row 0. */
 typedef struct dl_camera_desc {
     int32_t type;              /* DL_CAMERA_* */
     int32_t width, height;
@@ -134,6 +138,7 @@ typedef struct dl_camera_desc {
     float camera_to_world[16];
     float inv_size[2];
     float near_clip, far_clip;
+    float intrinsics[4];       /* DL_CAMERA_OPENCV: cx, cy, fx, fy */
 } dl_camera_desc;
 
 /* ImageBlock filter tabulation (src/core/bitmap/block.cpp:12-27):
@@ -192,7 +197,10 @@ enum {
                                       (progressive rendering, src/optix/cuda/integrator/path.cu:86-90) */
     DL_RENDER_GUIDED = 4,          /* sample directions from the learned proposal too (one-sample MIS
                                       with BSDF sampling) and train it online after every pass */
-    DL_RENDER_GUIDE_FROZEN = 8     /* with DL_RENDER_GUIDED: use the proposal without training it */
+    DL_RENDER_GUIDE_FROZEN = 8,    /* with DL_RENDER_GUIDED: use the proposal without training it */
+    DL_RENDER_NORMALS = 16         /* the "normals" integrator (src/tracer/integrator/normal.cpp:9-19):
+                                      per camera sample the component-wise absolute value of the
+                                      shading normal at the first hit, 0 on a miss */
 };
 
 typedef struct dl_render_stats {
@@ -269,6 +277,26 @@ dl_status dl_hip_memory_usage(void* ctx, uint64_t* current_bytes, uint64_t* peak
    the next render starts empty.  Replaces OptixScene::updateCamera / resize
    (src/optix/host/optix_scene.cpp:43-52). */
 dl_status dl_hip_set_camera(void* ctx, const dl_camera_desc* camera);
+
+/* Viewer settings changed between renders without re-uploading the scene,
+   as the OptiX renderer's context holds them (include/tracer/backend/
+   context.h:12-14, applied by OptixRenderer::updateOptixScene,
+   src/tracer/backend/optix_renderer.cpp:84-102). */
+typedef struct dl_view_options {
+    float bg_color[3];         /* colour of camera rays that miss when no visible environment map */
+    float epsilon;             /* ray offset: tmin of camera, continuation and shadow rays and the
+                                  shadow rays' end short of the light; 0 = 1e-4 (the CPU path's
+                                  Epsilon, the default) */
+    int32_t hide_lights;       /* nonzero: surfaces of area lights are invisible to camera rays and
+                                  transparent to every path ray (hitgroup_pgs.cu:55-64): camera rays
+                                  pass through without their emission; later rays collect it (MIS-
+                                  weighted) and pass through; neither counts as a bounce nor draws
+                                  random numbers.  Shadow rays are still blocked by them. */
+} dl_view_options;
+/* Sets the options (NULL restores the defaults: the uploaded scene's
+   bg_color, epsilon 0, lights shown); they hold until changed or until
+   another scene is uploaded or selected. */
+dl_status dl_hip_set_view_options(void* ctx, const dl_view_options* opts);
 
 /* Frees the context's scenes (current and resident), acceleration
    structures and render state (the context keeps only its fixed buffers;
@@ -389,6 +417,12 @@ void dl_camera_virtual(const float eye[3], const float lookat[3], const float up
    identity).  Returns DL_ERR_INVALID if the projection is not invertible. */
 dl_status dl_camera_perspective(const float to_world[16], float fov, float near_clip, float far_clip, int width,
                                 int height, dl_camera_desc* out);
+/* OpencvCamera constructor (opencv.cpp:9-24): extrinsic is the row-major
+   world-to-camera matrix; its inverse, the eye (the inverse's translation)
+   and the intrinsics (pixels) fill `out`.  Returns DL_ERR_INVALID if the
+   extrinsic matrix is not invertible or a focal length is zero. */
+dl_status dl_camera_opencv(const float extrinsic[16], float cx, float cy, float fx, float fy, int width, int height,
+                           dl_camera_desc* out);
 /* ReconstructionFilter + ImageBlock tabulation (rfilter.cpp:13-98,
  * block.cpp:12-27).  type: 0 gaussian(radius, stddev) 1 box 2 tent
  * 3 mitchell(radius, B, C). */

@@ -54,6 +54,8 @@ def lib():
             "dlo_scene_set_filter": (None, [_P, i32, f32, f32, f32]),
             "dlo_scene_get_filter_table": (None, [_P, _f32p]),
             "dlo_scene_set_background": (None, [_P, _f32p]),
+            "dlo_scene_set_camera_opencv": (None, [_P, _f32p, _f32p, _f32p, i32, i32]),
+            "dlo_scene_set_options": (None, [_P, i32, i32, f32]),
             "dlo_scene_build": (i32, [_P]),
             "dlo_scene_num_prims": (u32, [_P]),
             "dlo_trace_closest": (None, [_P, i32, _f32p, u32, _u32p, _f32p, _f32p, _f32p]),
@@ -156,6 +158,18 @@ class Scene:
         self.width, self.height = int(width), int(height)
         self.L.dlo_scene_set_camera_perspective(self.h, _f(sample_to_camera, 16), _f(camera_to_world, 16),
                                                 int(width), int(height), float(near), float(far))
+
+    def set_camera_opencv(self, camera_to_world, eye, intrinsics, width, height):
+        """OpencvCamera: the inverse extrinsic matrix (row-major), the eye and
+        (cx, cy, fx, fy) are inputs."""
+        self.width, self.height = int(width), int(height)
+        self.L.dlo_scene_set_camera_opencv(self.h, _f(camera_to_world, 16), _f(eye, 3), _f(intrinsics, 4),
+                                           int(width), int(height))
+
+    def set_options(self, normals=False, hide_lights=False, eps=0.0):
+        """normals: the NormalIntegrator; hide_lights: area-light surfaces
+        transparent to path rays; eps: ray offset (0 = 1e-4)."""
+        self.L.dlo_scene_set_options(self.h, int(bool(normals)), int(bool(hide_lights)), float(eps))
 
     def camera(self):
         out = np.zeros(12, np.float32)
@@ -303,10 +317,20 @@ def from_desc(desc, build=True, camera_matrices=None):
         s.set_camera_perspective(np.array(cd.sample_to_camera[:], np.float32),
                                  np.array(cd.camera_to_world[:], np.float32), cd.width, cd.height, cd.near_clip,
                                  cd.far_clip)
+    elif cd is not None and cd.type == 2:
+        s.set_camera_opencv(np.array(cd.camera_to_world[:], np.float32), np.array(cd.eye[:], np.float32),
+                            np.array(cd.intrinsics[:], np.float32), cd.width, cd.height)
     elif cd is not None:
         s.L.dlo_scene_set_camera_basis(s.h, _f(list(cd.eye) + list(cd.U) + list(cd.V) + list(cd.W), 12),
                                        int(cd.width), int(cd.height))
         s.width, s.height = int(cd.width), int(cd.height)
+    elif getattr(c, "type", "virtual") == "opencv":
+        # camera_matrices: (inverse extrinsic, eye) as the product computed
+        # them; otherwise a float64 inverse rounded to float32
+        if camera_matrices is None:
+            inv = np.linalg.inv(np.asarray(c.extrinsic, np.float64).reshape(4, 4)).astype(np.float32)
+            camera_matrices = (inv.reshape(16), inv[:3, 3].copy())
+        s.set_camera_opencv(camera_matrices[0], camera_matrices[1], (c.cx, c.cy, c.fx, c.fy), c.width, c.height)
     elif getattr(c, "type", "virtual") == "perspective":
         if camera_matrices is None:
             camera_matrices = perspective_matrices(c.fov, c.near, c.far, c.width, c.height, c.to_world)

@@ -115,8 +115,9 @@ struct Camera {
     int width = 0, height = 0;
     V3 eye, U, V, W;
     V2 inv_size;
-    float S[16] = {0}, C[16] = {0};  // perspective: sample->camera, camera->world (row-major)
+    float S[16] = {0}, C[16] = {0};  // perspective: sample->camera, camera->world (row-major); opencv: C
     float near_clip = 1e-4f, far_clip = 1e4f;
+    float intr[4] = {0, 0, 1, 1};     // opencv: cx, cy, fx, fy
 };
 
 struct Filter {
@@ -141,6 +142,9 @@ struct Scene {
     Camera cam;
     Filter filter;
     V3 bg{0, 0, 0};
+    bool normals = false;      // NormalIntegrator instead of PathIntegrator
+    bool hide_lights = false;  // area-light surfaces transparent to path rays
+    float eps = kEpsilon;      // ray offset
     std::vector<OcNode> oct;
     int oct_root = -1;
     bool built = false;
@@ -1099,17 +1103,27 @@ static V3 area_eval(const Emitter& e, const Its& its, V3 wi) {  // area.cpp:46-4
     return c > 0.f ? e.radiance : mk(0, 0, 0);
 }
 
+// With hide_lights, a ray that hits an area light's surface continues from
+// the hit point in the same direction (src/optix/cuda/hitgroup_pgs.cu:55-64:
+// fr = 1, eta = 1, p = its.p, wo = ray direction), drawing no random numbers
+// and counting no bounce; the camera ray does not see the light.
+static bool hidden(const Scene& s, const Its& its) { return s.hide_lights && s.meshes[its.mesh].emitter >= 0; }
+
 static V3 Li(const Scene& s, const RenderOpts& o, Sampler& sampler, Ray ray, uint64_t* counters) {
     float eta = 1.f;
     V3 result = mk(0, 0, 0), thr = mk(1, 1, 1);
     Its its;
-    counters[0]++;
-    if (!intersect(s, o, ray, its)) {
-        if (s.env >= 0) {  // miss at depth 0: src/optix/cuda/miss.cu:23-29
-            const Emitter& e = s.emitters[s.env];
-            result = e.visual ? env_eval(e, ray.d) : s.bg;
+    for (;;) {
+        counters[0]++;
+        if (!intersect(s, o, ray, its)) {
+            if (s.env >= 0) {  // miss at depth 0: src/optix/cuda/miss.cu:23-29
+                const Emitter& e = s.emitters[s.env];
+                result = e.visual ? env_eval(e, ray.d) : s.bg;
+            }
+            return result;
         }
-        return result;
+        if (!hidden(s, its)) break;
+        ray = make_ray(its.p, ray.d, s.eps, INFINITY);
     }
     {
         const Mesh& m = s.meshes[its.mesh];
@@ -1123,7 +1137,7 @@ static V3 Li(const Scene& s, const RenderOpts& o, Sampler& sampler, Ray ray, uin
         V2 lsmp = sampler.next2d();
         V3 emitter_val = sample_emitter_direction(s, its, lsmp, ds, sampler, o.accel);
         if (csum(emitter_val) > 0) {  // visibility  scene.cpp:165-169
-            Ray sr = make_ray(its.p, ds.d, kEpsilon, ds.dist - kEpsilon);
+            Ray sr = make_ray(its.p, ds.d, s.eps, ds.dist - s.eps);
             counters[1]++;
             float occluded = trace_any(s, o.accel, sr) ? 1.0f : 0.0f;
             emitter_val = emitter_val * (1 - occluded);
@@ -1148,10 +1162,13 @@ static V3 Li(const Scene& s, const RenderOpts& o, Sampler& sampler, Ray ray, uin
         thr = cmul(thr, fr);
         eta *= br.eta;
         if (is_zero(thr)) break;  // every later contribution is multiplied by zero
-        ray = make_ray(its.p, to_world(its.sh, br.wo), kEpsilon, INFINITY);
+        ray = make_ray(its.p, to_world(its.sh, br.wo), s.eps, INFINITY);
         Its nx;
-        counters[0]++;
-        if (intersect(s, o, ray, nx)) {
+        bool hit;
+        for (;;) {
+            counters[0]++;
+            hit = intersect(s, o, ray, nx);
+            if (!hit) break;
             const Mesh& m = s.meshes[nx.mesh];
             if (m.emitter >= 0) {  // path.cpp:72-84
                 const Emitter& e = s.emitters[m.emitter];
@@ -1162,6 +1179,10 @@ static V3 Li(const Scene& s, const RenderOpts& o, Sampler& sampler, Ray ray, uin
                 float mis = bsdf_is_diffuse(bsdf) ? mis_weight(bpdf, epdf) : 1.f;
                 result = result + cmul(thr * mis, ev);
             }
+            if (!hidden(s, nx)) break;
+            ray = make_ray(nx.p, ray.d, s.eps, INFINITY);
+        }
+        if (hit) {
             its = nx;
         } else {
             if (s.env >= 0) {  // miss at depth > 0: miss.cu:31-42
@@ -1214,14 +1235,34 @@ static Ray perspective_ray(const Camera& c, V2 pos) {
                     c.far_clip * inv_z);
 }
 
+// OpencvCamera::sampleRay as the GPU path casts it (include/optix/camera/
+// opencv.h:28-46): (pos - principal point) / focal length at z = 1, w = 1,
+// mapped by the inverse extrinsic matrix; the ray leaves the eye toward it.
+static Ray opencv_ray(const Camera& c, V2 pos, float eps) {
+    const float in[4] = {(pos.x - c.intr[0]) / c.intr[2], (pos.y - c.intr[1]) / c.intr[3], 1.0f, 1.0f};
+    float q[4];
+    mat4_apply(c.C, in, q);
+    const V3 d = normalized(mk(q[0], q[1], q[2]) - c.eye);
+    return make_ray(c.eye, d, eps, INFINITY);
+}
+
 // VirtualCamera::sampleRay  src/tracer/camera/virtual.cpp:52-65
-static Ray camera_ray(const Camera& c, V2 pos) {
+static Ray camera_ray(const Camera& c, V2 pos, float eps) {
     if (c.type == DLO_CAMERA_PERSPECTIVE) return perspective_ray(c, pos);
+    if (c.type == DLO_CAMERA_OPENCV) return opencv_ray(c, pos, eps);
     float lx = pos.x, ly = ((float)c.height - pos.y) - 1;
     float px = lx * c.inv_size.x * 2 - 1;
     float py = ly * c.inv_size.y * 2 - 1;
     V3 d = normalized((c.U * px + c.V * py) + c.W);
-    return make_ray(c.eye, d, kEpsilon, INFINITY);
+    return make_ray(c.eye, d, eps, INFINITY);
+}
+
+// NormalIntegrator::Li  src/tracer/integrator/normal.cpp:9-19
+static V3 Li_normals(const Scene& s, const RenderOpts& o, const Ray& ray, uint64_t* counters) {
+    Its its;
+    counters[0]++;
+    if (!intersect(s, o, ray, its)) return mk(0, 0, 0);
+    return mk(std::fabs(its.sh.n.x), std::fabs(its.sh.n.y), std::fabs(its.sh.n.z));
 }
 
 static inline uint64_t stream_state(uint32_t pixel, uint32_t seed) {
@@ -1237,8 +1278,9 @@ static V3 render_sample(const Scene& s, const RenderOpts& o, int x, int y, uint3
     sampler.rng.seed(stream_state((uint32_t)y * (uint32_t)s.cam.width + (uint32_t)x, seed), sidx);
     V2 ps = sampler.next2d();
     pos = V2{(float)x + ps.x, (float)y + ps.y};
-    sampler.next2d();  // aperture sample, unused by the virtual camera
-    Ray ray = camera_ray(s.cam, pos);
+    sampler.next2d();  // aperture sample, unused by every camera
+    Ray ray = camera_ray(s.cam, pos, s.eps);
+    if (s.normals) return Li_normals(s, o, ray, counters);
     return Li(s, o, sampler, ray, counters);
 }
 
@@ -1612,6 +1654,28 @@ void dlo_scene_set_camera_perspective(void* h, const float* sample_to_camera, co
     c.far_clip = far_clip;
 }
 
+// OpencvCamera (opencv.cpp:9-24) with its inverse extrinsic, eye and
+// intrinsics given.
+void dlo_scene_set_camera_opencv(void* h, const float* camera_to_world, const float* eye, const float* intrinsics,
+                                 int width, int height) {
+    Scene& s = *(Scene*)h;
+    Camera& c = s.cam;
+    c.type = DLO_CAMERA_OPENCV;
+    c.width = width;
+    c.height = height;
+    c.inv_size = V2{1.f / width, 1.f / height};
+    for (int i = 0; i < 16; i++) c.C[i] = camera_to_world[i];
+    c.eye = mk(eye[0], eye[1], eye[2]);
+    for (int i = 0; i < 4; i++) c.intr[i] = intrinsics[i];
+}
+
+void dlo_scene_set_options(void* h, int normals, int hide_lights, float eps) {
+    Scene& s = *(Scene*)h;
+    s.normals = normals != 0;
+    s.hide_lights = hide_lights != 0;
+    s.eps = eps > 0.0f ? eps : kEpsilon;
+}
+
 void dlo_scene_get_camera(void* h, float* out12) {
     Scene& s = *(Scene*)h;
     V3 v[4] = {s.cam.eye, s.cam.U, s.cam.V, s.cam.W};
@@ -1726,7 +1790,7 @@ void dlo_trace_shadow(void* h, int mode, const float* rays, uint32_t n, uint8_t*
 void dlo_camera_rays(void* h, const float* pos, uint32_t n, float* rays) {
     const Scene& s = *(Scene*)h;
     for (uint32_t i = 0; i < n; i++) {
-        Ray r = camera_ray(s.cam, V2{pos[2 * i], pos[2 * i + 1]});
+        Ray r = camera_ray(s.cam, V2{pos[2 * i], pos[2 * i + 1]}, s.eps);
         float vals[8] = {r.o.x, r.o.y, r.o.z, r.d.x, r.d.y, r.d.z, r.mint, r.maxt};
         for (int k = 0; k < 8; k++) rays[(size_t)k * n + i] = vals[k];
     }

@@ -45,7 +45,7 @@ enum {
     DLO_BSDF_CONDUCTOR = 5,   // alpha, eta[3], k[3]
 };
 enum { DLO_EMITTER_AREA = 0, DLO_EMITTER_POINT = 1, DLO_EMITTER_ENV = 2 };
-enum { DLO_CAMERA_VIRTUAL = 0, DLO_CAMERA_PERSPECTIVE = 1 };
+enum { DLO_CAMERA_VIRTUAL = 0, DLO_CAMERA_PERSPECTIVE = 1, DLO_CAMERA_OPENCV = 2 };
 enum { DLO_FILTER_GAUSSIAN = 0, DLO_FILTER_BOX = 1, DLO_FILTER_TENT = 2, DLO_FILTER_MITCHELL = 3 };
 // BRUTE: every triangle, lowest index on equal t (the semantic definition).
 // OCTREE: the reference octree exactly (last equal hit, unpadded slabs).
@@ -74,6 +74,15 @@ void dlo_scene_get_camera(void* scene, float* out12);
 void dlo_scene_set_filter(void* scene, int type, float radius, float p0, float p1);
 void dlo_scene_get_filter_table(void* scene, float* out33);
 void dlo_scene_set_background(void* scene, const float* rgb);
+/* OpencvCamera: camera_to_world = inverse extrinsic (row-major), eye,
+   intrinsics (cx, cy, fx, fy). */
+void dlo_scene_set_camera_opencv(void* scene, const float* camera_to_world, const float* eye,
+                                 const float* intrinsics, int width, int height);
+/* Integrator and viewer options: normals != 0 renders |shading normal| of
+   the first hit (normal.cpp:9-19); hide_lights != 0 makes area-light
+   surfaces transparent to path rays (hitgroup_pgs.cu:55-64); eps is the ray
+   offset (the CPU path's Epsilon 1e-4 by default). */
+void dlo_scene_set_options(void* scene, int normals, int hide_lights, float eps);
 int dlo_scene_build(void* scene);
 uint32_t dlo_scene_num_prims(void* scene);
 

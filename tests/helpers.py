@@ -91,19 +91,28 @@ def relmse(x, y):
     return float(np.mean((x - y) ** 2 / (y * y + 1e-2)))
 
 
-def paths_parity(backend, desc, samples, seed, builder=0, camera_matrices=None):
+def paths_parity(backend, desc, samples, seed, builder=0, camera_matrices=None, normals=False, view=None):
     """Per-path radiance of the HIP integrator against the oracle for every
     pixel of the given sample indices: at least 99.9% of paths bit-exact, the
-    rest within 1e-3 relative (binary64 libm vs device rounding)."""
+    rest within 1e-3 relative (binary64 libm vs device rounding).
+    normals: the normals integrator; view: keyword arguments of
+    Backend.set_view_options (bg_color, epsilon, hide_lights), applied to
+    both sides."""
     import oracle
     backend.upload_scene(desc)
     backend.build_accel(builder)
     osc = oracle.from_desc(desc, camera_matrices=camera_matrices)
+    view = dict(view or {})
+    if view:
+        backend.set_view_options(**view)
+        if view.get("bg_color") is not None:
+            osc.set_background(view["bg_color"])
+    osc.set_options(normals=normals, hide_lights=view.get("hide_lights", False), eps=view.get("epsilon", 0.0))
     H, W = desc.camera.height, desc.camera.width
     tot, same_all = 0, 0
     for s in samples:
         print("paths sample", s, flush=True)
-        g = backend.render_paths(sample=s, seed=seed)
+        g = backend.render_paths(sample=s, seed=seed, normals=normals)
         xys = np.array([(x, y, s) for y in range(H) for x in range(W)], np.int32)
         o = osc.render_samples(xys, seed=seed, max_depth=desc.max_depth, rr_depth=desc.rr_depth)
         same = (g.view(np.uint32) == o.view(np.uint32)).all(axis=1)

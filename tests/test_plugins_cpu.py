@@ -270,7 +270,7 @@ def test_two_envmaps_rejected(tmp_path):
 def test_export_errors(tmp_path):
     with pytest.raises(host.XmlError, match="There is not emitter in the scene."):
         load(tmp_path, "")
-    tpl = MINIMAL.replace('<integrator type="path"/>', '<integrator type="normals"/>')
+    tpl = MINIMAL.replace('<integrator type="path"/>', '<integrator type="whitted"/>')
     with pytest.raises(host.XmlError, match="not supported by the hip backend"):
         load(tmp_path, LIGHT, template=tpl)
     with pytest.raises(host.XmlError, match="Unsupported texture format"):

@@ -1,0 +1,139 @@
+"""The OptiX backend's extra features on the HIP path, per path against the
+oracle: the "opencv" camera, the "normals" integrator, and the viewer
+settings (hidden lights, ray epsilon, background colour) changed between
+renders without re-uploading the scene."""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import drawlab_amd
+from drawlab_amd import host, scenes
+from drawlab_amd.scene import Camera, EnvMap
+from helpers import paths_parity
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _cbox_opencv(w=64, h=48):
+    d = scenes.cornell_box(w, h, spp=4, max_depth=6, rr_depth=3)
+    d.camera = scenes.opencv_camera((0.278, 0.273, -0.8), (0.278, 0.273, 0.0), (0, 1, 0), 39.3, w, h)
+    d.camera_desc = drawlab_amd.camera_desc(d.camera)
+    return d
+
+
+def test_opencv_camera_paths_bitexact(backend):
+    d = _cbox_opencv()
+    paths_parity(backend, d, samples=[0, 3], seed=21)
+    img, _ = backend.render(spp=8, seed=1)
+    assert np.isfinite(img).all() and img.mean() > 0.01
+
+
+def test_opencv_camera_reproduces_the_virtual_camera(backend):
+    """An opencv camera with the virtual camera's eye, target and vertical
+    field of view, its principal point one pixel above the centre (the
+    virtual camera maps film row y to H - y - 1), casts the same rays up to
+    rounding: the two renders agree to relMSE 1e-10."""
+    d = scenes.cornell_box(64, 64, spp=64, max_depth=6, rr_depth=3)
+    backend.upload_scene(d)
+    backend.build_accel(drawlab_amd.BUILD_GPU)
+    a, _ = backend.render(spp=64, seed=3)
+    d2 = scenes.cornell_box(64, 64, spp=64, max_depth=6, rr_depth=3)
+    d2.camera = scenes.opencv_camera((0.278, 0.273, -0.8), (0.278, 0.273, 0.0), (0, 1, 0), 39.3, 64, 64)
+    d2.camera.cy -= 1
+    backend.upload_scene(d2)
+    backend.build_accel(drawlab_amd.BUILD_GPU)
+    b, _ = backend.render(spp=64, seed=3)
+    err = float(np.mean((a.astype(np.float64) - b) ** 2 / (a.astype(np.float64) ** 2 + 1e-2)))
+    print("relMSE opencv vs virtual", err, flush=True)
+    assert err < 1e-10
+
+
+def test_normals_integrator_paths_bitexact(backend):
+    d = scenes.cornell_box(48, 48, spp=4, max_depth=6, rr_depth=3)
+    paths_parity(backend, d, samples=[0, 1], seed=2, normals=True)
+    img, st = backend.render(spp=4, seed=2, normals=True)
+    assert st["shadow_rays"] == 0 and st["closest_rays"] == st["paths"]
+    assert img.max() <= 1.0 + 1e-6 and img.mean() > 0.2
+
+
+def test_normals_on_mesh_with_vertex_normals(backend):
+    d = scenes.displaced_sphere(width=64, height=36, grid=32, glass_level=1, env_size=(32, 16), max_depth=4)
+    paths_parity(backend, d, samples=[0], seed=8, normals=True)
+
+
+def test_hidden_lights_paths_bitexact(backend):
+    """The Cornell box light hidden: camera rays pass through it, later rays
+    collect its emission and pass through; with epsilon 1e-3 too."""
+    d = scenes.cornell_box(48, 48, spp=4, max_depth=6, rr_depth=3)
+    # look up at the light so camera rays cross it
+    d.camera = Camera(eye=(0.278, 0.05, 0.28), lookat=(0.278, 0.6, 0.28), up=(0, 0, 1), fov=60, width=48,
+                      height=48)
+    paths_parity(backend, d, samples=[0, 1], seed=5, view={"hide_lights": True})
+    paths_parity(backend, d, samples=[0], seed=6, view={"hide_lights": True, "epsilon": 1e-3})
+    paths_parity(backend, d, samples=[0], seed=6, view={"epsilon": 1e-3})
+
+
+def test_view_options_change_between_renders(backend):
+    """Options apply to the next render without re-uploading; reset restores
+    the scene's own render bit for bit; a new upload clears them."""
+    d = scenes.cornell_box(32, 32, spp=4, max_depth=5, rr_depth=3)
+    d.camera = Camera(eye=(0.278, 0.05, 0.28), lookat=(0.278, 0.6, 0.28), up=(0, 0, 1), fov=60, width=32,
+                      height=32)
+    backend.upload_scene(d)
+    backend.build_accel(drawlab_amd.BUILD_GPU)
+    a, _ = backend.render(spp=8, seed=1)
+    backend.set_view_options(hide_lights=True)
+    b, _ = backend.render(spp=8, seed=1)
+    assert a.max() > 10 and b.max() < a.max()          # the light no longer shows
+    backend.reset_view_options()
+    c, _ = backend.render(spp=8, seed=1)
+    assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
+    backend.set_view_options(hide_lights=True)
+    backend.upload_scene(d)
+    backend.build_accel(drawlab_amd.BUILD_GPU)
+    e, _ = backend.render(spp=8, seed=1)
+    assert np.array_equal(a.view(np.uint32), e.view(np.uint32))
+    with pytest.raises(drawlab_amd.DlError, match="epsilon"):
+        backend.set_view_options(epsilon=-1.0)
+
+
+def test_background_colour_override(backend):
+    """Camera rays that miss take the background colour when the environment
+    map is not visible (miss.cu:23-29); the override replaces the scene's."""
+    d = scenes.cornell_box(32, 32, spp=4, max_depth=4, rr_depth=3)
+    d.camera = Camera(eye=(0.278, 0.273, -0.8), lookat=(0.278, 0.273, -2.0), up=(0, 1, 0), fov=40,
+                      width=32, height=32)           # looking away from the box
+    d.envmap = EnvMap(np.full((8, 16, 3), 0.5, np.float32), 1.0, False)
+    d.bg_color = (0.1, 0.2, 0.3)
+    paths_parity(backend, d, samples=[0], seed=3)
+    paths_parity(backend, d, samples=[0], seed=3, view={"bg_color": (0.9, 0.5, 0.25)})
+    img, _ = backend.render(spp=2, seed=3)
+    assert np.allclose(img, (0.9, 0.5, 0.25), atol=1e-6)
+
+
+def test_xml_normals_and_opencv_render(tmp_path):
+    text = open(os.path.join(HERE, "golden", "scenes", "cbox.xml")).read()
+    a, b = text.index("<camera"), text.index("</camera>") + len("</camera>")
+    cam = """<camera type="opencv">
+      <float name="cx" value="32"/><float name="cy" value="32"/><float name="fx" value="90"/><float name="fy" value="90"/>
+      <integer name="width" value="64"/><integer name="height" value="64"/>
+      <transform name="extrinsic"><matrix value="-1 0 0 0.278  0 -1 0 0.273  0 0 1 0.8  0 0 0 1"/></transform>
+    </camera>"""
+    text = text[:a] + cam + text[b:]
+    text = text.replace('<integrator type="path">', '<integrator type="normals">')
+    shutil.copy(os.path.join(HERE, "golden", "scenes", "unit_cube.obj"), tmp_path / "unit_cube.obj")
+    p = tmp_path / "n.xml"
+    p.write_text(text)
+    xs = host.load_xml(str(p))
+    print("scene", xs.path, flush=True)
+    img, st = xs.render(spp=4)
+    assert st["shadow_rays"] == 0
+    import oracle
+    osc = oracle.from_desc(xs.desc)
+    osc.set_options(normals=True)
+    oimg, _ = osc.render(4, seed=xs.params.seed)
+    assert np.array_equal(img.view(np.uint32), oimg.view(np.uint32)) or np.allclose(img, oimg, atol=1e-6)
