@@ -181,12 +181,25 @@ struct Image {
     int width = 0, height = 0;
     std::vector<float> rgb;  // row-major, row 0 = top
 };
+// Wavefront OBJ corners de-indexed per distinct (v, t, n), before any
+// transform: positions (3 per corner), normals and texcoords (empty unless
+// every corner has one), triangles (3 corner indices each).
+struct ObjData {
+    std::vector<float> P, N, UV;
+    std::vector<uint32_t> F;
+};
+// Reads an OBJ file (through the binary cache in $DRAWLAB_MESH_CACHE when
+// that names a directory; *from_cache tells whether the cache answered).
+ObjData loadOBJ(const std::string& filename, bool* from_cache = nullptr);
+
 // Radiance .hdr (RGBE, run-length encoded or flat), .pfm, uncompressed .exr.
 Image loadImage(const std::string& filename);
 // Texture rows as the reference uploads them (row 0 at v = 0, 8-bit
 // samples / 255): .png / .exr / .pfm rows bottom up, .hdr top down.
 Image loadTextureImage(const std::string& filename);
 void savePFM(const std::string& filename, const Image& img);
+// 8-bit sRGB PNG, as Bitmap::savePNG writes it.
+void savePNG(const std::string& filename, const Image& img);
 void saveEXR(const std::string& filename, const Image& img);
 
 }  // namespace drawlab

@@ -61,7 +61,7 @@ void render_scene(const ExportedScene& ex, int device, int builder, int spp, flo
 }  // namespace
 
 // CPURenderer::renderAsync's contract (cpu_renderer.cpp:57-135): render the
-// scene, then write <scene file without extension>.exr (and .pfm).
+// scene, then write <scene file without extension>.exr and .png (and .pfm).
 void renderToFile(const Scene& scene, const std::string& scene_file, int device, int builder, int spp) {
     ExportedScene ex;
     exportScene(scene, ex);
@@ -76,6 +76,7 @@ void renderToFile(const Scene& scene, const std::string& scene_file, int device,
     const size_t dot = out.find_last_of('.');
     if (dot != std::string::npos && out.find('/', dot) == std::string::npos) out.erase(dot);
     saveEXR(out + ".exr", img);
+    savePNG(out + ".png", img);
     savePFM(out + ".pfm", img);
     std::printf("Render done: %dx%d, %.0f paths in %.1f ms (%.1f Mpaths/s); wrote %s.exr\n", W, H, st.paths,
                 st.ms_total, st.paths / (st.ms_total * 1e3), out.c_str());
@@ -137,8 +138,10 @@ int dlx_save_image(const char* path, const float* rgb, int width, int height) {
             savePFM(path, img);
         else if (endsWith(toLower(path), ".exr"))
             saveEXR(path, img);
+        else if (endsWith(toLower(path), ".png"))
+            savePNG(path, img);
         else
-            throw Exception("Unsupported output format \"%s\" (expected .exr or .pfm)", path);
+            throw Exception("Unsupported output format \"%s\" (expected .exr, .png or .pfm)", path);
     });
 }
 

@@ -8,7 +8,8 @@
 //   PNG (textures): non-interlaced, 8- or 16-bit grey, grey + alpha, RGB,
 //     RGBA or 8-bit palette; zlib stream inflated by the system zlib, rows
 //     unfiltered (None, Sub, Up, Average, Paeth), 16-bit samples reduced to
-//     their high byte (as stb_image returns them).
+//     their high byte (as stb_image returns them).  Written (renders): 8-bit
+//     sRGB RGB, unfiltered rows, deflated by zlib.
 // The reference reads images through stb_image / OpenEXR
 // (src/core/bitmap/bitmap.cpp:33-214), neither of which is vendored.
 #include <zlib.h>
@@ -383,6 +384,55 @@ Image loadImage(const std::string& filename) {
     if (endsWith(l, ".pfm")) return load_pfm(filename);
     if (endsWith(l, ".hdr") || endsWith(l, ".pic")) return load_hdr(filename);
     throw Exception("Unsupported image format \"%s\" (expected .hdr, .pfm or .exr)", filename);
+}
+
+// Bitmap::savePNG (src/core/bitmap/bitmap.cpp:190-214): each channel
+// through Color3f::toSRGB (include/core/bitmap/color.h:76-86), times 255,
+// clamped to [0, 255] and truncated to 8 bits; RGB rows top first.  Rows
+// are stored unfiltered in one zlib stream (stb picks its own filters; the
+// pixels are the same).
+void savePNG(const std::string& filename, const Image& img) {
+    const size_t stride = (size_t)img.width * 3 + 1;
+    std::vector<unsigned char> raw(stride * (size_t)img.height);
+    for (int y = 0; y < img.height; y++) {
+        unsigned char* row = &raw[(size_t)y * stride];
+        row[0] = 0;
+        for (int i = 0; i < img.width * 3; i++) {
+            const float v = img.rgb[(size_t)y * img.width * 3 + i];
+            const float srgb = v <= 0.0031308f ? 12.92f * v : (1.0f + 0.055f) * std::pow(v, 1.0f / 2.4f) - 0.055f;
+            const float c = std::min(std::max(255.f * srgb, 0.f), 255.f);
+            row[1 + i] = (unsigned char)c;
+        }
+    }
+    uLongf zlen = compressBound((uLong)raw.size());
+    std::vector<unsigned char> z(zlen);
+    if (compress2(z.data(), &zlen, raw.data(), (uLong)raw.size(), 6) != Z_OK)
+        throw Exception("Could not compress \"%s\"", filename);
+    z.resize(zlen);
+    std::ofstream out(filename, std::ios::binary);
+    if (!out) throw Exception("Could not write \"%s\"", filename);
+    auto chunk = [&](const char* type, const unsigned char* data, size_t n) {
+        const unsigned char len[4] = {(unsigned char)(n >> 24), (unsigned char)(n >> 16), (unsigned char)(n >> 8),
+                                      (unsigned char)n};
+        out.write(reinterpret_cast<const char*>(len), 4);
+        out.write(type, 4);
+        if (n) out.write(reinterpret_cast<const char*>(data), (std::streamsize)n);
+        uLong crc = crc32(0L, reinterpret_cast<const Bytef*>(type), 4);
+        if (n) crc = crc32(crc, data, (uInt)n);
+        const unsigned char c[4] = {(unsigned char)(crc >> 24), (unsigned char)(crc >> 16), (unsigned char)(crc >> 8),
+                                    (unsigned char)crc};
+        out.write(reinterpret_cast<const char*>(c), 4);
+    };
+    static const unsigned char sig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
+    out.write(reinterpret_cast<const char*>(sig), 8);
+    const uint32_t w = (uint32_t)img.width, h = (uint32_t)img.height;
+    const unsigned char ihdr[13] = {(unsigned char)(w >> 24), (unsigned char)(w >> 16), (unsigned char)(w >> 8),
+                                    (unsigned char)w, (unsigned char)(h >> 24), (unsigned char)(h >> 16),
+                                    (unsigned char)(h >> 8), (unsigned char)h, 8, 2, 0, 0, 0};
+    chunk("IHDR", ihdr, 13);
+    chunk("IDAT", z.data(), z.size());
+    chunk("IEND", nullptr, 0);
+    if (!out) throw Exception("Could not write \"%s\"", filename);
 }
 
 void savePFM(const std::string& filename, const Image& img) {

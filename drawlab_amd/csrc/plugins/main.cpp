@@ -2,7 +2,7 @@
 // backend.  Usage mirrors the reference:
 //   drawlab -s scene.xml [-b hip] [-t threads] [--device N] [--builder gpu|host_sah|lbvh] [--spp N]
 // The scene's directory is searched for relative resource paths; the image
-// is written beside the scene as <name>.exr and <name>.pfm.
+// is written beside the scene as <name>.exr, <name>.png and <name>.pfm.
 #include <cstdio>
 #include <cstring>
 #include <memory>

@@ -49,95 +49,33 @@ std::string Mesh::toString() const {
 
 // Wavefront OBJ (src/tracer/mesh/obj.cpp:17-87): positions, texcoords and
 // normals de-indexed per distinct (v, vt, vn) corner in order of first use,
-// polygons fanned into triangles, positions mapped by toWorld as points and
-// normals by its inverse transpose.
+// polygons fanned into triangles (obj_io.cpp), positions mapped by toWorld
+// as points and normals by its inverse transpose.
 class WavefrontOBJ : public Mesh {
 public:
     explicit WavefrontOBJ(const PropertyList& props) {
         const std::string filename = resolvePath(props.getString("filename"));
         const Transform trafo = props.getTransform("toWorld", Transform());
-        std::ifstream in(filename);
-        if (!in) throw Exception("Loadobj fail: could not open \"%s\"", filename);
-        std::vector<Vector3f> P, N;
-        std::vector<std::pair<float, float>> T;
-        struct Key {
-            int v, t, n;
-            bool operator==(const Key& o) const { return v == o.v && t == o.t && n == o.n; }
-        };
-        struct KeyHash {
-            size_t operator()(const Key& k) const {
-                return ((size_t)k.v * 73856093u) ^ ((size_t)(k.t + 1) * 19349663u) ^ ((size_t)(k.n + 1) * 83492791u);
-            }
-        };
-        std::unordered_map<Key, uint32_t, KeyHash> corner;
-        std::vector<Key> corners;
-        std::string line;
-        int lineno = 0;
-        auto fix = [&](int idx, size_t count) -> int {
-            if (idx > 0) return idx - 1;
-            if (idx < 0) return (int)count + idx;
-            throw Exception("Loadobj fail: index 0 at %s:%i", filename, lineno);
-        };
-        while (std::getline(in, line)) {
-            ++lineno;
-            std::istringstream ls(line);
-            std::string tag;
-            if (!(ls >> tag)) continue;
-            if (tag == "v") {
-                float x, y, z;
-                ls >> x >> y >> z;
-                P.emplace_back(x, y, z);
-            } else if (tag == "vn") {
-                float x, y, z;
-                ls >> x >> y >> z;
-                N.emplace_back(x, y, z);
-            } else if (tag == "vt") {
-                float u = 0, v = 0;
-                ls >> u >> v;
-                T.emplace_back(u, v);
-            } else if (tag == "f") {
-                std::vector<uint32_t> poly;
-                std::string tok;
-                while (ls >> tok) {
-                    Key k{-1, -1, -1};
-                    const size_t s1 = tok.find('/');
-                    k.v = fix(std::stoi(tok.substr(0, s1)), P.size());
-                    if (s1 != std::string::npos) {
-                        const size_t s2 = tok.find('/', s1 + 1);
-                        const std::string ts = tok.substr(s1 + 1, s2 == std::string::npos ? std::string::npos : s2 - s1 - 1);
-                        if (!ts.empty()) k.t = fix(std::stoi(ts), T.size());
-                        if (s2 != std::string::npos && s2 + 1 < tok.size()) k.n = fix(std::stoi(tok.substr(s2 + 1)), N.size());
-                    }
-                    if (k.v < 0 || k.v >= (int)P.size() || k.t >= (int)T.size() || k.n >= (int)N.size())
-                        throw Exception("Loadobj fail: index out of range at %s:%i", filename, lineno);
-                    auto it = corner.find(k);
-                    if (it == corner.end()) {
-                        it = corner.emplace(k, (uint32_t)corners.size()).first;
-                        corners.push_back(k);
-                    }
-                    poly.push_back(it->second);
-                }
-                for (size_t i = 1; i + 1 < poly.size(); ++i) {
-                    m_F.push_back(poly[0]);
-                    m_F.push_back(poly[i]);
-                    m_F.push_back(poly[i + 1]);
-                }
+        ObjData d = loadOBJ(filename);
+        const size_t nc = d.P.size() / 3;
+        m_V.resize(3 * nc);
+        for (size_t i = 0; i < nc; i++) {
+            const Point3f p = trafo.point(Point3f(d.P[3 * i], d.P[3 * i + 1], d.P[3 * i + 2]));
+            m_V[3 * i] = p[0];
+            m_V[3 * i + 1] = p[1];
+            m_V[3 * i + 2] = p[2];
+        }
+        if (!d.N.empty()) {
+            m_N.resize(3 * nc);
+            for (size_t i = 0; i < nc; i++) {
+                const Normal3f n = trafo.normal(Normal3f(d.N[3 * i], d.N[3 * i + 1], d.N[3 * i + 2]));
+                m_N[3 * i] = n[0];
+                m_N[3 * i + 1] = n[1];
+                m_N[3 * i + 2] = n[2];
             }
         }
-        bool has_n = !N.empty(), has_t = !T.empty();
-        for (const Key& k : corners) {
-            if (k.n < 0) has_n = false;
-            if (k.t < 0) has_t = false;
-        }
-        for (const Key& k : corners) {
-            const Point3f p = trafo.point(P[k.v]);
-            m_V.insert(m_V.end(), {p[0], p[1], p[2]});
-            if (has_n) {
-                const Normal3f n = trafo.normal(N[k.n]);
-                m_N.insert(m_N.end(), {n[0], n[1], n[2]});
-            }
-            if (has_t) m_UV.insert(m_UV.end(), {T[k.t].first, T[k.t].second});
-        }
+        m_UV = std::move(d.UV);
+        m_F = std::move(d.F);
         m_name = filename;
     }
 };

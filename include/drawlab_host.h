@@ -36,7 +36,8 @@ int dlx_describe(void* scene, char* buf, size_t n);
    sampleCount.  Replaces CPURenderer::renderAsync's render loop. */
 int dlx_render(void* scene, int device, int builder, int spp, float* rgb_out, dl_render_stats* stats);
 
-/* Images: .exr (uncompressed) / .pfm written, .exr / .pfm / .hdr read.
+/* Images: .exr (uncompressed) / .png (8-bit sRGB, Bitmap::savePNG) / .pfm
+   written, .exr / .pfm / .hdr read.
    dlx_load_image: call with rgb = NULL to get the size, then with a buffer
    of w*h*3 floats.  Replaces Bitmap::saveEXR / Bitmap(filename)
    (src/core/bitmap/bitmap.cpp:33-214). */

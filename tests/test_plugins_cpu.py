@@ -356,3 +356,139 @@ def test_cli_usage_and_errors(tmp_path):
     bad = write(tmp_path, "<foo/>")
     r = run_cli("-s", bad)
     assert r.returncode != 0 and 'unexpected tag "foo"' in r.stderr
+
+
+def test_png_output_is_srgb_8bit(tmp_path):
+    """Bitmap::savePNG (src/core/bitmap/bitmap.cpp:190-214): toSRGB
+    (color.h:76-86), x255, clamped, truncated; decoded by an independent
+    reader.  powf may differ from numpy's float32 power in the last bit, so
+    a value may land one step off where 255 * srgb is within rounding of an
+    integer."""
+    Image = pytest.importorskip("PIL.Image")
+    rng = np.random.default_rng(4)
+    img = (rng.random((9, 13, 3)) * 1.3).astype(np.float32)
+    img[0, 0] = (-1.0, 0.001, 0.0031308)
+    img[0, 1] = (np.inf, 0.0, 5.0)
+    host.save_image(tmp_path / "a.png", img)
+    got = np.asarray(Image.open(tmp_path / "a.png").convert("RGB")).astype(np.int64)
+    v = img
+    with np.errstate(invalid="ignore", over="ignore"):
+        srgb = np.where(v <= np.float32(0.0031308), np.float32(12.92) * v,
+                        np.float32(1.055) * np.power(v, np.float32(1.0 / 2.4), dtype=np.float32) - np.float32(0.055))
+        want = np.clip(np.float32(255.0) * srgb.astype(np.float32), 0, 255).astype(np.int64)
+    assert got.shape == (9, 13, 3)
+    diff = np.abs(got - want)
+    assert diff.max() <= 1 and (diff == 0).mean() > 0.99
+    assert got[0, 0].tolist() == [0, 3, 10] and got[0, 1].tolist() == [255, 0, 255]
+
+
+# ---------------------------------------------------------------------------
+# OBJ parsing: rounding of decimals and the binary cache
+# ---------------------------------------------------------------------------
+def _nearest_float32(text):
+    """Correctly rounded (ties to even) binary32 of a decimal string, by
+    exact rational comparison."""
+    from fractions import Fraction
+    x = Fraction(text)
+    f = np.float32(float(x))          # within one step of the answer
+    cands = {f, np.nextafter(f, np.float32(np.inf)), np.nextafter(f, np.float32(-np.inf))}
+    best = None
+    for c in cands:
+        if not np.isfinite(c):
+            continue
+        d = abs(Fraction(float(c)) - x)
+        key = (d, int(np.float32(c).view(np.uint32)) & 1)
+        if best is None or key < best[0]:
+            best = (key, c)
+    return best[1]
+
+
+def test_obj_float_parsing_is_correctly_rounded(tmp_path):
+    """Coordinates are the nearest binary32 of their decimal text (ties to
+    even): short and long decimals, exponents across the float range,
+    exact midpoints between neighbouring floats, and decimals a hair off a
+    midpoint (where rounding through binary64 first goes wrong)."""
+    from fractions import Fraction
+    rng = np.random.default_rng(12)
+    vals = []
+    for _ in range(2000):
+        kind = rng.integers(0, 6)
+        if kind == 0:
+            vals.append("%.*g" % (int(rng.integers(1, 12)), rng.normal() * 10.0 ** rng.integers(-8, 9)))
+        elif kind == 1:
+            vals.append("%.*f" % (int(rng.integers(0, 10)), rng.normal() * 100))
+        elif kind == 2:
+            vals.append("%.*e" % (int(rng.integers(0, 20)), rng.normal() * 10.0 ** rng.integers(-40, 38)))
+        elif kind == 3:
+            digits = "".join(str(d) for d in rng.integers(0, 10, size=int(rng.integers(1, 25))))
+            vals.append(("-" if rng.integers(0, 2) else "+") + digits[:3] + "." + digits[3:])
+        else:
+            f = np.float32(rng.normal() * 10.0 ** rng.integers(-5, 6))
+            g = np.nextafter(f, np.float32(np.inf))
+            mid = (Fraction(float(f)) + Fraction(float(g))) / 2
+            if kind == 5:   # off the midpoint by far less than a binary64 step
+                mid += Fraction(int(rng.choice([-1, 1])), 10 ** 40) * abs(mid)
+                mid = Fraction(round(mid * 10 ** 60), 10 ** 60)
+            vals.append(_frac_text(mid))
+    while len(vals) % 3:
+        vals.append("0.5")
+    lines = ["v %s %s %s" % tuple(vals[i:i + 3]) for i in range(0, len(vals), 3)]
+    lines.append("f " + " ".join(str(i + 1) for i in range(len(vals) // 3)))     # one fan over every position
+    (tmp_path / "f.obj").write_text("\n".join(lines) + "\n")
+    body = '<mesh type="obj"><string name="filename" value="%s"/></mesh>' % (tmp_path / "f.obj")
+    got = load(tmp_path, body + LIGHT).desc.meshes[0].V.reshape(-1)
+    want = np.array([_nearest_float32(v) for v in vals], np.float32)
+    bad = np.nonzero(got.view(np.uint32) != want.view(np.uint32))[0]
+    assert bad.size == 0, [(vals[i], got[i], want[i]) for i in bad[:5]]
+
+
+def _frac_text(fr):
+    """Exact decimal expansion of a dyadic fraction."""
+    from fractions import Fraction
+    neg = fr < 0
+    fr = abs(fr)
+    k = 0
+    while (fr * 10 ** k).denominator != 1:
+        k += 1
+    digits = str((fr * 10 ** k).numerator).rjust(k + 1, "0")
+    s = digits[:-k] + "." + digits[-k:] if k else digits
+    return ("-" if neg else "") + s
+
+
+def test_obj_mesh_cache(tmp_path, monkeypatch):
+    """With DRAWLAB_MESH_CACHE set, the parsed OBJ is stored once and read
+    back while the file's path, size and modification time match."""
+    cache = tmp_path / "cache"
+    cache.mkdir()
+    monkeypatch.setenv("DRAWLAB_MESH_CACHE", str(cache))
+    obj = tmp_path / "m.obj"
+    obj.write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nv 1 1 0\nvn 0 0 1\nvt 0 0\nvt 1 0\nvt 0 1\nvt 1 1\n"
+                   "f 1/1/1 2/2/1 4/4/1 3/3/1\n")
+    body = ('<mesh type="obj"><string name="filename" value="%s"/><transform name="toWorld">'
+            '<translate value="0, 0, 1"/></transform></mesh>' % obj)
+    first = load(tmp_path, body + LIGHT).desc.meshes[0]
+    files = list(cache.iterdir())
+    assert len(files) == 1 and files[0].suffix == ".dlmesh"
+    # answered from the cache: change a cached position, the load shows it
+    raw = bytearray(files[0].read_bytes())
+    header = 8 + 8 + 8 + 4 + len(str(obj.resolve()).encode()) + 8 + 8 + 4 + 4
+    struct.pack_into("<f", raw, header, 7.5)
+    files[0].write_bytes(bytes(raw))
+    again = load(tmp_path, body + LIGHT).desc.meshes[0]
+    assert again.V[0].tolist() == [7.5, 0.0, 1.0]                      # transform applied after the cache
+    assert np.array_equal(again.F, first.F) and np.array_equal(again.UV, first.UV)
+    # a changed source file is parsed again (and re-cached)
+    obj.write_text(obj.read_text() + "# touched\n")
+    os.utime(obj, ns=(1, 10 ** 18))
+    fresh = load(tmp_path, body + LIGHT).desc.meshes[0]
+    assert np.array_equal(fresh.V, first.V)
+    # an unreadable cache file is ignored
+    for f in cache.iterdir():
+        f.write_bytes(b"junk")
+    assert np.array_equal(load(tmp_path, body + LIGHT).desc.meshes[0].V, first.V)
+    # without the variable nothing is cached
+    monkeypatch.delenv("DRAWLAB_MESH_CACHE")
+    for f in cache.iterdir():
+        f.unlink()
+    load(tmp_path, body + LIGHT)
+    assert not list(cache.iterdir())

@@ -1,7 +1,7 @@
 """Scenes loaded from XML by the C++ plugin layer, rendered on the GPU:
 per-path parity with the oracle (virtual and perspective cameras), the
 HipRenderer path (dlx_render) against the Python binding, and the drawlab
-command line writing its OpenEXR image."""
+command line writing its OpenEXR and PNG images."""
 import os
 import shutil
 import subprocess
@@ -73,3 +73,6 @@ def test_cli_writes_exr(tmp_path):
     assert out.shape == (64, 64, 3)
     assert np.array_equal(out, img) and np.array_equal(pfm, img)
     assert np.isfinite(out).all() and out.mean() > 0.01
+    # and the 8-bit sRGB PNG the reference writes beside the EXR
+    host.save_image(tmp_path / "again.png", img)
+    assert (tmp_path / "cbox.png").read_bytes() == (tmp_path / "again.png").read_bytes()
