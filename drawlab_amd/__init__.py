@@ -7,6 +7,7 @@ callers.  There is no CPU fallback: without the built library (or without a
 GPU) every entry point raises.
 """
 import ctypes
+import dataclasses
 import os
 
 import numpy as np
@@ -28,7 +29,7 @@ EXPORTED_SYMBOLS = (
     "dl_hip_build_accel", "dl_hip_trace_closest", "dl_hip_trace_shadow", "dl_hip_trace_closest_counted",
     "dl_hip_render", "dl_hip_render_paths", "dl_hip_malloc", "dl_hip_free", "dl_hip_memcpy_h2d",
     "dl_hip_memcpy_d2h", "dl_hip_memcpy_d2d", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_camera_perspective",
-    "dl_camera_opencv", "dl_film_filter", "dl_hip_set_view_options",
+    "dl_camera_opencv", "dl_film_filter", "dl_hip_set_view_options", "dl_hip_render_batch",
     "dl_mesh_tangents", "dl_hip_memory_usage", "dl_hip_trim", "dl_hip_set_camera",
     "dl_hip_scene_keep", "dl_hip_scene_select", "dl_hip_scene_drop", "dl_hip_accel_export",
     "dl_hip_service_start", "dl_hip_service_add_scene", "dl_hip_service_submit", "dl_hip_service_poll",
@@ -69,6 +70,15 @@ class dl_camera_desc(ctypes.Structure):
                 ("W", ctypes.c_float * 3), ("sample_to_camera", ctypes.c_float * 16),
                 ("camera_to_world", ctypes.c_float * 16), ("inv_size", ctypes.c_float * 2),
                 ("near_clip", ctypes.c_float), ("far_clip", ctypes.c_float), ("intrinsics", ctypes.c_float * 4)]
+
+
+class dl_batch_item(ctypes.Structure):
+    _fields_ = [("camera", dl_camera_desc), ("spp", ctypes.c_int32), ("first_sample", ctypes.c_int32),
+                ("seed", ctypes.c_uint32), ("film", ctypes.c_int32), ("accumulate", ctypes.c_int32),
+                ("rgb_out", ctypes.c_void_p)]
+
+
+BATCH_MAX = 32
 
 
 class dl_view_options(ctypes.Structure):
@@ -145,6 +155,9 @@ def lib():
             "dl_camera_opencv": (i32, [P, c.c_float, c.c_float, c.c_float, c.c_float, i32, i32,
                                        c.POINTER(dl_camera_desc)]),
             "dl_hip_set_view_options": (i32, [P, c.POINTER(dl_view_options)]),
+            "dl_hip_set_camera": (i32, [P, c.POINTER(dl_camera_desc)]),
+            "dl_hip_render_batch": (i32, [P, c.POINTER(dl_render_params), c.POINTER(dl_batch_item), u32,
+                                          c.POINTER(dl_render_stats)]),
             "dl_film_filter": (None, [i32, c.c_float, c.c_float, c.c_float, c.POINTER(dl_film_desc)]),
             "dl_mesh_tangents": (None, [P, u32, P, u32, P, P]),
             "dl_hip_guide_init": (i32, [P, c.c_uint64, c.c_float, P]),
@@ -469,6 +482,40 @@ class Backend:
         self._check(self.L.dl_hip_render(self.h, ctypes.byref(p), _p(img), ctypes.byref(st)))
         return img, st.as_dict()
 
+    def set_camera(self, cam):
+        """Replaces the uploaded scene's camera and output size
+        (dl_hip_set_camera); cam: Camera or dl_camera_desc."""
+        cd = cam if isinstance(cam, dl_camera_desc) else camera_desc(cam)
+        self._check(self.L.dl_hip_set_camera(self.h, ctypes.byref(cd)))
+        if isinstance(cam, dl_camera_desc):
+            cam = dataclasses.replace(self.desc.camera, width=int(cd.width), height=int(cd.height))
+        self.desc = dataclasses.replace(self.desc, camera=cam)
+
+    def render_batch(self, items, max_depth=None, rr_depth=None, normals=False):
+        """One pass shared by several images of the uploaded scene
+        (dl_hip_render_batch).  items: dicts with camera (Camera or
+        dl_camera_desc), spp, first_sample (0), seed (0), film (slot),
+        accumulate (False), image (True: return the resolved image).
+        Returns (images, stats); images[i] is None when not asked for."""
+        d = self.desc
+        shared = self._params(1, 0, d.max_depth if max_depth is None else max_depth,
+                              d.rr_depth if rr_depth is None else rr_depth, flags=RENDER_NORMALS if normals else 0)
+        arr = (dl_batch_item * len(items))()
+        imgs = []
+        for i, it in enumerate(items):
+            cam = it["camera"]
+            cd = cam if isinstance(cam, dl_camera_desc) else camera_desc(cam)
+            a = arr[i]
+            a.camera = cd
+            a.spp, a.first_sample, a.seed = int(it["spp"]), int(it.get("first_sample", 0)), int(it.get("seed", 0))
+            a.film, a.accumulate = int(it["film"]), int(bool(it.get("accumulate", False)))
+            img = np.zeros((cd.height, cd.width, 3), np.float32) if it.get("image", True) else None
+            a.rgb_out = None if img is None else img.ctypes.data
+            imgs.append(img)
+        st = dl_render_stats()
+        self._check(self.L.dl_hip_render_batch(self.h, ctypes.byref(shared), arr, len(items), ctypes.byref(st)))
+        return imgs, st.as_dict()
+
     def set_view_options(self, bg_color=None, epsilon=0.0, hide_lights=False):
         """Viewer settings of the current scene (dl_hip_set_view_options):
         background colour (None: the scene's), ray offset (0: 1e-4) and
@@ -560,7 +607,8 @@ class dl_service_stats(ctypes.Structure):
     _fields_ = [("jobs_done", ctypes.c_uint64), ("jobs_failed", ctypes.c_uint64),
                 ("jobs_cancelled", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
                 ("peak_reserved_bytes", ctypes.c_uint64), ("estimate_misses", ctypes.c_uint64),
-                ("scene_uploads", ctypes.c_uint64), ("scene_selects", ctypes.c_uint64), ("paths", ctypes.c_double)]
+                ("scene_uploads", ctypes.c_uint64), ("scene_selects", ctypes.c_uint64), ("paths", ctypes.c_double),
+                ("batched_jobs", ctypes.c_uint64), ("shared_passes", ctypes.c_uint64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

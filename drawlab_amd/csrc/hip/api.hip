@@ -176,6 +176,7 @@ dl_status dl_hip_destroy(void* h) {
     if (!c) return DL_ERR_INVALID;
     hipSetDevice(c->device);
     free_render_state(c);
+    free_film_slots(c);
     free_guide(c);
     free_all_scenes(c);
     if (c->stack_ovf) dfree(c, c->stack_ovf);
@@ -440,6 +441,24 @@ dl_status dl_hip_render(void* h, const dl_render_params* p, float* rgb_host, dl_
     return render(c, p, rgb_host, stats, -1, nullptr);
 }
 
+dl_status dl_hip_render_batch(void* h, const dl_render_params* shared, const dl_batch_item* items, uint32_t n,
+                              dl_render_stats* stats) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !shared || !items) return DL_ERR_INVALID;
+    if (!c->has_accel) return set_error(c, DL_ERR_STATE, "render before build_accel");
+    if (n == 0 || n > DL_BATCH_MAX) return set_error(c, DL_ERR_INVALID, "a shared pass holds 1 to DL_BATCH_MAX images");
+    bool used[DL_BATCH_MAX] = {};
+    for (uint32_t i = 0; i < n; i++) {
+        DL_TRY(check_camera(c, items[i].camera));
+        if (items[i].spp <= 0) return set_error(c, DL_ERR_INVALID, "spp must be positive");
+        if (items[i].film < 0 || items[i].film >= DL_BATCH_MAX)
+            return set_error(c, DL_ERR_INVALID, "film slot out of range");
+        if (used[items[i].film]) return set_error(c, DL_ERR_INVALID, "two images of a pass share a film slot");
+        used[items[i].film] = true;
+    }
+    return render_batch(c, shared, items, n, stats);
+}
+
 dl_status dl_hip_render_paths(void* h, const dl_render_params* p, uint32_t sample, float* out) {
     Ctx* c = (Ctx*)h;
     if (!c || !p || !out) return DL_ERR_INVALID;
@@ -534,6 +553,7 @@ dl_status dl_hip_trim(void* h) {
     if (!c) return DL_ERR_INVALID;
     (void)hipSetDevice(c->device);
     free_render_state(c);
+    free_film_slots(c);
     free_guide(c);
     free_all_scenes(c);
     return DL_OK;

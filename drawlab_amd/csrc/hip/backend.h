@@ -133,6 +133,14 @@ struct Ctx {
     std::vector<void*> render_allocs;
     size_t render_capacity = 0;
     void* rs = nullptr;  // RenderState*
+    // films of shared-pass images (dl_hip_render_batch), by slot: float4
+    // accumulators and resolved RGB of w x h pixels (room for cap)
+    struct FilmSlot {
+        float* film = nullptr;
+        float* rgb = nullptr;
+        uint32_t w = 0, h = 0, cap = 0;
+    };
+    std::vector<FilmSlot> film_slots;
     void* guide = nullptr;  // GuideState* (learned directional proposal)
 
     // device memory held by this context (every allocation goes through
@@ -179,6 +187,9 @@ dl_status launch_trace_shadow(Ctx* c, const float* rays, uint32_t n, uint8_t* oc
 dl_status render(Ctx* c, const dl_render_params* p, float* rgb_host, dl_render_stats* stats, int paths_sample,
                  float* paths_out);
 void free_render_state(Ctx* c);
+dl_status render_batch(Ctx* c, const dl_render_params* shared, const dl_batch_item* items, uint32_t n,
+                       dl_render_stats* stats);
+void free_film_slots(Ctx* c);
 void invalidate_film(Ctx* c);  // the next render clears the film even with DL_RENDER_ACCUMULATE
 
 // guide.hip

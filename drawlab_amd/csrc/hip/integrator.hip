@@ -63,6 +63,7 @@ struct RenderBufs {
     float4 *thr, *rad, *nee;
     int4* aux;
     uint64_t* rng;
+    uint32_t* seq;  // sample index of the path (its pcg32 stream selector)
     float2* pos;
     uint8_t* occ;
     uint32_t* matq[2];
@@ -234,11 +235,16 @@ __device__ __forceinline__ void mat4_apply(const float* m, const float in[4], fl
     }
 }
 
-__global__ __launch_bounds__(kStageBlock) void k_raygen(PassParams pp, dl_camera_desc cam, RenderBufs rs) {
+// Camera paths pp.P of one image (pp.W x pp.H, samples pp.sample_base ...)
+// stored from path index `base` on; a pass of several images (one launch
+// each) queues `total` paths.
+__global__ __launch_bounds__(kStageBlock) void k_raygen(PassParams pp, dl_camera_desc cam, RenderBufs rs, uint32_t base,
+                                                        uint32_t total) {
     const uint32_t WH = pp.W * pp.H;
     const float invx = 1.f / (float)cam.width, invy = 1.f / (float)cam.height;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < pp.P; p += gridDim.x * blockDim.x) {
-        const uint32_t pix = p % WH, j = p / WH;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < pp.P; i += gridDim.x * blockDim.x) {
+        const uint32_t p = base + i;
+        const uint32_t pix = i % WH, j = i / WH;
         const uint32_t x = pix % pp.W, y = pix / pp.W;
         Pcg32 rng;
         rng.seed((uint64_t)pix + ((uint64_t)pp.seed << 32), (uint64_t)(pp.sample_base + j));
@@ -285,10 +291,11 @@ __global__ __launch_bounds__(kStageBlock) void k_raygen(PassParams pp, dl_camera
         rs.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
         rs.aux[p] = make_int4(0, 0, 0, 0);
         rs.rng[p] = rng.state;
+        rs.seq[p] = pp.sample_base + j;
         rs.pos[p] = make_float2(px, py);
         rs.rayq[p] = p;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) rs.cnt->ray = pp.P;
+    if (blockIdx.x == 0 && threadIdx.x == 0) rs.cnt->ray = total;
 }
 
 // Queued path rays: the ray of path q[i] lives in ray_o/ray_d (shadow: sh_o/sh_d).
@@ -475,7 +482,6 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                                                        GuideView gv) {
     const uint32_t n = rs.cnt->mat[cur][TYPE];
     const uint32_t* __restrict__ queue = rs.matq[cur] + (size_t)TYPE * pp.cap;
-    const uint32_t WH = pp.W * pp.H;
     constexpr int type = TYPE;
     __shared__ uint32_t s_p[kStageItems][kStageBlock];
     __shared__ uint32_t s_b[kStageItems][kStageBlock];
@@ -500,7 +506,7 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                 int4 aux = rs.aux[p];
                 Pcg32 rng;
                 rng.state = rs.rng[p];
-                rng.inc = ((uint64_t)(pp.sample_base + p / WH) << 1u) | 1u;
+                rng.inc = ((uint64_t)rs.seq[p] << 1u) | 1u;
                 Its its;
                 int mat;
                 float prm[8];
@@ -669,7 +675,6 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
     const uint32_t* counts = rs.cnt->mat[cur];
     const uint32_t n = queue_total(counts);
     const int next = cur ^ 1;
-    const uint32_t WH = pp.W * pp.H;
     __shared__ uint32_t s_p[kStageItems][kStageBlock];
     __shared__ uint32_t s_b[kStageItems][kStageBlock];
     const uint32_t items = stage_items(n);
@@ -758,7 +763,7 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
                         if (depth >= pp.rr_depth) {
                             Pcg32 rng;
                             rng.state = rs.rng[p];
-                            rng.inc = ((uint64_t)(pp.sample_base + p / WH) << 1u) | 1u;
+                            rng.inc = ((uint64_t)rs.seq[p] << 1u) | 1u;
                             const float pr = fmin0(cmax(scale(scale(thr, eta), eta)), 0.99f);
                             if (rng.next_float() > pr) {
                                 alive = false;
@@ -1019,6 +1024,15 @@ void invalidate_film(Ctx* c) {
     if (c->rs) ((RenderState*)c->rs)->film_valid = false;
 }
 
+void free_film_slots(Ctx* c) {
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& f : c->film_slots) {
+        (void)dfree(c, f.film);
+        (void)dfree(c, f.rgb);
+    }
+    c->film_slots.clear();
+}
+
 namespace {
 
 template <typename T>
@@ -1054,6 +1068,7 @@ dl_status ensure_state(Ctx* c, uint32_t P, uint32_t pixels) {
     DL_TRY(alloc(c, rs, &rs->b.nee, P));
     DL_TRY(alloc(c, rs, &rs->b.aux, P));
     DL_TRY(alloc(c, rs, &rs->b.rng, P));
+    DL_TRY(alloc(c, rs, &rs->b.seq, P));
     DL_TRY(alloc(c, rs, &rs->b.pos, P));
     DL_TRY(alloc(c, rs, &rs->b.occ, P));
     DL_TRY(alloc(c, rs, &rs->b.matq[0], (size_t)P * kTypes));
@@ -1131,6 +1146,95 @@ struct Timer {
 enum { kTagClosest = 0, kTagShadow = 1, kTagShade = 2, kTagFilm = 3, kTagOther = 4, kTagGuide = 5, kTagTrain = 6 };
 
 }  // namespace
+
+// The closest-hit trace of the camera rays k_raygen queued, the first hits,
+// then bounces (shading, continuation and shadow traces, post) until every
+// path of the pass has ended.
+dl_status run_bounces(Ctx* c, RenderState* rs, const PassParams& pp, const SceneView& sv, const BvhView& bvh,
+                      bool count, bool normals, bool hide, bool guided, const bool* guide_types, const GuideView& gv,
+                      const Net& net, int guide_grid, int stage_grid, int trace_grid, Timer& tm, double& closest,
+                      double& shadow) {
+    int cur = 0;
+    DL_TRY(tm.start(kTagClosest));
+    if (count)
+        k_trace_queue<true><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
+    else
+        k_trace_queue<false><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
+    DL_TRY(tm.stop());
+    DL_TRY(tm.start(kTagShade));
+    if (normals)
+        k_normals<<<stage_grid, kStageBlock, 0, c->stream>>>(sv, rs->b);
+    else if (hide)
+        k_post_camera<true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
+    else
+        k_post_camera<false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
+    DL_TRY(tm.stop());
+    DL_HIP_CHECK(c, hipGetLastError());
+    for (int iter = 0;; ++iter) {
+        DL_HIP_CHECK(c, hipMemcpyAsync(rs->cnt_host, rs->b.cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+        DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        uint32_t live = 0;
+        for (int t = 0; t < kTypes; t++) live += rs->cnt_host->mat[cur][t];
+        if (iter > 0) {
+            closest += rs->cnt_host->ray;
+            shadow += rs->cnt_host->shadow;
+        }
+        if (live == 0) break;
+        k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, cur ^ 1);
+        DL_TRY(tm.start(kTagShade));
+        if (rs->cnt_host->mat[cur][kPassQ] > 0)
+            k_pass<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
+        for (int t = 0; t < DL_BSDF_COUNT; t++) {
+            if (!rs->types_present[t] || rs->cnt_host->mat[cur][t] == 0) continue;
+            if (guided && guide_types[t]) {
+                DL_TRY(tm.stop());
+                DL_TRY(tm.start(kTagGuide));
+                if (gv.frac > 0.0f) switch (t) {
+                    case DL_BSDF_DIFFUSE: k_guide_infer<DL_BSDF_DIFFUSE><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
+                    case DL_BSDF_MICROFACET: k_guide_infer<DL_BSDF_MICROFACET><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
+                    case DL_BSDF_ANISO_GGX: k_guide_infer<DL_BSDF_ANISO_GGX><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
+                    case DL_BSDF_CONDUCTOR: k_guide_infer<DL_BSDF_CONDUCTOR><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
+                }
+                DL_TRY(tm.stop());
+                DL_TRY(tm.start(kTagShade));
+                switch (t) {
+                    case DL_BSDF_DIFFUSE: k_shade<DL_BSDF_DIFFUSE, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                    case DL_BSDF_MICROFACET: k_shade<DL_BSDF_MICROFACET, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                    case DL_BSDF_ANISO_GGX: k_shade<DL_BSDF_ANISO_GGX, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                    case DL_BSDF_CONDUCTOR: k_shade<DL_BSDF_CONDUCTOR, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                }
+                continue;
+            }
+            switch (t) {
+                case DL_BSDF_DIFFUSE: k_shade<DL_BSDF_DIFFUSE, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                case DL_BSDF_MIRROR: k_shade<DL_BSDF_MIRROR, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                case DL_BSDF_DIELECTRIC: k_shade<DL_BSDF_DIELECTRIC, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                case DL_BSDF_MICROFACET: k_shade<DL_BSDF_MICROFACET, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                case DL_BSDF_ANISO_GGX: k_shade<DL_BSDF_ANISO_GGX, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                case DL_BSDF_CONDUCTOR: k_shade<DL_BSDF_CONDUCTOR, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+            }
+        }
+        DL_TRY(tm.stop());
+        DL_TRY(tm.start(kTagClosest));
+        if (count)
+            k_trace_queue<true><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
+        else
+            k_trace_queue<false><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
+        DL_TRY(tm.stop());
+        DL_TRY(tm.start(kTagShadow));
+        k_shadow_queue<<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
+        DL_TRY(tm.stop());
+        DL_TRY(tm.start(kTagShade));
+        if (hide)
+            k_post<true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv.rec);
+        else
+            k_post<false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv.rec);
+        DL_TRY(tm.stop());
+        DL_HIP_CHECK(c, hipGetLastError());
+        cur ^= 1;
+    }
+    return DL_OK;
+}
 
 dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render_stats* stats, int paths_sample,
                  float* paths_out) {
@@ -1220,7 +1324,6 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
         pp.eps = eps;
         paths += pp.P;
         closest += pp.P;
-        int cur = 0;
         if (train) {
             // one record slot per (recorded path, depth < kGRecDepth)
             gv.rec_stride = std::max<uint32_t>(1u, (uint32_t)(((uint64_t)pp.P * kGRecDepth + kGuideRecords - 1) /
@@ -1230,86 +1333,10 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
         k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, 0);
         k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, 1);
         DL_TRY(tm.start(kTagOther));
-        k_raygen<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, c->cam, rs->b);
+        k_raygen<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, c->cam, rs->b, 0u, pp.P);
         DL_TRY(tm.stop());
-        DL_TRY(tm.start(kTagClosest));
-        if (count)
-            k_trace_queue<true><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
-        else
-            k_trace_queue<false><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
-        DL_TRY(tm.stop());
-        DL_TRY(tm.start(kTagShade));
-        if (normals)
-            k_normals<<<stage_grid, kStageBlock, 0, c->stream>>>(sv, rs->b);
-        else if (hide)
-            k_post_camera<true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
-        else
-            k_post_camera<false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
-        DL_TRY(tm.stop());
-        DL_HIP_CHECK(c, hipGetLastError());
-        for (int iter = 0;; ++iter) {
-            DL_HIP_CHECK(c, hipMemcpyAsync(rs->cnt_host, rs->b.cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-            DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-            uint32_t live = 0;
-            for (int t = 0; t < kTypes; t++) live += rs->cnt_host->mat[cur][t];
-            if (iter > 0) {
-                closest += rs->cnt_host->ray;
-                shadow += rs->cnt_host->shadow;
-            }
-            if (live == 0) break;
-            k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, cur ^ 1);
-            DL_TRY(tm.start(kTagShade));
-            if (rs->cnt_host->mat[cur][kPassQ] > 0)
-                k_pass<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
-            for (int t = 0; t < DL_BSDF_COUNT; t++) {
-                if (!rs->types_present[t] || rs->cnt_host->mat[cur][t] == 0) continue;
-                if (guided && guide_types[t]) {
-                    DL_TRY(tm.stop());
-                    DL_TRY(tm.start(kTagGuide));
-                    if (gv.frac > 0.0f) switch (t) {
-                        case DL_BSDF_DIFFUSE: k_guide_infer<DL_BSDF_DIFFUSE><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
-                        case DL_BSDF_MICROFACET: k_guide_infer<DL_BSDF_MICROFACET><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
-                        case DL_BSDF_ANISO_GGX: k_guide_infer<DL_BSDF_ANISO_GGX><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
-                        case DL_BSDF_CONDUCTOR: k_guide_infer<DL_BSDF_CONDUCTOR><<<guide_grid, kGWaves * 64, 0, c->stream>>>(pp, sv, rs->b, cur, gv, net); break;
-                    }
-                    DL_TRY(tm.stop());
-                    DL_TRY(tm.start(kTagShade));
-                    switch (t) {
-                        case DL_BSDF_DIFFUSE: k_shade<DL_BSDF_DIFFUSE, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                        case DL_BSDF_MICROFACET: k_shade<DL_BSDF_MICROFACET, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                        case DL_BSDF_ANISO_GGX: k_shade<DL_BSDF_ANISO_GGX, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                        case DL_BSDF_CONDUCTOR: k_shade<DL_BSDF_CONDUCTOR, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                    }
-                    continue;
-                }
-                switch (t) {
-                    case DL_BSDF_DIFFUSE: k_shade<DL_BSDF_DIFFUSE, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                    case DL_BSDF_MIRROR: k_shade<DL_BSDF_MIRROR, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                    case DL_BSDF_DIELECTRIC: k_shade<DL_BSDF_DIELECTRIC, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                    case DL_BSDF_MICROFACET: k_shade<DL_BSDF_MICROFACET, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                    case DL_BSDF_ANISO_GGX: k_shade<DL_BSDF_ANISO_GGX, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                    case DL_BSDF_CONDUCTOR: k_shade<DL_BSDF_CONDUCTOR, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                }
-            }
-            DL_TRY(tm.stop());
-            DL_TRY(tm.start(kTagClosest));
-            if (count)
-                k_trace_queue<true><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
-            else
-                k_trace_queue<false><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
-            DL_TRY(tm.stop());
-            DL_TRY(tm.start(kTagShadow));
-            k_shadow_queue<<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
-            DL_TRY(tm.stop());
-            DL_TRY(tm.start(kTagShade));
-            if (hide)
-                k_post<true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv.rec);
-            else
-                k_post<false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv.rec);
-            DL_TRY(tm.stop());
-            DL_HIP_CHECK(c, hipGetLastError());
-            cur ^= 1;
-        }
+        DL_TRY(run_bounces(c, rs, pp, sv, bvh, count, normals, hide, guided, guide_types, gv, net, guide_grid,
+                           stage_grid, trace_grid, tm, closest, shadow));
         if (paths_sample >= 0) break;
         if (train) {
             DL_TRY(tm.start(kTagTrain));
@@ -1374,6 +1401,145 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
         stats->nodes_closest = (double)rs->cnt_host->nodes;
         stats->tris_closest = (double)rs->cnt_host->tris;
         stats->train_steps = train_steps;
+    }
+    return DL_OK;
+}
+
+// One pass shared by several images of the current scene: each image's
+// camera paths are generated into consecutive ranges of the path state (its
+// pcg32 streams keyed by its own pixels, samples and seed), every path then
+// goes through the same queues and kernels as in render(), and each image's
+// film gathers its own range with the same kernel and order as a pass of
+// render() -- so an image equals its dl_hip_render with the same pass sizes.
+dl_status render_batch(Ctx* c, const dl_render_params* prm, const dl_batch_item* items, uint32_t n,
+                       dl_render_stats* stats) {
+    if (prm->flags & DL_RENDER_GUIDED) return set_error(c, DL_ERR_INVALID, "shared passes do not take DL_RENDER_GUIDED");
+    if (c->light_recs.empty()) return set_error(c, DL_ERR_INVALID, "There is not emitter in the scene.");
+    (void)hipSetDevice(c->device);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n; i++)
+        total += (uint64_t)items[i].camera.width * items[i].camera.height * (uint64_t)items[i].spp;
+    if (total > (1ull << 27)) return set_error(c, DL_ERR_INVALID, "pass too large");
+    const uint32_t P = (uint32_t)total;
+    DL_TRY(ensure_state(c, P, 1));
+    RenderState* rs = (RenderState*)c->rs;
+    if (c->film_slots.size() < (size_t)DL_BATCH_MAX) c->film_slots.resize(DL_BATCH_MAX);
+    for (uint32_t i = 0; i < n; i++) {
+        Ctx::FilmSlot& f = c->film_slots[items[i].film];
+        const uint32_t w = (uint32_t)items[i].camera.width, h = (uint32_t)items[i].camera.height;
+        if (items[i].accumulate && (f.w != w || f.h != h))
+            return set_error(c, DL_ERR_INVALID, "film slot accumulates an image of another size");
+        if (f.cap < w * h) {
+            (void)dfree(c, f.film);
+            (void)dfree(c, f.rgb);
+            f = Ctx::FilmSlot{};
+            DL_HIP_CHECK(c, dmalloc(c, (void**)&f.film, (size_t)w * h * sizeof(float4)));
+            DL_HIP_CHECK(c, dmalloc(c, (void**)&f.rgb, (size_t)w * h * 3 * sizeof(float)));
+            f.cap = w * h;
+        }
+        f.w = w;
+        f.h = h;
+    }
+    const SceneView sv = scene_view(c);
+    for (int t = 0; t < 8; t++) rs->types_present[t] = false;
+    for (const auto& m : c->mats) rs->types_present[m.type] = true;
+    const bool guide_types[8] = {false, false, false, false, false, false, false, false};
+    const BvhView bvh{c->ds.nodes, c->ds.tris};
+    const bool count = stats && (prm->flags & DL_RENDER_COUNT_TRAVERSAL);
+    const bool normals = (prm->flags & DL_RENDER_NORMALS) != 0;
+    const bool hide = c->view_set && c->view.hide_lights != 0;
+    const float eps = c->view_set && c->view.epsilon > 0.0f ? c->view.epsilon : kEpsilon;
+    const int stage_grid = std::max(1, c->num_cus) * 8;
+    const int trace_grid = (int)c->trace_blocks;
+    std::vector<int> tags;
+    Timer tm{c, rs, &tags};
+    double closest = 0, shadow = 0;
+    PassParams pp{};
+    pp.P = P;
+    pp.W = 1;
+    pp.H = 1;
+    pp.max_depth = prm->max_depth;
+    pp.rr_depth = prm->rr_depth;
+    pp.cap = P;
+    pp.eps = eps;
+    closest += P;
+    std::vector<PassParams> ipp(n);
+    std::vector<uint32_t> base(n);
+    DL_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
+    k_reset_all<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt);
+    k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, 0);
+    k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, 1);
+    DL_TRY(tm.start(kTagOther));
+    uint32_t off = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const dl_batch_item& it = items[i];
+        PassParams& q = ipp[i];
+        q = pp;
+        q.W = (uint32_t)it.camera.width;
+        q.H = (uint32_t)it.camera.height;
+        q.P = q.W * q.H * (uint32_t)it.spp;
+        q.sample_base = (uint32_t)it.first_sample;
+        q.seed = it.seed;
+        base[i] = off;
+        if (!it.accumulate)
+            k_clear_film<<<stage_grid, kStageBlock, 0, c->stream>>>((float4*)c->film_slots[it.film].film, q.W * q.H);
+        k_raygen<<<stage_grid, kStageBlock, 0, c->stream>>>(q, it.camera, rs->b, off, P);
+        off += q.P;
+    }
+    DL_TRY(tm.stop());
+    const GuideView gv{};
+    const Net net{};
+    DL_TRY(run_bounces(c, rs, pp, sv, bvh, count, normals, hide, false, guide_types, gv, net, 1, stage_grid,
+                       trace_grid, tm, closest, shadow));
+    DL_TRY(tm.start(kTagFilm));
+    for (uint32_t i = 0; i < n; i++) {
+        const dl_batch_item& it = items[i];
+        const PassParams& q = ipp[i];
+        RenderBufs fb = rs->b;
+        fb.rad += base[i];
+        fb.pos += base[i];
+        fb.film = (float4*)c->film_slots[it.film].film;
+        if ((int)ceilf(c->film.radius) + 1 <= kFilmHalo) {
+            const dim3 fgrid((q.W + kFilmTile - 1) / kFilmTile, (q.H + kFilmTile - 1) / kFilmTile);
+            k_film_tiled<<<fgrid, kFilmTile * kFilmTile, 0, c->stream>>>(q, (uint32_t)it.spp, c->film, fb);
+        } else {
+            k_film<<<stage_grid, kStageBlock, 0, c->stream>>>(q, (uint32_t)it.spp, c->film, fb);
+        }
+        if (it.rgb_out)
+            k_resolve<<<stage_grid, kStageBlock, 0, c->stream>>>((const float4*)c->film_slots[it.film].film,
+                                                                  c->film_slots[it.film].rgb, q.W * q.H);
+    }
+    DL_TRY(tm.stop());
+    DL_HIP_CHECK(c, hipGetLastError());
+    DL_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
+    DL_HIP_CHECK(c, hipEventSynchronize(c->ev1));
+    for (uint32_t i = 0; i < n; i++)
+        if (items[i].rgb_out)
+            DL_HIP_CHECK(c, hipMemcpy(items[i].rgb_out, c->film_slots[items[i].film].rgb,
+                                      (size_t)ipp[i].W * ipp[i].H * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        float ms = 0.f;
+        DL_HIP_CHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        stats->ms_total = ms;
+        for (size_t i = 0; i < tags.size(); i++) {
+            float e = 0.f;
+            DL_HIP_CHECK(c, hipEventElapsedTime(&e, rs->evs[2 * i], rs->evs[2 * i + 1]));
+            switch (tags[i]) {
+                case kTagClosest: stats->ms_closest += e; stats->launches_closest += 1; break;
+                case kTagShadow: stats->ms_shadow += e; stats->launches_shadow += 1; break;
+                case kTagShade: stats->ms_shade += e; break;
+                case kTagFilm: stats->ms_film += e; break;
+                default: stats->ms_other += e; break;
+            }
+        }
+        DL_HIP_CHECK(c, hipMemcpy(rs->cnt_host, rs->b.cnt, sizeof(Counters), hipMemcpyDeviceToHost));
+        stats->paths = P;
+        stats->closest_rays = closest;
+        stats->shadow_rays = shadow;
+        stats->invalid_samples = (double)rs->cnt_host->invalid;
+        stats->nodes_closest = (double)rs->cnt_host->nodes;
+        stats->tris_closest = (double)rs->cnt_host->tris;
     }
     return DL_OK;
 }

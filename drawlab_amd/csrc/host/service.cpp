@@ -2,8 +2,14 @@
 //
 // Lanes are worker threads, each owning one backend context (its own HIP
 // stream, scene, BVH and render state); kernels of jobs on different lanes
-// overlap on the device's hardware queues, which is what packs small jobs
-// into the GPU.  A lane keeps every scene it has uploaded resident
+// overlap on the device's hardware queues.  Within a lane, jobs of the
+// lane's scene share wavefront passes (dl_hip_render_batch): a lane that
+// starts a job keeps taking further queued jobs of the same scene and
+// compatible settings into its running group at every pass boundary, while
+// the pass stays within the pass size, so small jobs travel together
+// through one set of queues and kernels (continuous batching).  Each job's
+// passes have the sizes its standalone render would use, so its image is
+// still bit-identical to a single dl_hip_render call.  A lane keeps every scene it has uploaded resident
 // (dl_hip_scene_keep / dl_hip_scene_select: switching scenes costs no upload
 // and no BVH build), and prefers a queued job of one of its resident scenes
 // among the next `lanes` queue entries (bounded bypass, so no job waits
@@ -20,9 +26,10 @@
 // is compared with its reservation (estimate_misses counts violations).
 //
 // A job renders in passes of the same size dl_hip_render would choose
-// (spp_per_pass = 0 resolves to the same value), accumulating into the
-// lane's film, so its image is bit-identical to a single dl_hip_render
-// call; cancellation takes effect between passes.
+// (spp_per_pass = 0 resolves to the same value), accumulating into its film
+// slot (or, for a guided job, which renders alone, the context's film), so
+// its image is bit-identical to a single dl_hip_render call; cancellation
+// takes effect between passes.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -82,6 +89,7 @@ struct Lane {
     uint64_t resident_scene = 0;  // the context's current scene (0: none)
     std::map<uint64_t, uint64_t> kept;  // resident scenes -> device bytes they hold (measured at upload)
     uint64_t state_paths = 0, state_pixels = 0;  // render state the context holds
+    uint64_t slot_pixels[DL_BATCH_MAX] = {};     // film slots the context holds (pixels each)
     uint64_t reservation = 0;
     std::thread th;
 };
@@ -131,6 +139,31 @@ uint64_t reservation_for(const Lane& lane, const Job& job, const SceneEntry& s) 
     return lane.fixed_bytes + others + scene + state;
 }
 
+// Device bytes the lane may reach while running a group of shared passes of
+// at most `pass_paths` paths on scene `scene_id`, whose film slots need
+// slot_px[s] pixels (see file comment).
+uint64_t group_reservation(const Lane& lane, uint64_t scene_id, const SceneEntry& s, uint64_t pass_paths,
+                           const uint64_t* slot_px) {
+    uint64_t films = 0;
+    for (int k = 0; k < DL_BATCH_MAX; k++) films += std::max(lane.slot_pixels[k], slot_px[k]);
+    const uint64_t state = std::max(pass_paths, lane.state_paths) * kStateBytesPerPath +
+                           (lane.state_pixels + films) * kFilmBytesPerPixel + kSlackBytes;
+    uint64_t others = 0;
+    for (const auto& kv : lane.kept)
+        if (kv.first != scene_id) others += kv.second;
+    const auto it = lane.kept.find(scene_id);
+    const uint64_t scene = it != lane.kept.end() ? it->second : s.bytes_estimate();
+    return lane.fixed_bytes + others + scene + state;
+}
+
+// Jobs that can share a pass: same scene and the settings a pass shares.
+bool can_share(const Job& a, const Job& b) {
+    return a.scene == b.scene && a.params.max_depth == b.params.max_depth &&
+           a.params.rr_depth == b.params.rr_depth && !(a.params.flags & DL_RENDER_GUIDED) &&
+           !(b.params.flags & DL_RENDER_GUIDED) &&
+           (a.params.flags & DL_RENDER_NORMALS) == (b.params.flags & DL_RENDER_NORMALS);
+}
+
 // Pick the next job for `lane` (caller holds the lock): policy order, with a
 // preference for the lane's resident scene among the next `lanes` entries.
 std::deque<uint64_t>::iterator pick(Service& S, Lane& lane) {
@@ -166,11 +199,11 @@ void finish(Service& S, Job& job, int state, const std::string& error) {
     if (state == DL_JOB_CANCELLED) S.stats.jobs_cancelled++;
 }
 
-// Render one job on the lane's context (no lock held).  Returns "" or an error.
+// Make the job's scene current on the lane's context (no lock held).
 // switch: 0 the scene is current, 1 select a resident scene, 2 upload and
 // build (then keep it resident; *kept_bytes = the device bytes it added).
-std::string run_job(Service& S, Lane& lane, Job& job, const SceneEntry& scene, int switch_, uint64_t* kept_bytes,
-                    dl_render_stats& total) {
+std::string prepare_scene(Service& S, Lane& lane, const Job& job, const SceneEntry& scene, int switch_,
+                          uint64_t* kept_bytes) {
     void* c = lane.ctx;
     auto fail = [&]() { return std::string(dl_hip_last_error(c)); };
     if (switch_ == 1 && dl_hip_scene_select(c, job.scene) != DL_OK) return fail();
@@ -185,6 +218,16 @@ std::string run_job(Service& S, Lane& lane, Job& job, const SceneEntry& scene, i
         dl_hip_memory_usage(c, &after, nullptr);
         *kept_bytes = after > before ? after - before : 0;
     }
+    (void)S;
+    return "";
+}
+
+// Render one job alone on the lane's context, its scene current (no lock
+// held).  Returns "" or an error.
+std::string run_job(Service& S, Lane& lane, Job& job, dl_render_stats& total) {
+    (void)S;
+    void* c = lane.ctx;
+    auto fail = [&]() { return std::string(dl_hip_last_error(c)); };
     if (dl_hip_set_camera(c, &job.camera) != DL_OK) return fail();
     const uint64_t px = job_pixels(job);
     const uint32_t k = pass_spp(job.params, px);
@@ -210,6 +253,178 @@ std::string run_job(Service& S, Lane& lane, Job& job, const SceneEntry& scene, i
     return "";
 }
 
+// A group of shared passes led by `lead` (its scene current, unless
+// `prep_error` says why not; no lock held on entry or exit).  At every pass
+// boundary the group drops finished and cancelled jobs and takes queued jobs
+// that can share its passes while the pass stays within kPathsPerPass paths,
+// the group within DL_BATCH_MAX jobs and the lane's reservation within the
+// budget.
+struct Member {
+    Job* job;
+    uint32_t k = 0, done = 0;  // pass size and samples rendered
+    int slot = 0;
+    double paths = 0, ms = 0;
+};
+
+void run_group(Service& S, Lane& lane, Job* lead, const SceneEntry& scene, std::string prep_error, int switch_,
+               uint64_t kept_bytes) {
+    void* c = lane.ctx;
+    std::vector<Member> group;
+    const uint64_t lpx = job_pixels(*lead);
+    Member m0;
+    m0.job = lead;
+    m0.k = pass_spp(lead->params, lpx);
+    group.push_back(m0);
+    lead->rgb.assign(lpx * 3, 0.0f);
+    std::string error = prep_error;
+    uint64_t peak_all = 0, max_pass = 0;
+    bool ok = error.empty();
+    std::unique_lock<std::mutex> lk(S.m);
+    if (ok && switch_ == 2) lane.kept[lead->scene] = kept_bytes;
+    if (ok) lane.resident_scene = lead->scene;
+    while (!group.empty()) {
+        // drop cancelled jobs
+        for (size_t i = 0; i < group.size();) {
+            if (group[i].job->cancel.load() || !ok) {
+                Job& j = *group[i].job;
+                j.info.paths = group[i].paths;
+                j.info.ms_gpu = group[i].ms;
+                S.stats.paths += group[i].paths;
+                finish(S, j, ok ? DL_JOB_CANCELLED : DL_JOB_FAILED, ok ? "" : error);
+                group.erase(group.begin() + (long)i);
+                S.cv.notify_all();
+            } else {
+                ++i;
+            }
+        }
+        if (group.empty()) break;
+        // take queued jobs that can share the passes
+        uint64_t pass_paths = 0;
+        bool slot_used[DL_BATCH_MAX] = {};
+        uint64_t slot_px[DL_BATCH_MAX] = {};
+        for (const Member& g : group) {
+            const uint64_t px = job_pixels(*g.job);
+            pass_paths += px * std::min<uint32_t>(g.k, (uint32_t)g.job->params.spp - g.done);
+            slot_used[g.slot] = true;
+            slot_px[g.slot] = px;
+        }
+        for (auto qi = S.queue.begin(); qi != S.queue.end() && group.size() < (size_t)DL_BATCH_MAX;) {
+            Job& cand = *S.jobs[*qi];
+            const uint64_t px = job_pixels(cand);
+            const uint32_t k = pass_spp(cand.params, px);
+            if (!can_share(*lead, cand) || pass_paths + px * k > kPathsPerPass) {
+                ++qi;
+                continue;
+            }
+            int slot = 0;
+            while (slot_used[slot]) ++slot;
+            uint64_t try_px[DL_BATCH_MAX];
+            std::copy(slot_px, slot_px + DL_BATCH_MAX, try_px);
+            try_px[slot] = px;
+            const uint64_t need =
+                std::max(lane.reservation, group_reservation(lane, lead->scene, scene, pass_paths + px * k, try_px));
+            if (S.reserved_total() - lane.reservation + need > S.cfg.budget_bytes) {
+                ++qi;
+                continue;
+            }
+            lane.reservation = need;
+            S.stats.peak_reserved_bytes = std::max<uint64_t>(S.stats.peak_reserved_bytes, S.reserved_total());
+            cand.info.state = DL_JOB_RUNNING;
+            cand.info.lane = lane.index;
+            cand.info.t_start = S.now();
+            cand.info.reserved_bytes = need;
+            cand.rgb.assign(px * 3, 0.0f);
+            Member mm;
+            mm.job = &cand;
+            mm.k = k;
+            mm.slot = slot;
+            group.push_back(mm);
+            slot_used[slot] = true;
+            slot_px[slot] = px;
+            pass_paths += px * k;
+            S.stats.batched_jobs++;
+            qi = S.queue.erase(qi);
+        }
+        if (group.size() > 1) S.stats.shared_passes++;
+        // every member runs under the lane's current reservation
+        for (const Member& g : group)
+            g.job->info.reserved_bytes = std::max<uint64_t>(g.job->info.reserved_bytes, lane.reservation);
+        lk.unlock();
+
+        std::vector<dl_batch_item> items(group.size());
+        for (size_t i = 0; i < group.size(); i++) {
+            const Member& g = group[i];
+            const Job& j = *g.job;
+            const uint32_t kk = std::min<uint32_t>(g.k, (uint32_t)j.params.spp - g.done);
+            dl_batch_item& it = items[i];
+            it.camera = j.camera;
+            it.spp = (int32_t)kk;
+            it.first_sample = j.params.first_sample + (int32_t)g.done;
+            it.seed = j.params.seed;
+            it.film = g.slot;
+            it.accumulate = g.done > 0 ? 1 : 0;
+            it.rgb_out = g.done + kk >= (uint32_t)j.params.spp ? g.job->rgb.data() : nullptr;
+        }
+        dl_render_params shared = lead->params;
+        dl_render_stats st{};
+        const bool pass_ok = dl_hip_render_batch(c, &shared, items.data(), (uint32_t)items.size(), &st) == DL_OK;
+        if (!pass_ok) error = dl_hip_last_error(c);
+        uint64_t cur = 0, peak = 0;
+        dl_hip_memory_usage(c, &cur, &peak);
+        peak_all = std::max(peak_all, peak);
+        max_pass = std::max<uint64_t>(max_pass, (uint64_t)st.paths);
+
+        lk.lock();
+        if (!pass_ok) {
+            ok = false;
+            continue;  // the group fails at the top of the loop
+        }
+        for (size_t i = 0; i < group.size(); i++) {
+            Member& g = group[i];
+            const double p = (double)job_pixels(*g.job) * items[i].spp;
+            g.paths += p;
+            g.ms += st.paths > 0 ? st.ms_total * p / st.paths : 0.0;
+            g.done += (uint32_t)items[i].spp;
+            const uint64_t px = job_pixels(*g.job);
+            lane.slot_pixels[g.slot] = std::max(lane.slot_pixels[g.slot], px);
+            g.job->info.peak_bytes = std::max<uint64_t>(g.job->info.peak_bytes, peak);
+        }
+        for (size_t i = 0; i < group.size();) {
+            Member& g = group[i];
+            if (g.done >= (uint32_t)g.job->params.spp) {
+                Job& j = *g.job;
+                j.info.paths = g.paths;
+                j.info.ms_gpu = g.ms;
+                if (j.info.peak_bytes > j.info.reserved_bytes) S.stats.estimate_misses++;
+                S.stats.paths += g.paths;
+                finish(S, j, DL_JOB_DONE, "");
+                group.erase(group.begin() + (long)i);
+                S.cv.notify_all();
+            } else {
+                ++i;
+            }
+        }
+    }
+    lane.state_paths = std::max<uint64_t>(lane.state_paths, max_pass);
+    if (!ok) {
+        // the context may be left between scenes: start over from nothing
+        lk.unlock();
+        dl_hip_trim(c);
+        lk.lock();
+        lane.kept.clear();
+        lane.resident_scene = 0;
+        lane.state_paths = lane.state_pixels = 0;
+        for (auto& v : lane.slot_pixels) v = 0;
+    }
+    lk.unlock();
+    uint64_t cur = 0;
+    dl_hip_memory_usage(c, &cur, nullptr);
+    lk.lock();
+    lane.reservation = cur;
+    S.cv.notify_all();
+    (void)peak_all;
+}
+
 void lane_main(Service* Sp, Lane* lp) {
     Service& S = *Sp;
     Lane& lane = *lp;
@@ -229,6 +444,7 @@ void lane_main(Service* Sp, Lane* lp) {
             lane.resident_scene = 0;
             lane.kept.clear();
             lane.state_paths = lane.state_pixels = 0;
+            for (auto& v : lane.slot_pixels) v = 0;
             lane.reservation = cur;
             S.cv.notify_all();
         }
@@ -236,9 +452,15 @@ void lane_main(Service* Sp, Lane* lp) {
         if (it == S.queue.end()) continue;
         Job& job = *S.jobs[*it];
         const SceneEntry& scene = *S.scenes[job.scene];
+        const bool alone = (job.params.flags & DL_RENDER_GUIDED) != 0;  // guided jobs render alone
+        const uint64_t px = job_pixels(job);
+        uint64_t slot_px[DL_BATCH_MAX] = {};
+        slot_px[0] = px;
         // the lane starts from what it holds (its previous scene stays until
         // the upload replaces it), then may reach the estimate
-        const uint64_t need = std::max(lane.reservation, reservation_for(lane, job, scene));
+        const uint64_t est = alone ? reservation_for(lane, job, scene)
+                                   : group_reservation(lane, job.scene, scene, px * pass_spp(job.params, px), slot_px);
+        const uint64_t need = std::max(lane.reservation, est);
         const uint64_t others = S.reserved_total() - lane.reservation;
         if (others + need > S.cfg.budget_bytes) {
             // with every other lane at its floor (fixed buffers only) the job
@@ -273,41 +495,47 @@ void lane_main(Service* Sp, Lane* lp) {
         Job* jp = &job;
         lk.unlock();
 
-        // the context's peak counter restarts at each query: the second
-        // query gives this job's peak
+        // the context's peak counter restarts at each query: the next query
+        // gives the peak of this job (or group)
         dl_hip_memory_usage(lane.ctx, nullptr, nullptr);
-        dl_render_stats total{};
         uint64_t kept_bytes = 0;
-        std::string e = run_job(S, lane, *jp, scene, switch_, &kept_bytes, total);
-        uint64_t cur = 0, peak = 0;
-        dl_hip_memory_usage(lane.ctx, &cur, &peak);
-
-        lk.lock();
-        const bool ok = e.empty() || e == "cancelled";
-        if (switch_ == 2 && ok) lane.kept[jp->scene] = kept_bytes;
-        if (!ok) {
-            // the context may be left between scenes: start over from nothing
-            lk.unlock();
-            dl_hip_trim(lane.ctx);
-            dl_hip_memory_usage(lane.ctx, &cur, nullptr);
+        std::string e = prepare_scene(S, lane, *jp, scene, switch_, &kept_bytes);
+        if (alone) {
+            dl_render_stats total{};
+            if (e.empty()) e = run_job(S, lane, *jp, total);
+            uint64_t cur = 0, peak = 0;
+            dl_hip_memory_usage(lane.ctx, &cur, &peak);
             lk.lock();
-            lane.kept.clear();
-            lane.state_paths = lane.state_pixels = 0;
+            const bool ok = e.empty() || e == "cancelled";
+            if (switch_ == 2 && ok) lane.kept[jp->scene] = kept_bytes;
+            if (!ok) {
+                // the context may be left between scenes: start over from nothing
+                lk.unlock();
+                dl_hip_trim(lane.ctx);
+                dl_hip_memory_usage(lane.ctx, &cur, nullptr);
+                lk.lock();
+                lane.kept.clear();
+                lane.state_paths = lane.state_pixels = 0;
+                for (auto& v : lane.slot_pixels) v = 0;
+            }
+            lane.resident_scene = ok ? jp->scene : 0;
+            jp->info.paths = total.paths;
+            jp->info.ms_gpu = total.ms_total;
+            jp->info.peak_bytes = peak;
+            if (peak > need) S.stats.estimate_misses++;
+            S.stats.paths += total.paths;
+            if (e.empty())
+                finish(S, *jp, DL_JOB_DONE, "");
+            else if (e == "cancelled")
+                finish(S, *jp, DL_JOB_CANCELLED, "");
+            else
+                finish(S, *jp, DL_JOB_FAILED, e);
+            lane.reservation = cur;
+            S.cv.notify_all();
+            continue;
         }
-        lane.resident_scene = ok ? jp->scene : 0;
-        jp->info.paths = total.paths;
-        jp->info.ms_gpu = total.ms_total;
-        jp->info.peak_bytes = peak;
-        if (peak > need) S.stats.estimate_misses++;
-        S.stats.paths += total.paths;
-        if (e.empty())
-            finish(S, *jp, DL_JOB_DONE, "");
-        else if (e == "cancelled")
-            finish(S, *jp, DL_JOB_CANCELLED, "");
-        else
-            finish(S, *jp, DL_JOB_FAILED, e);
-        lane.reservation = cur;
-        S.cv.notify_all();
+        run_group(S, lane, jp, scene, e, switch_, kept_bytes);
+        lk.lock();
     }
 }
 

@@ -278,6 +278,35 @@ dl_status dl_hip_memory_usage(void* ctx, uint64_t* current_bytes, uint64_t* peak
    (src/optix/host/optix_scene.cpp:43-52). */
 dl_status dl_hip_set_camera(void* ctx, const dl_camera_desc* camera);
 
+/* ---- shared passes -------------------------------------------------------
+   One image of a pass that several images of the current scene share
+   (dl_hip_render_batch): its camera, the samples it adds in this pass and
+   the film slot that accumulates them.  Each image's samples and film are
+   exactly those of a dl_hip_render of it with the same pass sizes, so its
+   result is the same bit for bit. */
+enum { DL_BATCH_MAX = 32 };
+typedef struct dl_batch_item {
+    dl_camera_desc camera;     /* the image's camera and size */
+    int32_t spp;               /* samples per pixel added in this pass */
+    int32_t first_sample;      /* index of the first of them */
+    uint32_t seed;             /* stream key, as dl_render_params.seed */
+    int32_t film;              /* film slot, 0 .. DL_BATCH_MAX - 1 (distinct within a pass) */
+    int32_t accumulate;        /* nonzero: add to the slot's film (same image size); zero: start it empty */
+    float* rgb_out;            /* host buffer of width * height * 3: the resolved image after this pass, or NULL */
+} dl_batch_item;
+/* Renders one wavefront pass holding the paths of n images (1..DL_BATCH_MAX)
+   of the current scene: the paths of every image share the pass's queues
+   and kernels (camera rays, traversal, material-sorted shading, shadow
+   rays), then each image's film gathers its own samples.  `shared` gives
+   max_depth, rr_depth and flags (DL_RENDER_COUNT_TRAVERSAL and
+   DL_RENDER_NORMALS; DL_RENDER_GUIDED is rejected); its other fields are
+   ignored.  At most 2^27 paths (sum of width * height * spp).  The film
+   slots belong to the context and persist across calls and scene changes
+   until dl_hip_trim.  (No reference counterpart: the render service packs
+   jobs into shared passes with it.) */
+dl_status dl_hip_render_batch(void* ctx, const dl_render_params* shared, const dl_batch_item* items, uint32_t n,
+                              dl_render_stats* stats);
+
 /* Viewer settings changed between renders without re-uploading the scene,
    as the OptiX renderer's context holds them (include/tracer/backend/
    context.h:12-14, applied by OptixRenderer::updateOptixScene,
@@ -387,6 +416,8 @@ typedef struct dl_service_stats {
     uint64_t scene_uploads;        /* uploads + BVH builds performed */
     uint64_t scene_selects;        /* jobs that switched a lane to a scene it kept resident */
     double paths;
+    uint64_t batched_jobs;         /* jobs that joined a running group of shared passes */
+    uint64_t shared_passes;        /* passes that held more than one job */
 } dl_service_stats;
 
 dl_status dl_hip_service_start(const dl_service_config* cfg, void** svc);

@@ -1,0 +1,138 @@
+"""Shared wavefront passes (dl_hip_render_batch): several images of one
+scene -- different cameras, sizes, sample counts and seeds -- travel through
+one set of queues and kernels, and each image equals its standalone
+dl_hip_render with the same pass sizes bit for bit.  The render service
+packs queued jobs of a scene into such passes (continuous batching) and its
+images stay bit-identical to standalone renders."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import drawlab_amd
+from drawlab_amd import scenes
+from drawlab_amd.scene import Camera
+
+pytestmark = pytest.mark.gpu
+
+
+def _cams():
+    base = scenes.cornell_box(64, 48).camera
+    return [
+        base,
+        dataclasses.replace(base, width=40, height=40, eye=(0.3, 0.3, -0.7)),
+        scenes.opencv_camera((0.278, 0.273, -0.8), (0.278, 0.273, 0.0), (0, 1, 0), 39.3, 32, 24),
+        Camera(type="perspective", width=48, height=32, fov=45.0,
+               to_world=np.array([[-1, 0, 0, 0.278], [0, 1, 0, 0.273], [0, 0, 1, -0.8], [0, 0, 0, 1]], np.float32)),
+    ]
+
+
+def _standalone(backend, desc, cam, spp, seed, k, normals=False):
+    backend.set_camera(cam)
+    img, st = backend.render(spp=spp, seed=seed, spp_per_pass=k, normals=normals)
+    return img, st
+
+
+def test_shared_pass_images_equal_standalone_renders(backend):
+    d = scenes.cornell_box(64, 48, spp=4, max_depth=6, rr_depth=3)
+    backend.upload_scene(d)
+    backend.build_accel(drawlab_amd.BUILD_GPU)
+    cams = _cams()
+    spps, seeds = [4, 7, 2, 5], [3, 11, 0, 9]
+    items = [{"camera": c, "spp": s, "seed": e, "film": i} for i, (c, s, e) in enumerate(zip(cams, spps, seeds))]
+    imgs, st = backend.render_batch(items)
+    print("batch", st["paths"], st["ms_total"], flush=True)
+    assert st["paths"] == sum(c.width * c.height * s for c, s in zip(cams, spps))
+    for c, s, e, img in zip(cams, spps, seeds, imgs):
+        ref, _ = _standalone(backend, d, c, s, e, s)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), (c.type, c.width, s, e)
+        assert img.mean() > 0.01
+
+
+def test_shared_passes_accumulate_like_render_passes(backend):
+    """Two passes of 4 samples into a film slot equal render(spp=8,
+    spp_per_pass=4); the other slot's image in the same passes is untouched
+    by it."""
+    d = scenes.mis_plates(64, 36, max_depth=5)
+    backend.upload_scene(d)
+    backend.build_accel(drawlab_amd.BUILD_GPU)
+    a, b = d.camera, dataclasses.replace(d.camera, width=32, height=18)
+    backend.render_batch([{"camera": a, "spp": 4, "seed": 2, "film": 5, "image": False},
+                          {"camera": b, "spp": 3, "seed": 1, "film": 0}])
+    imgs, _ = backend.render_batch([{"camera": a, "spp": 4, "first_sample": 4, "seed": 2, "film": 5,
+                                     "accumulate": True},
+                                    {"camera": b, "spp": 6, "seed": 8, "film": 1}])
+    ref, _ = _standalone(backend, d, a, 8, 2, 4)
+    assert np.array_equal(imgs[0].view(np.uint32), ref.view(np.uint32))
+    ref_b, _ = _standalone(backend, d, b, 6, 8, 6)
+    assert np.array_equal(imgs[1].view(np.uint32), ref_b.view(np.uint32))
+
+
+def test_shared_pass_normals_and_hidden_lights(backend):
+    d = scenes.cornell_box(48, 48, spp=4, max_depth=5, rr_depth=3)
+    backend.upload_scene(d)
+    backend.build_accel(drawlab_amd.BUILD_GPU)
+    cams = _cams()[:2]
+    imgs, st = backend.render_batch([{"camera": c, "spp": 2, "seed": i, "film": i} for i, c in enumerate(cams)],
+                                    normals=True)
+    assert st["shadow_rays"] == 0
+    for i, (c, img) in enumerate(zip(cams, imgs)):
+        ref, _ = _standalone(backend, d, c, 2, i, 2, normals=True)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+    backend.set_view_options(hide_lights=True)
+    imgs, _ = backend.render_batch([{"camera": c, "spp": 3, "seed": i, "film": i} for i, c in enumerate(cams)])
+    for i, (c, img) in enumerate(zip(cams, imgs)):
+        ref, _ = _standalone(backend, d, c, 3, i, 3)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+    backend.reset_view_options()
+
+
+def test_shared_pass_errors(backend):
+    d = scenes.cornell_box(32, 32, spp=4, max_depth=5, rr_depth=3)
+    backend.upload_scene(d)
+    backend.build_accel(drawlab_amd.BUILD_GPU)
+    c = d.camera
+    with pytest.raises(drawlab_amd.DlError, match="share a film slot"):
+        backend.render_batch([{"camera": c, "spp": 1, "film": 3}, {"camera": c, "spp": 1, "film": 3}])
+    backend.render_batch([{"camera": c, "spp": 1, "film": 2}])
+    with pytest.raises(drawlab_amd.DlError, match="another size"):
+        backend.render_batch([{"camera": dataclasses.replace(c, width=16), "spp": 1, "film": 2,
+                               "accumulate": True}])
+    with pytest.raises(drawlab_amd.DlError, match="film slot out of range"):
+        backend.render_batch([{"camera": c, "spp": 1, "film": drawlab_amd.BATCH_MAX}])
+    with pytest.raises(drawlab_amd.DlError, match="pass too large"):
+        backend.render_batch([{"camera": dataclasses.replace(c, width=4096, height=4096), "spp": 9, "film": 0}])
+
+
+def test_service_packs_jobs_into_shared_passes(backend):
+    """Many small jobs of one scene on one lane: they join the running
+    group (stats), and every image is bit-identical to its standalone
+    render, including jobs of several passes and a second scene's jobs."""
+    cbox = scenes.cornell_box(48, 48, spp=8, max_depth=6, rr_depth=3)
+    plates = scenes.mis_plates(48, 27, max_depth=5)
+    jobs = [(cbox, 4 + i % 5, i, 0, 48 - 4 * (i % 3)) for i in range(10)]
+    jobs += [(plates, 6, 50, 2, 48), (cbox, 9, 60, 4, 40), (plates, 3, 70, 0, 32)]
+    svc = drawlab_amd.Service(lanes=1, policy=drawlab_amd.POLICY_FIFO)
+    try:
+        sid = {id(cbox): svc.add_scene(cbox), id(plates): svc.add_scene(plates)}
+        ids = []
+        for d, spp, seed, k, w in jobs:
+            cam = dataclasses.replace(d.camera, width=w, height=d.camera.height * w // 48)
+            ids.append(svc.submit(sid[id(d)], spp, seed=seed, max_depth=d.max_depth, rr_depth=d.rr_depth,
+                                  spp_per_pass=k, camera=cam))
+        infos = [svc.wait(j) for j in ids]
+        assert all(i["state"] == drawlab_amd.JOB_DONE for i in infos), infos
+        imgs = [svc.result(j) for j in ids]
+        st = svc.stats()
+        print("stats", st, flush=True)
+        assert st["batched_jobs"] >= 5 and st["shared_passes"] >= 1
+        assert st["estimate_misses"] == 0 and st["peak_reserved_bytes"] <= st["budget_bytes"]
+    finally:
+        svc.stop()
+    for (d, spp, seed, k, w), img, info in zip(jobs, imgs, infos):
+        cam = dataclasses.replace(d.camera, width=w, height=d.camera.height * w // 48)
+        backend.upload_scene(dataclasses.replace(d, camera=cam))
+        backend.build_accel(drawlab_amd.BUILD_GPU)
+        ref, rst = backend.render(spp=spp, seed=seed, spp_per_pass=k)
+        assert info["paths"] == rst["paths"]
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), (spp, seed, k, w)
