@@ -239,7 +239,8 @@ def service_bench(args):
     results = {}
 
     def run(lanes, pol, arrivals):
-        svc = drawlab_amd.Service(lanes=lanes, policy=pol)
+        # one lane renders one job at a time (no shared passes)
+        svc = drawlab_amd.Service(lanes=lanes, policy=pol, max_batch=1 if lanes == 1 else 0)
         try:
             sids = [svc.add_scene(d) for d in pool]
             lat, paths, span = _run_jobs(svc, sids, pool, jobs, arrivals)
@@ -327,7 +328,9 @@ def service_bench(args):
                                    "one_at_a_time": round(paths / span1 / 1e6, 1)},
         "service_stats": {"peak_reserved_bytes": int(stp["peak_reserved_bytes"]),
                           "estimate_misses": int(stp["estimate_misses"] + stn["estimate_misses"]),
-                          "scene_uploads": int(stp["scene_uploads"])},
+                          "scene_uploads": int(stp["scene_uploads"]),
+                          "batched_jobs": int(stp["batched_jobs"]), "shared_passes": int(stp["shared_passes"]),
+                          "batched_jobs_saturated": int(stn["batched_jobs"])},
         "roofline": None,
         "cpu_baseline": None,
     }

@@ -591,7 +591,7 @@ SERVICE_SYMBOLS = ("dl_hip_service_start", "dl_hip_service_add_scene", "dl_hip_s
 
 class dl_service_config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("lanes", ctypes.c_int32), ("policy", ctypes.c_int32),
-                ("builder", ctypes.c_int32), ("budget_bytes", ctypes.c_uint64)]
+                ("builder", ctypes.c_int32), ("budget_bytes", ctypes.c_uint64), ("max_batch", ctypes.c_int32)]
 
 
 class dl_job_info(ctypes.Structure):
@@ -644,9 +644,10 @@ class Service:
     """Many render jobs on one device: scenes registered once, jobs queued
     and run on `lanes` worker contexts under a device-memory budget."""
 
-    def __init__(self, device=0, lanes=4, policy=POLICY_FIFO, budget_bytes=200 << 30, builder=BUILD_GPU):
+    def __init__(self, device=0, lanes=4, policy=POLICY_FIFO, budget_bytes=200 << 30, builder=BUILD_GPU, max_batch=0):
         self.L = _service_lib()
-        cfg = dl_service_config(int(device), int(lanes), int(policy), int(builder), int(budget_bytes))
+        cfg = dl_service_config(int(device), int(lanes), int(policy), int(builder), int(budget_bytes),
+                                int(max_batch))
         h = ctypes.c_void_p()
         st = self.L.dl_hip_service_start(ctypes.byref(cfg), ctypes.byref(h))
         if st != 0:

@@ -308,7 +308,9 @@ void run_group(Service& S, Lane& lane, Job* lead, const SceneEntry& scene, std::
             slot_used[g.slot] = true;
             slot_px[g.slot] = px;
         }
-        for (auto qi = S.queue.begin(); qi != S.queue.end() && group.size() < (size_t)DL_BATCH_MAX;) {
+        const size_t max_group = S.cfg.max_batch > 0 ? (size_t)std::min(S.cfg.max_batch, (int)DL_BATCH_MAX)
+                                                     : (size_t)DL_BATCH_MAX;
+        for (auto qi = S.queue.begin(); qi != S.queue.end() && group.size() < max_group;) {
             Job& cand = *S.jobs[*qi];
             const uint64_t px = job_pixels(cand);
             const uint32_t k = pass_spp(cand.params, px);
@@ -559,6 +561,7 @@ dl_status dl_hip_service_start(const dl_service_config* cfg, void** svc) {
     if (cfg->lanes < 1 || cfg->lanes > 16) return err(nullptr, DL_ERR_INVALID, "lanes must be in [1, 16]");
     if (cfg->policy != DL_POLICY_FIFO && cfg->policy != DL_POLICY_SJF)
         return err(nullptr, DL_ERR_INVALID, "unknown scheduling policy");
+    if (cfg->max_batch < 0) return err(nullptr, DL_ERR_INVALID, "max_batch must be >= 0");
     std::unique_ptr<Service> S(new Service);
     S->cfg = *cfg;
     S->stats.budget_bytes = cfg->budget_bytes;

@@ -396,6 +396,7 @@ typedef struct dl_service_config {
     int32_t policy;            /* DL_POLICY_* */
     int32_t builder;           /* DL_BUILD_* for scene uploads */
     uint64_t budget_bytes;     /* device memory all lanes together may reserve */
+    int32_t max_batch;         /* jobs a lane packs into shared passes: 0 = DL_BATCH_MAX, 1 = one at a time */
 } dl_service_config;
 
 typedef struct dl_job_info {
