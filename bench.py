@@ -383,7 +383,10 @@ def guided_bench(args):
     from drawlab_amd import scenes
 
     W, H = (int(v) for v in args.guided_res.lower().split("x"))
-    desc = scenes.displaced_sphere(width=W, height=H)
+    if args.guided_scene == "chimney_room":
+        desc = scenes.chimney_room(W, H, max_depth=6, rr_depth=3)
+    else:
+        desc = scenes.displaced_sphere(width=W, height=H)
     be = drawlab_amd.Backend(0)
     be.upload_scene(desc)
     be.build_accel(drawlab_amd.BUILD_GPU)
@@ -436,7 +439,8 @@ def guided_bench(args):
         "vs_baseline": None,
         "dtype": "f32 (network bf16 with f32 accumulation)",
         "data": "synthetic (seeded procedural scene generated in code)",
-        "config": {"workload": "guided_vs_plain_displaced_sphere_2M", "width": W, "height": H,
+        "config": {"workload": "guided_vs_plain_" + ("chimney_room" if args.guided_scene == "chimney_room"
+                                                     else "displaced_sphere_2M"), "width": W, "height": H,
                    "reference_spp": args.ref_spp, "reference_seconds": round(t_ref, 2), "target_relmse": target,
                    "seeds_per_point": args.seeds,
                    "guide_steps_per_pass": args.guide_steps or "16 for the first 256 steps, then 4"},
@@ -468,6 +472,8 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=1024, help="calibration copy size used by the PMC runs")
     ap.add_argument("--workload", choices=("displaced_sphere", "service", "guided"), default="displaced_sphere")
     ap.add_argument("--guided-res", default="480x270", help="guided workload: image size WxH")
+    ap.add_argument("--guided-scene", choices=("displaced_sphere", "chimney_room"), default="displaced_sphere",
+                    help="guided workload: the 2M-triangle scene, or a room lit through a small ceiling patch")
     ap.add_argument("--ref-spp", type=int, default=65536, help="guided workload: reference samples per pixel")
     ap.add_argument("--max-spp", type=int, default=4096, help="guided workload: largest ladder render")
     ap.add_argument("--target-relmse", type=float, default=1e-3)

@@ -233,6 +233,39 @@ def opencv_camera(eye, lookat, up, fov, width, height):
                   cy=height / 2, fx=float(f), fy=float(f))
 
 
+def chimney_room(width=64, height=64, spp=64, max_depth=6, rr_depth=3):
+    """A closed diffuse room (unit cube, walls facing in) whose only light
+    sits at the bottom of an open-topped chimney in the middle of the floor:
+    it lights a small patch of the ceiling, and everything the camera sees is
+    lit indirectly from that patch -- a scene where directions worth
+    sampling are concentrated (used to show what the learned proposal
+    gains)."""
+    s = SceneDesc(name="chimney_room")
+    wall = s.add_bsdf(Bsdf.diffuse((0.75, 0.75, 0.75)))
+    dark = s.add_bsdf(Bsdf.diffuse((0.2, 0.2, 0.2)))
+    for p, e0, e1 in [((0, 0, 0), (0, 0, 1), (1, 0, 0)),    # floor, +y
+                      ((0, 1, 0), (1, 0, 0), (0, 0, 1)),    # ceiling, -y
+                      ((0, 0, 1), (0, 1, 0), (1, 0, 0)),    # back, -z
+                      ((0, 0, 0), (1, 0, 0), (0, 1, 0)),    # front, +z
+                      ((0, 0, 0), (0, 1, 0), (0, 0, 1)),    # x = 0, +x
+                      ((1, 0, 0), (0, 0, 1), (0, 1, 0))]:   # x = 1, -x
+        V, F, _ = quad(p, e0, e1)
+        s.add_mesh(V, F, wall, name="wall")
+    # chimney: four outward- and inward-facing plates around the light
+    lo, hi, top = 0.42, 0.58, 0.8
+    for p, e0, e1 in [((lo, 0, lo), (hi - lo, 0, 0), (0, top, 0)), ((lo, 0, hi), (0, top, 0), (hi - lo, 0, 0)),
+                      ((lo, 0, lo), (0, top, 0), (0, 0, hi - lo)), ((hi, 0, lo), (0, 0, hi - lo), (0, top, 0))]:
+        V, F, _ = quad(p, e0, e1)
+        s.add_mesh(np.concatenate([V, V]), np.concatenate([F, F[:, ::-1] + 4]), dark, name="chimney")
+    V, F, _ = quad((lo + 0.01, 0.05, lo + 0.01), (0, 0, hi - lo - 0.02), (hi - lo - 0.02, 0, 0))  # facing +y
+    s.add_mesh(V, F, dark, radiance=(3000.0, 2750.0, 2250.0), name="light")
+    s.camera = Camera(eye=(0.5, 0.5, 0.02), lookat=(0.5, 0.3, 1.0), up=(0, 1, 0), fov=70, width=width,
+                      height=height)
+    s.rfilter = RFilter(FILTER_GAUSSIAN, 2.0, 0.5)
+    s.spp, s.max_depth, s.rr_depth = spp, max_depth, rr_depth
+    return s
+
+
 def sky_envmap(width=1024, height=512, sun_dir=(0.5, 0.7071, 0.5), sun_peak=2e4, sun_sigma_deg=0.5):
     """Lat-long sky: gradient 0.1 (horizon and below) to 1.0 (zenith) plus a
     Gaussian sun lobe (peak sun_peak, sigma sun_sigma_deg)."""
