@@ -1287,7 +1287,8 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
             gv.inv[k] = c->bbox_inv[k];
         }
     }
-    const int guide_grid = std::max(1, c->num_cus) * 2;
+    // 53 KB of LDS per block (4 waves x 64 rows): three blocks fit a CU
+    const int guide_grid = std::max(1, c->num_cus) * 3;
     const int stage_grid = std::max(1, c->num_cus) * 8;
     const int trace_grid = (int)c->trace_blocks;
     std::vector<int> tags;
