@@ -51,7 +51,7 @@ sys.path.insert(0, ROOT)
 METRIC = "Mpaths/s + relMSE@equal-time (guided vs plain); p50/p99 job latency"
 WORKLOAD = "displaced_sphere_2M"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES = 128, 48, 32, 16  # 4-wide node, triangle, ray in, hit out
+NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES = 64, 48, 32, 16  # 4-wide node, triangle, ray in, hit out
 TRACE_KERNEL = "k_trace_queue<false>"
 COPY_KERNEL = "k_copy"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_queue_%s_traffic.json" % WORKLOAD)

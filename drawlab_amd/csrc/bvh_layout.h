@@ -15,12 +15,20 @@
 // rejects a box that holds a triangle Moller-Trumbore accepts, including
 // rays with zero direction components (no 0 * inf).
 //
-// 4-wide node (traversal), 128 bytes = 8 x 16-byte records, one 128-byte
-// line, child boxes as structure of arrays:
-//   q0 = lo.x of children 0..3, q1 = hi.x, q2 = lo.y, q3 = hi.y, q4 = lo.z,
-//   q5 = hi.z, q6 = ref of children 0..3, q7 = count of children 0..3
+// 4-wide node (traversal), 64 bytes = 4 x 16-byte records; child boxes on
+// a per-node grid of 8-bit steps (2^ex, 2^ey, 2^ez) from the node's
+// origin:
+//   r0 = (origin.x, origin.y, origin.z, exponent bits) -- exponent bits:
+//        ex | ey << 8 | ez << 16, each the biased binary32 exponent of the
+//        step, so the step is the float with those exponent bits
+//   r1 = (qlo.x, qhi.x, qlo.y, qhi.y), r2 = (qlo.z, qhi.z, count, 0)
+//        -- each word holds children 0..3 in bytes 0..3
+//   r3 = ref of children 0..3 (int32)
 // with ref / count as in the binary node; an unused slot is an empty leaf
-// (ref = ~0, count = 0).
+// (ref = ~0, count = 0).  A child's box is lo = origin + qlo * step,
+// hi = origin + qhi * step in binary32: q * step is exact, the sum rounds
+// once, and the quantiser chooses qlo, qhi by that same arithmetic so the
+// decoded box contains the child's (padded) float box.
 // Node 0 is the root; boxes are the binary nodes' padded boxes.
 //
 // Triangle: 48 bytes = 3 x 16-byte records, pre-gathered in leaf order:
@@ -35,7 +43,7 @@
 namespace dl {
 
 constexpr int kNodeBytes = 64;    // binary node (builders)
-constexpr int kNode4Bytes = 128;  // 4-wide node (traversal)
+constexpr int kNode4Bytes = 64;   // 4-wide node (traversal)
 constexpr int kTriBytes = 48;
 constexpr int kMaxLeafSize = 4;
 // Traversal stack: kLdsStack entries in LDS per lane (8 KB per 128-lane
@@ -56,8 +64,11 @@ struct BvhNodeH {
     int32_t r3[4];
 };
 struct Bvh4NodeH {
-    float q[6][4];
-    int32_t ref[4], cnt[4];
+    float origin[3];
+    uint32_t exps;
+    uint8_t qlo_x[4], qhi_x[4], qlo_y[4], qhi_y[4];
+    uint8_t qlo_z[4], qhi_z[4], cnt[4], pad[4];
+    int32_t ref[4];
 };
 struct BvhTriH {
     float t0[4], t1[4], t2[4];

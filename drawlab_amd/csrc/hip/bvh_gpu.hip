@@ -909,10 +909,93 @@ __global__ __launch_bounds__(kB) void k_c4_emit(const float4* __restrict__ bn, c
     }
 }
 
+// Quantises 4-wide nodes with float child boxes (8 float4 records: lo.x,
+// hi.x, lo.y, hi.y, lo.z, hi.z of children 0..3, refs, counts) into the
+// 64-byte traversal layout.  Per axis the origin is the least lo of the
+// used children and the step the smallest power of two whose 255 steps
+// span them; qlo / qhi are rounded outward and then checked with the
+// traversal's own decode (origin + q * step in binary32), stepping down /
+// up until the decoded box contains the float box (a wider step when 255
+// steps do not reach), so quantisation never shrinks a box.
+__device__ float q_decode(float o, uint32_t q, float s) { return o + (float)q * s; }
+
+__global__ __launch_bounds__(kB) void k_quantize4(const float4* __restrict__ in, float4* __restrict__ out, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4* np = in + 8 * (size_t)i;
+        float b[6][4];
+        for (int a = 0; a < 6; ++a) {
+            const float4 v = np[a];
+            b[a][0] = v.x;
+            b[a][1] = v.y;
+            b[a][2] = v.z;
+            b[a][3] = v.w;
+        }
+        const int4 ref = reinterpret_cast<const int4*>(np)[6];
+        const int4 cnt = reinterpret_cast<const int4*>(np)[7];
+        const int refs[4] = {ref.x, ref.y, ref.z, ref.w}, cnts[4] = {cnt.x, cnt.y, cnt.z, cnt.w};
+        bool used[4];
+        for (int j = 0; j < 4; ++j) used[j] = !(refs[j] == ~0 && cnts[j] == 0);
+        float origin[3];
+        uint32_t ebits[3], qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
+        for (int ax = 0; ax < 3; ++ax) {
+            float o = INFINITY, m = -INFINITY;
+            for (int j = 0; j < 4; ++j)
+                if (used[j]) {
+                    o = fminf(o, b[2 * ax][j]);
+                    m = fmaxf(m, b[2 * ax + 1][j]);
+                }
+            if (!(o <= m)) o = m = 0.0f;
+            origin[ax] = o;
+            // smallest exponent whose 255 steps cover m - o
+            const double ext = (double)m - (double)o;
+            int e = -126;
+            if (ext > 0.0) {
+                int ex;
+                (void)frexp(ext / 255.0, &ex);  // ext / 255 < 2^ex
+                e = max(-126, ex);
+            }
+            uint32_t lo4 = 0, hi4 = 0;
+            for (;; ++e) {
+                const float s = __uint_as_float((uint32_t)(e + 127) << 23);
+                bool fits = true;
+                lo4 = hi4 = 0;
+                for (int j = 0; j < 4; ++j) {
+                    uint32_t ql = 0, qh = 0;
+                    if (used[j]) {
+                        const float lo = b[2 * ax][j], hi = b[2 * ax + 1][j];
+                        double fl = floor(((double)lo - (double)o) / (double)s);
+                        double fh = ceil(((double)hi - (double)o) / (double)s);
+                        ql = (uint32_t)fmin(fmax(fl, 0.0), 255.0);
+                        qh = (uint32_t)fmin(fmax(fh, 0.0), 255.0);
+                        while (ql > 0 && q_decode(o, ql, s) > lo) --ql;
+                        while (qh < 255 && q_decode(o, qh, s) < hi) ++qh;
+                        if (q_decode(o, ql, s) > lo || q_decode(o, qh, s) < hi) fits = false;
+                    }
+                    lo4 |= ql << (8 * j);
+                    hi4 |= qh << (8 * j);
+                }
+                if (fits || e >= 127) break;
+            }
+            ebits[ax] = (uint32_t)(e + 127);
+            qlo[ax] = lo4;
+            qhi[ax] = hi4;
+        }
+        uint32_t c4 = 0;
+        for (int j = 0; j < 4; ++j) c4 |= ((uint32_t)cnts[j] & 0xffu) << (8 * j);
+        float4* dst = out + 4 * (size_t)i;
+        dst[0] = make_float4(origin[0], origin[1], origin[2], __uint_as_float(ebits[0] | ebits[1] << 8 | ebits[2] << 16));
+        reinterpret_cast<uint4*>(dst)[1] = make_uint4(qlo[0], qhi[0], qlo[1], qhi[1]);
+        reinterpret_cast<uint4*>(dst)[2] = make_uint4(qlo[2], qhi[2], c4, 0u);
+        reinterpret_cast<int4*>(dst)[3] = ref;
+    }
+}
+
+constexpr size_t kNode4FloatBytes = 128;  // 4-wide node with float child boxes (collapse output)
+
 }  // namespace
 
 // Replaces the context's binary nodes (num_binary internal nodes, root 0)
-// with 4-wide nodes; stats get the 4-wide node count and depth.
+// with quantised 4-wide nodes; stats get the 4-wide node count and depth.
 dl_status collapse_bvh4(Ctx* c, uint32_t num_binary, dl_accel_stats* stats) {
     const int cap = (int)std::max<uint32_t>(1, num_binary);
     Scratch sc{c};
@@ -928,7 +1011,7 @@ dl_status collapse_bvh4(Ctx* c, uint32_t num_binary, dl_accel_stats* stats) {
     void* temp = nullptr;
     DL_TRY(sc.get((char**)&temp, std::max<size_t>(1, scan_bytes)));
     void* p4 = nullptr;
-    DL_HIP_CHECK(c, dmalloc(c, &p4, (size_t)cap * kNode4Bytes));
+    DL_HIP_CHECK(c, dmalloc(c, &p4, (size_t)cap * kNode4FloatBytes));
     int* host_cnt = nullptr;
     DL_HIP_CHECK(c, hipHostMalloc((void**)&host_cnt, sizeof(int), hipHostMallocDefault));
     struct HostFree {
@@ -957,7 +1040,17 @@ dl_status collapse_bvh4(Ctx* c, uint32_t num_binary, dl_accel_stats* stats) {
         cur ^= 1;
     }
     dfree(c, c->ds.nodes);
-    c->ds.nodes = (float4*)p4;
+    c->ds.nodes = nullptr;
+    void* pq = nullptr;
+    if (dmalloc(c, &pq, (size_t)next * kNode4Bytes) != hipSuccess) {
+        dfree(c, p4);
+        return set_error(c, DL_ERR_OOM, "out of device memory for the 4-wide BVH");
+    }
+    k_quantize4<<<grid, kB, 0, c->stream>>>((const float4*)p4, (float4*)pq, next);
+    DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    DL_HIP_CHECK(c, hipGetLastError());
+    dfree(c, p4);
+    c->ds.nodes = (float4*)pq;
     stats->num_nodes = (uint32_t)next;
     stats->max_depth = (uint32_t)levels;
     return DL_OK;

@@ -87,6 +87,17 @@ __device__ __forceinline__ void cswap(float& ka, int& ra, float& kb, int& rb) {
 // tests (mesh.cpp:195, 204, 213) are combined into the returned predicate,
 // which accepts exactly the rays the early-out sequence accepts.  The triangle
 // is therefore fetched by three loads issued together.
+// Slab test of the child whose quantised planes sit at bit `sh` of the
+// node's records r1, r2: each plane is origin + q * step (q * step is
+// exact, the sum rounds once).
+__device__ __forceinline__ bool slab_q(const float4& r0, const uint4& r1, const uint4& r2, int sh, float sx, float sy,
+                                       float sz, const vec3& o, const vec3& rcp, float tmin, float tmax, float& n) {
+    return slab1(r0.x + (float)((r1.x >> sh) & 0xffu) * sx, r0.x + (float)((r1.y >> sh) & 0xffu) * sx,
+                 r0.y + (float)((r1.z >> sh) & 0xffu) * sy, r0.y + (float)((r1.w >> sh) & 0xffu) * sy,
+                 r0.z + (float)((r2.x >> sh) & 0xffu) * sz, r0.z + (float)((r2.y >> sh) & 0xffu) * sz, o, rcp, tmin,
+                 tmax, n);
+}
+
 __device__ __forceinline__ bool mt_test(const float4& a, const float4& b, const float4& c, const vec3& o,
                                         const vec3& d, float& u, float& v, float& t) {
     const vec3 p0 = v3(a.x, a.y, a.z);
@@ -176,16 +187,23 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
         if (!__ballot(active)) break;
         if (!active) continue;
 
-        const float4* np = bvh.nodes + 8 * (size_t)cur;
-        const float4 qx = np[0], qX = np[1], qy = np[2], qY = np[3], qz = np[4], qZ = np[5];
-        const int4 ref = reinterpret_cast<const int4*>(np)[6];
-        const int4 cnt4 = reinterpret_cast<const int4*>(np)[7];
+        const float4* np = bvh.nodes + 4 * (size_t)cur;
+        const float4 r0 = np[0];
+        const uint4 r1 = reinterpret_cast<const uint4*>(np)[1];
+        const uint4 r2 = reinterpret_cast<const uint4*>(np)[2];
+        const int4 ref = reinterpret_cast<const int4*>(np)[3];
         if (COUNT) cnt.nodes++;
+        // child boxes: origin + q * step (q * step exact, one rounding)
+        const uint32_t eb = __float_as_uint(r0.w);
+        const float sx = __uint_as_float((eb & 0xffu) << 23), sy = __uint_as_float(((eb >> 8) & 0xffu) << 23),
+                    sz = __uint_as_float(((eb >> 16) & 0xffu) << 23);
         float n0, n1, n2, n3;
-        const bool h0 = slab1(qx.x, qX.x, qy.x, qY.x, qz.x, qZ.x, o, rcp, tmin, best_t, n0);
-        const bool h1 = slab1(qx.y, qX.y, qy.y, qY.y, qz.y, qZ.y, o, rcp, tmin, best_t, n1);
-        const bool h2 = slab1(qx.z, qX.z, qy.z, qY.z, qz.z, qZ.z, o, rcp, tmin, best_t, n2);
-        const bool h3 = slab1(qx.w, qX.w, qy.w, qY.w, qz.w, qZ.w, o, rcp, tmin, best_t, n3);
+        const bool h0 = slab_q(r0, r1, r2, 0, sx, sy, sz, o, rcp, tmin, best_t, n0);
+        const bool h1 = slab_q(r0, r1, r2, 8, sx, sy, sz, o, rcp, tmin, best_t, n1);
+        const bool h2 = slab_q(r0, r1, r2, 16, sx, sy, sz, o, rcp, tmin, best_t, n2);
+        const bool h3 = slab_q(r0, r1, r2, 24, sx, sy, sz, o, rcp, tmin, best_t, n3);
+        const int4 cnt4 = make_int4((int)(r2.z & 0xffu), (int)((r2.z >> 8) & 0xffu), (int)((r2.z >> 16) & 0xffu),
+                                    (int)(r2.z >> 24));
         bool hit_any = false;
         // Triangles of all hit leaf children in one loop (the closest-hit
         // result is the minimum over (t, id) and so does not depend on the

@@ -8,9 +8,10 @@
 // compared with the in-kernel counters, together with the hits themselves.
 //
 // Layout read here (all little-endian 32-bit words):
-//   node (32 words): q0..q5 = lo.x, hi.x, lo.y, hi.y, lo.z, hi.z of children
-//     0..3 (4 floats each), ref[4] (int: >= 0 child node, < 0 leaf ~first
-//     triangle), cnt[4] (leaf triangle count)
+//   node (16 words): origin x, y, z (floats), step exponent bits (ex | ey << 8
+//     | ez << 16); qlo.x, qhi.x, qlo.y, qhi.y; qlo.z, qhi.z, count, 0 (each
+//     word: children 0..3 in bytes 0..3); ref[4] (int: >= 0 child node, < 0
+//     leaf ~first triangle).  Child plane = origin + q * step in binary32.
 //   triangle (12 words): p0.xyz, e1.x | e1.yz, e2.xy | e2.z, id (bits), 0, 0
 //   rays: 8 planes of n floats (ox, oy, oz, dx, dy, dz, tmin, tmax)
 // Arithmetic: binary32, no contraction (built with -ffp-contract=off),
@@ -99,16 +100,26 @@ void dlo_bvh4_replay(const float* nodes, const float* tris, const float* rays, u
             stack.clear();
             int cur = 0;
             for (;;) {
-                const float* np = nodes + 32 * (size_t)cur;
+                const float* np = nodes + 16 * (size_t)cur;
+                uint32_t w[16];
+                std::memcpy(w, np, sizeof(w));
                 int ref[4], cnt[4];
-                std::memcpy(ref, np + 24, sizeof(ref));
-                std::memcpy(cnt, np + 28, sizeof(cnt));
+                std::memcpy(ref, np + 12, sizeof(ref));
+                for (int c = 0; c < 4; c++) cnt[c] = (int)((w[10] >> (8 * c)) & 0xffu);
                 nv++;
+                float step[3];
+                for (int a = 0; a < 3; a++) {
+                    const uint32_t bits = ((w[3] >> (8 * a)) & 0xffu) << 23;
+                    std::memcpy(&step[a], &bits, 4);
+                }
+                auto plane = [&](int axis, uint32_t word, int c) {
+                    return np[axis] + (float)((word >> (8 * c)) & 0xffu) * step[axis];
+                };
                 float nn[4];
                 bool h[4];
                 for (int c = 0; c < 4; c++)
-                    h[c] = slab1(np[c], np[4 + c], np[8 + c], np[12 + c], np[16 + c], np[20 + c], o, rcp, tmin, best_t,
-                                 nn[c]);
+                    h[c] = slab1(plane(0, w[4], c), plane(0, w[5], c), plane(1, w[6], c), plane(1, w[7], c),
+                                 plane(2, w[8], c), plane(2, w[9], c), o, rcp, tmin, best_t, nn[c]);
                 // triangles of the hit leaf children, slot order
                 hit_any = false;
                 for (int c = 0; c < 4 && !(any && hit_any); c++) {

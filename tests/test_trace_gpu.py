@@ -78,7 +78,7 @@ def test_traversal_counts_match_cpu_replay(backend, builder):
     backend.upload_scene(desc)
     st = backend.build_accel(builder)
     nodes, tris = backend.accel_export()
-    assert nodes.size == st["num_nodes"] * 32 and tris.size == st["num_prims"] * 12
+    assert nodes.size == st["num_nodes"] * 16 and tris.size == st["num_prims"] * 12
     osc = oracle.from_desc(desc)
     batches = ray_batches(desc, osc, 3000, seed=7)
     nan_rays = np.array(batches["random"][:, :64])
