@@ -491,7 +491,7 @@ def main():
         return
     if args.summarize_counters:
         summarize_counters(args.summarize_counters[:-1], args.summarize_counters[-1], args.command,
-                           ["k_trace_queue<false>", "k_shadow_queue", "k_post", "k_shade<0, false>",
+                           ["k_trace_queue<false>", "k_shadow_queue", "k_post<false>", "k_shade<0, false>",
                             "k_shade<5, false>", "k_film_tiled"])
         return
     if args.summarize_pmc:
