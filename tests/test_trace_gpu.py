@@ -102,3 +102,60 @@ def test_traversal_counts_match_cpu_replay(backend, builder):
             assert not r[4].any()
     print("replayed node visits", total, flush=True)
     assert total > 0
+
+
+@pytest.mark.parametrize("builder", [drawlab_amd.BUILD_GPU, drawlab_amd.BUILD_HOST_SAH, drawlab_amd.BUILD_GPU_LBVH],
+                         ids=["gpu", "host_sah", "lbvh"])
+def test_quantised_boxes_contain_their_triangles(backend, builder):
+    """Every decoded child box of the exported 4-wide BVH (origin + q * step
+    in binary32, as the traversal decodes it) contains the vertices of every
+    triangle below it; the boxes are not degenerate padding of the whole
+    scene (the mean leaf box is far smaller than the scene)."""
+    desc = scenes.displaced_sphere(width=32, height=18, grid=64, glass_level=2, env_size=(32, 16))
+    backend.upload_scene(desc)
+    st = backend.build_accel(builder)
+    nodes, tris = backend.accel_export()
+    N = st["num_nodes"]
+    w = nodes.view(np.uint32).reshape(N, 16)
+    origin = nodes.reshape(N, 16)[:, :3]
+    step = ((w[:, 3:4] >> np.array([0, 8, 16], np.uint32)) & 0xFF) << 23
+    step = step.astype(np.uint32).view(np.float32)                          # (N, 3)
+    q = ((w[:, 4:10, None] >> (8 * np.arange(4, dtype=np.uint32))) & 0xFF).astype(np.float32)  # (N, 6, 4)
+    planes = np.empty((N, 6, 4), np.float32)
+    for k in range(6):
+        ax = k // 2
+        planes[:, k] = (origin[:, ax:ax + 1] + q[:, k] * step[:, ax:ax + 1]).astype(np.float32)
+    cnt = ((w[:, 10, None] >> (8 * np.arange(4, dtype=np.uint32))) & 0xFF).astype(np.int64)
+    ref = nodes.view(np.int32).reshape(N, 16)[:, 12:16].astype(np.int64)
+    t = tris.reshape(-1, 12)
+    p0 = t[:, 0:3]
+    e1 = np.stack([t[:, 3], t[:, 4], t[:, 5]], 1)
+    e2 = np.stack([t[:, 6], t[:, 7], t[:, 8]], 1)
+    verts = np.stack([p0, (p0 + e1).astype(np.float32), (p0 + e2).astype(np.float32)], 1)   # (T, 3, 3)
+    tlo, thi = verts.min(axis=1), verts.max(axis=1)
+    # subtree bounds, children after parents (breadth-first order): fold from the end
+    lo = np.full((N, 3), np.inf, np.float32)
+    hi = np.full((N, 3), -np.inf, np.float32)
+    slot_lo = np.full((N, 4, 3), np.inf, np.float32)
+    slot_hi = np.full((N, 4, 3), -np.inf, np.float32)
+    for i in range(N - 1, -1, -1):
+        for c in range(4):
+            r = ref[i, c]
+            if r >= 0:
+                assert r > i
+                slot_lo[i, c], slot_hi[i, c] = lo[r], hi[r]
+            elif cnt[i, c] > 0:
+                f = ~r
+                slot_lo[i, c] = tlo[f:f + cnt[i, c]].min(axis=0)
+                slot_hi[i, c] = thi[f:f + cnt[i, c]].max(axis=0)
+        lo[i] = slot_lo[i].min(axis=0)
+        hi[i] = slot_hi[i].max(axis=0)
+    used = (ref >= 0) | (cnt > 0)
+    box_lo = np.stack([planes[:, 0], planes[:, 2], planes[:, 4]], -1)        # (N, 4, 3)
+    box_hi = np.stack([planes[:, 1], planes[:, 3], planes[:, 5]], -1)
+    assert np.all(box_lo[used] <= slot_lo[used]) and np.all(box_hi[used] >= slot_hi[used])
+    leaf = (ref < 0) & (cnt > 0)
+    ext = (box_hi - box_lo)[leaf].max(axis=-1)
+    scene = float((hi[0] - lo[0]).max())
+    print("leaf box extent mean / scene", float(ext.mean()) / scene, flush=True)
+    assert ext.mean() < 0.05 * scene
