@@ -63,7 +63,7 @@ int main(int argc, char** argv) {
             usage();
             return 0;
         } else if (a == "--gui") {
-            std::fprintf(stderr, "Fatal error: --gui is not supported by this build\n");
+            std::fprintf(stderr, "Fatal error: --gui is not supported by the hip renderer\n");
             return 1;
         } else {
             std::fprintf(stderr, "unknown option: %s\n", a.c_str());
@@ -76,7 +76,7 @@ int main(int argc, char** argv) {
         return 1;
     }
     if (backend != "hip") {
-        std::fprintf(stderr, "Fatal error: unknown backend:  %s (this build renders with 'hip')\n", backend.c_str());
+        std::fprintf(stderr, "Fatal error: unknown backend:  %s (the renderer here is 'hip')\n", backend.c_str());
         return 1;
     }
     const int b = builder == "gpu" ? DL_BUILD_GPU : builder == "host_sah" ? DL_BUILD_HOST_SAH
