@@ -933,6 +933,8 @@ __global__ __launch_bounds__(kFilmTile* kFilmTile) void k_film_tiled(PassParams 
     __shared__ float4 s_rad[kFilmSpan * kFilmSpan];
     __shared__ float4 s_rel[kFilmSpan * kFilmSpan];  // plx, ply, block origin x, y (int bits)
     __shared__ int4 s_cov[kFilmSpan * kFilmSpan];    // covered film pixels: x lo, x hi, y lo, y hi
+    __shared__ float s_table[33];                    // the filter table, indexed per lane
+    if (threadIdx.x < 33) s_table[threadIdx.x] = film.table[threadIdx.x];
     const uint32_t W = pp.W, H = pp.H, WH = W * H;
     // A sample at P in [px, px + 1] reaches the film pixels with
     // ceil(P - 1/2 - r) <= q <= floor(P - 1/2 + r), i.e. within
@@ -978,8 +980,8 @@ __global__ __launch_bounds__(kFilmTile* kFilmTile) void k_film_tiled(PassParams 
                     const float4 L = s_rad[e];
                     const float4 rel = s_rel[e];
                     const int xl = qx - __float_as_int(rel.z), yl = qy - __float_as_int(rel.w);
-                    const float wx = film.table[(int)(fabsf((float)xl - rel.x) * film.lookup)];
-                    const float wy = film.table[(int)(fabsf((float)yl - rel.y) * film.lookup)];
+                    const float wx = s_table[(int)(fabsf((float)xl - rel.x) * film.lookup)];
+                    const float wy = s_table[(int)(fabsf((float)yl - rel.y) * film.lookup)];
                     acc.x += (L.x * wx) * wy;
                     acc.y += (L.y * wx) * wy;
                     acc.z += (L.z * wx) * wy;
