@@ -88,13 +88,17 @@ __device__ __forceinline__ void cswap(float& ka, int& ra, float& kb, int& rb) {
 // which accepts exactly the rays the early-out sequence accepts.  The triangle
 // is therefore fetched by three loads issued together.
 // Slab test of the child whose quantised planes sit at bit `sh` of the
-// node's records r1, r2: each plane is origin + q * step (q * step is
-// exact, the sum rounds once).
+// node's records r1, r2: each plane is origin + q * step.  q * step is
+// exact (an 8-bit integer times a power of two), so the fused
+// multiply-add rounds once, exactly like the separate product and sum the
+// quantiser checked.
+__device__ __forceinline__ float qplane(uint32_t word, int sh, float step, float origin) {
+    return __builtin_fmaf((float)((word >> sh) & 0xffu), step, origin);
+}
 __device__ __forceinline__ bool slab_q(const float4& r0, const uint4& r1, const uint4& r2, int sh, float sx, float sy,
                                        float sz, const vec3& o, const vec3& rcp, float tmin, float tmax, float& n) {
-    return slab1(r0.x + (float)((r1.x >> sh) & 0xffu) * sx, r0.x + (float)((r1.y >> sh) & 0xffu) * sx,
-                 r0.y + (float)((r1.z >> sh) & 0xffu) * sy, r0.y + (float)((r1.w >> sh) & 0xffu) * sy,
-                 r0.z + (float)((r2.x >> sh) & 0xffu) * sz, r0.z + (float)((r2.y >> sh) & 0xffu) * sz, o, rcp, tmin,
+    return slab1(qplane(r1.x, sh, sx, r0.x), qplane(r1.y, sh, sx, r0.x), qplane(r1.z, sh, sy, r0.y),
+                 qplane(r1.w, sh, sy, r0.y), qplane(r2.x, sh, sz, r0.z), qplane(r2.y, sh, sz, r0.z), o, rcp, tmin,
                  tmax, n);
 }
 
