@@ -227,6 +227,31 @@ def _run_jobs(svc, sids, pool, jobs, arrivals):
     return lat, sum(i["paths"] for i in infos), span
 
 
+def _run_standalone(pool, jobs, arrivals):
+    """The jobs rendered one after another the way a single render loads
+    its scene: upload, BVH build, render, on one context (no resident
+    scenes, no shared passes).  Returns per-job latencies and the makespan."""
+    import dataclasses
+    import drawlab_amd
+    be = drawlab_amd.Backend(0)
+    try:
+        t0 = time.perf_counter()
+        done = []
+        for k, ((si, w, h, spp), ta) in enumerate(zip(jobs, arrivals)):
+            d = time.perf_counter() - t0
+            if ta > d:
+                time.sleep(ta - d)
+            desc = pool[si]
+            be.upload_scene(dataclasses.replace(desc, camera=dataclasses.replace(desc.camera, width=w, height=h)))
+            be.build_accel(drawlab_amd.BUILD_GPU)
+            be.render(spp=spp, seed=k, max_depth=desc.max_depth, rr_depth=desc.rr_depth)
+            done.append(time.perf_counter() - t0)
+    finally:
+        be.close()
+    lat = np.array([f - max(a, 0.0) for f, a in zip(done, arrivals)])
+    return lat, done[-1] - min(arrivals)
+
+
 def service_bench(args):
     import drawlab_amd
     from drawlab_amd import scenes
@@ -295,6 +320,18 @@ def service_bench(args):
 
     r1 = rate_at(curve1, target)
     rn = rate_at(curven, target)
+    # the reference's flow: every job loads its scene (upload + BVH build)
+    lats, spans = _run_standalone(pool, jobs, zero)
+    print("standalone saturated: %.2f s, %.2f jobs/s" % (spans, len(jobs) / spans), file=sys.stderr, flush=True)
+    caps = len(jobs) / spans
+    curves = []
+    for f in (0.6, 0.8, 0.9):
+        ls, _ = _run_standalone(pool, jobs, unit / (f * caps))
+        curves.append((f * caps, float(np.percentile(ls, 99))))
+        print("standalone at %.2f jobs/s: p99 %.3f s" % (f * caps, curves[-1][1]), file=sys.stderr, flush=True)
+    target_s = curves[-1][1]
+    rs_ = rate_at(curves, target_s)
+    rn_s = rate_at(curven, target_s)
 
     def q(a):
         return {"p50": round(float(np.percentile(a, 50)), 4), "p99": round(float(np.percentile(a, 99)), 4),
@@ -323,7 +360,12 @@ def service_bench(args):
                       "ratio": (rn / r1) if (rn and r1) else None,
                       "note": "service rate is a lower bound when it stays within the p99 at every swept rate"},
         "saturated_jobs_per_s": {"service": round(len(jobs) / spann, 4),
-                                 "one_at_a_time": round(len(jobs) / span1, 4)},
+                                 "one_at_a_time": round(len(jobs) / span1, 4),
+                                 "standalone": round(len(jobs) / spans, 4)},
+        "standalone_equal_p99": {"p99_s": round(target_s, 3), "rate_standalone": rs_, "rate_service": rn_s,
+                                 "ratio": (rn_s / rs_) if (rn_s and rs_) else None,
+                                 "p99_vs_rate": [[round(r, 3), round(l, 3)] for r, l in curves],
+                                 "note": "standalone = per job: scene upload + BVH build + render, one context"},
         "saturated_mpaths_per_s": {"service": round(paths / spann / 1e6, 1),
                                    "one_at_a_time": round(paths / span1 / 1e6, 1)},
         "service_stats": {"peak_reserved_bytes": int(stp["peak_reserved_bytes"]),
