@@ -50,7 +50,8 @@ struct Counters {
     uint32_t shadow;  // shadow rays queued (block_append index 1)
     uint32_t work_ray;     // rays claimed by the closest-hit traversal
     uint32_t work_shadow;  // rays claimed by the shadow traversal
-    uint32_t pad[12];
+    uint32_t emit;         // continuations that hit an emitter (k_post -> k_post_emit)
+    uint32_t pad[11];
     unsigned long long nodes;
     unsigned long long tris;
     unsigned long long invalid;
@@ -69,6 +70,7 @@ struct RenderBufs {
     uint32_t* matq[2];
     uint32_t* rayq;
     uint32_t* shq;
+    uint32_t* emq;  // paths whose continuation hit an emitter (MIS-weighted emission pending)
     Counters* cnt;
     float4* film;
     float* rgb;
@@ -209,6 +211,7 @@ __global__ __launch_bounds__(kStageBlock) void k_reset(Counters* cnt, int next) 
         cnt->shadow = 0;
         cnt->work_ray = 0;
         cnt->work_shadow = 0;
+        cnt->emit = 0;
     }
 }
 
@@ -665,6 +668,42 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
 
 // NEE resolution, emitter hit / miss with MIS, Russian roulette
 // (path.cpp:46-56, 71-97; miss.cu:31-42).
+// A continuation that hit an area light: its MIS-weighted emission
+// (path.cpp:72-84) is added by k_post_emit, after the rest of k_post, from
+// the throughput before Russian roulette, kept in the path's NEE slot (the
+// NEE contribution has been consumed) -- this keeps the rarely needed
+// emitter pdf out of k_post's registers.
+__device__ __forceinline__ void defer_emission(const RenderBufs& rs, uint32_t p, vec3 thr, int light) {
+    rs.nee[p] = f4(thr, __int_as_float(light));
+    rs.emq[atomicAdd(&rs.cnt->emit, 1u)] = p;
+}
+
+// Emission of the continuations k_post deferred: rad += thr * mis * Le, in
+// the order k_post would have added it (after the NEE contribution).
+__global__ __launch_bounds__(kStageBlock) void k_post_emit(SceneView s, RenderBufs rs) {
+    const uint32_t n = rs.cnt->emit;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t p = rs.emq[i];
+        const float4 h = rs.hit[p];
+        const uint32_t prim = __float_as_uint(h.x);
+        const float4 t4 = rs.nee[p];
+        const int light = __float_as_int(t4.w);
+        const int4 aux = rs.aux[p];
+        const float prev_pdf = __int_as_float(aux.x);
+        const bool diffuse = ((uint32_t)aux.y & kFlagDiffuse) != 0;
+        const vec3 o = f3(rs.ray_o[p]);
+        const vec3 d = f3(rs.ray_d[p]);
+        vec3 hp, hn;
+        hit_point_normal(s, prim, h.z, h.w, hp, hn);
+        const vec3 ev = dot(hn, neg(d)) > 0.f ? pcol(s.lights[light].radiance) : v3(0, 0, 0);
+        const DirSample hs = dir_sample(o, hp, hn);
+        const float epdf = pdf_emitter_direction(s, light, hs);
+        const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
+        const vec3 rad = add(f3(rs.rad[p]), mul(scale(f3(t4), mis), ev));
+        rs.rad[p] = f4(rad, 0.f);
+    }
+}
+
 // HIDE: light surfaces are hidden (hitgroup_pgs.cu:55-64): a path ray that
 // hits one collects its emission as usual, unless it is still the camera
 // ray, and passes through (queue kPassQ) without a bounce; a camera ray that
@@ -709,7 +748,6 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
                     const bool diffuse = (flags & kFlagDiffuse) != 0;
                     const float4 h = rs.hit[p];
                     const uint32_t prim = __float_as_uint(h.x);
-                    const vec3 o = f3(rs.ray_o[p]);
                     const vec3 d = f3(rs.ray_d[p]);
                     const bool cam = HIDE && (flags & kFlagCamera) != 0;
                     if (prim == 0xffffffffu) {
@@ -727,16 +765,7 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
                     } else if (HIDE && (cam || s.meshes[s.prim_mesh[prim]].light >= 0)) {
                         const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
                         if (mr.light >= 0) {
-                            if (!cam) {
-                                vec3 hp, hn;
-                                hit_point_normal(s, prim, h.z, h.w, hp, hn);
-                                const vec3 ev =
-                                    dot(hn, neg(d)) > 0.f ? pcol(s.lights[mr.light].radiance) : v3(0, 0, 0);
-                                const DirSample hs = dir_sample(o, hp, hn);
-                                const float epdf = pdf_emitter_direction(s, mr.light, hs);
-                                const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
-                                rad = add(rad, mul(scale(thr, mis), ev));
-                            }
+                            if (!cam) defer_emission(rs, p, thr, mr.light);
                             aux.w = 0;  // the record's radiance before its continuation is already kept
                             rs.aux[p] = aux;
                             alive = true;
@@ -749,15 +778,7 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
                         }
                     } else {
                         const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
-                        if (mr.light >= 0) {
-                            vec3 hp, hn;
-                            hit_point_normal(s, prim, h.z, h.w, hp, hn);
-                            const vec3 ev = dot(hn, neg(d)) > 0.f ? pcol(s.lights[mr.light].radiance) : v3(0, 0, 0);
-                            const DirSample hs = dir_sample(o, hp, hn);
-                            const float epdf = pdf_emitter_direction(s, mr.light, hs);
-                            const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
-                            rad = add(rad, mul(scale(thr, mis), ev));
-                        }
+                        if (mr.light >= 0) defer_emission(rs, p, thr, mr.light);
                         alive = true;
                         const int depth = aux.z;
                         if (depth >= pp.rr_depth) {
@@ -1077,6 +1098,7 @@ dl_status ensure_state(Ctx* c, uint32_t P, uint32_t pixels) {
     DL_TRY(alloc(c, rs, &rs->b.matq[1], (size_t)P * kTypes));
     DL_TRY(alloc(c, rs, &rs->b.rayq, P));
     DL_TRY(alloc(c, rs, &rs->b.shq, P));
+    DL_TRY(alloc(c, rs, &rs->b.emq, P));
     DL_TRY(alloc(c, rs, &rs->b.cnt, 1));
     DL_TRY(alloc(c, rs, &rs->b.film, pixels));
     DL_TRY(alloc(c, rs, &rs->b.rgb, (size_t)pixels * 3));
@@ -1231,6 +1253,7 @@ dl_status run_bounces(Ctx* c, RenderState* rs, const PassParams& pp, const Scene
             k_post<true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv.rec);
         else
             k_post<false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv.rec);
+        k_post_emit<<<stage_grid, kStageBlock, 0, c->stream>>>(sv, rs->b);
         DL_TRY(tm.stop());
         DL_HIP_CHECK(c, hipGetLastError());
         cur ^= 1;
