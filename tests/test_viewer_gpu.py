@@ -77,6 +77,34 @@ def test_hidden_lights_paths_bitexact(backend):
     paths_parity(backend, d, samples=[0], seed=6, view={"epsilon": 1e-3})
 
 
+def _hidden_light_under_sky(visible):
+    """A floor, an area light hanging above it facing down and an environment
+    map: camera rays that pass through the hidden light leave the scene."""
+    from drawlab_amd.scene import Bsdf, RFilter, SceneDesc, FILTER_BOX
+    d = SceneDesc()
+    b = d.add_bsdf(Bsdf.diffuse())
+    d.add_mesh(np.float32([[-3, 0, -3], [3, 0, -3], [3, 0, 3], [-3, 0, 3]]), np.uint32([[0, 2, 1], [0, 3, 2]]), b)
+    d.add_mesh(np.float32([[-.5, 1, -.5], [.5, 1, -.5], [.5, 1, .5], [-.5, 1, .5]]), np.uint32([[0, 1, 2], [0, 2, 3]]),
+               b, radiance=(6.0, 5.0, 4.0))
+    env = np.random.default_rng(4).random((8, 16, 3)).astype(np.float32) * 2
+    d.envmap = EnvMap(env, 1.0, visible)
+    d.bg_color = (0.1, 0.2, 0.3)
+    d.camera = Camera(eye=(0.1, 0.5, -2.0), lookat=(0.0, 0.9, 0.0), up=(0, 1, 0), fov=60, width=32, height=32)
+    d.rfilter = RFilter(FILTER_BOX)
+    d.max_depth, d.rr_depth = 6, 3
+    return d
+
+
+@pytest.mark.parametrize("visible", [True, False], ids=["env_visible", "env_hidden"])
+def test_hidden_light_under_environment_paths_bitexact(backend, visible):
+    """Camera rays through the hidden light that then leave the scene take
+    the environment (or the background colour when it is not visible), later
+    rays collect the light's and the environment's emission with MIS."""
+    d = _hidden_light_under_sky(visible)
+    paths_parity(backend, d, samples=[0, 1], seed=7, view={"hide_lights": True})
+    paths_parity(backend, d, samples=[0], seed=7)
+
+
 def test_view_options_change_between_renders(backend):
     """Options apply to the next render without re-uploading; reset restores
     the scene's own render bit for bit; a new upload clears them."""
