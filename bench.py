@@ -5,8 +5,8 @@ Workload: the 2,097,152-triangle displaced sphere (GGX conductor) with a
 20,480-triangle glass sphere, a diffuse ground and a 1024x512 sky + sun
 environment map, rendered at 1920x1080 with max depth 16, 256 spp.  One step
 is one wavefront pass of the unidirectional MIS path tracer over the whole
-frame at 32 samples per pixel (66,355,200 paths, dl_hip_render's default
-pass size at this resolution), accumulated progressively; the default 8
+frame at 64 samples per pixel (132,710,400 paths, dl_hip_render's default
+pass size at this resolution), accumulated progressively; the default 4
 steps are the workload's 256 spp.
 
 Prints one JSON line: Mpaths/s over the timed steps, the roofline of the
@@ -103,7 +103,7 @@ def short_name(name):
 def summarize_stats(d, out_md, command, title=""):
     path = sorted(glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True))[0]
     rows = list(csv.DictReader(open(path)))
-    title = title or "%s bench (1920x1080, 32 spp per pass, max depth 16)" % WORKLOAD
+    title = title or "%s bench (1920x1080, 64 spp per pass, max depth 16)" % WORKLOAD
     lines = ["# Kernel time: %s" % title, "",
              "Command (repository root, MI355X):", "", "```", command, "```", "",
              "| kernel | calls | total ms | avg us | min us | max us | % |", "|---|---:|---:|---:|---:|---:|---:|"]
@@ -135,7 +135,7 @@ def summarize_pmc(fetch_dir, write_dir, out_json, copy_bytes, commands):
     write = w[TRACE_KERNEL][0] * 1024.0 * cw
     out = {
         "kernel": TRACE_KERNEL,
-        "workload": WORKLOAD + " (32 spp per pass)",
+        "workload": WORKLOAD + " (64 spp per pass)",
         "dispatches": f[TRACE_KERNEL][1],
         "fetch_kib_per_dispatch": f[TRACE_KERNEL][0],
         "write_kib_per_dispatch": w[TRACE_KERNEL][0],
@@ -174,7 +174,7 @@ def summarize_counters(dirs, out_md, command, kernels):
             s[0] += w * float(r["Counter_Value"])
             s[1] += w
             s[2] += 1
-    lines = ["# Hardware counters: %s bench (1920x1080, 32 spp per pass)" % WORKLOAD, "",
+    lines = ["# Hardware counters: %s bench (1920x1080, 64 spp per pass)" % WORKLOAD, "",
              "Commands (repository root, MI355X):", "", "```", command, "```", "",
              "Dispatch-time-weighted means over each kernel's dispatches.  VALUBusy: % of GPU time the vector ALUs "
              "issue; VALUUtilization: % of active lanes per VALU instruction; OccupancyPercent: resident waves as "
@@ -498,9 +498,9 @@ def guided_bench(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--spp-per-step", type=int, default=32)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp-per-step", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,

@@ -49,7 +49,7 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-constexpr uint64_t kPathsPerPass = 64u << 20;  // dl_hip_render's default pass size (paths)
+constexpr uint64_t kPathsPerPass = 64u << 20;  // paths per pass of a job or a shared group (half the largest pass)
 constexpr uint64_t kStateBytesPerPath = 240;    // path state + queues (integrator RenderState)
 constexpr uint64_t kFilmBytesPerPixel = 28;     // accumulation film (16 B) + resolved RGB (12 B)
 constexpr uint64_t kBytesPerVertex = 36;        // positions, normals, tangents
