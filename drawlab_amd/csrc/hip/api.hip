@@ -328,6 +328,14 @@ dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
     DL_TRY(upload(c, &ds.tan, tan.data(), tan.size()));
     DL_TRY(upload(c, &ds.idx, idx.data(), idx.size()));
     DL_TRY(upload(c, &ds.prim_mesh, prim_mesh.data(), prim_mesh.size()));
+    // per primitive: its material's type and whether its mesh is a light, so
+    // the path kernels classify a hit with one load
+    std::vector<uint8_t> prim_info(nf);
+    for (const MeshRec& r : c->mesh_recs) {
+        const uint8_t info = (uint8_t)((uint32_t)c->mats[r.material].type | (r.light >= 0 ? kPrimLight : 0u));
+        std::fill(prim_info.begin() + r.prim_offset, prim_info.begin() + r.prim_offset + r.num_prims, info);
+    }
+    DL_TRY(upload(c, &ds.prim_info, prim_info.data(), prim_info.size()));
     MeshRec* dm = nullptr;
     DL_TRY(upload(c, &dm, c->mesh_recs.data(), c->mesh_recs.size()));
     ds.meshes = dm;

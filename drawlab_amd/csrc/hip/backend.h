@@ -25,6 +25,7 @@ struct DevScene {
     float* tan = nullptr;        // 3 * nv   (zeros for meshes without tangents)
     uint32_t* idx = nullptr;     // 3 * nprims (global vertex indices)
     uint32_t* prim_mesh = nullptr;  // nprims -> mesh index
+    uint8_t* prim_info = nullptr;   // nprims -> material type | kPrimLight (one byte: 2 MB per 2M triangles)
     float* uv = nullptr;         // 2 * nv   (texture coordinates; zeros for meshes without)
     // bitmap textures: texels (rgb, 0) of all textures, per texture
     // (first texel, width, height, 0)
@@ -47,6 +48,10 @@ struct DevScene {
 };
 
 // Per-mesh record read by the shading kernels.
+// prim_info flag: the primitive's mesh is an area light.
+constexpr uint32_t kPrimLight = 8u;
+static_assert(DL_BSDF_COUNT <= (int)kPrimLight, "material types fit below the light flag");
+
 struct MeshRec {
     uint32_t prim_offset;
     uint32_t num_prims;

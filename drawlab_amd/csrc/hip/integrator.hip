@@ -348,11 +348,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_queue(BvhView bvh, Rende
     traverse_persistent<true, false>(bvh, n, &rs.cnt->work_shadow, stk, io, cnt);
 }
 
-__device__ __forceinline__ int material_of(const SceneView& s, uint32_t prim, int& mat) {
-    const uint32_t m = s.prim_mesh[prim];
-    mat = s.meshes[m].material;
-    return s.mats[mat].type;
-}
+// Material type and area-light flag of a primitive (one byte).
+__device__ __forceinline__ uint32_t prim_info(const SceneView& s, uint32_t prim) { return s.prim_info[prim]; }
+__device__ __forceinline__ int info_type(uint32_t info) { return (int)(info & (kPrimLight - 1u)); }
+__device__ __forceinline__ bool info_light(uint32_t info) { return (info & kPrimLight) != 0; }
+// the area light of a primitive whose info has kPrimLight
+__device__ __forceinline__ int light_of(const SceneView& s, uint32_t prim) { return s.meshes[s.prim_mesh[prim]].light; }
 
 // First intersection (path.cpp:21-29) and a camera-ray miss (miss.cu:23-29).
 // HIDE: light surfaces are hidden (hitgroup_pgs.cu:55-64): a camera ray that
@@ -384,21 +385,23 @@ __global__ __launch_bounds__(kStageBlock) void k_post_camera(PassParams pp, Scen
                         const LightRec& e = s.lights[s.env_light];
                         rad = e.env_visual ? env_eval(s, e, d) : v3(s.bg[0], s.bg[1], s.bg[2]);
                     }
-                } else if (HIDE && s.meshes[s.prim_mesh[prim]].light >= 0) {
-                    rs.aux[p] = make_int4(0, (int)(kFlagCont | kFlagCamera), 0, 0);
-                    alive = true;
-                    type = kPassQ;
                 } else {
-                    const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
-                    if (mr.light >= 0) {
-                        vec3 hp, hn;
-                        hit_point_normal(s, prim, h.z, h.w, hp, hn);
-                        const vec3 le = dot(hn, neg(d)) > 0.f ? pcol(s.lights[mr.light].radiance) : v3(0, 0, 0);
-                        rad = add(rad, mul(v3(1.f, 1.f, 1.f), le));
+                    const uint32_t info = prim_info(s, prim);
+                    if (HIDE && info_light(info)) {
+                        rs.aux[p] = make_int4(0, (int)(kFlagCont | kFlagCamera), 0, 0);
+                        alive = true;
+                        type = kPassQ;
+                    } else {
+                        if (info_light(info)) {
+                            vec3 hp, hn;
+                            hit_point_normal(s, prim, h.z, h.w, hp, hn);
+                            const vec3 le =
+                                dot(hn, neg(d)) > 0.f ? pcol(s.lights[light_of(s, prim)].radiance) : v3(0, 0, 0);
+                            rad = add(rad, mul(v3(1.f, 1.f, 1.f), le));
+                        }
+                        alive = !(pp.max_depth >= 0 && 0 + 2 > pp.max_depth);
+                        type = info_type(info);
                     }
-                    alive = !(pp.max_depth >= 0 && 0 + 2 > pp.max_depth);
-                    int mat;
-                    type = material_of(s, prim, mat);
                 }
                 rs.rad[p] = f4(rad, 0.f);
             }
@@ -762,45 +765,44 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
                                 rad = add(rad, mul(scale(thr, mis), ev));
                             }
                         }
-                    } else if (HIDE && (cam || s.meshes[s.prim_mesh[prim]].light >= 0)) {
-                        const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
-                        if (mr.light >= 0) {
-                            if (!cam) defer_emission(rs, p, thr, mr.light);
-                            aux.w = 0;  // the record's radiance before its continuation is already kept
-                            rs.aux[p] = aux;
-                            alive = true;
-                            ntype = kPassQ;
-                        } else {
-                            // the camera ray's first visible surface (as k_post_camera)
-                            alive = !(pp.max_depth >= 0 && 0 + 2 > pp.max_depth);
-                            int mat;
-                            ntype = material_of(s, prim, mat);
-                        }
                     } else {
-                        const MeshRec& mr = s.meshes[s.prim_mesh[prim]];
-                        if (mr.light >= 0) defer_emission(rs, p, thr, mr.light);
-                        alive = true;
-                        const int depth = aux.z;
-                        if (depth >= pp.rr_depth) {
-                            Pcg32 rng;
-                            rng.state = rs.rng[p];
-                            rng.inc = ((uint64_t)rs.seq[p] << 1u) | 1u;
-                            const float pr = fmin0(cmax(scale(scale(thr, eta), eta)), 0.99f);
-                            if (rng.next_float() > pr) {
-                                alive = false;
+                        const uint32_t info = prim_info(s, prim);
+                        if (HIDE && (cam || info_light(info))) {
+                            if (info_light(info)) {
+                                if (!cam) defer_emission(rs, p, thr, light_of(s, prim));
+                                aux.w = 0;  // the record's radiance before its continuation is already kept
+                                rs.aux[p] = aux;
+                                alive = true;
+                                ntype = kPassQ;
                             } else {
-                                thr = divs(thr, pr);
-                                rs.thr[p] = f4(thr, eta);
+                                // the camera ray's first visible surface (as k_post_camera)
+                                alive = !(pp.max_depth >= 0 && 0 + 2 > pp.max_depth);
+                                ntype = info_type(info);
                             }
-                            rs.rng[p] = rng.state;
-                        }
-                        if (alive) {
-                            const int nd = depth + 1;
-                            if (pp.max_depth >= 0 && nd + 2 > pp.max_depth) alive = false;
-                            aux.z = nd;
-                            rs.aux[p] = aux;
-                            int mat;
-                            ntype = material_of(s, prim, mat);
+                        } else {
+                            if (info_light(info)) defer_emission(rs, p, thr, light_of(s, prim));
+                            alive = true;
+                            const int depth = aux.z;
+                            if (depth >= pp.rr_depth) {
+                                Pcg32 rng;
+                                rng.state = rs.rng[p];
+                                rng.inc = ((uint64_t)rs.seq[p] << 1u) | 1u;
+                                const float pr = fmin0(cmax(scale(scale(thr, eta), eta)), 0.99f);
+                                if (rng.next_float() > pr) {
+                                    alive = false;
+                                } else {
+                                    thr = divs(thr, pr);
+                                    rs.thr[p] = f4(thr, eta);
+                                }
+                                rs.rng[p] = rng.state;
+                            }
+                            if (alive) {
+                                const int nd = depth + 1;
+                                if (pp.max_depth >= 0 && nd + 2 > pp.max_depth) alive = false;
+                                aux.z = nd;
+                                rs.aux[p] = aux;
+                                ntype = info_type(info);
+                            }
                         }
                     }
                 }
@@ -1128,6 +1130,7 @@ SceneView scene_view(Ctx* c) {
     s.tan = d.tan;
     s.idx = d.idx;
     s.prim_mesh = d.prim_mesh;
+    s.prim_info = d.prim_info;
     s.meshes = (const MeshRec*)d.meshes;
     s.mats = (const dl_material_desc*)d.materials;
     s.lights = (const LightRec*)d.lights;

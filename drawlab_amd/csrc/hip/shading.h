@@ -13,6 +13,7 @@ struct SceneView {
     const float* __restrict__ tan;
     const uint32_t* __restrict__ idx;
     const uint32_t* __restrict__ prim_mesh;
+    const uint8_t* __restrict__ prim_info;  // material type | kPrimLight
     const MeshRec* __restrict__ meshes;
     const dl_material_desc* __restrict__ mats;
     const LightRec* __restrict__ lights;
