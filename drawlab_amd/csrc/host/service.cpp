@@ -25,11 +25,12 @@
 // device memory never exceeds it.  After each job the lane's measured peak
 // is compared with its reservation (estimate_misses counts violations).
 //
-// A job renders in passes of the same size dl_hip_render would choose
-// (spp_per_pass = 0 resolves to the same value), accumulating into its film
-// slot (or, for a guided job, which renders alone, the context's film), so
-// its image is bit-identical to a single dl_hip_render call; cancellation
-// takes effect between passes.
+// A job renders in passes of its spp_per_pass, or of at most kPathsPerPass
+// paths when that is 0 (half dl_hip_render's own default, so that smaller
+// jobs can share a pass), accumulating into its film slot (or, for a guided
+// job, which renders alone, the context's film), so its image is
+// bit-identical to a single dl_hip_render call with that spp_per_pass;
+// cancellation takes effect between passes.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -119,7 +120,7 @@ struct Service {
 
 uint64_t job_pixels(const Job& j) { return (uint64_t)j.camera.width * j.camera.height; }
 
-// passes of the job: the pass size dl_hip_render uses for spp_per_pass = 0
+// samples per pixel per pass of the job
 uint32_t pass_spp(const dl_render_params& p, uint64_t pixels) {
     uint64_t k = p.spp_per_pass > 0 ? (uint64_t)p.spp_per_pass : std::max<uint64_t>(1, kPathsPerPass / pixels);
     return (uint32_t)std::min<uint64_t>(k, (uint64_t)p.spp);
