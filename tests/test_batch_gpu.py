@@ -136,3 +136,16 @@ def test_service_packs_jobs_into_shared_passes(backend):
         ref, rst = backend.render(spp=spp, seed=seed, spp_per_pass=k)
         assert info["paths"] == rst["paths"]
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), (spp, seed, k, w)
+
+
+def test_pass_split_does_not_change_the_image(backend):
+    """The film adds each pixel's samples in sample order whatever the pass
+    size (the per-path pcg32 streams depend only on pixel and sample), so a
+    plain render is bit-identical over any split into passes."""
+    d = scenes.cornell_box(48, 40, spp=9, max_depth=6, rr_depth=3)
+    backend.upload_scene(d)
+    backend.build_accel(drawlab_amd.BUILD_GPU)
+    ref, _ = backend.render(spp=9, seed=4, spp_per_pass=9)
+    for k in (4, 1, 0):
+        img, _ = backend.render(spp=9, seed=4, spp_per_pass=k)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), k
