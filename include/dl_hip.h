@@ -383,8 +383,10 @@ dl_status dl_hip_guide_train_step(void* ctx, const float* x, const float* dirs, 
  * included) resident for the next job of that scene.  Admission keeps the
  * device bytes reserved by all lanes within budget_bytes; a job that could
  * never fit fails.  Every job's image is bit-identical to dl_hip_render of
- * the same scene and parameters with the same spp_per_pass (a job with
- * spp_per_pass 0 renders in passes of at most 2^26 paths).
+ * the same scene and parameters (a plain image does not depend on the pass
+ * size; a guided one, whose network trains between passes, is identical for
+ * the same spp_per_pass: a job with spp_per_pass 0 renders in passes of at
+ * most 2^26 paths).
  * No reference counterpart: the reference renders one scene per process
  * (src/tracer/main.cpp:34-48); its closest analogue is the progressive
  * accumulation of src/optix/host/param_buffer.cpp:87-95. */
