@@ -954,8 +954,9 @@ constexpr int kFilmSpan = kFilmTile + 2 * kFilmHalo;
 __global__ __launch_bounds__(kFilmTile* kFilmTile) void k_film_tiled(PassParams pp, uint32_t k, dl_film_desc film,
                                                                      RenderBufs rs) {
     __shared__ float4 s_rad[kFilmSpan * kFilmSpan];
-    __shared__ float4 s_rel[kFilmSpan * kFilmSpan];  // plx, ply, block origin x, y (int bits)
-    __shared__ int4 s_cov[kFilmSpan * kFilmSpan];    // covered film pixels: x lo, x hi, y lo, y hi
+    __shared__ float2 s_rel[kFilmSpan * kFilmSpan];  // plx, ply
+    // covered film pixels relative to the tile origin: x lo, x hi, y lo, y hi
+    __shared__ short4 s_cov[kFilmSpan * kFilmSpan];
     __shared__ float s_table[33];                    // the filter table, indexed per lane
     if (threadIdx.x < 33) s_table[threadIdx.x] = film.table[threadIdx.x];
     const uint32_t W = pp.W, H = pp.H, WH = W * H;
@@ -989,20 +990,24 @@ __global__ __launch_bounds__(kFilmTile* kFilmTile) void k_film_tiled(PassParams 
                 const bool ok = sample_valid(L);
                 if (!ok && gx >= tx0 && gx < tx0 + kFilmTile && gy >= ty0 && gy < ty0 + kFilmTile) invalid++;
                 s_rad[e] = L;
-                s_rel[e] = make_float4(plx, ply, __int_as_float(ox - b), __int_as_float(oy - b));
-                s_cov[e] = ok ? make_int4(x0 + ox - b, x1 + ox - b, y0 + oy - b, y1 + oy - b) : make_int4(1, 0, 1, 0);
+                s_rel[e] = make_float2(plx, ply);
+                s_cov[e] = ok ? make_short4((short)(x0 + ox - b - tx0), (short)(x1 + ox - b - tx0),
+                                            (short)(y0 + oy - b - ty0), (short)(y1 + oy - b - ty0))
+                              : make_short4(1, 0, 1, 0);
             }
         }
         __syncthreads();
         if (inside) {
+            const int lx = qx - tx0, ly = qy - ty0;
             for (int py = max(0, qy - R); py <= min((int)H - 1, qy + R); ++py)
                 for (int px = max(0, qx - R); px <= min((int)W - 1, qx + R); ++px) {
                     const int e = (py - (ty0 - R)) * S + (px - (tx0 - R));
-                    const int4 cv = s_cov[e];
-                    if (qx < cv.x || qx > cv.y || qy < cv.z || qy > cv.w) continue;
+                    const short4 cv = s_cov[e];
+                    if (lx < cv.x || lx > cv.y || ly < cv.z || ly > cv.w) continue;
                     const float4 L = s_rad[e];
-                    const float4 rel = s_rel[e];
-                    const int xl = qx - __float_as_int(rel.z), yl = qy - __float_as_int(rel.w);
+                    const float2 rel = s_rel[e];
+                    // the sample's 32x32 image block origin, as staged
+                    const int xl = qx - ((px / 32) * 32 - b), yl = qy - ((py / 32) * 32 - b);
                     const float wx = s_table[(int)(fabsf((float)xl - rel.x) * film.lookup)];
                     const float wy = s_table[(int)(fabsf((float)yl - rel.y) * film.lookup)];
                     acc.x += (L.x * wx) * wy;
