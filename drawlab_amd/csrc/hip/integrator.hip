@@ -94,7 +94,7 @@ struct RenderState {
 // helpers
 // ---------------------------------------------------------------------------
 constexpr int kStageWaves = kStageBlock / 64;
-constexpr int kStageItems = 4;                          // items per thread per block-wide append
+constexpr int kStageItems = 8;                          // items per thread per block-wide append
 
 // Items per thread per loop trip: up to kStageItems, but no more than spread
 // the queue over the whole grid (short queues keep every block busy).
