@@ -68,6 +68,8 @@ def lib():
             "dlo_warp": (None, [i32, _f32p, u32, f32, f32, _f32p]),
             "dlo_optics": (None, [i32, _f32p, u32, _f32p]),
             "dlo_bsdf_query": (None, [_P, i32, _f32p, u32, _f32p]),
+            "dlo_emitter_query": (None, [_P, _f32p, u32, _f32p]),
+            "dlo_set_mutation": (None, [i32]),
             "dlo_env_tables": (None, [_P, _f32p, _u32p, _f32p]),
             "dlo_spiral_blocks": (u32, [i32, i32, _P]),
             "dlo_bvh4_replay": (None, [_f32p, _f32p, _f32p, u32, i32, _P, _P, _P, _P, _P, _P, _P]),
@@ -242,12 +244,49 @@ class Scene:
         self.L.dlo_bsdf_query(self.h, int(bsdf), inp.reshape(-1), n, out)
         return out.reshape(n, 12)
 
+    def emitter_query(self, p, n, u):
+        """Scene::sampleEmitterDirection at shading points p with shading
+        normals n and uniforms u (n, 3); returns a dict of spectrum (n, 3),
+        d (n, 3), dist, pdf (the sample's), pdf_dir (pdfEmitterDirection of
+        the sampled direction), emitter, delta."""
+        inp = np.concatenate([np.asarray(p, np.float32).reshape(-1, 3), np.asarray(n, np.float32).reshape(-1, 3),
+                              np.asarray(u, np.float32).reshape(-1, 3)], axis=1)
+        inp = np.ascontiguousarray(inp, dtype=np.float32)
+        k = inp.shape[0]
+        out = np.zeros(11 * k, np.float32)
+        self.L.dlo_emitter_query(self.h, inp.reshape(-1), k, out)
+        o = out.reshape(k, 11)
+        return {"spectrum": o[:, 0:3], "d": o[:, 3:6], "dist": o[:, 6], "pdf": o[:, 7], "pdf_dir": o[:, 8],
+                "emitter": o[:, 9].astype(np.int32), "delta": o[:, 10] != 0}
+
     def env_tables(self, w, h):
         q = np.zeros(w * h, np.float32)
         a = np.zeros(w * h, np.uint32)
         p = np.zeros(w * h, np.float32)
         self.L.dlo_env_tables(self.h, q, a, p)
         return q, a, p
+
+
+# Planted mistakes (oracle.h DLO_MUT_*): only the tests that show a pin of
+# the oracle fails on a wrong oracle set them.
+MUT_NONE, MUT_GGX_PDF_NO_WHZ, MUT_DIELECTRIC_SWAP_IOR, MUT_DIFFUSE_UNIFORM_PDF, MUT_EMITTER_HIT_NO_MIS, \
+    MUT_AREA_PDF_NO_DIST2, MUT_CONDUCTOR_SWAP_ETA_K, MUT_RR_NO_DIVIDE, MUT_MICROFACET_G_WI_TWICE, MUT_POINT_INV_DIST, \
+    MUT_ENV_PDF_TIMES_AREA, MUT_DIELECTRIC_NO_FRESNEL = range(12)
+
+
+class mutation:
+    """Context manager: the oracle with a planted mistake."""
+
+    def __init__(self, m):
+        self.m = int(m)
+
+    def __enter__(self):
+        lib().dlo_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().dlo_set_mutation(MUT_NONE)
+        return False
 
 
 def perspective_matrices(fov, near, far, width, height, to_world=None):

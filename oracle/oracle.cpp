@@ -24,6 +24,10 @@ using namespace dlo;
 
 namespace {
 
+// Planted mistake (DLO_MUT_*), set only by the tests of the oracle's pins.
+int g_mut = DLO_MUT_NONE;
+inline bool mut(int m) { return g_mut == m; }
+
 // ---------------------------------------------------------------------------
 // Scene containers
 // ---------------------------------------------------------------------------
@@ -757,7 +761,7 @@ static V3 bsdf_eval(const Bsdf& b, const BRec& r) {
             V3 wh = normalized(r.wi + r.wo);
             float D = sq_beckmann_pdf(wh, alpha);
             float F = fresnel(dot(r.wi, wh), b.p[2], b.p[1]);
-            float G = mf_g1(r.wi, wh, alpha) * mf_g1(r.wo, wh, alpha);
+            float G = mf_g1(r.wi, wh, alpha) * mf_g1(mut(DLO_MUT_MICROFACET_G_WI_TWICE) ? r.wi : r.wo, wh, alpha);
             float deno = 1 / (4 * r.wi.z * r.wo.z);
             return add_scalar(col(b.p + 3) * kInvPi, b.ks * D * F * G * deno);
         }
@@ -780,8 +784,10 @@ static V3 bsdf_eval(const Bsdf& b, const BRec& r) {
             float cosi = dot(r.wi, wh);
             float deno = 1.f / (4.f * r.wi.z * r.wo.z);
             float s = (D * G) * deno;
-            return mk(fresnel_conductor(cosi, b.p[1], b.p[4]) * s, fresnel_conductor(cosi, b.p[2], b.p[5]) * s,
-                      fresnel_conductor(cosi, b.p[3], b.p[6]) * s);
+            const int ie = mut(DLO_MUT_CONDUCTOR_SWAP_ETA_K) ? 4 : 1, ik = 5 - ie;
+            return mk(fresnel_conductor(cosi, b.p[ie], b.p[ik]) * s,
+                      fresnel_conductor(cosi, b.p[ie + 1], b.p[ik + 1]) * s,
+                      fresnel_conductor(cosi, b.p[ie + 2], b.p[ik + 2]) * s);
         }
         default:  // mirror / dielectric: discrete   mirror.cpp:12-15, dielectric.cpp:40-43
             return mk(0, 0, 0);
@@ -792,6 +798,7 @@ static float bsdf_pdf(const Bsdf& b, const BRec& r) {
     switch (b.type) {
         case DLO_BSDF_DIFFUSE:  // diffuse.cpp:34-48
             if (r.measure != kSolidAngle || r.wi.z <= 0 || r.wo.z <= 0) return 0.0f;
+            if (mut(DLO_MUT_DIFFUSE_UNIFORM_PDF)) return kInvTwoPi;
             return kInvPi * r.wo.z;
         case DLO_BSDF_MICROFACET: {  // microfacet.cpp:61-73
             if (r.measure != kSolidAngle || r.wi.z <= 0 || r.wo.z <= 0) return 0.0f;
@@ -804,13 +811,15 @@ static float bsdf_pdf(const Bsdf& b, const BRec& r) {
             if (r.measure != kSolidAngle || r.wi.z <= 0 || r.wo.z <= 0) return 0.f;
             V3 wh = normalized(r.wi + r.wo);
             float diff = (1 - 0.5f) * sq_cos_hemi_pdf(r.wo);
-            float spec = 0.5f * ggx_d(wh, b.p[6], b.p[7]) * wh.z / 4.f / dot(wh, r.wo);
+            float whz = mut(DLO_MUT_GGX_PDF_NO_WHZ) ? 1.0f : wh.z;
+            float spec = 0.5f * ggx_d(wh, b.p[6], b.p[7]) * whz / 4.f / dot(wh, r.wo);
             return diff + spec;
         }
         case DLO_BSDF_CONDUCTOR: {
             if (r.measure != kSolidAngle || r.wi.z <= 0 || r.wo.z <= 0) return 0.f;
             V3 wh = normalized(r.wi + r.wo);
-            return ggx_d(wh, b.p[0], b.p[0]) * wh.z / 4.f / dot(wh, r.wo);
+            float whz = mut(DLO_MUT_GGX_PDF_NO_WHZ) ? 1.0f : wh.z;
+            return ggx_d(wh, b.p[0], b.p[0]) * whz / 4.f / dot(wh, r.wo);
         }
         default:
             return 0.0f;
@@ -834,12 +843,14 @@ static V3 bsdf_sample(const Bsdf& b, BRec& r, V2 s) {
         case DLO_BSDF_DIELECTRIC: {  // dielectric.cpp:45-58
             r.measure = kDiscrete;
             float c = r.wi.z;
-            float ri = fresnel(c, b.p[1], b.p[0]);
+            const bool swap = mut(DLO_MUT_DIELECTRIC_SWAP_IOR);
+            const float int_ior = swap ? b.p[1] : b.p[0], ext_ior = swap ? b.p[0] : b.p[1];
+            float ri = mut(DLO_MUT_DIELECTRIC_NO_FRESNEL) ? 0.0f : fresnel(c, ext_ior, int_ior);
             if (s.x < ri) {
                 r.wo = mk(-r.wi.x, -r.wi.y, r.wi.z);
                 r.eta = 1;
             } else {
-                float eta = b.p[1] / b.p[0];
+                float eta = ext_ior / int_ior;
                 r.wo = normalized(refract(r.wi, eta));
                 r.eta = c > 0.f ? eta : 1 / eta;
             }
@@ -975,13 +986,19 @@ struct Sampler {
     V2 next2d() { return rng.next2d(); }
     float next1d() { return rng.next_float(); }
 };
+// A sampler whose one-dimensional draw is given (emitter queries).
+struct FixedSampler {
+    float x;
+    float next1d() const { return x; }
+};
 
 // Emitter::sampleDirection dispatch:
 //  area   src/tracer/emitter/area.cpp:11-37
 //  point  include/optix/light/point.h:15-31 (receiver cosine, delta)
 //  env    include/optix/light/envmap.h:32-79 (third uniform drawn here)
+template <class SamplerT>
 static V3 emitter_sample(const Scene& s, const Emitter& e, V2 smp, const Its& its, DirSample& ds,
-                         Sampler& sampler) {
+                         SamplerT& sampler) {
     if (e.type == DLO_EMITTER_AREA) {
         const Mesh& m = s.meshes[e.mesh];
         V3 position, normal;
@@ -989,7 +1006,8 @@ static V3 emitter_sample(const Scene& s, const Emitter& e, V2 smp, const Its& it
         ds = dir_sample(its.p, position, normal);
         float pA = m.normalization;
         float dp = std::max(dot(normal, -ds.d), 0.0f);
-        float pw = dp != 0 ? pA * ds.dist * ds.dist / dp : 0;
+        float d2 = mut(DLO_MUT_AREA_PDF_NO_DIST2) ? 1.0f : ds.dist * ds.dist;
+        float pw = dp != 0 ? pA * d2 / dp : 0;
         ds.pdf = pw;
         return pw != 0 ? (e.radiance * std::max(dot(its.sh.n, ds.d), 0.0f)) / pw : mk(0, 0, 0);
     }
@@ -999,6 +1017,7 @@ static V3 emitter_sample(const Scene& s, const Emitter& e, V2 smp, const Its& it
         ds.delta = true;
         float inv = 1.0f / ds.dist;
         float c = std::max(dot(its.sh.n, ds.d), 0.0f);
+        if (mut(DLO_MUT_POINT_INV_DIST)) return c > 0.f ? (e.intensity * c) * inv : mk(0, 0, 0);
         return c > 0.f ? ((e.intensity * c) * inv) * inv : mk(0, 0, 0);
     }
     // environment map
@@ -1066,7 +1085,8 @@ static float pdf_emitter_direction(const Scene& s, int e, const DirSample& ds) {
         // AreaLight::pdfDirection  area.cpp:39-44
         float pA = s.meshes[em.mesh].normalization;
         float dp = std::max(dot(ds.n, -ds.d), 0.0f);
-        p = dp != 0 ? pA * ds.dist * ds.dist / dp : 0;
+        float d2 = mut(DLO_MUT_AREA_PDF_NO_DIST2) ? 1.0f : ds.dist * ds.dist;
+        p = dp != 0 ? pA * d2 / dp : 0;
     } else if (em.type == DLO_EMITTER_POINT) {
         p = 0;
     } else {
@@ -1176,7 +1196,7 @@ static V3 Li(const Scene& s, const RenderOpts& o, Sampler& sampler, Ray ray, uin
                 DirSample hs = dir_sample(ray.o, nx.p, nx.sh.n);
                 float epdf = pdf_emitter_direction(s, m.emitter, hs);
                 float bpdf = bsdf_pdf(bsdf, br);
-                float mis = bsdf_is_diffuse(bsdf) ? mis_weight(bpdf, epdf) : 1.f;
+                float mis = bsdf_is_diffuse(bsdf) && !mut(DLO_MUT_EMITTER_HIT_NO_MIS) ? mis_weight(bpdf, epdf) : 1.f;
                 result = result + cmul(thr * mis, ev);
             }
             if (!hidden(s, nx)) break;
@@ -1192,7 +1212,7 @@ static V3 Li(const Scene& s, const RenderOpts& o, Sampler& sampler, Ray ray, uin
                 hs.d = ray.d;
                 float epdf = pdf_emitter_direction(s, s.env, hs);
                 float bpdf = bsdf_pdf(bsdf, br);
-                float mis = bsdf_is_diffuse(bsdf) ? mis_weight(bpdf, epdf) : 1.f;
+                float mis = bsdf_is_diffuse(bsdf) && !mut(DLO_MUT_EMITTER_HIT_NO_MIS) ? mis_weight(bpdf, epdf) : 1.f;
                 result = result + cmul(thr * mis, ev);
             }
             break;
@@ -1201,7 +1221,7 @@ static V3 Li(const Scene& s, const RenderOpts& o, Sampler& sampler, Ray ray, uin
         if (depth >= o.rr_depth) {
             float p = std::min(cmax((thr * eta) * eta), 0.99f);
             if (sampler.next1d() > p) break;
-            thr = thr / p;
+            if (!mut(DLO_MUT_RR_NO_DIVIDE)) thr = thr / p;
         }
     }
     return result;
@@ -1584,7 +1604,7 @@ int dlo_scene_add_envmap(void* h, const float* rgb, int w, int hgt, float scale,
     const float inv = 1.0f / sum;
     for (uint32_t i = 0; i < size; ++i) {
         const float* p = &e.rgb[3 * i];
-        e.pdf[i] = std::max(p[0], std::max(p[1], p[2])) * inv;
+        e.pdf[i] = (mut(DLO_MUT_ENV_PDF_TIMES_AREA) ? imp[i] : std::max(p[0], std::max(p[1], p[2]))) * inv;
     }
     s.env = (int)s.emitters.size();
     s.emitters.push_back(std::move(e));
@@ -1967,6 +1987,49 @@ void dlo_bsdf_query(void* h, int bsdf, const float* in, uint32_t n, float* out) 
         memcpy(o, vals, sizeof(vals));
     }
 }
+// Emitter sampling at shading points (tests of the emitters): the scene's
+// emitter pick, the emitter's sample, and the pdf of the sampled direction
+// as the emitter-hit MIS evaluates it.  The environment map's third uniform
+// is in[8].
+void dlo_emitter_query(void* h, const float* in, uint32_t n, float* out) {
+    const Scene& s = *(Scene*)h;
+    for (uint32_t i = 0; i < n; i++) {
+        const float* a = in + 9 * (size_t)i;
+        float* o = out + 11 * (size_t)i;
+        Its its;
+        its.p = mk(a[0], a[1], a[2]);
+        its.sh = frame_from_normal(mk(a[3], a[4], a[5]));
+        its.geo = its.sh;
+        its.mesh = -1;
+        its.tri = -1;
+        FixedSampler fs{a[8]};
+        DirSample ds;
+        ds.d = ds.n = ds.o = mk(0, 0, 0);
+        V3 spec = mk(0, 0, 0);
+        size_t ne = s.emitters.size();
+        if (ne > 0) {
+            V2 smp{a[6], a[7]};
+            int index = 0;
+            float scale = 1.0f;
+            if (ne > 1) {
+                index = (int)std::min((size_t)(smp.x * (float)ne), ne - 1);
+                smp.x = smp.x * (float)ne - (float)index;
+                scale = (float)ne;
+            }
+            spec = emitter_sample(s, s.emitters[index], smp, its, ds, fs);
+            if (ne > 1) {
+                spec = spec * scale;
+                ds.pdf *= 1.f / (float)ne;
+            }
+            ds.emitter = index;
+        }
+        float pdir = ds.emitter >= 0 && !ds.delta ? pdf_emitter_direction(s, ds.emitter, ds) : 0.0f;
+        float vals[11] = {spec.x, spec.y, spec.z, ds.d.x, ds.d.y, ds.d.z, ds.dist, ds.pdf, pdir, (float)ds.emitter,
+                          ds.delta ? 1.0f : 0.0f};
+        memcpy(o, vals, sizeof(vals));
+    }
+}
+void dlo_set_mutation(int m) { g_mut = m; }
 // Environment map internals: q, alias, pdf tables (n = w*h).
 void dlo_env_tables(void* h, float* q, uint32_t* alias, float* pdf) {
     const Scene& s = *(Scene*)h;

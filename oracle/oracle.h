@@ -100,8 +100,32 @@ void dlo_pcg32_float(uint64_t initstate, uint64_t initseq, uint32_t n, float* ou
 void dlo_warp(int which, const float* in, uint32_t n, float a0, float a1, float* out);
 void dlo_optics(int which, const float* in, uint32_t n, float* out);
 void dlo_bsdf_query(void* scene, int bsdf, const float* in, uint32_t n, float* out);
+// Scene::sampleEmitterDirection + pdfEmitterDirection at shading points.
+// in: p(3) shading normal(3) uniforms(3) per query (the third uniform is
+// the environment map's extra draw); out: spectrum(3) d(3) dist pdf
+// pdf_of_direction emitter delta = 11 floats.
+void dlo_emitter_query(void* scene, const float* in, uint32_t n, float* out);
 void dlo_env_tables(void* scene, float* q, uint32_t* alias, float* pdf);
 uint32_t dlo_spiral_blocks(int W, int H, int* out);
+
+// Planted mistakes, used only by the tests that show each pin of the oracle
+// fails when the oracle is wrong.  0 (none) everywhere else.  Environment
+// tables take the setting when the map is added.
+enum {
+    DLO_MUT_NONE = 0,
+    DLO_MUT_GGX_PDF_NO_WHZ = 1,         // GGX lobe pdf D/(4 wh.wo) without wh.z
+    DLO_MUT_DIELECTRIC_SWAP_IOR = 2,    // intIOR and extIOR exchanged
+    DLO_MUT_DIFFUSE_UNIFORM_PDF = 3,    // diffuse pdf 1/(2 pi) for a cosine sampler
+    DLO_MUT_EMITTER_HIT_NO_MIS = 4,     // emitter hits weighted 1 beside MIS-weighted NEE
+    DLO_MUT_AREA_PDF_NO_DIST2 = 5,      // area light solid-angle pdf without dist^2
+    DLO_MUT_CONDUCTOR_SWAP_ETA_K = 6,   // conductor Fresnel with eta and k exchanged
+    DLO_MUT_RR_NO_DIVIDE = 7,           // Russian roulette survivors not divided by p
+    DLO_MUT_MICROFACET_G_WI_TWICE = 8,  // Beckmann G = G1(wi) G1(wi)
+    DLO_MUT_POINT_INV_DIST = 9,         // point light I cos / d instead of / d^2
+    DLO_MUT_ENV_PDF_TIMES_AREA = 10,    // environment pdf times the pixel's solid angle
+    DLO_MUT_DIELECTRIC_NO_FRESNEL = 11, // dielectric always refracts (no Fresnel choice)
+};
+void dlo_set_mutation(int m);
 
 #ifdef __cplusplus
 }
