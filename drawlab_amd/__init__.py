@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libdrawlab_hip.so")
 REPO_ROOT = os.path.dirname(_HERE)
 
 BUILD_GPU, BUILD_HOST_SAH, BUILD_GPU_LBVH = 0, 1, 2
-RENDER_NORMALS = 16
+RENDER_GUIDED, RENDER_NORMALS = 4, 16
 LIGHT_AREA, LIGHT_POINT, LIGHT_ENV = 0, 1, 2
 
 EXPORTED_SYMBOLS = (
@@ -669,11 +669,13 @@ class Service:
         self.sizes[sid.value] = (sd.camera.height, sd.camera.width)
         return sid.value
 
-    def submit(self, scene_id, spp, seed=0, max_depth=-1, rr_depth=4, spp_per_pass=0, first_sample=0, camera=None):
+    def submit(self, scene_id, spp, seed=0, max_depth=-1, rr_depth=4, spp_per_pass=0, first_sample=0, camera=None,
+               guided=False):
         """camera: None (the scene's camera) or a drawlab_amd.scene.Camera /
-        dl_camera_desc for this job (e.g. another resolution)."""
+        dl_camera_desc for this job (e.g. another resolution); guided: render
+        with the learned proposal (the job renders alone on its lane)."""
         p = dl_render_params(int(spp), int(seed), int(max_depth), int(rr_depth), int(spp_per_pass),
-                             int(first_sample), 0)
+                             int(first_sample), RENDER_GUIDED if guided else 0)
         jid = ctypes.c_uint64()
         cd = None
         if camera is not None:

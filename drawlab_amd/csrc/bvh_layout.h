@@ -49,15 +49,20 @@ constexpr int kMaxLeafSize = 4;
 // Traversal stack: kLdsStack entries in LDS per lane (8 KB per 128-lane
 // block, so LDS does not limit occupancy below the 8 waves/SIMD the VGPR
 // budget allows), the rest in a global overflow buffer.  No binary tree is
-// deeper than kMaxBinaryDepth internal levels (the host builder caps depth
-// at kMaxDepth, an LBVH over 64-bit keys has at most 64 levels; PLOC is
-// checked), so the 4-wide tree has at most as many levels, and a 4-wide
-// visit pushes at most 3 entries: kMaxStack = 3 x kMaxBinaryDepth entries
-// always suffice.
+// deeper than kMaxBinaryDepth internal levels (the host builder: SAH splits
+// down to depth kMaxDepth, then object-median splits until a leaf holds at
+// most kMaxLeafCount triangles, at most 24 more levels for 2^32 triangles;
+// the GPU builders' depth is checked and a deeper tree is rebuilt by the
+// host builder), so the 4-wide tree has at most as many levels, and a
+// 4-wide visit pushes at most 3 entries: kMaxStack = 3 x kMaxBinaryDepth
+// entries always suffice.
 constexpr int kLdsStack = 16;
 constexpr int kMaxBinaryDepth = 64;
 constexpr int kMaxStack = 3 * kMaxBinaryDepth;
-constexpr int kMaxDepth = 48;
+constexpr int kMaxDepth = 40;
+// Triangles of one leaf: the 4-wide node stores each child's count in 8 bits.
+constexpr int kMaxLeafCount = 255;
+static_assert(kMaxDepth + 24 <= kMaxBinaryDepth, "median levels below the SAH depth cap");
 
 struct BvhNodeH {
     float r0[4], r1[4], r2[4];

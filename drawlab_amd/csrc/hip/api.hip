@@ -85,6 +85,10 @@ void free_stashed(Ctx* c, SceneState& s) {
 // Camera descriptor checks shared by upload and set_camera.
 dl_status check_camera(Ctx* c, const dl_camera_desc& cam) {
     if (cam.width <= 0 || cam.height <= 0) return set_error(c, DL_ERR_INVALID, "bad camera size");
+    // one sample per pixel must fit the largest pass (2^27 paths); pixel
+    // indices are 32-bit throughout the integrator and film
+    if ((uint64_t)cam.width * (uint64_t)cam.height > (1ull << 27))
+        return set_error(c, DL_ERR_INVALID, "camera has more than 2^27 pixels");
     if (cam.type != DL_CAMERA_VIRTUAL && cam.type != DL_CAMERA_PERSPECTIVE && cam.type != DL_CAMERA_OPENCV)
         return set_error(c, DL_ERR_INVALID, "unknown camera type");
     if (cam.type == DL_CAMERA_OPENCV) {
@@ -383,9 +387,26 @@ dl_status dl_hip_build_accel(void* h, int builder, dl_accel_stats* stats) {
         c->ds.tris = nullptr;
     }
     dl_accel_stats st{};
-    if (builder == DL_BUILD_GPU || builder == DL_BUILD_GPU_LBVH) {
-        DL_TRY(build_bvh_gpu(c, &st, builder));
-    } else if (builder == DL_BUILD_HOST_SAH) {
+    if (builder != DL_BUILD_GPU && builder != DL_BUILD_GPU_LBVH && builder != DL_BUILD_HOST_SAH)
+        return set_error(c, DL_ERR_INVALID, "unknown builder");
+    bool host = builder == DL_BUILD_HOST_SAH;
+    if (!host) {
+        c->bvh_too_deep = false;
+        const dl_status s = build_bvh_gpu(c, &st, builder);
+        if (s != DL_OK && !c->bvh_too_deep) return s;
+        if (s != DL_OK) {
+            // a tree deeper than the traversal stack allows (pathological
+            // geometry): rebuilt by the host builder, whose depth is bounded
+            dfree(c, c->ds.nodes);
+            dfree(c, c->ds.tris);
+            c->ds.nodes = nullptr;
+            c->ds.tris = nullptr;
+            st = dl_accel_stats{};
+            host = true;
+            c->err.clear();
+        }
+    }
+    if (host) {
         auto t0 = std::chrono::steady_clock::now();
         HostBvh bvh;
         build_bvh_host(c->tri_verts.data(), c->nprims, bvh);
@@ -403,8 +424,6 @@ dl_status dl_hip_build_accel(void* h, int builder, dl_accel_stats* stats) {
         st.num_leaves = bvh.num_leaves;
         st.max_depth = bvh.max_depth;
         st.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    } else {
-        return set_error(c, DL_ERR_INVALID, "unknown builder");
     }
     {
         const auto t0 = std::chrono::steady_clock::now();

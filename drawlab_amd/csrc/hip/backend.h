@@ -107,6 +107,7 @@ struct Ctx {
     char devname[256] = {0};
     int num_cus = 0;
     double last_kernel_ms = 0.0;
+    bool bvh_too_deep = false;  // the last GPU build's tree exceeded kMaxBinaryDepth
     // traversal stack overflow (entries kLdsStack..kMaxStack of every lane of
     // a persistent traversal grid)
     int* stack_ovf = nullptr;

@@ -424,7 +424,10 @@ dl_status build_bvh_lbvh(Ctx* c, dl_accel_stats* stats) {
     stats->num_leaves = hb[7];
     stats->max_depth = hb[6];
     stats->build_ms = ms;
-    if (hb[6] > (uint32_t)kMaxBinaryDepth) return set_error(c, DL_ERR_STATE, "LBVH deeper than the traversal stack");
+    if (hb[6] > (uint32_t)kMaxBinaryDepth) {
+        c->bvh_too_deep = true;
+        return set_error(c, DL_ERR_STATE, "LBVH deeper than the traversal stack");
+    }
     return DL_OK;
 }
 
@@ -795,7 +798,10 @@ dl_status build_bvh_ploc(Ctx* c, dl_accel_stats* stats) {
     stats->num_leaves = hb[7];
     stats->max_depth = hb[6];
     stats->build_ms = ms;
-    if (hb[6] > (uint32_t)kMaxBinaryDepth) return set_error(c, DL_ERR_STATE, "PLOC tree deeper than the traversal stack");
+    if (hb[6] > (uint32_t)kMaxBinaryDepth) {
+        c->bvh_too_deep = true;
+        return set_error(c, DL_ERR_STATE, "PLOC tree deeper than the traversal stack");
+    }
     return DL_OK;
 }
 

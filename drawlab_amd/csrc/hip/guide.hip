@@ -21,6 +21,7 @@
 #include <cstring>
 #include <vector>
 
+#include "../render_sizes.h"
 #include "backend.h"
 #include "guide.h"
 
@@ -413,6 +414,20 @@ __global__ void k_resample(const float* __restrict__ rec, uint32_t R, const floa
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// the device bytes the render service reserves for a guided render
+static_assert(kGuideBatch == dl_sizes::kGuideBatchRows && kChunk == (int)dl_sizes::kGuideChunkRows &&
+                  kScanTile == (int)dl_sizes::kGuideScanTile && kGParams == dl_sizes::kGuideParams &&
+                  kGWSlot == dl_sizes::kGuideWSlot,
+              "guided render sizes");
+static_assert(dl_sizes::kGuideNetBytes == 4ull * kGParams * sizeof(float) + 5 * 64 * sizeof(float) +
+                                              2ull * 5 * kGWSlot * sizeof(uint16_t) + 4 * sizeof(float),
+              "guide_init allocations");
+static_assert(dl_sizes::kGuideTrainBytes ==
+                  (uint64_t)kGuideBatch * ((glayer_in(0) + 4 * glayer_in(1)) * sizeof(uint16_t) +
+                                           (kGOut + 2 * 64) * sizeof(float) + (kGIn + 3 + 1) * sizeof(float)) +
+                      (uint64_t)(kGuideBatch / kChunk) * (64 * 96 + 64) * sizeof(float),
+              "ensure_train and training batch allocations");
+
 struct GuideState {
     // training batch built from render records
     uint32_t batch_cap = 0;

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "../render_sizes.h"
 #include "backend.h"
 #include "guide.h"
 #include "shading.h"
@@ -1113,7 +1114,15 @@ dl_status ensure_state(Ctx* c, uint32_t P, uint32_t pixels) {
     return DL_OK;
 }
 
-constexpr uint32_t kGuideRecords = 1u << 23;  // training record slots per pass (96 B each)
+constexpr uint32_t kGuideRecords = dl_sizes::kGuideRecordSlots;  // training record slots per pass (96 B each)
+static_assert(kGRecFloats == dl_sizes::kGuideRecordFloats && kGMix == dl_sizes::kGuideMixFloats,
+              "guided buffer sizes the render service reserves");
+// the per-path bytes ensure_state allocates, as the render service reserves them
+static_assert(9 * sizeof(float4) + sizeof(uint64_t) + sizeof(uint32_t) + sizeof(float2) + sizeof(uint8_t) +
+                      2 * kTypes * sizeof(uint32_t) + 3 * sizeof(uint32_t) ==
+                  dl_sizes::kStateBytesPerPath &&
+              kTypes == dl_sizes::kMatQueueSegs && sizeof(float4) + 3 * sizeof(float) == dl_sizes::kFilmBytesPerPixel,
+              "path state bytes the render service reserves");
 
 dl_status ensure_guide_bufs(Ctx* c, RenderState* rs, uint32_t cap) {
     if (rs->gmix_cap < cap) {

@@ -55,6 +55,7 @@ struct Child {
 constexpr int kBins = 32;
 
 struct Builder {
+    int sah_depth = kMaxDepth;
     std::vector<Ref> refs;
     std::vector<BvhNodeH> nodes;
     std::atomic<uint32_t> next_node{0};
@@ -166,10 +167,15 @@ struct Builder {
             cb.grow_pt(refs[i].c);
         }
         uint32_t n = e - b;
-        if (n <= (uint32_t)kMaxLeafSize || depth + 1 >= (uint32_t)kMaxDepth) return make_leaf(b, e, box, depth);
-        uint32_t mid = split(b, e, cb);
+        if (n <= (uint32_t)kMaxLeafSize) return make_leaf(b, e, box, depth);
+        // Below the SAH depth cap, object-median splits until a leaf fits
+        // the 4-wide node's 8-bit triangle count.
+        const bool capped = depth + 1 >= (uint32_t)sah_depth;
+        if (capped && n <= (uint32_t)kMaxLeafCount) return make_leaf(b, e, box, depth);
+        uint32_t mid = capped ? b : split(b, e, cb);
         if (mid == b) {
-            // Coincident centroids: median split in index order.
+            // Coincident centroids (or the depth cap): median split, ties in
+            // index order.
             int ax = 0;
             for (int i = 1; i < 3; i++)
                 if (cb.hi[i] - cb.lo[i] > cb.hi[ax] - cb.lo[ax]) ax = i;
@@ -198,8 +204,9 @@ struct Builder {
 
 }  // namespace
 
-void build_bvh_host(const float* tri_verts, uint32_t num_tris, HostBvh& out) {
+void build_bvh_host(const float* tri_verts, uint32_t num_tris, HostBvh& out, int sah_depth) {
     Builder bld;
+    bld.sah_depth = std::min(sah_depth, kMaxDepth);
     bld.refs.resize(num_tris);
     for (uint32_t i = 0; i < num_tris; i++) {
         const float* p = tri_verts + 9 * (size_t)i;

@@ -16,6 +16,8 @@ struct HostBvh {
 };
 
 // tri_verts: 9 floats per triangle (p0, p1, p2), indexed by global primitive id.
-void build_bvh_host(const float* tri_verts, uint32_t num_tris, HostBvh& out);
+// sah_depth: depth below which nodes are split at the object median instead
+// of by SAH (kMaxDepth; smaller values only in the builder's tests).
+void build_bvh_host(const float* tri_verts, uint32_t num_tris, HostBvh& out, int sah_depth = kMaxDepth);
 
 }  // namespace dl

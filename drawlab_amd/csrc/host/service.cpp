@@ -45,14 +45,15 @@
 #include <vector>
 
 #include "../../../include/dl_hip.h"
+#include "../render_sizes.h"
 
 namespace {
 
 using Clock = std::chrono::steady_clock;
 
 constexpr uint64_t kPathsPerPass = 64u << 20;  // paths per pass of a job or a shared group (half the largest pass)
-constexpr uint64_t kStateBytesPerPath = 240;    // path state + queues (integrator RenderState)
-constexpr uint64_t kFilmBytesPerPixel = 28;     // accumulation film (16 B) + resolved RGB (12 B)
+constexpr uint64_t kStateBytesPerPath = dl_sizes::kStateBytesPerPath;  // path state + queues
+constexpr uint64_t kFilmBytesPerPixel = dl_sizes::kFilmBytesPerPixel;  // accumulation film + resolved RGB
 constexpr uint64_t kBytesPerVertex = 36;        // positions, normals, tangents
 constexpr uint64_t kBytesPerPrim = 16 + 112 + 256;  // indices + mesh map, BVH nodes + triangles, build scratch
 constexpr uint64_t kBytesPerTexel = 32;         // environment texels + alias tables
@@ -130,8 +131,11 @@ uint32_t pass_spp(const dl_render_params& p, uint64_t pixels) {
 uint64_t reservation_for(const Lane& lane, const Job& job, const SceneEntry& s) {
     const uint64_t px = job_pixels(job);
     const uint64_t paths = px * pass_spp(job.params, px);
-    const uint64_t state = std::max(paths, lane.state_paths) * kStateBytesPerPath +
-                           std::max(px, lane.state_pixels) * kFilmBytesPerPixel + kSlackBytes;
+    uint64_t state = std::max(paths, lane.state_paths) * kStateBytesPerPath +
+                     std::max(px, lane.state_pixels) * kFilmBytesPerPixel + kSlackBytes;
+    // a guided render adds its mixtures, training records, network and
+    // training batch
+    if (job.params.flags & DL_RENDER_GUIDED) state += dl_sizes::guide_bytes(std::max(paths, lane.state_paths));
     uint64_t others = 0;
     for (const auto& kv : lane.kept)
         if (kv.first != job.scene) others += kv.second;

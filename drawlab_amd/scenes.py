@@ -415,6 +415,45 @@ def mis_plates(width=1280, height=720, spp=512, max_depth=6):
     return s
 
 
+def material_zoo(width=64, height=48, spp=16, max_depth=10, filter_radius=2.0, filter_sigma=0.5):
+    """Every BSDF of the reference side by side: a mirror sphere, Beckmann
+    microfacet spheres (the default kd 0.5 and a rough one with a coloured
+    kd and extIOR 1.2), a microfacet box with kd 0 (specular only), a glass
+    sphere with intIOR 1.7 inside a medium of extIOR 1.33, a GGX conductor
+    sphere, a diffuse floor and a default-microfacet back wall, lit by an
+    area light and a point light."""
+    s = SceneDesc(name="material_zoo")
+    floor = s.add_bsdf(Bsdf.diffuse((0.6, 0.55, 0.5)))
+    wall = s.add_bsdf(Bsdf.microfacet())
+    mirror = s.add_bsdf(Bsdf.mirror())
+    mf_rough = s.add_bsdf(Bsdf.microfacet(0.4, 1.7, 1.2, (0.2, 0.3, 0.1)))
+    mf_spec = s.add_bsdf(Bsdf.microfacet(0.25, 1.5, 1.0, (0.0, 0.0, 0.0)))
+    glass = s.add_bsdf(Bsdf.dielectric(1.7, 1.33))
+    cond = s.add_bsdf(Bsdf.conductor(0.15))
+    lamp = s.add_bsdf(Bsdf.diffuse((0.0, 0.0, 0.0)))
+    V, F, _ = quad((-3.0, 0.0, -3.0), (0, 0, 6.0), (6.0, 0, 0))
+    s.add_mesh(V, F, floor, name="floor")
+    V, F, _ = quad((-3.0, 0.0, -2.0), (6.0, 0, 0), (0, 4.0, 0))
+    s.add_mesh(V, F, wall, name="wall")
+    spheres = [(mirror, (-1.6, 0.5, -0.6)), (mf_rough, (-0.5, 0.5, -0.9)), (glass, (0.6, 0.5, -0.4)),
+               (cond, (1.7, 0.5, -0.8))]
+    for m, c in spheres:
+        V, F = icosphere(3, radius=0.45, center=c)
+        s.add_mesh(V, F, m, name="sphere")
+    V, F = box((-0.9, 0.0, 0.4), (-0.3, 0.5, 1.0))
+    s.add_mesh(V, F, mf_spec, name="box")
+    V, F = icosphere(2, radius=0.3, center=(0.4, 0.35, 0.8))
+    s.add_mesh(V, F, wall, name="default_microfacet_sphere")
+    V, F, _ = quad((-0.6, 3.2, -0.8), (1.2, 0, 0), (0, 0, 0.9))
+    s.add_mesh(V, F, lamp, radiance=(14.0, 12.0, 9.0), name="light")
+    s.point_lights.append(((2.0, 2.5, 2.0), (6.0, 7.0, 9.0)))
+    s.camera = Camera(eye=(0.0, 2.0, 5.0), lookat=(0.0, 0.6, 0.0), up=(0, 1, 0), fov=45.0, width=width,
+                      height=height)
+    s.rfilter = RFilter(FILTER_GAUSSIAN, float(filter_radius), float(filter_sigma))
+    s.spp, s.max_depth, s.rr_depth = spp, max_depth, 3
+    return s
+
+
 def random_scene(seed, target_tris=50_000, width=512, height=512, spp=64):
     """Procedural job scene: displaced spheres and boxes with random diffuse /
     GGX-conductor / dielectric materials and 1-8 spherical area lights."""

@@ -176,7 +176,10 @@ typedef struct dl_accel_stats {
 } dl_accel_stats;
 
 /* DL_BUILD_GPU: PLOC clustering on the device (SAH-like quality);
-   DL_BUILD_HOST_SAH: binned SAH on the host; DL_BUILD_GPU_LBVH: Karras LBVH. */
+   DL_BUILD_HOST_SAH: binned SAH on the host (object-median splits below
+   depth 40, leaves of at most 255 triangles); DL_BUILD_GPU_LBVH: Karras LBVH.
+   A GPU-built tree deeper than 64 levels (pathological geometry) is rebuilt
+   by the host builder within the same call. */
 enum { DL_BUILD_GPU = 0, DL_BUILD_HOST_SAH = 1, DL_BUILD_GPU_LBVH = 2 };
 
 typedef struct dl_render_params {

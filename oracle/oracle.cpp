@@ -1006,8 +1006,8 @@ static V3 emitter_sample(const Scene& s, const Emitter& e, V2 smp, const Its& it
         ds = dir_sample(its.p, position, normal);
         float pA = m.normalization;
         float dp = std::max(dot(normal, -ds.d), 0.0f);
-        float d2 = mut(DLO_MUT_AREA_PDF_NO_DIST2) ? 1.0f : ds.dist * ds.dist;
-        float pw = dp != 0 ? pA * d2 / dp : 0;
+        // pA * dist * dist / dp, evaluated left to right as area.cpp:29
+        float pw = dp == 0 ? 0 : mut(DLO_MUT_AREA_PDF_NO_DIST2) ? pA / dp : pA * ds.dist * ds.dist / dp;
         ds.pdf = pw;
         return pw != 0 ? (e.radiance * std::max(dot(its.sh.n, ds.d), 0.0f)) / pw : mk(0, 0, 0);
     }
@@ -1085,8 +1085,7 @@ static float pdf_emitter_direction(const Scene& s, int e, const DirSample& ds) {
         // AreaLight::pdfDirection  area.cpp:39-44
         float pA = s.meshes[em.mesh].normalization;
         float dp = std::max(dot(ds.n, -ds.d), 0.0f);
-        float d2 = mut(DLO_MUT_AREA_PDF_NO_DIST2) ? 1.0f : ds.dist * ds.dist;
-        p = dp != 0 ? pA * d2 / dp : 0;
+        p = dp == 0 ? 0 : mut(DLO_MUT_AREA_PDF_NO_DIST2) ? pA / dp : pA * ds.dist * ds.dist / dp;
     } else if (em.type == DLO_EMITTER_POINT) {
         p = 0;
     } else {

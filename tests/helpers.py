@@ -141,6 +141,12 @@ def tile_z(batches_a, batches_b, t=16):
     (z, degrees of freedom of the smaller side)."""
     A = np.stack([tile_means(b, t) for b in batches_a])
     B = np.stack([tile_means(b, t) for b in batches_b])
+    return tile_z_means(A, B)
+
+
+def tile_z_means(A, B):
+    """tile_z from per-batch tile means (K, tiles_y, tiles_x, 3)."""
+    A, B = np.asarray(A, np.float64), np.asarray(B, np.float64)
     va = A.var(axis=0, ddof=1) / len(A)
     vb = B.var(axis=0, ddof=1) / len(B)
     return (A.mean(axis=0) - B.mean(axis=0)) / np.sqrt(va + vb + 1e-30), min(len(A), len(B)) - 1

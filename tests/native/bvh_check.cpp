@@ -1,7 +1,8 @@
 // Structural checks of the host BVH builder, run on the CPU before any GPU
 // traversal: every child reference is in range, leaves cover each primitive
 // exactly once, every triangle lies inside all its ancestors' child boxes,
-// and no path is deeper than the traversal stack.
+// no leaf holds more triangles than a 4-wide node can count, and no path is
+// deeper than the traversal stack.
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -10,9 +11,9 @@
 
 using namespace dl;
 
-extern "C" int bvh_check(const float* tri_verts, uint32_t n, uint32_t* stats) {
+extern "C" int bvh_check(const float* tri_verts, uint32_t n, uint32_t* stats, int sah_depth) {
     HostBvh b;
-    build_bvh_host(tri_verts, n, b);
+    build_bvh_host(tri_verts, n, b, sah_depth);
     const uint32_t nn = (uint32_t)b.nodes.size();
     std::vector<int> seen(n, 0);
     struct Item { int node; int depth; float lo[3], hi[3]; };
@@ -24,7 +25,7 @@ extern "C" int bvh_check(const float* tri_verts, uint32_t n, uint32_t* stats) {
         Item it = st.back();
         st.pop_back();
         if (it.node < 0 || (uint32_t)it.node >= nn) return 1;
-        if (it.depth > kMaxDepth) return 2;
+        if (it.depth > kMaxBinaryDepth) return 2;
         maxd = it.depth > (int)maxd ? it.depth : maxd;
         const BvhNodeH& nd = b.nodes[it.node];
         for (int c = 0; c < 2; c++) {
@@ -39,6 +40,7 @@ extern "C" int bvh_check(const float* tri_verts, uint32_t n, uint32_t* stats) {
                 leaves++;
                 uint32_t first = (uint32_t)(~ref);
                 if (cnt < 0 || first + (uint32_t)cnt > n) return 4;
+                if (cnt > kMaxLeafCount) return 10;
                 for (int k = 0; k < cnt; k++) {
                     const BvhTriH& t = b.tris[first + k];
                     uint32_t id;

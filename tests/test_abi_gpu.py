@@ -53,6 +53,8 @@ def test_malformed_scenes_and_parameters(backend):
         (lambda x: setattr(x.meshes[0], "bsdf", 5), "mesh material index out of range"),
         (lambda x: setattr(x.meshes[0], "F", np.uint32([[0, 1, 7]])), "vertex index out of range"),
         (lambda x: setattr(x, "camera", dataclasses.replace(x.camera, width=0)), "bad camera size"),
+        (lambda x: setattr(x, "camera", dataclasses.replace(x.camera, width=65536, height=65537)),
+         "more than 2\\^27 pixels"),
     ]:
         bad = _one_triangle()
         mutate(bad)
@@ -61,6 +63,11 @@ def test_malformed_scenes_and_parameters(backend):
     # the context still renders a good scene
     backend.upload_scene(d)
     backend.build_accel(drawlab_amd.BUILD_GPU)
+    # a camera whose pixel count would wrap 32-bit pixel indices is refused
+    # before any kernel runs
+    with pytest.raises(drawlab_amd.DlError, match="more than 2\\^27 pixels"):
+        backend.set_camera(dataclasses.replace(d.camera, width=65536, height=65537))
+    backend.set_camera(d.camera)
     with pytest.raises(drawlab_amd.DlError, match="spp must be positive"):
         backend.render(spp=0)
     with pytest.raises(drawlab_amd.DlError, match="pass too large"):

@@ -124,3 +124,21 @@ def test_trained_network_gains_over_plain_sampling(backend):
           flush=True)
     assert frozen < plain / 2
     assert online < plain / 1.5
+
+
+def test_guided_on_the_displaced_sphere_workload_unbiased(backend):
+    """The learned proposal on the 2,097,152-triangle displaced-sphere scene
+    (conductor, glass, ground under the sky and sun; reduced resolution):
+    independent guided batches, each training the network online as a
+    render does, agree per tile with independent plain batches."""
+    d = scenes.displaced_sphere(width=96, height=64, max_depth=16)
+    backend.upload_scene(d)
+    backend.build_accel(drawlab_amd.BUILD_GPU)
+    plain, _ = _batches(backend, 16, 256, 800)
+    backend.guide_init(seed=7)
+    guided, st = _batches(backend, 16, 256, 900, guided=True)
+    print("guided stats", st, flush=True)
+    assert st["train_steps"] > 0 and st["ms_guide"] > 0
+    for img in guided:
+        assert np.isfinite(img).all()
+    assert_unbiased(tile_z(guided, plain))

@@ -184,3 +184,75 @@ def test_sjf_runs_short_jobs_first():
             assert is_["t_start"] <= ib["t_start"]
     finally:
         svc.stop()
+
+
+def test_service_workload_mix(backend):
+    """The service's own job mix, bounded: jobs drawn by scenes.service_jobs
+    over the service scene pool (the Cornell box, the MIS plates, the
+    2,097,152-triangle displaced sphere and procedural scenes of 10k to
+    500k triangles), every pool scene at least once and a 1920x1080 job
+    among them, at a few samples per pixel, spread over four lanes.  Every
+    job's image is bit-identical to its standalone render, and no lane's
+    device memory exceeds its reservation or the budget."""
+    import dataclasses
+    pool = scenes.service_scene_pool()
+    picked, seen, full_hd = [], set(), False
+    for si, w, h, spp in scenes.service_jobs(512):
+        if si in seen and (full_hd or (w, h) != (1920, 1080)):
+            continue
+        seen.add(si)
+        full_hd |= (w, h) == (1920, 1080)
+        picked.append((si, w, h, min(spp, 4)))
+        if len(seen) == len(pool) and full_hd:
+            break
+    assert len(seen) == len(pool) and full_hd
+    print("jobs", picked, flush=True)
+    svc = drawlab_amd.Service(lanes=4, policy=drawlab_amd.POLICY_FIFO, budget_bytes=200 << 30)
+    try:
+        sids = [svc.add_scene(d) for d in pool]
+        ids = []
+        for k, (si, w, h, spp) in enumerate(picked):
+            d = pool[si]
+            cam = dataclasses.replace(d.camera, width=w, height=h)
+            ids.append(svc.submit(sids[si], spp, seed=100 + k, max_depth=d.max_depth, rr_depth=d.rr_depth,
+                                  camera=cam))
+        infos = [svc.wait(j) for j in ids]
+        assert all(i["state"] == drawlab_amd.JOB_DONE for i in infos), infos
+        imgs = [svc.result(j) for j in ids]
+        st = svc.stats()
+    finally:
+        svc.stop()
+    print("stats", st, flush=True)
+    assert st["estimate_misses"] == 0 and st["peak_reserved_bytes"] <= st["budget_bytes"]
+    assert len({i["lane"] for i in infos}) > 1
+    for i in infos:
+        assert i["peak_bytes"] <= i["reserved_bytes"], i
+    for k, ((si, w, h, spp), img, info) in enumerate(zip(picked, imgs, infos)):
+        d = pool[si]
+        print("standalone", k, d.name, w, h, spp, flush=True)
+        backend.upload_scene(dataclasses.replace(d, camera=dataclasses.replace(d.camera, width=w, height=h)))
+        backend.build_accel(drawlab_amd.BUILD_GPU)
+        ref, rst = backend.render(spp=spp, seed=100 + k, max_depth=d.max_depth, rr_depth=d.rr_depth)
+        assert img.shape == (h, w, 3) and info["paths"] == rst["paths"] == w * h * spp
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), (k, d.name)
+
+
+def test_guided_job_within_its_reservation():
+    """A guided job (which renders alone and allocates mixtures, training
+    records, the network and a training batch beside the path state) stays
+    within the device bytes the service reserved for it."""
+    cbox = scenes.cornell_box(64, 64, spp=16, max_depth=8, rr_depth=3)
+    svc = drawlab_amd.Service(lanes=2)
+    try:
+        s = svc.add_scene(cbox)
+        g = svc.submit(s, 16, seed=4, max_depth=8, rr_depth=3, guided=True)
+        p = svc.submit(s, 16, seed=5, max_depth=8, rr_depth=3)
+        ig, ip = svc.wait(g), svc.wait(p)
+        st = svc.stats()
+    finally:
+        svc.stop()
+    print("guided job", ig, "plain job", ip, "stats", st, flush=True)
+    assert ig["state"] == drawlab_amd.JOB_DONE and ip["state"] == drawlab_amd.JOB_DONE
+    assert ig["peak_bytes"] <= ig["reserved_bytes"] and ip["peak_bytes"] <= ip["reserved_bytes"]
+    assert st["estimate_misses"] == 0
+    assert ig["reserved_bytes"] > ip["reserved_bytes"] + (1 << 30)  # records + training batch

@@ -159,3 +159,46 @@ def test_quantised_boxes_contain_their_triangles(backend, builder):
     scene = float((hi[0] - lo[0]).max())
     print("leaf box extent mean / scene", float(ext.mean()) / scene, flush=True)
     assert ext.mean() < 0.05 * scene
+
+
+def _exponential_chain(n=100):
+    """n triangles at distances and sizes doubling along x: agglomerative
+    clustering merges them one at a time into a chain n - 1 levels deep."""
+    from drawlab_amd.scene import Bsdf, SceneDesc
+    s = 2.0 ** np.arange(n)
+    base = np.array([[0.0, 0.0, 0.0], [0.1, 0.0, 0.0], [0.0, 0.1, 0.05]])
+    V = (base[None] * s[:, None, None] + np.stack([s, 0.3 * s, 0.0 * s], 1)[:, None, :]).reshape(-1, 3)
+    d = SceneDesc()
+    d.add_mesh(V.astype(np.float32), np.arange(3 * n, dtype=np.uint32).reshape(n, 3), d.add_bsdf(Bsdf.diffuse()))
+    return d, V.reshape(n, 3, 3)
+
+
+def test_too_deep_gpu_tree_is_rebuilt_on_the_host(backend):
+    """A GPU-built tree deeper than the traversal stack allows is rebuilt by
+    the host builder within dl_hip_build_accel: the result is the host
+    builder's BVH, and hits stay bit-exact."""
+    d, tris = _exponential_chain()
+    backend.upload_scene(d)
+    backend.build_accel(drawlab_amd.BUILD_HOST_SAH)
+    host_nodes, host_tris = backend.accel_export()
+    st = backend.build_accel(drawlab_amd.BUILD_GPU)
+    print("accel", st, flush=True)
+    gpu_nodes, gpu_tris = backend.accel_export()
+    assert np.array_equal(gpu_nodes.view(np.uint32), host_nodes.view(np.uint32))
+    assert np.array_equal(gpu_tris.view(np.uint32), host_tris.view(np.uint32))
+    # rays straight down onto each triangle's centroid and its vertices
+    c = tris.mean(1)
+    scale = np.abs(tris).max(axis=(1, 2))
+    targets = np.concatenate([c, tris[:, 0], tris[:, 1], tris[:, 2]]).astype(np.float32)
+    up = np.tile(scale, 4)[:, None] * np.array([[0.0, 0.0, 1.0]])
+    o = (targets + up).astype(np.float32)
+    rays = np.zeros((8, len(o)), np.float32)
+    rays[0:3] = o.T
+    rays[3:6] = np.array([0.0, 0.0, -1.0], np.float32)[:, None]
+    rays[6], rays[7] = 0.0, np.inf
+    osc = oracle.from_desc(d)
+    g = backend.trace_closest(rays)
+    ref = osc.trace_closest(rays, oracle.ACCEL_BRUTE)
+    nbad, idx = compare_hits(g, ref)
+    assert nbad == 0, idx[:8]
+    assert (ref[0] != 0xFFFFFFFF).mean() > 0.2
