@@ -27,6 +27,7 @@ LIGHT_AREA, LIGHT_POINT, LIGHT_ENV = 0, 1, 2
 EXPORTED_SYMBOLS = (
     "dl_hip_create", "dl_hip_destroy", "dl_hip_last_error", "dl_hip_device_name", "dl_hip_upload_scene",
     "dl_hip_build_accel", "dl_hip_trace_closest", "dl_hip_trace_shadow", "dl_hip_trace_closest_counted",
+    "dl_hip_trace_closest_noalloc",
     "dl_hip_render", "dl_hip_render_paths", "dl_hip_malloc", "dl_hip_free", "dl_hip_memcpy_h2d",
     "dl_hip_memcpy_d2h", "dl_hip_memcpy_d2d", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_camera_perspective",
     "dl_camera_opencv", "dl_film_filter", "dl_hip_set_view_options", "dl_hip_render_batch",
@@ -140,6 +141,7 @@ def lib():
             "dl_hip_build_accel": (i32, [P, i32, c.POINTER(dl_accel_stats)]),
             "dl_hip_trace_closest": (i32, [P, P, u32, P, P, P, P]),
             "dl_hip_trace_closest_counted": (i32, [P, P, u32, P, P, P, P, P, P]),
+            "dl_hip_trace_closest_noalloc": (i32, [P, P, u32, P, P, P, P]),
             "dl_hip_trace_shadow": (i32, [P, P, u32, P]),
             "dl_hip_render": (i32, [P, c.POINTER(dl_render_params), P, c.POINTER(dl_render_stats)]),
             "dl_hip_render_paths": (i32, [P, c.POINTER(dl_render_params), u32, P]),

@@ -168,8 +168,14 @@ dl_status dl_hip_create(int device, void** out) {
     e = dmalloc(c, (void**)&c->stack_ovf, sizeof(int) * (size_t)c->trace_blocks * kTraceBlockHost * (kMaxStack - kLdsStack));
     if (e == hipSuccess) e = dmalloc(c, (void**)&c->work, 4 * sizeof(uint32_t));
     if (e != hipSuccess) {
-        delete c;
+        dl_hip_destroy(c);
         return DL_ERR_OOM;
+    }
+    // the device address of the allocation-free entry point's scratch, once
+    // per device, with the device current
+    if (noalloc_scratch_init(c) != DL_OK) {
+        dl_hip_destroy(c);
+        return DL_ERR_HIP;
     }
     *out = c;
     return DL_OK;
@@ -444,6 +450,14 @@ dl_status dl_hip_trace_closest(void* h, const float* rays, uint32_t n, uint32_t*
     if (!c || !rays || !prim || !t || !u || !v) return DL_ERR_INVALID;
     if (!c->has_accel) return set_error(c, DL_ERR_STATE, "trace before build_accel");
     return launch_trace_closest(c, rays, n, prim, t, u, v, nullptr, nullptr);
+}
+
+dl_status dl_hip_trace_closest_noalloc(void* h, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u,
+                                       float* v) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !rays || !prim || !t || !u || !v) return DL_ERR_INVALID;
+    if (!c->has_accel) return set_error(c, DL_ERR_STATE, "trace before build_accel");
+    return launch_trace_closest_noalloc(c, rays, n, prim, t, u, v);
 }
 
 dl_status dl_hip_trace_closest_counted(void* h, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u,

@@ -1,6 +1,8 @@
 // Batch traversal kernels behind dl_hip_trace_closest / dl_hip_trace_shadow.
 // Rays are 8 planes of n floats (32 B/ray in), hits 4 planes (16 B/ray out),
 // occlusion 1 B/ray out.
+#include <mutex>
+
 #include "backend.h"
 #include "trace.h"
 
@@ -58,9 +60,14 @@ __global__ void k_zero_work(uint32_t* w) {
     if (threadIdx.x < 4) w[threadIdx.x] = 0;
 }
 
+// Lanes of a persistent grid step through ray indices by the grid size; a
+// 32-bit index must not wrap past n.
+constexpr uint32_t kMaxBatchRays = 1u << 31;
+
 dl_status launch_trace_closest(Ctx* c, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u, float* v,
                                uint32_t* cn, uint32_t* ct) {
     if (n == 0) return DL_OK;
+    if (n > kMaxBatchRays) return set_error(c, DL_ERR_INVALID, "more than 2^31 rays in one batch");
     BvhView bvh{c->ds.nodes, c->ds.tris};
     const uint32_t blocks = c->trace_blocks;
     DL_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
@@ -81,12 +88,69 @@ dl_status launch_trace_closest(Ctx* c, const float* rays, uint32_t n, uint32_t* 
 
 dl_status launch_trace_shadow(Ctx* c, const float* rays, uint32_t n, uint8_t* occ) {
     if (n == 0) return DL_OK;
+    if (n > kMaxBatchRays) return set_error(c, DL_ERR_INVALID, "more than 2^31 rays in one batch");
     BvhView bvh{c->ds.nodes, c->ds.tris};
     const uint32_t blocks = c->trace_blocks;
     DL_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
     BatchIO io{rays, (size_t)n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, occ};
     k_zero_work<<<1, 64, 0, c->stream>>>(c->work);
     k_trace_shadow<<<blocks, kTraceBlock, 0, c->stream>>>(bvh, io, n, c->work, c->stack_ovf);
+    DL_HIP_CHECK(c, hipGetLastError());
+    DL_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
+    DL_HIP_CHECK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    DL_HIP_CHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_kernel_ms = ms;
+    return DL_OK;
+}
+
+// ---- allocation-free closest hits (dl_hip_trace_closest_noalloc) ---------
+// The closest-hit traversal's scratch -- the per-lane stack overflow of a
+// persistent grid of kNoAllocBlocks blocks, then its work counter -- in a
+// static array that each device holds until the process ends (92.3 MB)
+// instead of the context's buffers.  Host code never passes the array's
+// name to a kernel: it stands for a host-side shadow there.  Its device
+// address is looked up once per device at context creation
+// (noalloc_scratch_init) and kept in g_scratch.
+constexpr uint32_t kNoAllocBlocks = 1024;
+constexpr size_t kNoAllocOvfInts = (size_t)kNoAllocBlocks * kTraceBlock * (kMaxStack - kLdsStack);
+constexpr size_t kNoAllocInts = kNoAllocOvfInts + 64;
+static __device__ __align__(256) int scratch[kNoAllocInts];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+// With c->device current: the device address of this device's scratch.
+dl_status noalloc_scratch_init(Ctx* c) {
+    if (c->device < 0 || c->device >= kMaxDevices) return set_error(c, DL_ERR_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[c->device]) return DL_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(c, e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_scratch[c->device] = (int*)p;
+    return DL_OK;
+}
+
+dl_status launch_trace_closest_noalloc(Ctx* c, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u,
+                                       float* v) {
+    if (n == 0) return DL_OK;
+    if (n > kMaxBatchRays) return set_error(c, DL_ERR_INVALID, "more than 2^31 rays in one batch");
+    int* s = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (c->device >= 0 && c->device < kMaxDevices) s = g_scratch[c->device];
+    }
+    if (!s) return set_error(c, DL_ERR_STATE, "no scratch address for this device");
+    BvhView bvh{c->ds.nodes, c->ds.tris};
+    BatchIO io{rays, (size_t)n, prim, t, u, v, nullptr, nullptr, nullptr};
+    uint32_t* work = (uint32_t*)(s + kNoAllocOvfInts);
+    DL_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
+    k_zero_work<<<1, 64, 0, c->stream>>>(work);
+    k_trace_closest<false><<<kNoAllocBlocks, kTraceBlock, 0, c->stream>>>(bvh, io, n, work, s);
     DL_HIP_CHECK(c, hipGetLastError());
     DL_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
     DL_HIP_CHECK(c, hipEventSynchronize(c->ev1));

@@ -244,6 +244,19 @@ dl_status dl_hip_build_accel(void* ctx, int builder, dl_accel_stats* stats);
 dl_status dl_hip_trace_closest(void* ctx, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u, float* v);
 dl_status dl_hip_trace_shadow(void* ctx, const float* rays, uint32_t n, uint8_t* occluded);
 /* Same with counters: per-ray node visits and triangle tests (uint32 each). */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): dl_hip_trace_closest with the traversal's scratch (the
+ * stack overflow of a fixed persistent grid of 1024 blocks and its work
+ * counter) in a static per-device array of 92.3 MB, held until the process
+ * ends, instead of the context's buffers.  The call allocates nothing;
+ * inputs and outputs are the caller's device pointers, as for
+ * dl_hip_trace_closest, and the results are identical.  n is at most 2^31
+ * (DL_ERR_INVALID beyond).  Every call on one device uses that device's one
+ * array: calls that overlap on one GPU (several contexts or threads) race.
+ * Returns after the kernel has finished. */
+dl_status dl_hip_trace_closest_noalloc(void* ctx, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u,
+                                       float* v);
 dl_status dl_hip_trace_closest_counted(void* ctx, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u,
                                        float* v, uint32_t* node_visits, uint32_t* tri_tests);
 
