@@ -1,11 +1,15 @@
-// Scene XML loader: a small XML reader plus the tag semantics of the
-// reference's loadFromXML (src/core/parser/parser.cpp:19-387): object tags
-// are instantiated by `type` through the ObjectFactory with the properties of
-// their child tags, children are attached with addChild/setParent, then the
-// object is activated; `id` registers an object for later <ref id=".."/>;
-// <transform> accumulates translate / scale / rotate / matrix / lookat
-// operations, each applied on the left.  Errors are drawlab::Exception with
-// the file name and line/column, as the reference reports them.
+// Scene XML loader: a small XML reader and a table-driven loader for the
+// scene format of the reference's loadFromXML (src/core/parser/parser.cpp:
+// 19-387).  Each tag has a role (object, reference, property, <transform>,
+// transform operation) with its attributes and handler in one table; where a
+// role may appear is a second table.  Objects are instantiated by `type`
+// through the ObjectFactory with the properties their children set, get
+// their child objects, and are activated; `id` names an object for a later
+// <ref id=".."/>; a <transform> is the product of its operations, each
+// applied on the left.  The tag names and the error messages (with the file
+// name and line/column) are the reference's, so scenes and their errors read
+// the same.
+#include <algorithm>
 #include <cctype>
 #include <cstring>
 #include <cmath>
@@ -187,34 +191,223 @@ private:
     size_t pos_ = 0;
 };
 
-enum ETag {
-    TScene = Object::EScene,
-    TMesh = Object::EMesh,
-    TBSDF = Object::EBSDF,
-    TPhase = Object::EPhaseFunction,
-    TEmitter = Object::EEmitter,
-    TMedium = Object::EMedium,
-    TCamera = Object::ECamera,
-    TIntegrator = Object::EIntegrator,
-    TSampler = Object::ESampler,
-    TTest = Object::ETest,
-    TRFilter = Object::EReconstructionFilter,
-    TTexture = Object::ETexture,
-    TBoolean = Object::EClassTypeCount,
-    TInteger,
-    TFloat,
-    TString,
-    TPoint,
-    TVector,
-    TColor,
-    TTransform,
-    TTranslate,
-    TMatrix,
-    TRotate,
-    TScale,
-    TLookAt,
-    TRef,
-    TInvalid
+// ---- what each tag is ------------------------------------------------------
+// Every element plays one role.  Objects are built by the factory from the
+// properties their children set; a reference names an object built earlier;
+// a property sets one entry of its parent object's PropertyList; a
+// <transform> property is the product of the transform operations inside it.
+enum class Role { Object, Reference, Property, Transform, TransformOp };
+
+// Where an element sits: at the root, inside an object, inside a
+// <transform>, or inside any other element (a property or an operation).
+enum class Place { Root, InObject, InTransform, InLeaf };
+
+using PropertySetter = void (*)(PropertyList&, const std::string& name, const std::string& value);
+using OpMatrix = Matrix4f (*)(const XmlNode&);
+
+struct TagSpec {
+    Role role;
+    Object::EClassType cls;                 // Role::Object
+    std::vector<std::string> required;      // attributes
+    std::vector<std::string> optional;
+    PropertySetter set = nullptr;           // Role::Property
+    OpMatrix op = nullptr;                  // Role::TransformOp
+};
+
+Vector3f uniform_or_vector(const std::string& v) {
+    // <scale value="s"/> is a uniform scale, as the documented form reads
+    const std::vector<std::string> tok = tokenize(v);
+    return tok.size() == 1 ? Vector3f(toFloat(tok[0])) : toVector3f(v);
+}
+
+// Camera-to-world of a camera at `origin` looking at `target`: columns are
+// the camera's x axis (up x forward), its y axis (forward x x) and forward.
+Matrix4f look_at(const XmlNode& n) {
+    const Vector3f eye = toVector3f(n.attr("origin"));
+    const Vector3f fwd = normalized(toVector3f(n.attr("target")) - eye);
+    const Vector3f xaxis = normalized(cross(normalized(toVector3f(n.attr("up"))), fwd));
+    const Vector3f yaxis = normalized(cross(fwd, xaxis));
+    Matrix4f m;
+    const Vector3f* cols[4] = {&xaxis, &yaxis, &fwd, &eye};
+    for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 3; r++) m.m[r][c] = (*cols[c])[r];
+    m.m[3][3] = 1.f;
+    return m;
+}
+
+Matrix4f matrix_of(const XmlNode& n) {
+    const std::vector<std::string> tok = tokenize(n.attr("value"), ", \t\r\n");
+    if (tok.size() != 16) throw Exception("Expected 16 values");
+    Matrix4f m;
+    for (size_t k = 0; k < 16; ++k) m.m[k / 4][k % 4] = toFloat(tok[k]);
+    return m;
+}
+
+const std::map<std::string, TagSpec>& tag_table() {
+    static const std::map<std::string, TagSpec> t = [] {
+        std::map<std::string, TagSpec> m;
+        const std::vector<std::string> obj_req = {"type"}, obj_opt = {"name", "id"}, nv = {"name", "value"};
+        const std::pair<const char*, Object::EClassType> objects[] = {
+            {"scene", Object::EScene},          {"mesh", Object::EMesh},        {"bsdf", Object::EBSDF},
+            {"emitter", Object::EEmitter},      {"camera", Object::ECamera},    {"medium", Object::EMedium},
+            {"phase", Object::EPhaseFunction},  {"integrator", Object::EIntegrator},
+            {"sampler", Object::ESampler},      {"rfilter", Object::EReconstructionFilter},
+            {"test", Object::ETest},            {"texture", Object::ETexture}};
+        for (const auto& o : objects) m[o.first] = TagSpec{Role::Object, o.second, obj_req, obj_opt};
+        m["ref"] = TagSpec{Role::Reference, Object::EClassTypeCount, {"id"}, {}};
+        auto prop = [&](const char* tag, PropertySetter f) {
+            m[tag] = TagSpec{Role::Property, Object::EClassTypeCount, nv, {}, f};
+        };
+        prop("string", [](PropertyList& l, const std::string& k, const std::string& v) { l.setString(k, v); });
+        prop("float", [](PropertyList& l, const std::string& k, const std::string& v) { l.setFloat(k, toFloat(v)); });
+        prop("integer", [](PropertyList& l, const std::string& k, const std::string& v) { l.setInteger(k, toInt(v)); });
+        prop("boolean", [](PropertyList& l, const std::string& k, const std::string& v) { l.setBoolean(k, toBool(v)); });
+        prop("point", [](PropertyList& l, const std::string& k, const std::string& v) { l.setPoint(k, toVector3f(v)); });
+        prop("vector", [](PropertyList& l, const std::string& k, const std::string& v) { l.setVector(k, toVector3f(v)); });
+        prop("color", [](PropertyList& l, const std::string& k, const std::string& v) { l.setColor(k, toVector3f(v)); });
+        m["transform"] = TagSpec{Role::Transform, Object::EClassTypeCount, {"name"}, {}};
+        auto op = [&](const char* tag, std::vector<std::string> req, OpMatrix f) {
+            m[tag] = TagSpec{Role::TransformOp, Object::EClassTypeCount, std::move(req), {}, nullptr, f};
+        };
+        op("translate", {"value"}, [](const XmlNode& n) { return Matrix4f::translation(toVector3f(n.attr("value"))); });
+        op("scale", {"value"}, [](const XmlNode& n) { return Matrix4f::scaling(uniform_or_vector(n.attr("value"))); });
+        op("rotate", {"angle", "axis"}, [](const XmlNode& n) {
+            return Matrix4f::rotation(toVector3f(n.attr("axis")), degToRad(toFloat(n.attr("angle"))));
+        });
+        op("matrix", {"value"}, matrix_of);
+        op("lookat", {"origin", "target", "up"}, look_at);
+        return m;
+    }();
+    return t;
+}
+
+// Which roles may appear where, and the message for those that may not.
+enum class Misplaced { No, RootNotObject, OnlyTransformOps, NeedsObjectParent };
+Misplaced placement(Place where, Role role) {
+    static const Misplaced ok = Misplaced::No, root = Misplaced::RootNotObject, ops = Misplaced::OnlyTransformOps,
+                           obj = Misplaced::NeedsObjectParent;
+    //                          Object Reference Property Transform TransformOp
+    static const Misplaced table[4][5] = {{ok, ok, root, root, root},   // Root
+                                          {ok, ok, ok, ok, ops},        // InObject
+                                          {ops, ops, ops, ops, ok},     // InTransform
+                                          {obj, obj, obj, obj, ops}};   // InLeaf
+    return table[(int)where][(int)role];
+}
+
+class SceneLoader {
+public:
+    SceneLoader(const std::string& filename, std::function<std::string(size_t)> where)
+        : file_(filename), where_(std::move(where)) {}
+
+    Object* load(const XmlNode& root) {
+        PropertyList unused;
+        return visit(root, Place::Root, unused, nullptr);
+    }
+
+private:
+    template <typename... Args>
+    Exception error(const char* fmt, const Args&... args) const {
+        return Exception(("Error while parsing \"%s\": " + std::string(fmt)).c_str(), file_, args...);
+    }
+
+    void check_attributes(const XmlNode& n, const TagSpec& spec) const {
+        std::set<std::string> missing(spec.required.begin(), spec.required.end());
+        for (const auto& a : n.attrs) {
+            if (missing.erase(a.first)) continue;
+            if (std::find(spec.optional.begin(), spec.optional.end(), a.first) != spec.optional.end()) continue;
+            throw error("unexpected attribute \"%s\" in \"%s\" at %s", a.first, n.name, where_(n.offset));
+        }
+        if (!missing.empty())
+            throw error("missing attribute \"%s\" in \"%s\" at %s", *missing.begin(), n.name, where_(n.offset));
+    }
+
+    // Parses `node` found at `where`; a property lands in `props`, an
+    // operation multiplies `*tf`; objects and references are returned.
+    Object* visit(const XmlNode& node, Place where, PropertyList& props, Matrix4f* tf) {
+        if (node.kind == XmlNode::Comment) return nullptr;
+        if (node.kind != XmlNode::Element) throw error("unexpected content at %s", where_(node.offset));
+        const auto& table = tag_table();
+        const auto it = table.find(node.name);
+        if (it == table.end()) throw error("unexpected tag \"%s\" at %s", node.name, where_(node.offset));
+        const TagSpec& spec = it->second;
+        switch (placement(where, spec.role)) {
+            case Misplaced::RootNotObject:
+                throw error("root element \"%s\" must be a object (at %s)", node.name, where_(node.offset));
+            case Misplaced::OnlyTransformOps:
+                throw error("transform nodes can only contain transform operations (at %s)", where_(node.offset));
+            case Misplaced::NeedsObjectParent:
+                throw error("node \"%s\" requires a object as parent (at %s)", node.name, where_(node.offset));
+            case Misplaced::No: break;
+        }
+        if (spec.role == Role::Reference) {
+            check_attributes(node, spec);
+            const auto found = ids_.find(node.attr("id"));
+            if (found == ids_.end())
+                throw error("Can't find the referred node %s  (at %s)", node.attr("id"), where_(node.offset));
+            return found->second;
+        }
+        try {
+            switch (spec.role) {
+                case Role::Object: return build_object(node, spec);
+                case Role::Property:
+                    visit_children(node, Place::InLeaf, props, nullptr);
+                    check_attributes(node, spec);
+                    spec.set(props, node.attr("name"), node.attr("value"));
+                    return nullptr;
+                case Role::Transform: {
+                    Matrix4f product = Matrix4f::identity();  // each operation multiplies on the left
+                    visit_children(node, Place::InTransform, props, &product);
+                    check_attributes(node, spec);
+                    props.setTransform(node.attr("name"), Transform(product));
+                    return nullptr;
+                }
+                case Role::TransformOp:
+                    visit_children(node, Place::InLeaf, props, nullptr);
+                    check_attributes(node, spec);
+                    *tf = spec.op(node) * *tf;
+                    return nullptr;
+                case Role::Reference: break;
+            }
+        } catch (const Exception& e) {
+            const std::string msg = e.what();
+            if (msg.rfind("Error while parsing", 0) == 0) throw;
+            throw error("%s (at %s)", msg, where_(node.offset));
+        }
+        return nullptr;
+    }
+
+    std::vector<Object*> visit_children(const XmlNode& node, Place where, PropertyList& props, Matrix4f* tf) {
+        std::vector<Object*> objs;
+        for (const XmlNode& ch : node.children)
+            if (Object* o = visit(ch, where, props, tf)) objs.push_back(o);
+        return objs;
+    }
+
+    Object* build_object(const XmlNode& node, const TagSpec& spec) {
+        PropertyList props;
+        const std::vector<Object*> kids = visit_children(node, Place::InObject, props, nullptr);
+        XmlNode decl = node;
+        if (spec.cls == Object::EScene && !decl.has_attr("type")) decl.attrs.emplace_back("type", "scene");
+        check_attributes(decl, spec);
+        Object* obj = ObjectFactory::createInstance(decl.attr("type"), props, decl.attr("name"));
+        if (obj->getClassType() != spec.cls)
+            throw Exception("Unexpectedly constructed an object of type <%s> (expected type <%s>): %s",
+                            Object::classTypeName(obj->getClassType()), Object::classTypeName(spec.cls),
+                            obj->toString());
+        for (Object* k : kids) {
+            obj->addChild(k);
+            k->setParent(obj);
+        }
+        obj->activate();
+        const std::string id = decl.attr("id");
+        if (!id.empty() && !ids_.emplace(id, obj).second)
+            throw error("the node id has been used. (at %s)", where_(node.offset));
+        return obj;
+    }
+
+    std::string file_;
+    std::function<std::string(size_t)> where_;
+    std::map<std::string, Object*> ids_;  // objects with an id, for <ref>
 };
 
 }  // namespace
@@ -246,198 +439,7 @@ Object* loadFromXML(const std::string& filename) {
     } catch (const Exception& e) {
         throw Exception("Error while parsing \"%s\": %s", filename, e.what());
     }
-
-    static const std::map<std::string, ETag> tags = {
-        {"scene", TScene},     {"mesh", TMesh},           {"bsdf", TBSDF},         {"emitter", TEmitter},
-        {"camera", TCamera},   {"medium", TMedium},       {"phase", TPhase},       {"integrator", TIntegrator},
-        {"sampler", TSampler}, {"rfilter", TRFilter},     {"test", TTest},         {"texture", TTexture},
-        {"boolean", TBoolean}, {"integer", TInteger},     {"float", TFloat},       {"string", TString},
-        {"point", TPoint},     {"vector", TVector},       {"color", TColor},       {"transform", TTransform},
-        {"translate", TTranslate}, {"matrix", TMatrix},   {"rotate", TRotate},     {"scale", TScale},
-        {"lookat", TLookAt},   {"ref", TRef}};
-
-    auto check_attributes = [&](const XmlNode& node, std::set<std::string> attrs,
-                                const std::set<std::string>& opt = {}) {
-        for (const auto& a : node.attrs) {
-            auto it = attrs.find(a.first);
-            if (it == attrs.end()) {
-                if (!opt.count(a.first))
-                    throw Exception("Error while parsing \"%s\": unexpected attribute \"%s\" in \"%s\" at %s", filename,
-                                    a.first, node.name, where(node.offset));
-            } else {
-                attrs.erase(it);
-            }
-        }
-        if (!attrs.empty())
-            throw Exception("Error while parsing \"%s\": missing attribute \"%s\" in \"%s\" at %s", filename,
-                            *attrs.begin(), node.name, where(node.offset));
-    };
-
-    Transform transform;
-    std::map<std::string, Object*> pool;
-
-    std::function<Object*(const XmlNode&, PropertyList&, int)> parseTag = [&](const XmlNode& node, PropertyList& list,
-                                                                            int parentTag) -> Object* {
-        if (node.kind == XmlNode::Comment) return nullptr;
-        if (node.kind != XmlNode::Element)
-            throw Exception("Error while parsing \"%s\": unexpected content at %s", filename, where(node.offset));
-        auto it = tags.find(node.name);
-        if (it == tags.end())
-            throw Exception("Error while parsing \"%s\": unexpected tag \"%s\" at %s", filename, node.name,
-                            where(node.offset));
-        const int tag = it->second;
-
-        const bool hasParent = parentTag != TInvalid;
-        const bool parentIsObject = hasParent && parentTag < Object::EClassTypeCount;
-        const bool currentIsObject = tag < Object::EClassTypeCount || tag == TRef;
-        const bool parentIsTransform = parentTag == TTransform;
-        const bool currentIsTransformOp =
-            tag == TTranslate || tag == TRotate || tag == TScale || tag == TLookAt || tag == TMatrix;
-
-        if (!hasParent && !currentIsObject)
-            throw Exception("Error while parsing \"%s\": root element \"%s\" must be a object (at %s)", filename,
-                            node.name, where(node.offset));
-        if (parentIsTransform != currentIsTransformOp)
-            throw Exception("Error while parsing \"%s\": transform nodes can only contain transform operations (at %s)",
-                            filename, where(node.offset));
-        if (hasParent && !parentIsObject && !(parentIsTransform && currentIsTransformOp))
-            throw Exception("Error while parsing \"%s\": node \"%s\" requires a object as parent (at %s)", filename,
-                            node.name, where(node.offset));
-
-        XmlNode self = node;
-        if (tag == TScene && !self.has_attr("type")) self.attrs.emplace_back("type", "scene");
-        if (tag == TTransform) transform.setIdentity();
-
-        if (tag == TRef) {
-            check_attributes(self, {"id"});
-            const std::string id = self.attr("id");
-            auto p = pool.find(id);
-            if (p == pool.end())
-                throw Exception("Error while parsing \"%s\": Can't find the referred node %s  (at %s)", filename, id,
-                                where(node.offset));
-            return p->second;
-        }
-
-        PropertyList props;
-        std::vector<Object*> children;
-        for (const XmlNode& ch : self.children) {
-            Object* c = parseTag(ch, props, tag);
-            if (c) children.push_back(c);
-        }
-
-        Object* result = nullptr;
-        try {
-            if (currentIsObject) {
-                check_attributes(self, {"type"}, {"name", "id"});
-                result = ObjectFactory::createInstance(self.attr("type"), props, self.attr("name"));
-                if ((int)result->getClassType() != tag)
-                    throw Exception("Unexpectedly constructed an object of type <%s> (expected type <%s>): %s",
-                                    Object::classTypeName(result->getClassType()),
-                                    Object::classTypeName((Object::EClassType)tag), result->toString());
-                for (Object* c : children) {
-                    result->addChild(c);
-                    c->setParent(result);
-                }
-                result->activate();
-                const std::string id = self.attr("id");
-                if (!id.empty()) {
-                    if (pool.count(id))
-                        throw Exception("Error while parsing \"%s\": the node id has been used. (at %s)", filename,
-                                        where(node.offset));
-                    pool[id] = result;
-                }
-            } else {
-                switch (tag) {
-                    case TString:
-                        check_attributes(self, {"name", "value"});
-                        list.setString(self.attr("name"), self.attr("value"));
-                        break;
-                    case TFloat:
-                        check_attributes(self, {"name", "value"});
-                        list.setFloat(self.attr("name"), toFloat(self.attr("value")));
-                        break;
-                    case TInteger:
-                        check_attributes(self, {"name", "value"});
-                        list.setInteger(self.attr("name"), toInt(self.attr("value")));
-                        break;
-                    case TBoolean:
-                        check_attributes(self, {"name", "value"});
-                        list.setBoolean(self.attr("name"), toBool(self.attr("value")));
-                        break;
-                    case TPoint:
-                        check_attributes(self, {"name", "value"});
-                        list.setPoint(self.attr("name"), toVector3f(self.attr("value")));
-                        break;
-                    case TVector:
-                        check_attributes(self, {"name", "value"});
-                        list.setVector(self.attr("name"), toVector3f(self.attr("value")));
-                        break;
-                    case TColor:
-                        check_attributes(self, {"name", "value"});
-                        list.setColor(self.attr("name"), toVector3f(self.attr("value")));
-                        break;
-                    case TTransform:
-                        check_attributes(self, {"name"});
-                        list.setTransform(self.attr("name"), transform);
-                        break;
-                    case TTranslate:
-                        check_attributes(self, {"value"});
-                        transform = Transform(Matrix4f::translation(toVector3f(self.attr("value"))) * transform.matrix());
-                        break;
-                    case TMatrix: {
-                        check_attributes(self, {"value"});
-                        const std::vector<std::string> tok = tokenize(self.attr("value"), ", \t\r\n");
-                        if (tok.size() != 16) throw Exception("Expected 16 values");
-                        Matrix4f m;
-                        for (int i = 0; i < 4; ++i)
-                            for (int j = 0; j < 4; ++j) m.m[i][j] = toFloat(tok[i * 4 + j]);
-                        transform = Transform(m * transform.matrix());
-                    } break;
-                    case TScale: {
-                        check_attributes(self, {"value"});
-                        // a single value is a uniform scale (the documented
-                        // <scale value="5"/> form)
-                        const std::vector<std::string> tok = tokenize(self.attr("value"));
-                        const Vector3f v = tok.size() == 1 ? Vector3f(toFloat(tok[0])) : toVector3f(self.attr("value"));
-                        transform = Transform(Matrix4f::scaling(v) * transform.matrix());
-                    } break;
-                    case TRotate: {
-                        check_attributes(self, {"angle", "axis"});
-                        const float angle = degToRad(toFloat(self.attr("angle")));
-                        const Vector3f axis = toVector3f(self.attr("axis"));
-                        transform = Transform(Matrix4f::rotation(axis, angle) * transform.matrix());
-                    } break;
-                    case TLookAt: {
-                        check_attributes(self, {"origin", "target", "up"});
-                        const Vector3f origin = toVector3f(self.attr("origin"));
-                        const Vector3f target = toVector3f(self.attr("target"));
-                        const Vector3f up = toVector3f(self.attr("up"));
-                        const Vector3f dir = normalized(target - origin);
-                        const Vector3f left = normalized(cross(normalized(up), dir));
-                        const Vector3f newUp = normalized(cross(dir, left));
-                        Matrix4f t;
-                        for (int r = 0; r < 3; r++) {
-                            t.m[r][0] = left[r];
-                            t.m[r][1] = newUp[r];
-                            t.m[r][2] = dir[r];
-                            t.m[r][3] = origin[r];
-                        }
-                        t.m[3][3] = 1.f;
-                        transform = Transform(t * transform.matrix());
-                    } break;
-                    default: throw Exception("Unhandled element \"%s\"", self.name);
-                }
-            }
-        } catch (const Exception& e) {
-            const std::string msg = e.what();
-            if (msg.rfind("Error while parsing", 0) == 0) throw;
-            throw Exception("Error while parsing \"%s\": %s (at %s)", filename, msg, where(node.offset));
-        }
-        return result;
-    };
-
-    PropertyList list;
-    return parseTag(root, list, TInvalid);
+    return SceneLoader(filename, where).load(root);
 }
 
 }  // namespace drawlab

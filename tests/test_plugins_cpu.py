@@ -239,6 +239,19 @@ ERRORS = [
     ("<vector name=\"x\" value=\"1, 2\"/>", "Expected 3 values", False),
     ("<mesh type=\"rectangle\" bad=\"1\"/>", 'unexpected attribute "bad" in "mesh"', False),
     ("<translate value=\"1,2,3\"/>", "transform nodes can only contain transform operations", False),
+    # where each kind of element may appear
+    ("<mesh type=\"rectangle\"><transform name=\"toWorld\"><float name=\"x\" value=\"1\"/></transform></mesh>",
+     "transform nodes can only contain transform operations", False),
+    ("<mesh type=\"rectangle\"><float name=\"x\" value=\"1\"><bsdf type=\"diffuse\"/></float></mesh>",
+     'node "bsdf" requires a object as parent', False),
+    ("<mesh type=\"rectangle\"><float name=\"x\" value=\"1\"><translate value=\"1,2,3\"/></float></mesh>",
+     "transform nodes can only contain transform operations", False),
+    ("<mesh type=\"rectangle\"><transform name=\"toWorld\"><translate value=\"1,2,3\"><float name=\"x\" "
+     "value=\"1\"/></translate></transform></mesh>", 'node "float" requires a object as parent', False),
+    ("<mesh type=\"rectangle\"><float name=\"x\"/></mesh>", 'missing attribute "value" in "float"', False),
+    ("<transform name=\"t\"><translate value=\"1,2,3\"/></transform>", "must be a object", True),
+    ("<mesh type=\"rectangle\"><transform name=\"toWorld\"><matrix value=\"1 0 0\"/></transform></mesh>",
+     "Expected 16 values", False),
     ("<float name=\"x\" value=\"1\"/>", None, True),   # property as root
     ("<scene><camera type=\"virtual\"><vector name=\"eye\" value=\"0,0,1\"/><vector name=\"lookat\" value=\"0,0,0\"/>"
      "<vector name=\"up\" value=\"0,1,0\"/></camera></scene>", "No integrator was specified!", True),
