@@ -21,6 +21,12 @@ spp = floor(16 * 64^U); Poisson arrivals at 0.9x the one-at-a-time service
 rate.  Prints p50/p99 job latency of the multi-lane service and of
 one-at-a-time rendering, and both saturated throughputs (jobs/s).
 
+`--workload trace`: the traversal micro-bench on the same geometry -- three
+ray sets (2,073,600 primary rays, 2^22 diffuse-bounce rays and 2^22 shadow
+rays toward the sky and sun, both from the points the primary rays hit)
+through the batch traversal kernels, each with its own HBM roofline
+fraction in algorithmic bytes.
+
 `--workload guided`: the learned proposal against plain rendering at equal
 wall time on the same scene at 480x270 -- a 65,536-spp plain reference, then
 ladders of 16..4096 spp of both methods (the guided network trained from
@@ -52,6 +58,26 @@ METRIC = "Mpaths/s + relMSE@equal-time (guided vs plain); p50/p99 job latency"
 WORKLOAD = "displaced_sphere_2M"
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES = 64, 48, 32, 16  # 4-wide node, triangle, ray in, hit out
+OCC_BYTES = 1  # shadow query result
+# Path state each stage reads and writes per path or per shading vertex
+# (structure-of-arrays records of the integrator, 16-byte vectors unless
+# stated): camera ray generation writes ray origin and direction,
+# throughput, radiance, aux, the pcg32 state (8), sample index (4), film
+# position (8) and a queue entry (4); the first-hit stage reads the queue
+# entry, hit and direction and writes radiance and a material-queue entry;
+# a shading vertex reads its queue entry, hit, direction, throughput,
+# radiance, aux, pcg32 state and sample index and writes the shadow ray's
+# origin and direction and its NEE term, the continuation ray, throughput,
+# radiance, pcg32 state, aux and two queue entries; its post-processing
+# reads aux, radiance, NEE term, occlusion byte, throughput, hit, direction,
+# pcg32 state and sample index and writes radiance, aux, throughput, pcg32
+# state and a queue entry; the film gather reads each sample's position and
+# radiance once and writes a pixel per 64 samples.
+PATH_STATE_BYTES = {"raygen": 5 * 16 + 8 + 4 + 8 + 4,                         # 104
+                    "first_hit": 4 + 2 * 16 + 16 + 4,                          # 56
+                    "shade": (4 + 5 * 16 + 8 + 4) + (8 * 16 + 8 + 2 * 4),      # 96 read + 144 written
+                    "post": (4 + 6 * 16 + 1 + 8 + 4) + (3 * 16 + 8 + 4),       # 113 read + 60 written
+                    "film": 8 + 16 + 16 / 64.0}
 TRACE_KERNEL = "k_trace_queue<false>"
 COPY_KERNEL = "k_copy"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_queue_%s_traffic.json" % WORKLOAD)
@@ -68,6 +94,25 @@ def traffic_from_profiles():
         return None
 
 
+def host_cpus():
+    """CPUs this process may run on (its affinity mask: the machine's share
+    given to this job), the machine's logical CPU count and the CPU model."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return avail, os.cpu_count() or avail, model
+
+
 def cpu_baseline(desc, target_s, threads):
     import oracle
 
@@ -82,15 +127,79 @@ def cpu_baseline(desc, target_s, threads):
     blocks = int(min(total_blocks, max(threads, threads * target_s / wave_s)))
     _, st = osc.render(1, seed=7, max_depth=desc.max_depth, rr_depth=desc.rr_depth, threads=threads,
                        max_blocks=blocks, want_image=False)
+    avail, logical, model = host_cpus()
     return {
         "value": st["paths"] / st["seconds"] / 1e6,
         "unit": "Mpaths/s",
         "cores": st["threads"],
+        "cpus_available": avail,
+        "hardware_concurrency": logical,
+        "cpu_model": model,
         "kind": "port",
         "sample": "first %d of %d 32x32 blocks of the spiral order (frame centre), 1 spp, %d paths, %.1f s; "
                   "reference octree (robust slabs) built in %.1f s, not timed" % (
                       blocks, total_blocks, int(st["paths"]), st["seconds"], build_s),
     }
+
+
+def path_bytes(agg, bytes_closest, bytes_shadow):
+    """Algorithmic bytes per path: its traversal queries (closest-hit and
+    shadow, at their measured bytes per query) plus the path state its
+    stages read and write (PATH_STATE_BYTES; every continuation ray comes
+    from one shading vertex)."""
+    paths = agg["paths"]
+    vertices = agg["closest_rays"] - paths
+    trav = (agg["closest_rays"] * (bytes_closest + 4) + agg["shadow_rays"] * (bytes_shadow + 4)) / paths
+    state = (paths * (PATH_STATE_BYTES["raygen"] + PATH_STATE_BYTES["first_hit"] + PATH_STATE_BYTES["film"]) +
+             vertices * (PATH_STATE_BYTES["shade"] + PATH_STATE_BYTES["post"])) / paths
+    return {"traversal": round(trav, 1), "path_state": round(state, 1), "total": trav + state,
+            "closest_rays_per_path": round(agg["closest_rays"] / paths, 3),
+            "shadow_rays_per_path": round(agg["shadow_rays"] / paths, 3)}
+
+
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "bench_%s_counters.md" % WORKLOAD)
+
+
+def counters_from_profiles(kernel):
+    """Occupancy, VALU busy and L2 hit rate of `kernel` from the committed
+    hardware-counter summary (None when absent)."""
+    try:
+        with open(COUNTERS_FILE) as f:
+            rows = [l for l in f if l.startswith("| ")]
+    except OSError:
+        return None
+    head = [c.strip() for c in rows[0].strip().strip("|").split("|")]
+    for r in rows[1:]:
+        cells = [c.strip() for c in r.strip().strip("|").split("|")]
+        if cells[0] == kernel:
+            d = dict(zip(head, cells))
+            try:
+                return {"occupancy_pct": float(d["OccupancyPercent"]), "valu_busy_pct": float(d["VALUBusy"]),
+                        "l2_hit_pct": float(d["L2 hit %"])}
+            except (KeyError, ValueError):
+                return None
+    return None
+
+
+def limiter_from_profiles(kernel, traffic, ms_per_launch):
+    """What limits the kernel below the HBM roofline, from its counters:
+    actual HBM traffic near the peak means bandwidth; otherwise a busy VALU
+    with misses to HBM means decode/ALU work behind dependent fetches, an
+    idle one dependent-fetch latency."""
+    c = counters_from_profiles(kernel)
+    if c is None:
+        return None
+    actual = traffic / (ms_per_launch * 1e-3) / 1e9 if traffic else None
+    if actual is not None and actual > 0.8 * HBM_PEAK_GBS:
+        what = "HBM bandwidth"
+    elif c["valu_busy_pct"] >= 50:
+        what = "VALU (plane decode, slab and triangle tests) interleaved with dependent node fetches"
+    else:
+        what = "latency of dependent fetches"
+    c["what"] = what
+    if actual is not None:
+        c["actual_hbm_gbs"] = round(actual, 1)
+    return c
 
 
 def short_name(name):
@@ -495,6 +604,187 @@ def guided_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def primary_rays(desc, seed=0x5EED):
+    """One camera ray per pixel of the frame through a uniformly jittered
+    film position (VirtualCamera::sampleRay, src/tracer/camera/virtual.cpp:
+    52-65); 8 planes of n floats."""
+    import drawlab_amd
+
+    rng = np.random.default_rng(seed)
+    f32 = np.float32
+    cd = drawlab_amd.camera_desc(desc.camera)
+    W, H = desc.camera.width, desc.camera.height
+    eye, U, V, Wv = (np.array(getattr(cd, k)[:], f32) for k in ("eye", "U", "V", "W"))
+    ys, xs = np.divmod(np.arange(W * H), W)
+    px = (xs + rng.random(W * H)).astype(f32)
+    py = (ys + rng.random(W * H)).astype(f32)
+    dx = (px * f32(1.0 / W) * f32(2) - f32(1)).astype(f32)
+    dy = (((f32(H) - py) - f32(1)) * f32(1.0 / H) * f32(2) - f32(1)).astype(f32)
+    d = dx[:, None] * U + dy[:, None] * V + Wv
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(f32)
+    return _planes(np.broadcast_to(eye, d.shape), d, 1e-4, np.inf)
+
+
+def secondary_rays(desc, primary, prim, u, v, n=1 << 22, seed=0x5EED + 1):
+    """Rays leaving the surface points the primary rays hit (n drawn with
+    replacement; barycentric hit points, face normals turned toward the
+    camera):
+      bounce  cosine-distributed directions about the normal, as diffuse
+              bounces leave;
+      shadow  directions drawn from the sky-and-sun map in proportion to
+              max(rgb) times the pixel's solid angle (the render's emitter
+              sampling; about three quarters go to the sun), directions
+              below the surface redrawn, ending 1e16 away as the render's
+              environment shadow rays do."""
+    rng = np.random.default_rng(seed)
+    Vs = np.concatenate([m.V for m in desc.meshes]).astype(np.float64)
+    offs = np.cumsum([0] + [m.V.shape[0] for m in desc.meshes])[:-1]
+    F = np.concatenate([m.F.astype(np.int64) + o for m, o in zip(desc.meshes, offs)])
+    hits = np.flatnonzero(prim != 0xFFFFFFFF)
+    pick = hits[rng.integers(0, hits.size, n)]
+    f = prim[pick].astype(np.int64)
+    P0, P1, P2 = Vs[F[f, 0]], Vs[F[f, 1]], Vs[F[f, 2]]
+    b1, b2 = u[pick].astype(np.float64)[:, None], v[pick].astype(np.float64)[:, None]
+    o = P0 * (1 - b1 - b2) + P1 * b1 + P2 * b2
+    nrm = np.cross(P1 - P0, P2 - P0)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    dcam = primary[3:6, pick].T.astype(np.float64)
+    nrm *= np.where((nrm * dcam).sum(1) > 0, -1.0, 1.0)[:, None]
+    t = np.where(np.abs(nrm[:, :1]) > 0.9, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    t = np.cross(nrm, t)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    b = np.cross(nrm, t)
+    r, phi = np.sqrt(rng.random(n)), 2 * np.pi * rng.random(n)
+    z = np.sqrt(np.maximum(0.0, 1 - r * r))
+    dd = t * (r * np.cos(phi))[:, None] + b * (r * np.sin(phi))[:, None] + nrm * z[:, None]
+    bounce = _planes(o, dd, 1e-4, np.inf)
+
+    env = desc.envmap.rgb.astype(np.float64)
+    h, w = env.shape[:2]
+    th = np.pi * np.arange(h + 1) / h
+    band = (np.cos(th[:-1]) - np.cos(th[1:]))[:, None] * np.ones((1, w))
+    pcdf = np.cumsum((env.max(axis=2) * band).reshape(-1))
+    pcdf /= pcdf[-1]
+    dirs = np.zeros((n, 3))
+    todo = np.arange(n)
+    for _ in range(64):
+        if not todo.size:
+            break
+        k = np.minimum(np.searchsorted(pcdf, rng.random(todo.size)), len(pcdf) - 1)
+        yy, xx = np.divmod(k, w)
+        ph = (xx + rng.random(todo.size)) / w * 2 * np.pi - np.pi
+        c0, c1 = np.cos(th[yy]), np.cos(th[yy + 1])
+        ct = c0 + (c1 - c0) * rng.random(todo.size)
+        st = np.sqrt(np.maximum(0.0, 1 - ct * ct))
+        cand = np.stack([np.cos(ph) * st, ct, np.sin(ph) * st], 1)
+        ok = (cand * nrm[todo]).sum(1) > 0
+        dirs[todo[ok]] = cand[ok]
+        todo = todo[~ok]
+    dirs[todo] = nrm[todo]  # points that see no part of the map above them (none in this scene)
+    shadow = _planes(o, dirs, 1e-4, 1e16)
+    return bounce, shadow
+
+
+def _planes(o, d, tmin, tmax):
+    n = d.shape[0]
+    r = np.empty((8, n), np.float32)
+    r[0:3] = np.asarray(o, np.float32).T
+    r[3:6] = np.asarray(d, np.float32).T
+    r[6] = tmin
+    r[7] = tmax
+    return r
+
+
+def trace_bench(args):
+    """Traversal micro-bench: each ray set traced by the batch kernels
+    (dl_hip_trace_closest / dl_hip_trace_shadow, persistent grids over
+    device-resident rays), HIP-event time of the kernel on the context's
+    stream, median of --steps runs after --warmup; node visits and triangle
+    tests per ray from the counting variants (untimed), giving algorithmic
+    bytes per ray B = 32 + 16 (shadow: 1) + 64 N_node + 48 N_tri."""
+    import drawlab_amd
+    from drawlab_amd import scenes
+
+    desc = scenes.displaced_sphere()
+    t0 = time.perf_counter()
+    primary = primary_rays(desc)
+    be = drawlab_amd.Backend(0)
+    be.upload_scene(desc)
+    accel = be.build_accel(drawlab_amd.BUILD_GPU)
+    prim, _, u, v = be.trace_closest(primary)
+    bounce, shadow = secondary_rays(desc, primary, prim, u, v)
+    gen_s = time.perf_counter() - t0
+    sets = {"primary": primary, "bounce": bounce, "shadow": shadow}
+    res = {}
+    total_rays, total_ms = 0, 0.0
+    for name, rays in sets.items():
+        n = rays.shape[1]
+        any_hit = name == "shadow"
+        d_rays = be.alloc(rays.nbytes)
+        d_rays.upload(np.ascontiguousarray(rays))
+        outs = [be.alloc(max(1, n)) if any_hit else be.alloc(4 * n) for _ in range(1 if any_hit else 4)]
+        cnt = [be.alloc(4 * n) for _ in range(2)]
+        if any_hit:
+            be.trace_shadow_dev(d_rays, n, outs[0], *cnt)
+        else:
+            be.trace_closest_dev(d_rays, n, *outs, *cnt)
+        be.sync()
+        nodes = cnt[0].download(np.uint32, n).astype(np.float64).mean()
+        tris = cnt[1].download(np.uint32, n).astype(np.float64).mean()
+        ms = []
+        for k in range(args.warmup + args.steps):
+            if any_hit:
+                be.trace_shadow_dev(d_rays, n, outs[0])
+            else:
+                be.trace_closest_dev(d_rays, n, *outs)
+            be.sync()
+            if k >= args.warmup:
+                ms.append(be.last_kernel_ms())
+        hits = outs[0].download(np.uint8 if any_hit else np.uint32, n)
+        hit_frac = float((hits != 0).mean() if any_hit else (hits != 0xFFFFFFFF).mean())
+        for b in outs + cnt + [d_rays]:
+            b.free()
+        med = float(np.median(ms))
+        bpr = RAY_BYTES + (OCC_BYTES if any_hit else HIT_BYTES) + NODE_BYTES * nodes + TRI_BYTES * tris
+        gbs = n * bpr / (med * 1e-3) / 1e9
+        res[name] = {"rays": n, "kernel": "k_trace_shadow" if any_hit else "k_trace_closest<false>",
+                     "ms": round(med, 4), "mrays_per_s": round(n / (med * 1e-3) / 1e6, 1),
+                     "node_visits_per_ray": round(nodes, 3), "tri_tests_per_ray": round(tris, 3),
+                     "bytes_per_ray": round(bpr, 1), "achieved_gbs": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), ("occluded" if any_hit else "hit") + "_fraction":
+                         round(hit_frac, 4)}
+        total_rays += n
+        total_ms += med
+        print("%-8s %9d rays  %8.3f ms  %8.1f Mrays/s  %6.1f B/ray  %7.1f GB/s (%.3f of HBM peak)" % (
+            name, n, med, res[name]["mrays_per_s"], bpr, gbs, gbs / HBM_PEAK_GBS), file=sys.stderr, flush=True)
+    value = total_rays / (total_ms * 1e-3) / 1e6
+    dom = max(res, key=lambda k: res[k]["ms"])
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene and ray sets generated in code)",
+        "config": {"workload": "traversal_" + WORKLOAD, "triangles": int(desc.num_triangles),
+                   "bvh": "gpu-ploc, quantised 4-wide nodes", "bvh_build_ms": round(accel["build_ms"], 2),
+                   "ray_generation_s": round(gen_s, 1)},
+        "ray_sets": res,
+        "roofline": {"bound": "hbm", "kernel": res[dom]["kernel"] + " (%s rays)" % dom,
+                     "achieved": res[dom]["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": res[dom]["frac"], "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    be.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -512,7 +802,8 @@ def main():
     ap.add_argument("--command", default="", help="command line recorded in a summary")
     ap.add_argument("--title", default="", help="title of a kernel-time summary")
     ap.add_argument("--copy-mib", type=int, default=1024, help="calibration copy size used by the PMC runs")
-    ap.add_argument("--workload", choices=("displaced_sphere", "service", "guided"), default="displaced_sphere")
+    ap.add_argument("--workload", choices=("displaced_sphere", "service", "guided", "trace"),
+                    default="displaced_sphere")
     ap.add_argument("--guided-res", default="480x270", help="guided workload: image size WxH")
     ap.add_argument("--guided-scene", choices=("displaced_sphere", "chimney_room"), default="displaced_sphere",
                     help="guided workload: the 2M-triangle scene, or a room lit through a small ceiling patch")
@@ -549,6 +840,9 @@ def main():
     if args.workload == "guided":
         guided_bench(args)
         return
+    if args.workload == "trace":
+        trace_bench(args)
+        return
 
     import drawlab_amd
     from drawlab_amd import scenes
@@ -560,11 +854,15 @@ def main():
     be.upload_scene(desc)
     accel = be.build_accel(drawlab_amd.BUILD_GPU)
 
-    # Untimed pass with in-kernel counters: traversal work per closest-hit ray.
+    # Untimed pass with in-kernel counters: traversal work per closest-hit
+    # and per shadow ray.
     _, cst = be.render(spp=S, spp_per_pass=S, seed=1 << 20, count_traversal=True, want_image=False)
     nodes_per_ray = cst["nodes_closest"] / cst["closest_rays"]
     tris_per_ray = cst["tris_closest"] / cst["closest_rays"]
     bytes_per_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray
+    nodes_per_shadow = cst["nodes_shadow"] / max(1.0, cst["shadow_rays"])
+    tris_per_shadow = cst["tris_shadow"] / max(1.0, cst["shadow_rays"])
+    bytes_per_shadow = RAY_BYTES + OCC_BYTES + NODE_BYTES * nodes_per_shadow + TRI_BYTES * tris_per_shadow
 
     for i in range(args.warmup):
         be.render(spp=S, spp_per_pass=S, first_sample=i * S, accumulate=i > 0, want_image=False)
@@ -587,6 +885,15 @@ def main():
     ms_per_launch = agg["ms_closest"] / launches
     achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
     traffic = traffic_from_profiles()
+    sh_launches = max(1.0, agg["launches_shadow"])
+    sh_bytes_per_launch = agg["shadow_rays"] * bytes_per_shadow / sh_launches
+    sh_ms_per_launch = agg["ms_shadow"] / sh_launches
+    sh_achieved = sh_bytes_per_launch / (sh_ms_per_launch * 1e-3) / 1e9
+    # rays: average algorithmic bytes per traversal query; paths: per path
+    rays = agg["closest_rays"] + agg["shadow_rays"]
+    bytes_per_query = (agg["closest_rays"] * bytes_per_ray + agg["shadow_rays"] * bytes_per_shadow) / rays
+    mrays = rays / seconds / 1e6
+    bpath = path_bytes(agg, bytes_per_ray, bytes_per_shadow)
 
     if args.calibration_copy_mib > 0:
         n = args.calibration_copy_mib << 20
@@ -599,7 +906,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(desc, args.cpu_seconds, min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(desc, args.cpu_seconds, host_cpus()[0])
 
     out = {
         "metric": METRIC,
@@ -626,9 +933,18 @@ def main():
             "bvh": "gpu-ploc",
             "bvh_build_ms": round(accel["build_ms"], 2),
         },
-        "mrays_per_s": round((agg["closest_rays"] + agg["shadow_rays"]) / seconds / 1e6, 2),
+        "mrays_per_s": round(mrays, 2),
+        # Mrays/s and Mpaths/s against the rates the HBM roofline allows at
+        # the algorithmic bytes of a query and of a path
+        "mrays_roofline": {"bytes_per_query": round(bytes_per_query, 1),
+                           "roofline_mrays_per_s": round(HBM_PEAK_GBS * 1e3 / bytes_per_query, 1),
+                           "frac": round(mrays / (HBM_PEAK_GBS * 1e3 / bytes_per_query), 4)},
+        "mpaths_roofline": {"bytes_per_path": round(bpath["total"], 1), "breakdown": bpath,
+                            "roofline_mpaths_per_s": round(HBM_PEAK_GBS * 1e3 / bpath["total"], 1),
+                            "frac": round(value / (HBM_PEAK_GBS * 1e3 / bpath["total"]), 4)},
         "roofline": {
             "bound": "hbm",
+            "limiter": limiter_from_profiles(TRACE_KERNEL, traffic, ms_per_launch),
             "kernel": TRACE_KERNEL + " (closest hit)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
@@ -641,6 +957,21 @@ def main():
             "tri_tests_per_ray": round(tris_per_ray, 2),
             "launches": int(launches),
             "avg_launch_ms": round(ms_per_launch, 4),
+        },
+        "roofline_shadow": {
+            "bound": "hbm",
+            "limiter": limiter_from_profiles("k_shadow_queue", None, sh_ms_per_launch),
+            "kernel": "k_shadow_queue (any hit)",
+            "achieved": round(sh_achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(sh_achieved / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes_per_launch": round(sh_bytes_per_launch),
+            "bytes_per_ray": round(bytes_per_shadow, 1),
+            "node_visits_per_ray": round(nodes_per_shadow, 2),
+            "tri_tests_per_ray": round(tris_per_shadow, 2),
+            "launches": int(sh_launches),
+            "avg_launch_ms": round(sh_ms_per_launch, 4),
         },
         "stage_ms": {k: round(agg[k], 1) for k in ("ms_closest", "ms_shadow", "ms_shade", "ms_film", "ms_other")},
         "cpu_baseline": cpu,

@@ -27,7 +27,7 @@ LIGHT_AREA, LIGHT_POINT, LIGHT_ENV = 0, 1, 2
 EXPORTED_SYMBOLS = (
     "dl_hip_create", "dl_hip_destroy", "dl_hip_last_error", "dl_hip_device_name", "dl_hip_upload_scene",
     "dl_hip_build_accel", "dl_hip_trace_closest", "dl_hip_trace_shadow", "dl_hip_trace_closest_counted",
-    "dl_hip_trace_closest_noalloc",
+    "dl_hip_trace_closest_noalloc", "dl_hip_trace_shadow_counted",
     "dl_hip_render", "dl_hip_render_paths", "dl_hip_malloc", "dl_hip_free", "dl_hip_memcpy_h2d",
     "dl_hip_memcpy_d2h", "dl_hip_memcpy_d2d", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_camera_perspective",
     "dl_camera_opencv", "dl_film_filter", "dl_hip_set_view_options", "dl_hip_render_batch",
@@ -115,7 +115,7 @@ class dl_render_stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
         "paths", "closest_rays", "shadow_rays", "invalid_samples", "ms_total", "ms_closest", "ms_shadow", "ms_shade",
         "ms_film", "ms_other", "launches_closest", "launches_shadow", "nodes_closest", "tris_closest", "ms_guide",
-        "ms_train", "train_steps")]
+        "ms_train", "train_steps", "nodes_shadow", "tris_shadow")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -143,6 +143,7 @@ def lib():
             "dl_hip_trace_closest_counted": (i32, [P, P, u32, P, P, P, P, P, P]),
             "dl_hip_trace_closest_noalloc": (i32, [P, P, u32, P, P, P, P]),
             "dl_hip_trace_shadow": (i32, [P, P, u32, P]),
+            "dl_hip_trace_shadow_counted": (i32, [P, P, u32, P, P, P]),
             "dl_hip_render": (i32, [P, c.POINTER(dl_render_params), P, c.POINTER(dl_render_stats)]),
             "dl_hip_render_paths": (i32, [P, c.POINTER(dl_render_params), u32, P]),
             "dl_hip_malloc": (i32, [P, c.c_size_t, c.POINTER(P)]),
@@ -386,8 +387,11 @@ class Backend:
         else:
             self._check(self.L.dl_hip_trace_closest(self.h, d_rays.ptr, n, d_prim.ptr, d_t.ptr, d_u.ptr, d_v.ptr))
 
-    def trace_shadow_dev(self, d_rays, n, d_occ):
-        self._check(self.L.dl_hip_trace_shadow(self.h, d_rays.ptr, n, d_occ.ptr))
+    def trace_shadow_dev(self, d_rays, n, d_occ, d_nodes=None, d_tris=None):
+        if d_nodes is not None:
+            self._check(self.L.dl_hip_trace_shadow_counted(self.h, d_rays.ptr, n, d_occ.ptr, d_nodes.ptr, d_tris.ptr))
+        else:
+            self._check(self.L.dl_hip_trace_shadow(self.h, d_rays.ptr, n, d_occ.ptr))
 
     def sync(self):
         self._check(self.L.dl_hip_sync(self.h))
@@ -452,14 +456,18 @@ class Backend:
             return prim, t, u, v, cnt[0].download(np.uint32, n), cnt[1].download(np.uint32, n)
         return prim, t, u, v
 
-    def trace_shadow(self, rays):
+    def trace_shadow(self, rays, counted=False):
         rays = np.ascontiguousarray(np.asarray(rays, dtype=np.float32).reshape(8, -1))
         n = rays.shape[1]
         d_rays = self.alloc(rays.nbytes)
         d_rays.upload(rays)
         d_occ = self.alloc(max(1, n))
-        self.trace_shadow_dev(d_rays, n, d_occ)
-        return d_occ.download(np.uint8, n)
+        cnt = [self.alloc(4 * max(1, n)) for _ in range(2)] if counted else [None, None]
+        self.trace_shadow_dev(d_rays, n, d_occ, *cnt)
+        occ = d_occ.download(np.uint8, n)
+        if counted:
+            return occ, cnt[0].download(np.uint32, n), cnt[1].download(np.uint32, n)
+        return occ
 
     # -- rendering ---------------------------------------------------------
     def _params(self, spp, seed, max_depth, rr_depth, spp_per_pass=0, first_sample=0, flags=0):

@@ -475,6 +475,14 @@ dl_status dl_hip_trace_shadow(void* h, const float* rays, uint32_t n, uint8_t* o
     return launch_trace_shadow(c, rays, n, occ);
 }
 
+dl_status dl_hip_trace_shadow_counted(void* h, const float* rays, uint32_t n, uint8_t* occ, uint32_t* nodes,
+                                      uint32_t* tris) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !rays || !occ || !nodes || !tris) return DL_ERR_INVALID;
+    if (!c->has_accel) return set_error(c, DL_ERR_STATE, "trace before build_accel");
+    return launch_trace_shadow(c, rays, n, occ, nodes, tris);
+}
+
 dl_status dl_hip_render(void* h, const dl_render_params* p, float* rgb_host, dl_render_stats* stats) {
     Ctx* c = (Ctx*)h;
     if (!c || !p) return DL_ERR_INVALID;

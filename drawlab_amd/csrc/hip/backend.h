@@ -187,7 +187,8 @@ dl_status hip_fail(Ctx* c, hipError_t e, const char* what);
 // trace.hip launchers
 dl_status launch_trace_closest(Ctx* c, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u, float* v,
                                uint32_t* cn, uint32_t* ct);
-dl_status launch_trace_shadow(Ctx* c, const float* rays, uint32_t n, uint8_t* occ);
+dl_status launch_trace_shadow(Ctx* c, const float* rays, uint32_t n, uint8_t* occ, uint32_t* cn = nullptr,
+                              uint32_t* ct = nullptr);
 dl_status noalloc_scratch_init(Ctx* c);
 dl_status launch_trace_closest_noalloc(Ctx* c, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u,
                                        float* v);

@@ -56,7 +56,8 @@ struct Counters {
     unsigned long long nodes;
     unsigned long long tris;
     unsigned long long invalid;
-    unsigned long long pad2;
+    unsigned long long nodes_sh;  // shadow rays: node visits, triangle tests (counting renders)
+    unsigned long long tris_sh;
 };
 
 // Device buffers of a render (plain pointers: passed by value to kernels).
@@ -321,7 +322,7 @@ struct QueueIO {
                                               uint32_t) const {
         hit[q[i]] = make_float4(__uint_as_float(prim), t, u, v);
     }
-    __device__ __forceinline__ void store_occ(uint32_t i, bool o) const { occ[q[i]] = o ? 1 : 0; }
+    __device__ __forceinline__ void store_occ(uint32_t i, bool o, uint32_t, uint32_t) const { occ[q[i]] = o ? 1 : 0; }
 };
 
 template <bool COUNT>
@@ -339,6 +340,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(BvhView bvh, Render
     }
 }
 
+template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow_queue(BvhView bvh, RenderBufs rs, int* __restrict__ ovf) {
     __shared__ int lds[kLdsStack * kTraceBlock];
     const uint32_t n = rs.cnt->shadow;
@@ -346,7 +348,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_queue(BvhView bvh, Rende
     LaneStack stk{lds, (int)threadIdx.x, kTraceBlock, ovf + g, G};
     QueueIO io{rs.shq, rs.sh_o, rs.sh_d, nullptr, rs.occ};
     TraceCounters cnt;
-    traverse_persistent<true, false>(bvh, n, &rs.cnt->work_shadow, stk, io, cnt);
+    traverse_persistent<true, COUNT>(bvh, n, &rs.cnt->work_shadow, stk, io, cnt);
+    if (COUNT) {
+        atomicAdd(&rs.cnt->nodes_sh, (unsigned long long)cnt.nodes);
+        atomicAdd(&rs.cnt->tris_sh, (unsigned long long)cnt.tris);
+    }
 }
 
 // Material type and area-light flag of a primitive (one byte).
@@ -1263,7 +1269,10 @@ dl_status run_bounces(Ctx* c, RenderState* rs, const PassParams& pp, const Scene
             k_trace_queue<false><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
         DL_TRY(tm.stop());
         DL_TRY(tm.start(kTagShadow));
-        k_shadow_queue<<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
+        if (count)
+            k_shadow_queue<true><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
+        else
+            k_shadow_queue<false><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
         DL_TRY(tm.stop());
         DL_TRY(tm.start(kTagShade));
         if (hide)
@@ -1443,6 +1452,8 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
         stats->invalid_samples = (double)rs->cnt_host->invalid;
         stats->nodes_closest = (double)rs->cnt_host->nodes;
         stats->tris_closest = (double)rs->cnt_host->tris;
+        stats->nodes_shadow = (double)rs->cnt_host->nodes_sh;
+        stats->tris_shadow = (double)rs->cnt_host->tris_sh;
         stats->train_steps = train_steps;
     }
     return DL_OK;
@@ -1583,6 +1594,8 @@ dl_status render_batch(Ctx* c, const dl_render_params* prm, const dl_batch_item*
         stats->invalid_samples = (double)rs->cnt_host->invalid;
         stats->nodes_closest = (double)rs->cnt_host->nodes;
         stats->tris_closest = (double)rs->cnt_host->tris;
+        stats->nodes_shadow = (double)rs->cnt_host->nodes_sh;
+        stats->tris_shadow = (double)rs->cnt_host->tris_sh;
     }
     return DL_OK;
 }

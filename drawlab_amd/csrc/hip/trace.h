@@ -140,7 +140,7 @@ __device__ __forceinline__ uint32_t claim_next(uint32_t* counter, bool want) {
 //
 // IO provides: load(i, o, d, tmin, tmax) -> bool (false: skip the ray),
 // store_hit(i, prim, t, u, v, nodes, tris) for closest hits and
-// store_occ(i, occluded) for any-hit queries.
+// store_occ(i, occluded, nodes, tris) for any-hit queries.
 template <bool ANY, bool COUNT, class IO>
 __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t n, uint32_t* work, LaneStack& stk,
                                                     IO& io, TraceCounters& total) {
@@ -177,7 +177,7 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
                     if (isnan(o.x) || isnan(o.y) || isnan(o.z) || isnan(d.x) || isnan(d.y) || isnan(d.z) || isnan(tmin) ||
                         isnan(tmax)) {
                         if (ANY)
-                            io.store_occ(ray, false);
+                            io.store_occ(ray, false, 0, 0);
                         else
                             io.store_hit(ray, best, best_t, 0.0f, 0.0f, 0, 0);
                     } else {
@@ -262,7 +262,7 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
         }
         if (done) {
             if (ANY)
-                io.store_occ(ray, hit_any);
+                io.store_occ(ray, hit_any, cnt.nodes, cnt.tris);
             else
                 io.store_hit(ray, best, best_t, bu, bv, cnt.nodes, cnt.tris);  // a miss keeps t = tmax
             if (COUNT) {

@@ -33,7 +33,13 @@ struct BatchIO {
             ct[i] = nt;
         }
     }
-    __device__ __forceinline__ void store_occ(uint32_t i, bool o) const { occ[i] = o ? 1 : 0; }
+    __device__ __forceinline__ void store_occ(uint32_t i, bool o, uint32_t nn, uint32_t nt) const {
+        occ[i] = o ? 1 : 0;
+        if (cn) {
+            cn[i] = nn;
+            ct[i] = nt;
+        }
+    }
 };
 
 // Persistent grids (gridDim.x == Ctx::trace_blocks) with per-lane refill.
@@ -47,13 +53,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(BvhView bvh, Batc
     traverse_persistent<false, COUNT>(bvh, n, work, stk, io, tot);
 }
 
+template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(BvhView bvh, BatchIO io, uint32_t n, uint32_t* work,
                                                               int* __restrict__ ovf) {
     __shared__ int lds[kLdsStack * kTraceBlock];
     const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x, G = gridDim.x * kTraceBlock;
     LaneStack stk{lds, (int)threadIdx.x, kTraceBlock, ovf + g, G};
     TraceCounters tot;
-    traverse_persistent<true, false>(bvh, n, work, stk, io, tot);
+    traverse_persistent<true, COUNT>(bvh, n, work, stk, io, tot);
 }
 
 __global__ void k_zero_work(uint32_t* w) {
@@ -86,15 +93,18 @@ dl_status launch_trace_closest(Ctx* c, const float* rays, uint32_t n, uint32_t* 
     return DL_OK;
 }
 
-dl_status launch_trace_shadow(Ctx* c, const float* rays, uint32_t n, uint8_t* occ) {
+dl_status launch_trace_shadow(Ctx* c, const float* rays, uint32_t n, uint8_t* occ, uint32_t* cn, uint32_t* ct) {
     if (n == 0) return DL_OK;
     if (n > kMaxBatchRays) return set_error(c, DL_ERR_INVALID, "more than 2^31 rays in one batch");
     BvhView bvh{c->ds.nodes, c->ds.tris};
     const uint32_t blocks = c->trace_blocks;
     DL_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
-    BatchIO io{rays, (size_t)n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, occ};
+    BatchIO io{rays, (size_t)n, nullptr, nullptr, nullptr, nullptr, cn, ct, occ};
     k_zero_work<<<1, 64, 0, c->stream>>>(c->work);
-    k_trace_shadow<<<blocks, kTraceBlock, 0, c->stream>>>(bvh, io, n, c->work, c->stack_ovf);
+    if (cn)
+        k_trace_shadow<true><<<blocks, kTraceBlock, 0, c->stream>>>(bvh, io, n, c->work, c->stack_ovf);
+    else
+        k_trace_shadow<false><<<blocks, kTraceBlock, 0, c->stream>>>(bvh, io, n, c->work, c->stack_ovf);
     DL_HIP_CHECK(c, hipGetLastError());
     DL_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
     DL_HIP_CHECK(c, hipEventSynchronize(c->ev1));

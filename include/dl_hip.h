@@ -224,6 +224,8 @@ typedef struct dl_render_stats {
     double ms_guide;           /* with DL_RENDER_GUIDED: proposal inference kernels */
     double ms_train;           /* with DL_RENDER_GUIDED: online training (batches, gradients, Adam) */
     double train_steps;        /* with DL_RENDER_GUIDED: Adam steps taken during the render */
+    double nodes_shadow;       /* with DL_RENDER_COUNT_TRAVERSAL: node visits of shadow rays */
+    double tris_shadow;        /* with DL_RENDER_COUNT_TRAVERSAL: triangle tests of shadow rays */
 } dl_render_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -243,7 +245,6 @@ dl_status dl_hip_build_accel(void* ctx, int builder, dl_accel_stats* stats);
  * the context's stream unless synchronised with dl_hip_sync. */
 dl_status dl_hip_trace_closest(void* ctx, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u, float* v);
 dl_status dl_hip_trace_shadow(void* ctx, const float* rays, uint32_t n, uint8_t* occluded);
-/* Same with counters: per-ray node visits and triangle tests (uint32 each). */
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): dl_hip_trace_closest with the traversal's scratch (the
@@ -257,8 +258,12 @@ dl_status dl_hip_trace_shadow(void* ctx, const float* rays, uint32_t n, uint8_t*
  * Returns after the kernel has finished. */
 dl_status dl_hip_trace_closest_noalloc(void* ctx, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u,
                                        float* v);
+/* Same with counters: per-ray node visits and triangle tests (uint32 each),
+ * the traversal work of the algorithmic byte count (bench, CPU replay). */
 dl_status dl_hip_trace_closest_counted(void* ctx, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u,
                                        float* v, uint32_t* node_visits, uint32_t* tri_tests);
+dl_status dl_hip_trace_shadow_counted(void* ctx, const float* rays, uint32_t n, uint8_t* occluded,
+                                      uint32_t* node_visits, uint32_t* tri_tests);
 
 /* ---- rendering --------------------------------------------------------- */
 /* Renders spp samples per pixel; writes W*H*3 floats (row-major, row 0 =

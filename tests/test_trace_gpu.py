@@ -70,8 +70,9 @@ def test_displaced_sphere_full_size_hits(backend, builder):
 @pytest.mark.parametrize("builder", [drawlab_amd.BUILD_GPU, drawlab_amd.BUILD_HOST_SAH, drawlab_amd.BUILD_GPU_LBVH],
                          ids=["gpu", "host_sah", "lbvh"])
 def test_traversal_counts_match_cpu_replay(backend, builder):
-    """The node visits and triangle tests of every ray -- the counts behind
-    the roofline's algorithmic bytes -- and the hits themselves equal a CPU
+    """The node visits and triangle tests of every ray, closest-hit and
+    shadow -- the counts behind the rooflines' algorithmic bytes -- and the
+    hits themselves equal a CPU
     replay of the traversal over the exported 4-wide BVH (oracle/replay.cpp),
     including rays with NaN components, which finish with no visit."""
     desc = scenes.displaced_sphere(width=64, height=36, grid=128, glass_level=3, env_size=(64, 32))
@@ -88,8 +89,11 @@ def test_traversal_counts_match_cpu_replay(backend, builder):
     for name, rays in batches.items():
         rays = np.ascontiguousarray(rays, np.float32)
         if name == "shadow":
-            occ, _, _ = oracle.bvh4_replay(nodes, tris, rays, any_hit=True)
-            assert np.array_equal(backend.trace_shadow(rays), occ), name
+            r = oracle.bvh4_replay(nodes, tris, rays, any_hit=True)
+            g = backend.trace_shadow(rays, counted=True)
+            assert np.array_equal(backend.trace_shadow(rays), r[0]), name
+            for a, b, what in zip(g, r, ("occlusion", "node visits", "triangle tests")):
+                assert np.array_equal(a, b), "shadow %s differ" % what
             continue
         g = backend.trace_closest(rays, counted=True)
         r = oracle.bvh4_replay(nodes, tris, rays)
