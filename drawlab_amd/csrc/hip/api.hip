@@ -164,7 +164,12 @@ dl_status dl_hip_create(int device, void** out) {
     }
     snprintf(c->devname, sizeof(c->devname), "%s (%s)", prop.name, prop.gcnArchName);
     c->num_cus = prop.multiProcessorCount;
-    c->trace_blocks = (uint32_t)std::max(1, c->num_cus) * 16u;
+    // persistent traversal grids: every block resident at once (at most 16
+    // per CU, 8 waves per SIMD), as each kernel's registers allow
+    const uint32_t cus = (uint32_t)std::max(1, c->num_cus);
+    c->trace_blocks = cus * (uint32_t)kTraceBlocksPerCu;
+    c->trace_grid = cus * (uint32_t)std::min(render_trace_blocks_per_cu(false), batch_trace_blocks_per_cu(false));
+    c->trace_grid_count = cus * (uint32_t)std::min(render_trace_blocks_per_cu(true), batch_trace_blocks_per_cu(true));
     e = dmalloc(c, (void**)&c->stack_ovf, sizeof(int) * (size_t)c->trace_blocks * kTraceBlockHost * (kMaxStack - kLdsStack));
     if (e == hipSuccess) e = dmalloc(c, (void**)&c->work, 4 * sizeof(uint32_t));
     if (e != hipSuccess) {

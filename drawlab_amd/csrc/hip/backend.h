@@ -14,6 +14,7 @@
 namespace dl {
 
 constexpr int kTraceBlockHost = 128;  // threads per traversal block (trace.h kTraceBlock)
+constexpr int kTraceBlocksPerCu = 16;  // at most: 8 waves per SIMD
 
 // Device copy of the scene in the layouts the kernels read.  All arrays are
 // indexed by global primitive id (mesh order, then face order) or global
@@ -111,7 +112,9 @@ struct Ctx {
     // traversal stack overflow (entries kLdsStack..kMaxStack of every lane of
     // a persistent traversal grid)
     int* stack_ovf = nullptr;
-    uint32_t trace_blocks = 0;
+    uint32_t trace_blocks = 0;      // blocks the stack overflow holds (kTraceBlocksPerCu per CU)
+    uint32_t trace_grid = 0;        // blocks of a persistent traversal launch: all resident at once
+    uint32_t trace_grid_count = 0;  // the same for the counting variants
     uint32_t* work = nullptr;  // work counters of the batch traversal kernels
 
     // scene
@@ -190,6 +193,8 @@ dl_status launch_trace_closest(Ctx* c, const float* rays, uint32_t n, uint32_t* 
 dl_status launch_trace_shadow(Ctx* c, const float* rays, uint32_t n, uint8_t* occ, uint32_t* cn = nullptr,
                               uint32_t* ct = nullptr);
 dl_status noalloc_scratch_init(Ctx* c);
+int render_trace_blocks_per_cu(bool counting);
+int batch_trace_blocks_per_cu(bool counting);
 dl_status launch_trace_closest_noalloc(Ctx* c, const float* rays, uint32_t n, uint32_t* prim, float* t, float* u,
                                        float* v);
 

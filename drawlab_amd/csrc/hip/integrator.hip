@@ -355,6 +355,20 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_queue(BvhView bvh, Rende
     }
 }
 
+// Resident blocks per CU of the render's traversal kernels (the fewer of
+// the closest-hit and shadow kernel), without or with traversal counters:
+// the persistent grids are sized to it, so no block waits for a second round.
+int render_trace_blocks_per_cu(bool counting) {
+    int a = 0, b = 0;
+    hipError_t e = counting ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace_queue<true>, kTraceBlock, 0)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace_queue<false>, kTraceBlock, 0);
+    if (e == hipSuccess)
+        e = counting ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_queue<true>, kTraceBlock, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_queue<false>, kTraceBlock, 0);
+    if (e != hipSuccess) return kTraceBlocksPerCu;
+    return std::max(1, std::min(kTraceBlocksPerCu, std::min(a, b)));
+}
+
 // Material type and area-light flag of a primitive (one byte).
 __device__ __forceinline__ uint32_t prim_info(const SceneView& s, uint32_t prim) { return s.prim_info[prim]; }
 __device__ __forceinline__ int info_type(uint32_t info) { return (int)(info & (kPrimLight - 1u)); }
@@ -1341,7 +1355,7 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
     // 53 KB of LDS per block (4 waves x 64 rows): three blocks fit a CU
     const int guide_grid = std::max(1, c->num_cus) * 3;
     const int stage_grid = std::max(1, c->num_cus) * 8;
-    const int trace_grid = (int)c->trace_blocks;
+    const int trace_grid = (int)(count ? c->trace_grid_count : c->trace_grid);
     std::vector<int> tags;
     Timer tm{c, rs, &tags};
     double closest = 0, shadow = 0, paths = 0;
@@ -1504,7 +1518,7 @@ dl_status render_batch(Ctx* c, const dl_render_params* prm, const dl_batch_item*
     const bool hide = c->view_set && c->view.hide_lights != 0;
     const float eps = c->view_set && c->view.epsilon > 0.0f ? c->view.epsilon : kEpsilon;
     const int stage_grid = std::max(1, c->num_cus) * 8;
-    const int trace_grid = (int)c->trace_blocks;
+    const int trace_grid = (int)(count ? c->trace_grid_count : c->trace_grid);
     std::vector<int> tags;
     Timer tm{c, rs, &tags};
     double closest = 0, shadow = 0;

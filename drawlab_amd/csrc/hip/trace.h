@@ -1,5 +1,5 @@
 // BVH traversal for gfx950: one ray per lane, 4-wide nodes (bvh_layout.h:
-// one 128-byte line per visit, four slab tests, the nearest hit internal
+// one 64-byte node per visit, four slab tests, the nearest hit internal
 // child next and the others stacked farthest first), persistent lanes with
 // per-lane ray refill,
 // per-lane traversal stack staged in LDS with a lane-interleaved layout
@@ -59,18 +59,6 @@ struct LaneStack {
     }
 };
 
-// Slab test of one child box (lo, hi per axis): entry distance n and
-// whether [n, far] meets [tmin, tmax] (far widened by kFarSlack).
-__device__ __forceinline__ bool slab1(float lx, float hx, float ly, float hy, float lz, float hz, const vec3& o,
-                                      const vec3& rcp, float tmin, float tmax, float& n) {
-    const float a0 = (lx - o.x) * rcp.x, b0 = (hx - o.x) * rcp.x;
-    const float a1 = (ly - o.y) * rcp.y, b1 = (hy - o.y) * rcp.y;
-    const float a2 = (lz - o.z) * rcp.z, b2 = (hz - o.z) * rcp.z;
-    n = fmaxf(fmaxf(fminf(a0, b0), fminf(a1, b1)), fmaxf(fminf(a2, b2), tmin));
-    const float f = fminf(fminf(fmaxf(a0, b0), fmaxf(a1, b1)), fminf(fmaxf(a2, b2), tmax));
-    return n <= f * kFarSlack;
-}
-
 // Compare-exchange of (key, ref) pairs: smaller key first.
 __device__ __forceinline__ void cswap(float& ka, int& ra, float& kb, int& rb) {
     const bool s = kb < ka;
@@ -87,6 +75,18 @@ __device__ __forceinline__ void cswap(float& ka, int& ra, float& kb, int& rb) {
 // tests (mesh.cpp:195, 204, 213) are combined into the returned predicate,
 // which accepts exactly the rays the early-out sequence accepts.  The triangle
 // is therefore fetched by three loads issued together.
+// Slab test of one child box (lo, hi per axis): entry distance n and
+// whether [n, far] meets [tmin, tmax] (far widened by kFarSlack).
+__device__ __forceinline__ bool slab1(float lx, float hx, float ly, float hy, float lz, float hz, const vec3& o,
+                                      const vec3& rcp, float tmin, float tmax, float& n) {
+    const float a0 = (lx - o.x) * rcp.x, b0 = (hx - o.x) * rcp.x;
+    const float a1 = (ly - o.y) * rcp.y, b1 = (hy - o.y) * rcp.y;
+    const float a2 = (lz - o.z) * rcp.z, b2 = (hz - o.z) * rcp.z;
+    n = fmaxf(fmaxf(fminf(a0, b0), fminf(a1, b1)), fmaxf(fminf(a2, b2), tmin));
+    const float f = fminf(fminf(fmaxf(a0, b0), fmaxf(a1, b1)), fminf(fmaxf(a2, b2), tmax));
+    return n <= f * kFarSlack;
+}
+
 // Slab test of the child whose quantised planes sit at bit `sh` of the
 // node's records r1, r2: each plane is origin + q * step.  q * step is
 // exact (an 8-bit integer times a power of two), so the fused
@@ -136,7 +136,18 @@ __device__ __forceinline__ uint32_t claim_next(uint32_t* counter, bool want) {
 // in the loop until the whole wave has run out of rays; a lane whose ray
 // finishes starts its next ray at once (lane g takes rays g, g + G, g + 2G,
 // ...: no atomic round trip on the refill path), so a wave never idles
-// behind its slowest ray.  Each iteration visits one node.
+// behind its slowest ray.
+//
+// Each iteration a lane takes one step: it visits one node, or tests one of
+// the triangles of the leaves its last node visit hit.  A node visit
+// decides the next node (the nearest hit internal child, or a stack pop)
+// with the best t of that moment, then the lane works through that visit's
+// triangles, one per iteration, before it visits the next node -- the
+// sequence of node visits and triangle tests of a ray is the one a
+// node-by-node loop makes (and the CPU replay follows), but a wave no longer
+// waits at every node for its lane with the most leaf triangles: a
+// triangle test costs one iteration of the lane that needs it, one
+// dependent fetch each.
 //
 // IO provides: load(i, o, d, tmin, tmax) -> bool (false: skip the ray),
 // store_hit(i, prim, t, u, v, nodes, tris) for closest hits and
@@ -153,6 +164,14 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
     float tmin = 0.f, best_t = 0.f, bu = 0.f, bv = 0.f;
     uint32_t best = 0xffffffffu;
     int sp = 0, cur = 0;
+    // triangles of the last visited node's hit leaves: leaf references, the
+    // triangle count of each hit leaf child packed one byte each (0 for the
+    // others; a leaf holds at most kMaxLeafCount = 255), their total and the
+    // next one to test; `last`: no node left to visit once they are tested
+    int4 lref = make_int4(0, 0, 0, 0);
+    uint32_t lcnt = 0;
+    int ltri = 0, lj = 0;
+    bool last = false;
     TraceCounters cnt;
     for (;;) {
         const bool need = !active && !exhausted;
@@ -170,6 +189,9 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
                     bu = bv = 0.0f;
                     sp = 0;
                     cur = 0;
+                    lcnt = 0;
+                    ltri = lj = 0;
+                    last = false;
                     cnt = TraceCounters();
                     // A NaN anywhere in the ray makes every Moller-Trumbore
                     // comparison false (no hit), but would pass every slab
@@ -191,33 +213,15 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
         if (!__ballot(active)) break;
         if (!active) continue;
 
-        const float4* np = bvh.nodes + 4 * (size_t)cur;
-        const float4 r0 = np[0];
-        const uint4 r1 = reinterpret_cast<const uint4*>(np)[1];
-        const uint4 r2 = reinterpret_cast<const uint4*>(np)[2];
-        const int4 ref = reinterpret_cast<const int4*>(np)[3];
-        if (COUNT) cnt.nodes++;
-        // child boxes: origin + q * step (q * step exact, one rounding)
-        const uint32_t eb = __float_as_uint(r0.w);
-        const float sx = __uint_as_float((eb & 0xffu) << 23), sy = __uint_as_float(((eb >> 8) & 0xffu) << 23),
-                    sz = __uint_as_float(((eb >> 16) & 0xffu) << 23);
-        float n0, n1, n2, n3;
-        const bool h0 = slab_q(r0, r1, r2, 0, sx, sy, sz, o, rcp, tmin, best_t, n0);
-        const bool h1 = slab_q(r0, r1, r2, 8, sx, sy, sz, o, rcp, tmin, best_t, n1);
-        const bool h2 = slab_q(r0, r1, r2, 16, sx, sy, sz, o, rcp, tmin, best_t, n2);
-        const bool h3 = slab_q(r0, r1, r2, 24, sx, sy, sz, o, rcp, tmin, best_t, n3);
-        const int4 cnt4 = make_int4((int)(r2.z & 0xffu), (int)((r2.z >> 8) & 0xffu), (int)((r2.z >> 16) & 0xffu),
-                                    (int)(r2.z >> 24));
-        bool hit_any = false;
-        // Triangles of all hit leaf children in one loop (the closest-hit
-        // result is the minimum over (t, id) and so does not depend on the
-        // order).
-        const int s0 = (h0 && ref.x < 0) ? cnt4.x : 0;
-        const int s1 = s0 + ((h1 && ref.y < 0) ? cnt4.y : 0);
-        const int s2 = s1 + ((h2 && ref.z < 0) ? cnt4.z : 0);
-        const int ntri = s2 + ((h3 && ref.w < 0) ? cnt4.w : 0);
-        for (int j = 0; j < ntri; ++j) {
-            const int k = j < s0 ? ~ref.x + j : j < s1 ? ~ref.y + (j - s0) : j < s2 ? ~ref.z + (j - s1) : ~ref.w + (j - s2);
+        bool done = false, hit_any = false;
+        if (lj < ltri) {
+            // one triangle of the last visit's hit leaves, in slot order
+            const int ls0 = (int)(lcnt & 0xffu);
+            const int ls1 = ls0 + (int)((lcnt >> 8) & 0xffu);
+            const int ls2 = ls1 + (int)((lcnt >> 16) & 0xffu);
+            const int j = lj++;
+            const int k = j < ls0 ? ~lref.x + j : j < ls1 ? ~lref.y + (j - ls0) : j < ls2 ? ~lref.z + (j - ls1)
+                                                                                    : ~lref.w + (j - ls2);
             const float4* tp = bvh.tris + 3 * k;
             const float4 ta = tp[0], tb = tp[1], tc = tp[2];
             if (COUNT) cnt.tris++;
@@ -225,40 +229,63 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
             if (mt_test(ta, tb, tc, o, d, u, v, t) && t >= tmin && t <= best_t) {
                 if (ANY) {
                     hit_any = true;
-                    break;
-                }
-                const uint32_t id = __float_as_uint(tc.y);
-                if (t < best_t || id < best) {
-                    best_t = t;
-                    best = id;
-                    bu = u;
-                    bv = v;
+                } else {
+                    const uint32_t id = __float_as_uint(tc.y);
+                    if (t < best_t || id < best) {
+                        best_t = t;
+                        best = id;
+                        bu = u;
+                        bv = v;
+                    }
                 }
             }
-        }
-        // hit internal children: keys = entry distances (+inf: not taken),
-        // sorted by a 5-exchange network; the nearest is visited next and
-        // the others are stacked farthest first
-        float k0 = (h0 && ref.x >= 0) ? n0 : INFINITY, k1 = (h1 && ref.y >= 0) ? n1 : INFINITY;
-        float k2 = (h2 && ref.z >= 0) ? n2 : INFINITY, k3 = (h3 && ref.w >= 0) ? n3 : INFINITY;
-        int c0 = ref.x, c1 = ref.y, c2 = ref.z, c3 = ref.w;
-        cswap(k0, c0, k1, c1);
-        cswap(k2, c2, k3, c3);
-        cswap(k0, c0, k2, c2);
-        cswap(k1, c1, k3, c3);
-        cswap(k1, c1, k2, c2);
-        bool done = false;
-        if (ANY && hit_any) {
-            done = true;
-        } else if (k0 != INFINITY) {
-            if (k3 != INFINITY) stk.push(sp, c3);
-            if (k2 != INFINITY) stk.push(sp, c2);
-            if (k1 != INFINITY) stk.push(sp, c1);
-            cur = c0;
-        } else if (sp == 0) {
-            done = true;
+            done = hit_any || (last && lj == ltri);
         } else {
-            cur = stk.pop(sp);
+            const float4* np = bvh.nodes + 4 * (size_t)cur;
+            const float4 r0 = np[0];
+            const uint4 r1 = reinterpret_cast<const uint4*>(np)[1];
+            const uint4 r2 = reinterpret_cast<const uint4*>(np)[2];
+            const int4 ref = reinterpret_cast<const int4*>(np)[3];
+            if (COUNT) cnt.nodes++;
+            // child boxes: origin + q * step (q * step exact, one rounding)
+            const uint32_t eb = __float_as_uint(r0.w);
+            const float sx = __uint_as_float((eb & 0xffu) << 23), sy = __uint_as_float(((eb >> 8) & 0xffu) << 23),
+                        sz = __uint_as_float(((eb >> 16) & 0xffu) << 23);
+            float n0, n1, n2, n3;
+            const bool h0 = slab_q(r0, r1, r2, 0, sx, sy, sz, o, rcp, tmin, best_t, n0);
+            const bool h1 = slab_q(r0, r1, r2, 8, sx, sy, sz, o, rcp, tmin, best_t, n1);
+            const bool h2 = slab_q(r0, r1, r2, 16, sx, sy, sz, o, rcp, tmin, best_t, n2);
+            const bool h3 = slab_q(r0, r1, r2, 24, sx, sy, sz, o, rcp, tmin, best_t, n3);
+            // triangles of the hit leaf children, tested in the next
+            // iterations (the closest hit is the minimum over (t, id), so
+            // their order does not change it)
+            lref = ref;
+            lcnt = r2.z & (((h0 && ref.x < 0) ? 0xffu : 0u) | ((h1 && ref.y < 0) ? 0xff00u : 0u) |
+                           ((h2 && ref.z < 0) ? 0xff0000u : 0u) | ((h3 && ref.w < 0) ? 0xff000000u : 0u));
+            ltri = (int)((lcnt & 0xffu) + ((lcnt >> 8) & 0xffu) + ((lcnt >> 16) & 0xffu) + (lcnt >> 24));
+            lj = 0;
+            // hit internal children: keys = entry distances (+inf: not
+            // taken), sorted by a 5-exchange network; the nearest is visited
+            // next and the others are stacked farthest first
+            float k0 = (h0 && ref.x >= 0) ? n0 : INFINITY, k1 = (h1 && ref.y >= 0) ? n1 : INFINITY;
+            float k2 = (h2 && ref.z >= 0) ? n2 : INFINITY, k3 = (h3 && ref.w >= 0) ? n3 : INFINITY;
+            int c0 = ref.x, c1 = ref.y, c2 = ref.z, c3 = ref.w;
+            cswap(k0, c0, k1, c1);
+            cswap(k2, c2, k3, c3);
+            cswap(k0, c0, k2, c2);
+            cswap(k1, c1, k3, c3);
+            cswap(k1, c1, k2, c2);
+            if (k0 != INFINITY) {
+                if (k3 != INFINITY) stk.push(sp, c3);
+                if (k2 != INFINITY) stk.push(sp, c2);
+                if (k1 != INFINITY) stk.push(sp, c1);
+                cur = c0;
+            } else if (sp == 0) {
+                last = true;
+            } else {
+                cur = stk.pop(sp);
+            }
+            done = last && ltri == 0;
         }
         if (done) {
             if (ANY)

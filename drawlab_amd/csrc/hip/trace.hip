@@ -1,6 +1,7 @@
 // Batch traversal kernels behind dl_hip_trace_closest / dl_hip_trace_shadow.
 // Rays are 8 planes of n floats (32 B/ray in), hits 4 planes (16 B/ray out),
 // occlusion 1 B/ray out.
+#include <algorithm>
 #include <mutex>
 
 #include "backend.h"
@@ -42,7 +43,7 @@ struct BatchIO {
     }
 };
 
-// Persistent grids (gridDim.x == Ctx::trace_blocks) with per-lane refill.
+// Persistent grids (Ctx::trace_grid / trace_grid_count blocks) with per-lane refill.
 template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(BvhView bvh, BatchIO io, uint32_t n, uint32_t* work,
                                                                int* __restrict__ ovf) {
@@ -63,6 +64,19 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(BvhView bvh, Batch
     traverse_persistent<true, COUNT>(bvh, n, work, stk, io, tot);
 }
 
+// Resident blocks per CU of the batch traversal kernels (the fewer of the
+// closest-hit and any-hit kernel), without or with traversal counters.
+int batch_trace_blocks_per_cu(bool counting) {
+    int a = 0, b = 0;
+    hipError_t e = counting ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace_closest<true>, kTraceBlock, 0)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace_closest<false>, kTraceBlock, 0);
+    if (e == hipSuccess)
+        e = counting ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace_shadow<true>, kTraceBlock, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace_shadow<false>, kTraceBlock, 0);
+    if (e != hipSuccess) return kTraceBlocksPerCu;
+    return std::max(1, std::min(kTraceBlocksPerCu, std::min(a, b)));
+}
+
 __global__ void k_zero_work(uint32_t* w) {
     if (threadIdx.x < 4) w[threadIdx.x] = 0;
 }
@@ -76,7 +90,7 @@ dl_status launch_trace_closest(Ctx* c, const float* rays, uint32_t n, uint32_t* 
     if (n == 0) return DL_OK;
     if (n > kMaxBatchRays) return set_error(c, DL_ERR_INVALID, "more than 2^31 rays in one batch");
     BvhView bvh{c->ds.nodes, c->ds.tris};
-    const uint32_t blocks = c->trace_blocks;
+    const uint32_t blocks = cn ? c->trace_grid_count : c->trace_grid;
     DL_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
     BatchIO io{rays, (size_t)n, prim, t, u, v, cn, ct, nullptr};
     k_zero_work<<<1, 64, 0, c->stream>>>(c->work);
@@ -97,7 +111,7 @@ dl_status launch_trace_shadow(Ctx* c, const float* rays, uint32_t n, uint8_t* oc
     if (n == 0) return DL_OK;
     if (n > kMaxBatchRays) return set_error(c, DL_ERR_INVALID, "more than 2^31 rays in one batch");
     BvhView bvh{c->ds.nodes, c->ds.tris};
-    const uint32_t blocks = c->trace_blocks;
+    const uint32_t blocks = cn ? c->trace_grid_count : c->trace_grid;
     DL_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
     BatchIO io{rays, (size_t)n, nullptr, nullptr, nullptr, nullptr, cn, ct, occ};
     k_zero_work<<<1, 64, 0, c->stream>>>(c->work);
