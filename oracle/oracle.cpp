@@ -8,6 +8,22 @@
 // anisotropic GGX BSDF) the restatement follows the reference device code,
 // evaluated with the CPU path's arithmetic conventions.  The decisions where
 // the two reference backends disagree are listed in oracle.h.
+//
+// How each part is pinned (beyond restating the cited lines):
+//   warps, fresnel/refract/reflect  bit-exact against the reference's own
+//       wrap.cpp / math.cpp compiled from its sources (oracle/ref.mk)
+//   pcg32       published known answers and a big-integer restatement
+//   MT, octree  dyadic hand cases; octree == brute force on edge, axis and
+//       random ray batches (tests/test_oracle_cpu.py)
+//   BSDFs       integral of pdf = valid-sample fraction, mean weight =
+//       integral of f cos, chi-square of sampled directions, reciprocity,
+//       textbook closed forms of D, G and Fresnel, Snell / TIR / Fresnel
+//       frequency (tests/test_oracle_pins_cpu.py)
+//   emitters    area pick chi-square, Lambert polygon irradiance, point
+//       irradiance, environment direction chi-square, emitter pick
+//   Li          white furnaces (closed emitting box, spheres under a
+//       constant sky) and a point-lit plane, per path
+//   Each of these fails when a mistake is planted (dlo_set_mutation).
 #include "oracle.h"
 #include "omath.h"
 
