@@ -76,7 +76,7 @@ class dl_camera_desc(ctypes.Structure):
 class dl_batch_item(ctypes.Structure):
     _fields_ = [("camera", dl_camera_desc), ("spp", ctypes.c_int32), ("first_sample", ctypes.c_int32),
                 ("seed", ctypes.c_uint32), ("film", ctypes.c_int32), ("accumulate", ctypes.c_int32),
-                ("rgb_out", ctypes.c_void_p)]
+                ("rgb_out", ctypes.c_void_p), ("scene", ctypes.c_uint64)]
 
 
 BATCH_MAX = 32
@@ -501,15 +501,17 @@ class Backend:
             cam = dataclasses.replace(self.desc.camera, width=int(cd.width), height=int(cd.height))
         self.desc = dataclasses.replace(self.desc, camera=cam)
 
-    def render_batch(self, items, max_depth=None, rr_depth=None, normals=False):
-        """One pass shared by several images of the uploaded scene
-        (dl_hip_render_batch).  items: dicts with camera (Camera or
-        dl_camera_desc), spp, first_sample (0), seed (0), film (slot),
-        accumulate (False), image (True: return the resolved image).
-        Returns (images, stats); images[i] is None when not asked for."""
+    def render_batch(self, items, max_depth=None, rr_depth=None, normals=False, count_traversal=False):
+        """One pass shared by several images (dl_hip_render_batch).
+        items: dicts with camera (Camera or dl_camera_desc), spp,
+        first_sample (0), seed (0), film (slot), accumulate (False), image
+        (True: return the resolved image), scene (0: the current scene, or
+        the key of a resident scene, see scene_keep()).  Returns (images, stats);
+        images[i] is None when not asked for."""
         d = self.desc
         shared = self._params(1, 0, d.max_depth if max_depth is None else max_depth,
-                              d.rr_depth if rr_depth is None else rr_depth, flags=RENDER_NORMALS if normals else 0)
+                              d.rr_depth if rr_depth is None else rr_depth,
+                              flags=(RENDER_NORMALS if normals else 0) | (1 if count_traversal else 0))
         arr = (dl_batch_item * len(items))()
         imgs = []
         for i, it in enumerate(items):
@@ -519,6 +521,7 @@ class Backend:
             a.camera = cd
             a.spp, a.first_sample, a.seed = int(it["spp"]), int(it.get("first_sample", 0)), int(it.get("seed", 0))
             a.film, a.accumulate = int(it["film"]), int(bool(it.get("accumulate", False)))
+            a.scene = int(it.get("scene", 0))
             img = np.zeros((cd.height, cd.width, 3), np.float32) if it.get("image", True) else None
             a.rgb_out = None if img is None else img.ctypes.data
             imgs.append(img)
@@ -601,7 +604,8 @@ SERVICE_SYMBOLS = ("dl_hip_service_start", "dl_hip_service_add_scene", "dl_hip_s
 
 class dl_service_config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("lanes", ctypes.c_int32), ("policy", ctypes.c_int32),
-                ("builder", ctypes.c_int32), ("budget_bytes", ctypes.c_uint64), ("max_batch", ctypes.c_int32)]
+                ("builder", ctypes.c_int32), ("budget_bytes", ctypes.c_uint64), ("max_batch", ctypes.c_int32),
+                ("single_scene_passes", ctypes.c_int32)]
 
 
 class dl_job_info(ctypes.Structure):
@@ -618,7 +622,8 @@ class dl_service_stats(ctypes.Structure):
                 ("jobs_cancelled", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
                 ("peak_reserved_bytes", ctypes.c_uint64), ("estimate_misses", ctypes.c_uint64),
                 ("scene_uploads", ctypes.c_uint64), ("scene_selects", ctypes.c_uint64), ("paths", ctypes.c_double),
-                ("batched_jobs", ctypes.c_uint64), ("shared_passes", ctypes.c_uint64)]
+                ("batched_jobs", ctypes.c_uint64), ("shared_passes", ctypes.c_uint64),
+                ("cross_scene_passes", ctypes.c_uint64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -654,10 +659,11 @@ class Service:
     """Many render jobs on one device: scenes registered once, jobs queued
     and run on `lanes` worker contexts under a device-memory budget."""
 
-    def __init__(self, device=0, lanes=4, policy=POLICY_FIFO, budget_bytes=200 << 30, builder=BUILD_GPU, max_batch=0):
+    def __init__(self, device=0, lanes=4, policy=POLICY_FIFO, budget_bytes=200 << 30, builder=BUILD_GPU, max_batch=0,
+                 single_scene_passes=False):
         self.L = _service_lib()
         cfg = dl_service_config(int(device), int(lanes), int(policy), int(builder), int(budget_bytes),
-                                int(max_batch))
+                                int(max_batch), int(bool(single_scene_passes)))
         h = ctypes.c_void_p()
         st = self.L.dl_hip_service_start(ctypes.byref(cfg), ctypes.byref(h))
         if st != 0:

@@ -59,9 +59,12 @@ void swap_scene(Ctx* c, SceneState& s) {
     std::swap(c->ds, s.ds);
     std::swap(c->accel, s.accel);
     std::swap(c->scene_allocs, s.scene_allocs);
+    std::swap(c->num_textures, s.num_textures);
+    std::swap(c->num_texels, s.num_texels);
 }
 
 void free_scene(Ctx* c) {
+    free_scene_union(c);
     if (c->stream) hipStreamSynchronize(c->stream);
     // the acceleration structure is allocated by the builders, outside
     // scene_allocs
@@ -75,6 +78,7 @@ void free_scene(Ctx* c) {
 
 // Frees one resident (stashed) scene.
 void free_stashed(Ctx* c, SceneState& s) {
+    free_scene_union(c);
     if (c->stream) hipStreamSynchronize(c->stream);
     if (s.ds.nodes) dfree(c, s.ds.nodes);
     if (s.ds.tris) dfree(c, s.ds.tris);
@@ -210,6 +214,7 @@ dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
     Ctx* c = (Ctx*)h;
     if (!c || !sd) return DL_ERR_INVALID;
     hipSetDevice(c->device);
+    free_scene_union(c);  // its member "current scene without a key" is about to change
     release_current(c);
     if (sd->num_meshes == 0) return set_error(c, DL_ERR_INVALID, "scene has no meshes");
     DL_TRY(check_camera(c, sd->camera));
@@ -377,6 +382,8 @@ dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
             info[t] = make_int4((int)o, td.width, td.height, 0);
             o += n;
         }
+        c->num_textures = sd->num_textures;
+        c->num_texels = o;
         DL_TRY(upload(c, &ds.texels, tx.data(), tx.size()));
         DL_TRY(upload(c, &ds.tex_info, info.data(), info.size()));
         DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));  // tx / info leave scope
@@ -391,6 +398,7 @@ dl_status dl_hip_build_accel(void* h, int builder, dl_accel_stats* stats) {
     if (!c) return DL_ERR_INVALID;
     if (!c->has_scene) return set_error(c, DL_ERR_STATE, "build_accel before upload_scene");
     hipSetDevice(c->device);
+    free_scene_union(c);  // it may hold this scene's previous BVH
     if (c->ds.nodes) {
         dfree(c, c->ds.nodes);
         dfree(c, c->ds.tris);
@@ -627,6 +635,8 @@ dl_status dl_hip_scene_keep(void* h, uint64_t key) {
     if (!c) return DL_ERR_INVALID;
     if (key == 0) return set_error(c, DL_ERR_INVALID, "scene key 0 is reserved");
     if (!c->has_accel) return set_error(c, DL_ERR_STATE, "scene_keep before build_accel");
+    (void)hipSetDevice(c->device);
+    free_scene_union(c);  // members are named by key
     auto it = c->stash.find(key);
     if (it != c->stash.end()) {  // an older scene under the same key is replaced
         free_stashed(c, it->second);

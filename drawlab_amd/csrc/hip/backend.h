@@ -74,6 +74,20 @@ struct LightRec {
     int32_t env_w, env_h;
     float env_scale;
     int32_t env_visual;
+    uint32_t env_base;  // first texel of the environment in the env_* arrays
+    int32_t pad[3];
+};
+
+// Lights, background and BVH root of one scene of a layout that holds
+// several (a pass shared by images of different scenes); paths carry the
+// index of their scene's record.
+struct SceneRec {
+    int32_t light0;      // first light of the scene in the light array
+    int32_t num_lights;
+    int32_t env_light;   // its environment light (index into the light array), or -1
+    int32_t root;        // its BVH root node
+    float bg[3];
+    int32_t pad;
 };
 
 // Everything a context holds for one scene (geometry, BVH, materials,
@@ -98,6 +112,41 @@ struct SceneState {
     DevScene ds;
     dl_accel_stats accel{};
     std::vector<void*> scene_allocs;
+    uint32_t num_textures = 0;
+    uint64_t num_texels = 0;
+};
+
+// What a pass over several scenes reads of one of them, current or resident.
+struct SceneParts {
+    uint32_t nprims = 0, nverts = 0;
+    const std::vector<MeshRec>* meshes = nullptr;
+    const std::vector<dl_material_desc>* mats = nullptr;
+    const std::vector<LightRec>* lights = nullptr;
+    int env_light = -1;
+    const float* bg = nullptr;  // background (the view options' when set)
+    bool hide = false;          // view options: hidden lights
+    float eps = 0.0f;           // ray offset
+    const dl_film_desc* film = nullptr;
+    const DevScene* ds = nullptr;
+    const dl_accel_stats* accel = nullptr;
+    uint32_t num_textures = 0;
+    uint64_t num_texels = 0;
+};
+
+// Device copy of several resident scenes in one layout (their geometry,
+// materials, lights, environments, textures and BVHs concatenated, every
+// index rebased), so that the paths of images of different scenes share one
+// wavefront pass: each path carries the index of its scene's record
+// (SceneRec: its lights, environment, background and BVH root).  Built from
+// the scenes' device arrays; a member that changes (rebuilt, replaced,
+// dropped) invalidates it.
+struct SceneUnion {
+    std::vector<uint64_t> keys;  // member scenes in record order (0: the current scene when it has no key)
+    std::vector<int32_t> light0, root;  // per member: first light, BVH root node
+    DevScene ds;
+    SceneRec* recs = nullptr;  // device, one per member
+    int32_t* roots = nullptr;  // device, one per member
+    std::vector<void*> allocs;
 };
 
 struct Ctx {
@@ -135,8 +184,11 @@ struct Ctx {
     DevScene ds;
     dl_accel_stats accel{};
     std::vector<void*> scene_allocs;
+    uint32_t num_textures = 0;
+    uint64_t num_texels = 0;
     uint64_t scene_key = 0;                 // nonzero: keep the current scene resident when replaced
     std::map<uint64_t, SceneState> stash;   // resident scenes by key
+    SceneUnion* uni = nullptr;              // resident scenes in one layout (passes over several scenes)
 
     // render scratch (reused across calls, grown on demand)
     std::vector<void*> render_allocs;
@@ -217,6 +269,16 @@ void guide_net_ptrs(Ctx* c, const uint16_t** w, const uint16_t** wt, const float
 // one training step on R records; rad: final path radiance (float4 per path)
 dl_status guide_train_records(Ctx* c, const float* records, uint32_t R, const float4* rad, int steps);
 int guide_step_count(Ctx* c);  // Adam steps taken since guide_init
+
+// scene_union.hip
+// The scene of key k (0: the current scene) as a pass reads it; false if
+// there is none.
+bool scene_parts(Ctx* c, uint64_t key, SceneParts& out);
+// A union holding every scene of `keys` (normalised: 0 stands for the
+// current scene when it has no key); reused while it holds them all,
+// otherwise rebuilt over every scene resident on the context.
+dl_status scene_union_for(Ctx* c, const std::vector<uint64_t>& keys, SceneUnion** out);
+void free_scene_union(Ctx* c);
 
 // bvh_gpu.hip
 dl_status build_bvh_gpu(Ctx* c, dl_accel_stats* stats, int method);

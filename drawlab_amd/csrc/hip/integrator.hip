@@ -78,6 +78,8 @@ struct RenderBufs {
     float* rgb;
     float* gmix;  // guided rendering: mixture (40 floats) per entry of the queue being shaded
     float* grec;  // guided rendering: training records of the pass
+    uint8_t* sid;  // scene of the path (a pass over several scenes)
+    const int32_t* roots;  // BVH root node per scene (nullptr: one scene, root node 0)
 };
 
 struct RenderState {
@@ -87,7 +89,10 @@ struct RenderState {
     bool film_valid = false;       // film holds the accumulation of earlier calls
     bool types_present[8] = {false, false, false, false, false, false, false, false};
     uint32_t gmix_cap = 0, grec_cap = 0;  // guided rendering buffers (entries)
-    Counters* cnt_host = nullptr;  // pinned
+    // pinned counter snapshots: two taken in turn after each bounce (the host
+    // reads one while the GPU runs the next bounce), one for final statistics
+    Counters* cnt_host = nullptr;
+    hipEvent_t snap_ev[2] = {nullptr, nullptr};
     std::vector<void*> allocs;
     std::vector<hipEvent_t> evs;
 };
@@ -200,6 +205,7 @@ struct PassParams {
     int rr_depth;
     uint32_t cap;         // queue segment capacity
     float eps;            // ray offset (tmin; shadow rays end this short of the light)
+    uint32_t scene;       // scene record of the paths k_raygen starts (a pass over several scenes)
 };
 
 // ---------------------------------------------------------------------------
@@ -298,6 +304,7 @@ __global__ __launch_bounds__(kStageBlock) void k_raygen(PassParams pp, dl_camera
         rs.rng[p] = rng.state;
         rs.seq[p] = pp.sample_base + j;
         rs.pos[p] = make_float2(px, py);
+        rs.sid[p] = (uint8_t)pp.scene;
         rs.rayq[p] = p;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) rs.cnt->ray = total;
@@ -310,6 +317,10 @@ struct QueueIO {
     const float4* __restrict__ rd;
     float4* __restrict__ hit;
     uint8_t* __restrict__ occ;
+    const uint8_t* __restrict__ sid;
+    const int32_t* __restrict__ roots;
+    // first node of the ray's scene
+    __device__ __forceinline__ int root(uint32_t i) const { return roots ? roots[sid[q[i]]] : 0; }
     __device__ __forceinline__ void load(uint32_t i, vec3& o, vec3& d, float& tmin, float& tmax) const {
         const uint32_t p = q[i];
         const float4 a = ro[p], b = rd[p];
@@ -331,7 +342,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(BvhView bvh, Render
     const uint32_t n = rs.cnt->ray;
     const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x, G = gridDim.x * kTraceBlock;
     LaneStack stk{lds, (int)threadIdx.x, kTraceBlock, ovf + g, G};
-    QueueIO io{rs.rayq, rs.ray_o, rs.ray_d, rs.hit, nullptr};
+    QueueIO io{rs.rayq, rs.ray_o, rs.ray_d, rs.hit, nullptr, rs.sid, rs.roots};
     TraceCounters cnt;
     traverse_persistent<false, COUNT>(bvh, n, &rs.cnt->work_ray, stk, io, cnt);
     if (COUNT) {
@@ -346,7 +357,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_queue(BvhView bvh, Rende
     const uint32_t n = rs.cnt->shadow;
     const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x, G = gridDim.x * kTraceBlock;
     LaneStack stk{lds, (int)threadIdx.x, kTraceBlock, ovf + g, G};
-    QueueIO io{rs.shq, rs.sh_o, rs.sh_d, nullptr, rs.occ};
+    QueueIO io{rs.shq, rs.sh_o, rs.sh_d, nullptr, rs.occ, rs.sid, rs.roots};
     TraceCounters cnt;
     traverse_persistent<true, COUNT>(bvh, n, &rs.cnt->work_shadow, stk, io, cnt);
     if (COUNT) {
@@ -402,9 +413,10 @@ __global__ __launch_bounds__(kStageBlock) void k_post_camera(PassParams pp, Scen
                 const vec3 d = f3(rs.ray_d[p]);
                 vec3 rad = v3(0, 0, 0);
                 if (prim == 0xffffffffu) {
-                    if (s.env_light >= 0) {
-                        const LightRec& e = s.lights[s.env_light];
-                        rad = e.env_visual ? env_eval(s, e, d) : v3(s.bg[0], s.bg[1], s.bg[2]);
+                    const SceneRec sc = path_scene(s, p);
+                    if (sc.env_light >= 0) {
+                        const LightRec& e = s.lights[sc.env_light];
+                        rad = e.env_visual ? env_eval(s, e, d) : v3(sc.bg[0], sc.bg[1], sc.bg[2]);
                     }
                 } else {
                     const uint32_t info = prim_info(s, prim);
@@ -547,15 +559,16 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                 ds.pdf = 0.0f;
                 ds.delta = false;
                 vec3 spectrum = v3(0, 0, 0);
-                const int L = s.num_lights;
+                const SceneRec sc = path_scene(s, p);
+                const int L = sc.num_lights;
                 if (L == 1) {
-                    spectrum = emitter_sample(s, s.lights[0], lx, ly, its, ds, rng);
+                    spectrum = emitter_sample(s, s.lights[sc.light0], lx, ly, its, ds, rng);
                 } else if (L > 1) {
                     const float epdf = 1.f / (float)L;
                     uint32_t idx = (uint32_t)(lx * (float)L);
                     idx = idx < (uint32_t)(L - 1) ? idx : (uint32_t)(L - 1);
                     lx = lx * (float)L - (float)idx;
-                    spectrum = emitter_sample(s, s.lights[idx], lx, ly, its, ds, rng);
+                    spectrum = emitter_sample(s, s.lights[sc.light0 + (int)idx], lx, ly, its, ds, rng);
                     spectrum = scale(spectrum, (float)L);
                     ds.pdf *= epdf;
                 }
@@ -721,7 +734,7 @@ __global__ __launch_bounds__(kStageBlock) void k_post_emit(SceneView s, RenderBu
         hit_point_normal(s, prim, h.z, h.w, hp, hn);
         const vec3 ev = dot(hn, neg(d)) > 0.f ? pcol(s.lights[light].radiance) : v3(0, 0, 0);
         const DirSample hs = dir_sample(o, hp, hn);
-        const float epdf = pdf_emitter_direction(s, light, hs);
+        const float epdf = pdf_emitter_direction(s, light, path_scene(s, p).num_lights, hs);
         const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
         const vec3 rad = add(f3(rs.rad[p]), mul(scale(f3(t4), mis), ev));
         rs.rad[p] = f4(rad, 0.f);
@@ -775,13 +788,15 @@ __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s
                     const vec3 d = f3(rs.ray_d[p]);
                     const bool cam = HIDE && (flags & kFlagCamera) != 0;
                     if (prim == 0xffffffffu) {
-                        if (s.env_light >= 0) {
-                            const LightRec& e = s.lights[s.env_light];
+                        const SceneRec sc = path_scene(s, p);
+                        if (sc.env_light >= 0) {
+                            const LightRec& e = s.lights[sc.env_light];
                             if (cam) {
-                                rad = add(rad, mul(thr, e.env_visual ? env_eval(s, e, d) : v3(s.bg[0], s.bg[1], s.bg[2])));
+                                rad = add(rad,
+                                          mul(thr, e.env_visual ? env_eval(s, e, d) : v3(sc.bg[0], sc.bg[1], sc.bg[2])));
                             } else {
                                 float epdf;
-                                const vec3 ev = env_eval_pdf(s, e, d, epdf);
+                                const vec3 ev = env_eval_pdf(s, e, sc.num_lights, d, epdf);
                                 const float mis = diffuse ? mis_weight(prev_pdf, epdf) : 1.f;
                                 rad = add(rad, mul(scale(thr, mis), ev));
                             }
@@ -1066,6 +1081,8 @@ void free_render_state(Ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void* p : rs->allocs) (void)dfree(c, p);
     if (rs->cnt_host) (void)hipHostFree(rs->cnt_host);
+    for (hipEvent_t e : rs->snap_ev)
+        if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : rs->evs) (void)hipEventDestroy(e);
     delete rs;
     c->rs = nullptr;
@@ -1122,6 +1139,7 @@ dl_status ensure_state(Ctx* c, uint32_t P, uint32_t pixels) {
     DL_TRY(alloc(c, rs, &rs->b.seq, P));
     DL_TRY(alloc(c, rs, &rs->b.pos, P));
     DL_TRY(alloc(c, rs, &rs->b.occ, P));
+    DL_TRY(alloc(c, rs, &rs->b.sid, P));
     DL_TRY(alloc(c, rs, &rs->b.matq[0], (size_t)P * kTypes));
     DL_TRY(alloc(c, rs, &rs->b.matq[1], (size_t)P * kTypes));
     DL_TRY(alloc(c, rs, &rs->b.rayq, P));
@@ -1130,7 +1148,8 @@ dl_status ensure_state(Ctx* c, uint32_t P, uint32_t pixels) {
     DL_TRY(alloc(c, rs, &rs->b.cnt, 1));
     DL_TRY(alloc(c, rs, &rs->b.film, pixels));
     DL_TRY(alloc(c, rs, &rs->b.rgb, (size_t)pixels * 3));
-    DL_HIP_CHECK(c, hipHostMalloc((void**)&rs->cnt_host, sizeof(Counters), hipHostMallocDefault));
+    DL_HIP_CHECK(c, hipHostMalloc((void**)&rs->cnt_host, 3 * sizeof(Counters), hipHostMallocDefault));
+    for (hipEvent_t& e : rs->snap_ev) DL_HIP_CHECK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return DL_OK;
 }
 
@@ -1138,7 +1157,7 @@ constexpr uint32_t kGuideRecords = dl_sizes::kGuideRecordSlots;  // training rec
 static_assert(kGRecFloats == dl_sizes::kGuideRecordFloats && kGMix == dl_sizes::kGuideMixFloats,
               "guided buffer sizes the render service reserves");
 // the per-path bytes ensure_state allocates, as the render service reserves them
-static_assert(9 * sizeof(float4) + sizeof(uint64_t) + sizeof(uint32_t) + sizeof(float2) + sizeof(uint8_t) +
+static_assert(9 * sizeof(float4) + sizeof(uint64_t) + sizeof(uint32_t) + sizeof(float2) + 2 * sizeof(uint8_t) +
                       2 * kTypes * sizeof(uint32_t) + 3 * sizeof(uint32_t) ==
                   dl_sizes::kStateBytesPerPath &&
               kTypes == dl_sizes::kMatQueueSegs && sizeof(float4) + 3 * sizeof(float) == dl_sizes::kFilmBytesPerPixel,
@@ -1156,9 +1175,11 @@ dl_status ensure_guide_bufs(Ctx* c, RenderState* rs, uint32_t cap) {
     return DL_OK;
 }
 
-SceneView scene_view(Ctx* c) {
+// The kernels' view of a device layout; the light count, environment and
+// background are those of a layout holding one scene (a layout holding
+// several gives them per scene, in SceneView::scenes).
+SceneView layout_view(const DevScene& d, int num_lights, int env_light, const float* bg) {
     SceneView s;
-    const DevScene& d = c->ds;
     s.pos = d.pos;
     s.nrm = d.nrm;
     s.tan = d.tan;
@@ -1176,10 +1197,16 @@ SceneView scene_view(Ctx* c) {
     s.uv = d.uv;
     s.texels = d.texels;
     s.tex_info = d.tex_info;
-    s.num_lights = (int)c->light_recs.size();
-    s.env_light = c->env_light;
-    for (int k = 0; k < 3; k++) s.bg[k] = c->view_set ? c->view.bg_color[k] : c->bg[k];
+    s.num_lights = num_lights;
+    s.env_light = env_light;
+    for (int k = 0; k < 3; k++) s.bg[k] = bg[k];
+    s.scenes = nullptr;
+    s.sid = nullptr;
     return s;
+}
+
+SceneView scene_view(Ctx* c) {
+    return layout_view(c->ds, (int)c->light_recs.size(), c->env_light, c->view_set ? c->view.bg_color : c->bg);
 }
 
 struct Timer {
@@ -1215,7 +1242,6 @@ dl_status run_bounces(Ctx* c, RenderState* rs, const PassParams& pp, const Scene
                       bool count, bool normals, bool hide, bool guided, const bool* guide_types, const GuideView& gv,
                       const Net& net, int guide_grid, int stage_grid, int trace_grid, Timer& tm, double& closest,
                       double& shadow) {
-    int cur = 0;
     DL_TRY(tm.start(kTagClosest));
     if (count)
         k_trace_queue<true><<<trace_grid, kTraceBlock, 0, c->stream>>>(bvh, rs->b, c->stack_ovf);
@@ -1226,27 +1252,22 @@ dl_status run_bounces(Ctx* c, RenderState* rs, const PassParams& pp, const Scene
     if (normals)
         k_normals<<<stage_grid, kStageBlock, 0, c->stream>>>(sv, rs->b);
     else if (hide)
-        k_post_camera<true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
+        k_post_camera<true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, 0);
     else
-        k_post_camera<false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
+        k_post_camera<false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, 0);
     DL_TRY(tm.stop());
     DL_HIP_CHECK(c, hipGetLastError());
-    for (int iter = 0;; ++iter) {
-        DL_HIP_CHECK(c, hipMemcpyAsync(rs->cnt_host, rs->b.cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-        DL_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-        uint32_t live = 0;
-        for (int t = 0; t < kTypes; t++) live += rs->cnt_host->mat[cur][t];
-        if (iter > 0) {
-            closest += rs->cnt_host->ray;
-            shadow += rs->cnt_host->shadow;
-        }
-        if (live == 0) break;
+    if (normals) return DL_OK;  // no bounces
+    // One bounce: shading of the paths in queue `cur`, their continuation and
+    // shadow traces, and post (which queues the survivors in cur ^ 1).  Every
+    // kernel reads its queue length from the device counters, so a bounce
+    // over empty queues launches and does nothing.
+    auto bounce = [&](int cur) -> dl_status {
         k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, cur ^ 1);
         DL_TRY(tm.start(kTagShade));
-        if (rs->cnt_host->mat[cur][kPassQ] > 0)
-            k_pass<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
+        if (hide) k_pass<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
         for (int t = 0; t < DL_BSDF_COUNT; t++) {
-            if (!rs->types_present[t] || rs->cnt_host->mat[cur][t] == 0) continue;
+            if (!rs->types_present[t]) continue;
             if (guided && guide_types[t]) {
                 DL_TRY(tm.stop());
                 DL_TRY(tm.start(kTagGuide));
@@ -1296,8 +1317,39 @@ dl_status run_bounces(Ctx* c, RenderState* rs, const PassParams& pp, const Scene
         k_post_emit<<<stage_grid, kStageBlock, 0, c->stream>>>(sv, rs->b);
         DL_TRY(tm.stop());
         DL_HIP_CHECK(c, hipGetLastError());
-        cur ^= 1;
+        return DL_OK;
+    };
+    // Snapshot k (pinned slot k & 1) is copied after bounce k (k = -1: after
+    // the first hits) and holds the queue bounce k + 1 shades and the rays
+    // bounce k traced.  The host queues bounce k + 1 before it waits for
+    // snapshot k, so the GPU never drains between bounces; the bounce queued
+    // after the last live one finds empty queues.
+    auto snapshot = [&](int k) -> dl_status {
+        const int slot = k & 1;
+        DL_HIP_CHECK(c, hipMemcpyAsync(rs->cnt_host + slot, rs->b.cnt, sizeof(Counters), hipMemcpyDeviceToHost,
+                                       c->stream));
+        DL_HIP_CHECK(c, hipEventRecord(rs->snap_ev[slot], c->stream));
+        return DL_OK;
+    };
+    DL_TRY(snapshot(-1));
+    for (int k = 0;; ++k) {
+        // queue bounce k, then learn from snapshot k - 1 whether it had work
+        const int cur = k & 1;
+        DL_TRY(bounce(cur));
+        DL_TRY(snapshot(k));
+        const int slot = (k - 1) & 1;
+        DL_HIP_CHECK(c, hipEventSynchronize(rs->snap_ev[slot]));
+        const Counters& s = rs->cnt_host[slot];
+        if (k > 0) {
+            closest += s.ray;
+            shadow += s.shadow;
+        }
+        uint32_t live = 0;
+        for (int t = 0; t < kTypes; t++) live += s.mat[cur][t];
+        if (live == 0) break;  // bounce k (queued) is empty, and so is every later one
     }
+    // the last snapshot's slot is written by a copy still in flight
+    for (hipEvent_t e : rs->snap_ev) DL_HIP_CHECK(c, hipEventSynchronize(e));
     return DL_OK;
 }
 
@@ -1326,6 +1378,7 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
     const uint32_t cap = WH * k;
     DL_TRY(ensure_state(c, cap, WH));
     RenderState* rs = (RenderState*)c->rs;
+    rs->b.roots = nullptr;  // one scene: every ray starts at node 0
     const SceneView sv = scene_view(c);
     for (int t = 0; t < 8; t++) rs->types_present[t] = false;
     bool guide_types[8] = {false, false, false, false, false, false, false, false};
@@ -1378,7 +1431,7 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
         uint32_t kk = std::min(k, total_spp - done);
         if (schedule) kk = std::min(kk, pass == 0 ? 1u : 1u << std::min<uint32_t>(pass - 1, 30));
         if (guided) gv.frac = (train && guide_step_count(c) == 0) ? 0.0f : kGuideFraction;
-        PassParams pp;
+        PassParams pp{};
         pp.P = WH * kk;
         pp.W = W;
         pp.H = H;
@@ -1482,12 +1535,32 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
 dl_status render_batch(Ctx* c, const dl_render_params* prm, const dl_batch_item* items, uint32_t n,
                        dl_render_stats* stats) {
     if (prm->flags & DL_RENDER_GUIDED) return set_error(c, DL_ERR_INVALID, "shared passes do not take DL_RENDER_GUIDED");
-    if (c->light_recs.empty()) return set_error(c, DL_ERR_INVALID, "There is not emitter in the scene.");
     (void)hipSetDevice(c->device);
+    // the pass's scenes (an item's key 0: the current scene), in item order
+    std::vector<uint64_t> keys;
+    std::vector<uint32_t> item_key(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const uint64_t k = items[i].scene != 0 ? items[i].scene : c->scene_key;
+        const auto it = std::find(keys.begin(), keys.end(), k);
+        item_key[i] = (uint32_t)(it - keys.begin());
+        if (it == keys.end()) keys.push_back(k);
+    }
+    std::vector<SceneParts> parts(keys.size());
+    for (size_t k = 0; k < keys.size(); k++) {
+        if (!scene_parts(c, keys[k], parts[k]))
+            return set_error(c, DL_ERR_STATE, keys[k] == c->scene_key ? "render_batch before build_accel"
+                                                                     : "no resident scene with this key");
+        if (parts[k].lights->empty()) return set_error(c, DL_ERR_INVALID, "There is not emitter in the scene.");
+        if (parts[k].hide != parts[0].hide || parts[k].eps != parts[0].eps)
+            return set_error(c, DL_ERR_INVALID, "scenes of a shared pass differ in hide_lights or epsilon");
+    }
     uint64_t total = 0;
     for (uint32_t i = 0; i < n; i++)
         total += (uint64_t)items[i].camera.width * items[i].camera.height * (uint64_t)items[i].spp;
     if (total > (1ull << 27)) return set_error(c, DL_ERR_INVALID, "pass too large");
+    // several scenes: one layout holding them all, a record per scene
+    SceneUnion* uni = nullptr;
+    if (keys.size() > 1) DL_TRY(scene_union_for(c, keys, &uni));
     const uint32_t P = (uint32_t)total;
     DL_TRY(ensure_state(c, P, 1));
     RenderState* rs = (RenderState*)c->rs;
@@ -1508,15 +1581,47 @@ dl_status render_batch(Ctx* c, const dl_render_params* prm, const dl_batch_item*
         f.w = w;
         f.h = h;
     }
-    const SceneView sv = scene_view(c);
+    SceneView sv;
+    BvhView bvh;
+    std::vector<uint32_t> item_rec(n, 0);  // scene record of each item's paths
+    std::vector<SceneRec> recs;
+    if (!uni) {
+        const SceneParts& s = parts[0];
+        sv = layout_view(*s.ds, (int)s.lights->size(), s.env_light, s.bg);
+        bvh = BvhView{s.ds->nodes, s.ds->tris};
+        rs->b.roots = nullptr;
+    } else {
+        sv = layout_view(uni->ds, 0, -1, parts[0].bg);
+        recs.resize(uni->keys.size());
+        for (size_t m = 0; m < uni->keys.size(); m++) {
+            SceneParts mp;
+            if (!scene_parts(c, uni->keys[m], mp)) return set_error(c, DL_ERR_STATE, "resident scene lost");
+            SceneRec& r = recs[m];
+            r.light0 = uni->light0[m];
+            r.num_lights = (int32_t)mp.lights->size();
+            r.env_light = mp.env_light >= 0 ? uni->light0[m] + mp.env_light : -1;
+            r.root = uni->root[m];
+            for (int k = 0; k < 3; k++) r.bg[k] = mp.bg[k];
+            r.pad = 0;
+        }
+        DL_HIP_CHECK(c, hipMemcpyAsync(uni->recs, recs.data(), recs.size() * sizeof(SceneRec), hipMemcpyHostToDevice,
+                                       c->stream));
+        for (uint32_t i = 0; i < n; i++)
+            item_rec[i] = (uint32_t)(std::find(uni->keys.begin(), uni->keys.end(), keys[item_key[i]]) -
+                                     uni->keys.begin());
+        sv.scenes = uni->recs;
+        sv.sid = rs->b.sid;
+        bvh = BvhView{uni->ds.nodes, uni->ds.tris};
+        rs->b.roots = uni->roots;
+    }
     for (int t = 0; t < 8; t++) rs->types_present[t] = false;
-    for (const auto& m : c->mats) rs->types_present[m.type] = true;
+    for (const SceneParts& s : parts)
+        for (const auto& m : *s.mats) rs->types_present[m.type] = true;
     const bool guide_types[8] = {false, false, false, false, false, false, false, false};
-    const BvhView bvh{c->ds.nodes, c->ds.tris};
     const bool count = stats && (prm->flags & DL_RENDER_COUNT_TRAVERSAL);
     const bool normals = (prm->flags & DL_RENDER_NORMALS) != 0;
-    const bool hide = c->view_set && c->view.hide_lights != 0;
-    const float eps = c->view_set && c->view.epsilon > 0.0f ? c->view.epsilon : kEpsilon;
+    const bool hide = parts[0].hide;
+    const float eps = parts[0].eps;
     const int stage_grid = std::max(1, c->num_cus) * 8;
     const int trace_grid = (int)(count ? c->trace_grid_count : c->trace_grid);
     std::vector<int> tags;
@@ -1548,6 +1653,7 @@ dl_status render_batch(Ctx* c, const dl_render_params* prm, const dl_batch_item*
         q.P = q.W * q.H * (uint32_t)it.spp;
         q.sample_base = (uint32_t)it.first_sample;
         q.seed = it.seed;
+        q.scene = item_rec[i];
         base[i] = off;
         if (!it.accumulate)
             k_clear_film<<<stage_grid, kStageBlock, 0, c->stream>>>((float4*)c->film_slots[it.film].film, q.W * q.H);
@@ -1567,11 +1673,12 @@ dl_status render_batch(Ctx* c, const dl_render_params* prm, const dl_batch_item*
         fb.rad += base[i];
         fb.pos += base[i];
         fb.film = (float4*)c->film_slots[it.film].film;
-        if ((int)ceilf(c->film.radius) + 1 <= kFilmHalo) {
+        const dl_film_desc& film = *parts[item_key[i]].film;  // the image's scene's reconstruction filter
+        if ((int)ceilf(film.radius) + 1 <= kFilmHalo) {
             const dim3 fgrid((q.W + kFilmTile - 1) / kFilmTile, (q.H + kFilmTile - 1) / kFilmTile);
-            k_film_tiled<<<fgrid, kFilmTile * kFilmTile, 0, c->stream>>>(q, (uint32_t)it.spp, c->film, fb);
+            k_film_tiled<<<fgrid, kFilmTile * kFilmTile, 0, c->stream>>>(q, (uint32_t)it.spp, film, fb);
         } else {
-            k_film<<<stage_grid, kStageBlock, 0, c->stream>>>(q, (uint32_t)it.spp, c->film, fb);
+            k_film<<<stage_grid, kStageBlock, 0, c->stream>>>(q, (uint32_t)it.spp, film, fb);
         }
         if (it.rgb_out)
             k_resolve<<<stage_grid, kStageBlock, 0, c->stream>>>((const float4*)c->film_slots[it.film].film,

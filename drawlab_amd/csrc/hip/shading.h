@@ -28,7 +28,26 @@ struct SceneView {
     int num_lights;
     int env_light;
     float bg[3];
+    // a layout holding several scenes: per-scene records and each path's
+    // scene index (nullptr: one scene, described by the fields above)
+    const SceneRec* __restrict__ scenes;
+    const uint8_t* __restrict__ sid;
 };
+
+// The record of the scene path p belongs to.
+__device__ __forceinline__ SceneRec path_scene(const SceneView& s, uint32_t p) {
+    if (s.sid) return s.scenes[s.sid[p]];
+    SceneRec r;
+    r.light0 = 0;
+    r.num_lights = s.num_lights;
+    r.env_light = s.env_light;
+    r.root = 0;
+    r.bg[0] = s.bg[0];
+    r.bg[1] = s.bg[1];
+    r.bg[2] = s.bg[2];
+    r.pad = 0;
+    return r;
+}
 
 __device__ __forceinline__ vec3 ld3(const float* p, uint32_t i) { return v3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 
@@ -547,8 +566,9 @@ __device__ __forceinline__ vec3 env_lookup(const SceneView& s, const LightRec& e
     const int ix = (int)fx0, iy = (int)fy0;
     const int x0 = wrap_index(ix, w), x1 = wrap_index(ix + 1, w);
     const int y0 = wrap_index(iy, h), y1 = wrap_index(iy + 1, h);
-    const float4 c00 = s.env_rgb[y0 * w + x0], c10 = s.env_rgb[y0 * w + x1];
-    const float4 c01 = s.env_rgb[y1 * w + x0], c11 = s.env_rgb[y1 * w + x1];
+    const float4* __restrict__ tx = s.env_rgb + e.env_base;
+    const float4 c00 = tx[y0 * w + x0], c10 = tx[y0 * w + x1];
+    const float4 c01 = tx[y1 * w + x0], c11 = tx[y1 * w + x1];
     const float w00 = (1.0f - ax) * (1.0f - ay), w10 = ax * (1.0f - ay);
     const float w01 = (1.0f - ax) * ay, w11 = ax * ay;
     vec3 r = add(add(add(scale(v3(c00.x, c00.y, c00.z), w00), scale(v3(c10.x, c10.y, c10.z), w10)),
@@ -571,21 +591,21 @@ __device__ __forceinline__ vec3 env_eval(const SceneView& s, const LightRec& e, 
 __device__ __forceinline__ float env_pdf_uv(const SceneView& s, const LightRec& e, float u, float v) {
     const int x = min((int)(u * (float)e.env_w), e.env_w - 1);
     const int y = min((int)(v * (float)e.env_h), e.env_h - 1);
-    return s.env_pdf[(size_t)y * e.env_w + x];
+    return s.env_pdf[e.env_base + (size_t)y * e.env_w + x];
 }
 __device__ __forceinline__ float env_pdf(const SceneView& s, const LightRec& e, vec3 d) {
     float u, v;
     env_coords(d, u, v);
     return env_pdf_uv(s, e, u, v);
 }
-// Radiance and the emitter-sampling pdf (scene level, divided by the emitter
-// count as Scene::pdfEmitterDirection does) of direction d, from one
-// evaluation of the lat-long coordinates.
-__device__ __forceinline__ vec3 env_eval_pdf(const SceneView& s, const LightRec& e, vec3 d, float& pdf) {
+// Radiance and the emitter-sampling pdf (scene level, divided by the
+// scene's emitter count nl as Scene::pdfEmitterDirection does) of direction
+// d, from one evaluation of the lat-long coordinates.
+__device__ __forceinline__ vec3 env_eval_pdf(const SceneView& s, const LightRec& e, int nl, vec3 d, float& pdf) {
     float u, v;
     env_coords(d, u, v);
     const float p = env_pdf_uv(s, e, u, v);
-    pdf = s.num_lights == 1 ? p : p / (float)s.num_lights;
+    pdf = nl == 1 ? p : p / (float)nl;
     return env_lookup(s, e, u, v);
 }
 
@@ -648,16 +668,16 @@ __device__ inline vec3 emitter_sample(const SceneView& s, const LightRec& e, flo
     const uint32_t idx = min((uint32_t)(sx * (float)size), size - 1);
     uint32_t env_idx;
     float xy = sy;
-    const float q = s.env_q[idx];
+    const float q = s.env_q[e.env_base + idx];
     if (xy < q) {
         env_idx = idx;
         xy /= q;
     } else {
-        env_idx = s.env_alias[idx];
+        env_idx = s.env_alias[e.env_base + idx];
         xy = (xy - q) / (1.0f - q);
     }
     const uint32_t py = env_idx / (uint32_t)e.env_w, px = env_idx % (uint32_t)e.env_w;
-    ds.pdf = s.env_pdf[env_idx];
+    ds.pdf = s.env_pdf[e.env_base + env_idx];
     const float u = ((float)px + xy) / (float)e.env_w;
     const float phi = (float)((double)u * 2.0 * (double)kPi - (double)kPi);
     const float step = kPi / (float)e.env_h;
@@ -683,8 +703,9 @@ __device__ __forceinline__ float mis_weight(float a, float b) {  // path.cpp:104
     return a / (a + b);
 }
 
-// Scene::pdfEmitterDirection (scene.cpp:172-179) for emitter `l`.
-__device__ __forceinline__ float pdf_emitter_direction(const SceneView& s, int l, const DirSample& ds) {
+// Scene::pdfEmitterDirection (scene.cpp:172-179) for emitter `l` of a scene
+// with nl emitters.
+__device__ __forceinline__ float pdf_emitter_direction(const SceneView& s, int l, int nl, const DirSample& ds) {
     const LightRec& e = s.lights[l];
     float p;
     if (e.type == DL_LIGHT_AREA) {
@@ -696,8 +717,8 @@ __device__ __forceinline__ float pdf_emitter_direction(const SceneView& s, int l
     } else {
         p = env_pdf(s, e, ds.d);
     }
-    if (s.num_lights == 1) return p;
-    return p / (float)s.num_lights;
+    if (nl == 1) return p;
+    return p / (float)nl;
 }
 
 }  // namespace dl

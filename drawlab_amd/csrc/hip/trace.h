@@ -149,8 +149,8 @@ __device__ __forceinline__ uint32_t claim_next(uint32_t* counter, bool want) {
 // triangle test costs one iteration of the lane that needs it, one
 // dependent fetch each.
 //
-// IO provides: load(i, o, d, tmin, tmax) -> bool (false: skip the ray),
-// store_hit(i, prim, t, u, v, nodes, tris) for closest hits and
+// IO provides: load(i, o, d, tmin, tmax), root(i) (the node traversal
+// starts from), store_hit(i, prim, t, u, v, nodes, tris) for closest hits and
 // store_occ(i, occluded, nodes, tris) for any-hit queries.
 template <bool ANY, bool COUNT, class IO>
 __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t n, uint32_t* work, LaneStack& stk,
@@ -188,7 +188,7 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
                     best = 0xffffffffu;
                     bu = bv = 0.0f;
                     sp = 0;
-                    cur = 0;
+                    cur = io.root(slot);
                     lcnt = 0;
                     ltri = lj = 0;
                     last = false;

@@ -23,6 +23,7 @@ struct BatchIO {
         tmin = rays[6 * N + i];
         tmax = rays[7 * N + i];
     }
+    __device__ __forceinline__ int root(uint32_t) const { return 0; }
     __device__ __forceinline__ void store_hit(uint32_t i, uint32_t p, float t, float u, float v, uint32_t nn,
                                               uint32_t nt) const {
         prim[i] = p;

@@ -2,14 +2,17 @@
 //
 // Lanes are worker threads, each owning one backend context (its own HIP
 // stream, scene, BVH and render state); kernels of jobs on different lanes
-// overlap on the device's hardware queues.  Within a lane, jobs of the
-// lane's scene share wavefront passes (dl_hip_render_batch): a lane that
-// starts a job keeps taking further queued jobs of the same scene and
-// compatible settings into its running group at every pass boundary, while
-// the pass stays within the pass size, so small jobs travel together
-// through one set of queues and kernels (continuous batching).  Each job's
-// passes have the sizes its standalone render would use, so its image is
-// still bit-identical to a single dl_hip_render call.  A lane keeps every scene it has uploaded resident
+// overlap on the device's hardware queues.  Within a lane, jobs share
+// wavefront passes (dl_hip_render_batch): a lane that starts a job keeps
+// taking further queued jobs with compatible settings into its running group
+// at every pass boundary, while the pass stays within the pass size, so small
+// jobs travel together through one set of queues and kernels (continuous
+// batching).  A joining job's scene is the group's or any scene resident on
+// the lane: the paths of a pass over several scenes carry their scene's index
+// into one device layout holding the lane's resident scenes (unless
+// single_scene_passes asks for one scene per pass).  Each job's passes have
+// the sizes its standalone render would use, so its image is still
+// bit-identical to a single dl_hip_render call.  A lane keeps every scene it has uploaded resident
 // (dl_hip_scene_keep / dl_hip_scene_select: switching scenes costs no upload
 // and no BVH build), and prefers a queued job of one of its resident scenes
 // among the next `lanes` queue entries (bounded bypass, so no job waits
@@ -127,8 +130,11 @@ uint32_t pass_spp(const dl_render_params& p, uint64_t pixels) {
     return (uint32_t)std::min<uint64_t>(k, (uint64_t)p.spp);
 }
 
-// Device bytes the lane may reach while rendering `job` (see file comment).
-uint64_t reservation_for(const Lane& lane, const Job& job, const SceneEntry& s) {
+uint64_t union_reservation(const Service& S, const Lane& lane, uint64_t scene_id, const SceneEntry& s);
+
+// Device bytes the lane may reach while rendering `job` (see file comment;
+// a layout of several scenes that an earlier group left is still held).
+uint64_t reservation_for(const Service& S, const Lane& lane, const Job& job, const SceneEntry& s) {
     const uint64_t px = job_pixels(job);
     const uint64_t paths = px * pass_spp(job.params, px);
     uint64_t state = std::max(paths, lane.state_paths) * kStateBytesPerPath +
@@ -141,14 +147,32 @@ uint64_t reservation_for(const Lane& lane, const Job& job, const SceneEntry& s) 
         if (kv.first != job.scene) others += kv.second;
     const auto it = lane.kept.find(job.scene);
     const uint64_t scene = it != lane.kept.end() ? it->second : s.bytes_estimate();
-    return lane.fixed_bytes + others + scene + state;
+    return lane.fixed_bytes + others + scene + state + union_reservation(S, lane, job.scene, s);
+}
+
+// Device bytes of the layout that holds every resident scene of the lane in
+// one (built by the first pass over several scenes, kept while they stay):
+// about the bytes those scenes hold.  Counted once the lane holds two scenes,
+// as a pass over several of them may build it at any time.
+uint64_t union_reservation(const Service& S, const Lane& lane, uint64_t scene_id, const SceneEntry& s) {
+    if (S.cfg.single_scene_passes) return 0;
+    uint64_t sum = 0, n = 0;
+    for (const auto& kv : lane.kept) {
+        sum += kv.second;
+        n++;
+    }
+    if (!lane.kept.count(scene_id)) {
+        sum += s.bytes_estimate();
+        n++;
+    }
+    return n >= 2 ? sum + kSlackBytes : 0;
 }
 
 // Device bytes the lane may reach while running a group of shared passes of
-// at most `pass_paths` paths on scene `scene_id`, whose film slots need
-// slot_px[s] pixels (see file comment).
-uint64_t group_reservation(const Lane& lane, uint64_t scene_id, const SceneEntry& s, uint64_t pass_paths,
-                           const uint64_t* slot_px) {
+// at most `pass_paths` paths led by a job of scene `scene_id`, whose film
+// slots need slot_px[s] pixels (see file comment).
+uint64_t group_reservation(const Service& S, const Lane& lane, uint64_t scene_id, const SceneEntry& s,
+                           uint64_t pass_paths, const uint64_t* slot_px) {
     uint64_t films = 0;
     for (int k = 0; k < DL_BATCH_MAX; k++) films += std::max(lane.slot_pixels[k], slot_px[k]);
     const uint64_t state = std::max(pass_paths, lane.state_paths) * kStateBytesPerPath +
@@ -158,14 +182,16 @@ uint64_t group_reservation(const Lane& lane, uint64_t scene_id, const SceneEntry
         if (kv.first != scene_id) others += kv.second;
     const auto it = lane.kept.find(scene_id);
     const uint64_t scene = it != lane.kept.end() ? it->second : s.bytes_estimate();
-    return lane.fixed_bytes + others + scene + state;
+    return lane.fixed_bytes + others + scene + state + union_reservation(S, lane, scene_id, s);
 }
 
-// Jobs that can share a pass: same scene and the settings a pass shares.
-bool can_share(const Job& a, const Job& b) {
-    return a.scene == b.scene && a.params.max_depth == b.params.max_depth &&
-           a.params.rr_depth == b.params.rr_depth && !(a.params.flags & DL_RENDER_GUIDED) &&
-           !(b.params.flags & DL_RENDER_GUIDED) &&
+// Jobs that can share a pass: the settings a pass shares, and either the
+// same scene or (passes over several scenes) b's scene resident on the lane
+// that runs a's group.
+bool can_share(const Service& S, const Lane& lane, const Job& a, const Job& b) {
+    const bool scene_ok = a.scene == b.scene || (!S.cfg.single_scene_passes && lane.kept.count(b.scene) > 0);
+    return scene_ok && a.params.max_depth == b.params.max_depth && a.params.rr_depth == b.params.rr_depth &&
+           !(a.params.flags & DL_RENDER_GUIDED) && !(b.params.flags & DL_RENDER_GUIDED) &&
            (a.params.flags & DL_RENDER_NORMALS) == (b.params.flags & DL_RENDER_NORMALS);
 }
 
@@ -319,7 +345,7 @@ void run_group(Service& S, Lane& lane, Job* lead, const SceneEntry& scene, std::
             Job& cand = *S.jobs[*qi];
             const uint64_t px = job_pixels(cand);
             const uint32_t k = pass_spp(cand.params, px);
-            if (!can_share(*lead, cand) || pass_paths + px * k > kPathsPerPass) {
+            if (!can_share(S, lane, *lead, cand) || pass_paths + px * k > kPathsPerPass) {
                 ++qi;
                 continue;
             }
@@ -329,7 +355,7 @@ void run_group(Service& S, Lane& lane, Job* lead, const SceneEntry& scene, std::
             std::copy(slot_px, slot_px + DL_BATCH_MAX, try_px);
             try_px[slot] = px;
             const uint64_t need =
-                std::max(lane.reservation, group_reservation(lane, lead->scene, scene, pass_paths + px * k, try_px));
+                std::max(lane.reservation, group_reservation(S, lane, lead->scene, scene, pass_paths + px * k, try_px));
             if (S.reserved_total() - lane.reservation + need > S.cfg.budget_bytes) {
                 ++qi;
                 continue;
@@ -353,6 +379,11 @@ void run_group(Service& S, Lane& lane, Job* lead, const SceneEntry& scene, std::
             qi = S.queue.erase(qi);
         }
         if (group.size() > 1) S.stats.shared_passes++;
+        for (const Member& g : group)
+            if (g.job->scene != lead->scene) {
+                S.stats.cross_scene_passes++;
+                break;
+            }
         // every member runs under the lane's current reservation
         for (const Member& g : group)
             g.job->info.reserved_bytes = std::max<uint64_t>(g.job->info.reserved_bytes, lane.reservation);
@@ -371,6 +402,7 @@ void run_group(Service& S, Lane& lane, Job* lead, const SceneEntry& scene, std::
             it.film = g.slot;
             it.accumulate = g.done > 0 ? 1 : 0;
             it.rgb_out = g.done + kk >= (uint32_t)j.params.spp ? g.job->rgb.data() : nullptr;
+            it.scene = j.scene;  // every scene the lane uploads is kept under its service id
         }
         dl_render_params shared = lead->params;
         dl_render_stats st{};
@@ -465,8 +497,9 @@ void lane_main(Service* Sp, Lane* lp) {
         slot_px[0] = px;
         // the lane starts from what it holds (its previous scene stays until
         // the upload replaces it), then may reach the estimate
-        const uint64_t est = alone ? reservation_for(lane, job, scene)
-                                   : group_reservation(lane, job.scene, scene, px * pass_spp(job.params, px), slot_px);
+        const uint64_t est = alone ? reservation_for(S, lane, job, scene)
+                                   : group_reservation(S, lane, job.scene, scene, px * pass_spp(job.params, px),
+                                                       slot_px);
         const uint64_t need = std::max(lane.reservation, est);
         const uint64_t others = S.reserved_total() - lane.reservation;
         if (others + need > S.cfg.budget_bytes) {

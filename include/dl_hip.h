@@ -300,11 +300,11 @@ dl_status dl_hip_memory_usage(void* ctx, uint64_t* current_bytes, uint64_t* peak
 dl_status dl_hip_set_camera(void* ctx, const dl_camera_desc* camera);
 
 /* ---- shared passes -------------------------------------------------------
-   One image of a pass that several images of the current scene share
-   (dl_hip_render_batch): its camera, the samples it adds in this pass and
-   the film slot that accumulates them.  Each image's samples and film are
-   exactly those of a dl_hip_render of it with the same pass sizes, so its
-   result is the same bit for bit. */
+   One image of a pass that several images share (dl_hip_render_batch): its
+   scene, its camera, the samples it adds in this pass and the film slot
+   that accumulates them.  Each image's samples and film are exactly those
+   of a dl_hip_render of it (its scene current) with the same pass sizes, so
+   its result is the same bit for bit. */
 enum { DL_BATCH_MAX = 32 };
 typedef struct dl_batch_item {
     dl_camera_desc camera;     /* the image's camera and size */
@@ -314,17 +314,24 @@ typedef struct dl_batch_item {
     int32_t film;              /* film slot, 0 .. DL_BATCH_MAX - 1 (distinct within a pass) */
     int32_t accumulate;        /* nonzero: add to the slot's film (same image size); zero: start it empty */
     float* rgb_out;            /* host buffer of width * height * 3: the resolved image after this pass, or NULL */
+    uint64_t scene;            /* key of a resident scene (dl_hip_scene_keep), or 0: the current scene */
 } dl_batch_item;
 /* Renders one wavefront pass holding the paths of n images (1..DL_BATCH_MAX)
-   of the current scene: the paths of every image share the pass's queues
-   and kernels (camera rays, traversal, material-sorted shading, shadow
-   rays), then each image's film gathers its own samples.  `shared` gives
-   max_depth, rr_depth and flags (DL_RENDER_COUNT_TRAVERSAL and
-   DL_RENDER_NORMALS; DL_RENDER_GUIDED is rejected); its other fields are
-   ignored.  At most 2^27 paths (sum of width * height * spp).  The film
-   slots belong to the context and persist across calls and scene changes
-   until dl_hip_trim.  (No reference counterpart: the render service packs
-   jobs into shared passes with it.) */
+   of the current scene or of resident scenes: the paths of every image
+   share the pass's queues and kernels (camera rays, traversal, material-
+   sorted shading, shadow rays), then each image's film gathers its own
+   samples with its scene's filter.  Images of different scenes are rendered
+   from one device layout holding every resident scene (built on first use,
+   about as many device bytes as those scenes' geometry, BVHs, lights and
+   textures, and kept until a resident scene changes or dl_hip_trim), each
+   path carrying its scene's index.  The scenes of one pass must agree in
+   hide_lights and epsilon (dl_view_options).  `shared` gives max_depth,
+   rr_depth and flags (DL_RENDER_COUNT_TRAVERSAL and DL_RENDER_NORMALS;
+   DL_RENDER_GUIDED is rejected); its other fields are ignored.  At most
+   2^27 paths (sum of width * height * spp).  The film slots belong to the
+   context and persist across calls and scene changes until dl_hip_trim.
+   The current scene stays current.  (No reference counterpart: the render
+   service packs jobs into shared passes with it.) */
 dl_status dl_hip_render_batch(void* ctx, const dl_render_params* shared, const dl_batch_item* items, uint32_t n,
                               dl_render_stats* stats);
 
@@ -421,6 +428,9 @@ typedef struct dl_service_config {
     int32_t builder;           /* DL_BUILD_* for scene uploads */
     uint64_t budget_bytes;     /* device memory all lanes together may reserve */
     int32_t max_batch;         /* jobs a lane packs into shared passes: 0 = DL_BATCH_MAX, 1 = one at a time */
+    int32_t single_scene_passes; /* nonzero: only jobs of one scene share a pass; zero: jobs of every scene
+                                  resident on the lane do (one layout holding those scenes, reserved
+                                  as about their bytes again) */
 } dl_service_config;
 
 typedef struct dl_job_info {
@@ -443,6 +453,7 @@ typedef struct dl_service_stats {
     double paths;
     uint64_t batched_jobs;         /* jobs that joined a running group of shared passes */
     uint64_t shared_passes;        /* passes that held more than one job */
+    uint64_t cross_scene_passes;   /* passes that held jobs of more than one scene */
 } dl_service_stats;
 
 dl_status dl_hip_service_start(const dl_service_config* cfg, void** svc);

@@ -103,3 +103,29 @@ def test_host_library_exports_every_declared_symbol():
     L = host.host_lib()
     for name in decl:
         assert hasattr(L, name), name
+
+
+def test_ctypes_structures_match_the_c_layout(tmp_path):
+    """Every ctypes mirror of an include/dl_hip.h structure has the C
+    structure's size and each field's offset: a C program built from the
+    header (its fields named after the mirror's) prints them."""
+    import subprocess
+    structs = [getattr(drawlab_amd, n) for n in dir(drawlab_amd)
+               if n.startswith("dl_") and isinstance(getattr(drawlab_amd, n), type)
+               and issubclass(getattr(drawlab_amd, n), ctypes.Structure)]
+    assert len(structs) >= 15
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "dl_hip.h"', 'int main(void) {']
+    for S in structs:
+        lines.append('printf("%%s %%zu\\n", "%s", sizeof(%s));' % (S.__name__, S.__name__))
+        for f in S._fields_:
+            lines.append('printf("%%s %%zu\\n", "%s.%s", offsetof(%s, %s));' % (S.__name__, f[0], S.__name__, f[0]))
+    lines += ["return 0;", "}"]
+    src, exe = tmp_path / "layout.c", tmp_path / "layout"
+    src.write_text("\n".join(lines) + "\n")
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                  text=True).stdout.splitlines())
+    for S in structs:
+        assert int(out[S.__name__]) == ctypes.sizeof(S), S.__name__
+        for f in S._fields_:
+            assert int(out["%s.%s" % (S.__name__, f[0])]) == getattr(S, f[0]).offset, (S.__name__, f[0])

@@ -149,3 +149,142 @@ def test_pass_split_does_not_change_the_image(backend):
     for k in (4, 1, 0):
         img, _ = backend.render(spp=9, seed=4, spp_per_pass=k)
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), k
+
+
+def _resident(backend, descs):
+    """Upload and keep each scene under keys 1, 2, ...; the last stays current."""
+    for k, d in enumerate(descs, 1):
+        backend.upload_scene(d)
+        backend.build_accel(drawlab_amd.BUILD_GPU)
+        backend.scene_keep(k)
+
+
+def _reference(backend, key, cam, spp, seed, max_depth, rr_depth, k=None):
+    assert backend.scene_select(key)
+    backend.set_camera(cam)
+    img, st = backend.render(spp=spp, seed=seed, spp_per_pass=spp if k is None else k, max_depth=max_depth,
+                             rr_depth=rr_depth)
+    return img
+
+
+def _cross_scene_set():
+    return [scenes.cornell_box(48, 40, spp=4, max_depth=8, rr_depth=3),
+            scenes.material_zoo(40, 30),
+            scenes.textured_box(32, 32),
+            scenes.displaced_sphere(48, 27, spp=4, max_depth=8, grid=48, glass_level=2, env_size=(64, 32)),
+            scenes.mis_plates(48, 27, max_depth=6)]
+
+
+def test_pass_over_several_scenes_equals_each_scenes_own_render(backend):
+    """Images of five resident scenes -- area, point, several and
+    environment lights, textures and normal maps, every BSDF type, two
+    reconstruction filters -- share one pass (their scenes in one layout,
+    each path carrying its scene's record), and each image equals the
+    render of its scene alone bit for bit.  Two items of one scene and an
+    item naming the current scene by key 0 are in the same pass."""
+    descs = _cross_scene_set()
+    _resident(backend, descs)
+    md, rr = 7, 3
+    keys = [1, 2, 3, 4, 0, 1]
+    cams = [descs[0].camera, descs[1].camera, descs[2].camera, descs[3].camera, descs[4].camera,
+            dataclasses.replace(descs[0].camera, width=24, height=20, eye=(0.3, 0.3, -0.7))]
+    spps, seeds = [3, 2, 4, 2, 3, 5], [1, 2, 3, 4, 5, 6]
+    items = [{"camera": c, "spp": s, "seed": e, "film": i, "scene": k}
+             for i, (c, s, e, k) in enumerate(zip(cams, spps, seeds, keys))]
+    imgs, st = backend.render_batch(items, max_depth=md, rr_depth=rr)
+    print("cross-scene batch", st["paths"], st["ms_total"], flush=True)
+    assert st["paths"] == sum(c.width * c.height * s for c, s in zip(cams, spps))
+    for key, c, s, e, img in zip(keys, cams, spps, seeds, imgs):
+        ref = _reference(backend, key or 5, c, s, e, md, rr)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), (key, c.width, s, e)
+        assert img.mean() > 0.005, key
+
+
+def test_cross_scene_layout_follows_resident_scenes(backend):
+    """The layout holding several scenes is reused while its scenes stay,
+    and rebuilt when one of them changes: a scene replaced under its key
+    renders as the new scene, a dropped scene is refused, and a pass with
+    counted traversal, normals or hidden lights over several scenes still
+    equals the standalone renders."""
+    descs = _cross_scene_set()[:3]
+    _resident(backend, descs)
+    md, rr = 6, 3
+    item = lambda k, i, s=2, e=0: {"camera": descs[k - 1].camera, "spp": s, "seed": e, "film": i, "scene": k}
+    imgs, _ = backend.render_batch([item(1, 0), item(2, 1)], max_depth=md, rr_depth=rr)
+    imgs2, st = backend.render_batch([item(1, 0), item(2, 1)], max_depth=md, rr_depth=rr, count_traversal=True)
+    assert all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(imgs, imgs2))
+    assert st["nodes_closest"] > 0 and st["nodes_shadow"] > 0
+    # replace scene 2 by the MIS plates under the same key
+    plates = scenes.mis_plates(40, 24, max_depth=6)
+    backend.upload_scene(plates)
+    backend.build_accel(drawlab_amd.BUILD_GPU)
+    backend.scene_keep(2)
+    descs[1] = plates
+    imgs, _ = backend.render_batch([item(1, 0, 3, 7), item(2, 1, 3, 8), item(3, 2, 2, 9)], max_depth=md, rr_depth=rr)
+    for k, img, s, e in ((1, imgs[0], 3, 7), (2, imgs[1], 3, 8), (3, imgs[2], 2, 9)):
+        ref = _reference(backend, k, descs[k - 1].camera, s, e, md, rr)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), k
+    imgs, _ = backend.render_batch([item(1, 0), item(3, 1)], max_depth=md, rr_depth=rr, normals=True)
+    for k, img in ((1, imgs[0]), (3, imgs[1])):
+        assert backend.scene_select(k)
+        backend.set_camera(descs[k - 1].camera)
+        ref, _ = backend.render(spp=2, seed=0, spp_per_pass=2, max_depth=md, rr_depth=rr, normals=True)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), k
+    for k in (1, 2, 3):
+        assert backend.scene_select(k)
+        backend.set_view_options(hide_lights=True)
+    imgs, _ = backend.render_batch([item(1, 0, 2, 4), item(2, 1, 2, 5)], max_depth=md, rr_depth=rr)
+    for k, img, e in ((1, imgs[0], 4), (2, imgs[1], 5)):
+        ref = _reference(backend, k, descs[k - 1].camera, 2, e, md, rr)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), k
+    assert backend.scene_select(3)
+    backend.reset_view_options()
+    with pytest.raises(drawlab_amd.DlError, match="differ in hide_lights or epsilon"):
+        backend.render_batch([item(1, 0), item(3, 1)], max_depth=md, rr_depth=rr)
+    backend.scene_drop(1)
+    with pytest.raises(drawlab_amd.DlError, match="no resident scene with this key"):
+        backend.render_batch([item(1, 0), item(3, 1)], max_depth=md, rr_depth=rr)
+    backend.scene_drop(0)
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_service_shares_passes_across_resident_scenes(backend, single):
+    """Once a lane holds three scenes, queued jobs of all three join one
+    group and share passes (cross_scene_passes), unless
+    single_scene_passes; every image stays bit-identical to its
+    standalone render and no lane exceeds its reservation."""
+    descs = [scenes.cornell_box(40, 32, spp=4, max_depth=6, rr_depth=3), scenes.mis_plates(40, 24, max_depth=6),
+             scenes.material_zoo(32, 24, max_depth=6)]
+    svc = drawlab_amd.Service(lanes=1, policy=drawlab_amd.POLICY_FIFO, single_scene_passes=single)
+    jobs, ids = [], []
+    try:
+        sids = [svc.add_scene(d) for d in descs]
+        # warm-up: one job per scene makes all three resident on the lane
+        for i, (d, s) in enumerate(zip(descs, sids)):
+            jobs.append((i, 2, 100 + i))
+            ids.append(svc.submit(s, 2, seed=100 + i, max_depth=6, rr_depth=3))
+        for j in ids:
+            svc.wait(j)
+        for r in range(3):
+            for i, s in enumerate(sids):
+                spp, seed = 3 + (i + r) % 3, 10 * r + i
+                jobs.append((i, spp, seed))
+                ids.append(svc.submit(s, spp, seed=seed, max_depth=6, rr_depth=3, spp_per_pass=2))
+        infos = [svc.wait(j) for j in ids]
+        assert all(i["state"] == drawlab_amd.JOB_DONE for i in infos), infos
+        imgs = [svc.result(j) for j in ids]
+        st = svc.stats()
+        print("stats", single, st, flush=True)
+        if single:
+            assert st["cross_scene_passes"] == 0
+        else:
+            assert st["cross_scene_passes"] >= 1 and st["shared_passes"] >= 1
+        assert st["scene_uploads"] == 3
+        assert st["estimate_misses"] == 0 and st["peak_reserved_bytes"] <= st["budget_bytes"]
+    finally:
+        svc.stop()
+    for (i, spp, seed), img in zip(jobs, imgs):
+        backend.upload_scene(descs[i])
+        backend.build_accel(drawlab_amd.BUILD_GPU)
+        ref, _ = backend.render(spp=spp, seed=seed, spp_per_pass=2 if seed < 100 else 0, max_depth=6, rr_depth=3)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), (i, spp, seed)
