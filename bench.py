@@ -374,7 +374,8 @@ def service_bench(args):
 
     def run(lanes, pol, arrivals):
         # one lane renders one job at a time (no shared passes)
-        svc = drawlab_amd.Service(lanes=lanes, policy=pol, max_batch=1 if lanes == 1 else 0)
+        svc = drawlab_amd.Service(lanes=lanes, policy=pol, max_batch=1 if lanes == 1 else 0,
+                                  single_scene_passes=args.single_scene_passes)
         try:
             sids = [svc.add_scene(d) for d in pool]
             lat, paths, span = _run_jobs(svc, sids, pool, jobs, arrivals)
@@ -481,7 +482,11 @@ def service_bench(args):
                           "estimate_misses": int(stp["estimate_misses"] + stn["estimate_misses"]),
                           "scene_uploads": int(stp["scene_uploads"]),
                           "batched_jobs": int(stp["batched_jobs"]), "shared_passes": int(stp["shared_passes"]),
-                          "batched_jobs_saturated": int(stn["batched_jobs"])},
+                          "cross_scene_passes": int(stp["cross_scene_passes"]),
+                          "batched_jobs_saturated": int(stn["batched_jobs"]),
+                          "shared_passes_saturated": int(stn["shared_passes"]),
+                          "cross_scene_passes_saturated": int(stn["cross_scene_passes"]),
+                          "single_scene_passes": bool(args.single_scene_passes)},
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -816,6 +821,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=64, help="service workload: number of jobs")
     ap.add_argument("--lanes", type=int, default=4, help="service workload: worker lanes")
     ap.add_argument("--policy", choices=("fifo", "sjf"), default="fifo")
+    ap.add_argument("--single-scene-passes", action="store_true",
+                    help="service workload: only jobs of one scene share a pass")
     ap.add_argument("--rate-sweep", type=float, nargs="*", default=[0.6, 0.8, 1.0, 1.2, 1.4],
                     help="service workload: arrival rates (x one-at-a-time capacity) of the p99 sweep")
     args = ap.parse_args()

@@ -32,7 +32,7 @@ EXPORTED_SYMBOLS = (
     "dl_hip_memcpy_d2h", "dl_hip_memcpy_d2d", "dl_hip_sync", "dl_hip_last_kernel_ms", "dl_camera_virtual", "dl_camera_perspective",
     "dl_camera_opencv", "dl_film_filter", "dl_hip_set_view_options", "dl_hip_render_batch",
     "dl_mesh_tangents", "dl_hip_memory_usage", "dl_hip_trim", "dl_hip_set_camera",
-    "dl_hip_scene_keep", "dl_hip_scene_select", "dl_hip_scene_drop", "dl_hip_accel_export",
+    "dl_hip_scene_keep", "dl_hip_scene_select", "dl_hip_scene_drop", "dl_hip_scene_bytes", "dl_hip_accel_export",
     "dl_hip_service_start", "dl_hip_service_add_scene", "dl_hip_service_submit", "dl_hip_service_poll",
     "dl_hip_service_wait", "dl_hip_service_result", "dl_hip_service_cancel", "dl_hip_service_stats",
     "dl_hip_service_stop", "dl_hip_service_last_error",
@@ -172,6 +172,7 @@ def lib():
             "dl_hip_scene_keep": (i32, [P, c.c_uint64]),
             "dl_hip_scene_select": (i32, [P, c.c_uint64]),
             "dl_hip_scene_drop": (i32, [P, c.c_uint64]),
+            "dl_hip_scene_bytes": (i32, [P, c.c_uint64, c.POINTER(c.c_uint64)]),
             "dl_hip_accel_export": (i32, [P, P, P]),
         }
         for name, (res, args) in sigs.items():
@@ -433,6 +434,12 @@ class Backend:
 
     def scene_drop(self, key):
         self._check(self.L.dl_hip_scene_drop(self.h, int(key)))
+
+    def scene_bytes(self, key=0):
+        """Device bytes held by the scene kept under `key` (0: the current one)."""
+        b = ctypes.c_uint64()
+        self._check(self.L.dl_hip_scene_bytes(self.h, int(key), ctypes.byref(b)))
+        return b.value
 
     def last_kernel_ms(self):
         ms = ctypes.c_double()

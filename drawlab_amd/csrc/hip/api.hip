@@ -675,6 +675,23 @@ dl_status dl_hip_scene_drop(void* h, uint64_t key) {
     if (key != 0 && c->scene_key == key) free_scene(c);
     return DL_OK;
 }
+dl_status dl_hip_scene_bytes(void* h, uint64_t key, uint64_t* bytes) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !bytes) return DL_ERR_INVALID;
+    const bool current = c->has_scene && (key == 0 || key == c->scene_key);
+    const auto it = c->stash.find(key);
+    if (!current && (key == 0 || it == c->stash.end())) return set_error(c, DL_ERR_STATE, "no such scene");
+    const std::vector<void*>& allocs = current ? c->scene_allocs : it->second.scene_allocs;
+    const DevScene& ds = current ? c->ds : it->second.ds;
+    auto size_of = [&](const void* p) -> uint64_t {
+        const auto a = c->dev_allocs.find(const_cast<void*>(p));
+        return a == c->dev_allocs.end() ? 0 : a->second;
+    };
+    uint64_t sum = size_of(ds.nodes) + size_of(ds.tris);
+    for (void* p : allocs) sum += size_of(p);
+    *bytes = sum;
+    return DL_OK;
+}
 dl_status dl_hip_sync(void* h) {
     Ctx* c = (Ctx*)h;
     if (!c) return DL_ERR_INVALID;

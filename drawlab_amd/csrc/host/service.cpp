@@ -57,7 +57,7 @@ using Clock = std::chrono::steady_clock;
 constexpr uint64_t kPathsPerPass = 64u << 20;  // paths per pass of a job or a shared group (half the largest pass)
 constexpr uint64_t kStateBytesPerPath = dl_sizes::kStateBytesPerPath;  // path state + queues
 constexpr uint64_t kFilmBytesPerPixel = dl_sizes::kFilmBytesPerPixel;  // accumulation film + resolved RGB
-constexpr uint64_t kBytesPerVertex = 36;        // positions, normals, tangents
+constexpr uint64_t kBytesPerVertex = 44;        // positions, normals, tangents, texture coordinates
 constexpr uint64_t kBytesPerPrim = 16 + 112 + 256;  // indices + mesh map, BVH nodes + triangles, build scratch
 constexpr uint64_t kBytesPerTexel = 32;         // environment texels + alias tables
 constexpr uint64_t kSlackBytes = 4u << 20;
@@ -92,8 +92,8 @@ struct Lane {
     void* ctx = nullptr;
     uint64_t fixed_bytes = 0;     // context buffers that exist without a scene
     uint64_t resident_scene = 0;  // the context's current scene (0: none)
-    std::map<uint64_t, uint64_t> kept;  // resident scenes -> device bytes they hold (measured at upload)
-    uint64_t state_paths = 0, state_pixels = 0;  // render state the context holds
+    std::map<uint64_t, uint64_t> kept;  // resident scenes -> device bytes they hold (dl_hip_scene_bytes)
+    uint64_t state_paths = 0, state_pixels = 0;  // render state the context holds (paths, film pixels)
     uint64_t slot_pixels[DL_BATCH_MAX] = {};     // film slots the context holds (pixels each)
     uint64_t reservation = 0;
     std::thread th;
@@ -239,15 +239,11 @@ std::string prepare_scene(Service& S, Lane& lane, const Job& job, const SceneEnt
     auto fail = [&]() { return std::string(dl_hip_last_error(c)); };
     if (switch_ == 1 && dl_hip_scene_select(c, job.scene) != DL_OK) return fail();
     if (switch_ == 2) {
-        uint64_t before = 0, after = 0;
-        // the current scene stays resident (it was kept), so the new one adds
-        // its own bytes only
-        dl_hip_memory_usage(c, &before, nullptr);
+        // the current scene stays resident (it was kept)
         if (dl_hip_upload_scene(c, &scene.desc) != DL_OK) return fail();
         if (dl_hip_build_accel(c, S.cfg.builder, nullptr) != DL_OK) return fail();
         if (dl_hip_scene_keep(c, job.scene) != DL_OK) return fail();
-        dl_hip_memory_usage(c, &after, nullptr);
-        *kept_bytes = after > before ? after - before : 0;
+        if (dl_hip_scene_bytes(c, job.scene, kept_bytes) != DL_OK) return fail();
     }
     (void)S;
     return "";
@@ -418,6 +414,9 @@ void run_group(Service& S, Lane& lane, Job* lead, const SceneEntry& scene, std::
             ok = false;
             continue;  // the group fails at the top of the loop
         }
+        // the render state keeps the capacity of the group's largest pass:
+        // later reservations of the group start from it
+        lane.state_paths = std::max<uint64_t>(lane.state_paths, (uint64_t)st.paths);
         for (size_t i = 0; i < group.size(); i++) {
             Member& g = group[i];
             const double p = (double)job_pixels(*g.job) * items[i].spp;

@@ -377,10 +377,14 @@ dl_status dl_hip_accel_export(void* ctx, void* nodes, void* tris);
    lights, environment, camera and film as they were -- without upload or
    build (DL_ERR_STATE when no scene is kept under that key).
    dl_hip_scene_drop frees the scene kept under `key` (0: every kept scene
-   that is not current; the current scene too when it is the one named). */
+   that is not current; the current scene too when it is the one named).
+   dl_hip_scene_bytes gives the device bytes the scene kept under `key` (0:
+   the current scene) holds: its geometry, materials, lights, environment,
+   textures and BVH (DL_ERR_STATE when there is no such scene). */
 dl_status dl_hip_scene_keep(void* ctx, uint64_t key);
 dl_status dl_hip_scene_select(void* ctx, uint64_t key);
 dl_status dl_hip_scene_drop(void* ctx, uint64_t key);
+dl_status dl_hip_scene_bytes(void* ctx, uint64_t key, uint64_t* bytes);
 
 /* ---- learned directional proposal ----------------------------------------
  * A mixture of 8 von Mises-Fisher lobes per shading point predicted by a
