@@ -323,7 +323,13 @@ def _run_jobs(svc, sids, pool, jobs, arrivals):
         cam = dataclasses.replace(desc.camera, width=w, height=h)
         ids.append(svc.submit(sids[si], spp, seed=len(ids), max_depth=desc.max_depth, rr_depth=desc.rr_depth,
                               camera=cam))
-    infos = [svc.wait(j) for j in ids]
+        if len(ids) % 64 == 0:
+            print("  %d jobs submitted at %.1f s" % (len(ids), time.perf_counter() - t0), file=sys.stderr, flush=True)
+    infos = []
+    for j in ids:
+        infos.append(svc.wait(j))
+        if len(infos) % 64 == 0:
+            print("  %d jobs done at %.1f s" % (len(infos), time.perf_counter() - t0), file=sys.stderr, flush=True)
     for j in ids:
         svc.result(j)
     for (si, w, h, spp), i in zip(jobs, infos):
@@ -431,17 +437,20 @@ def service_bench(args):
     r1 = rate_at(curve1, target)
     rn = rate_at(curven, target)
     # the reference's flow: every job loads its scene (upload + BVH build)
-    lats, spans = _run_standalone(pool, jobs, zero)
-    print("standalone saturated: %.2f s, %.2f jobs/s" % (spans, len(jobs) / spans), file=sys.stderr, flush=True)
-    caps = len(jobs) / spans
     curves = []
-    for f in (0.6, 0.8, 0.9):
-        ls, _ = _run_standalone(pool, jobs, unit / (f * caps))
-        curves.append((f * caps, float(np.percentile(ls, 99))))
-        print("standalone at %.2f jobs/s: p99 %.3f s" % (f * caps, curves[-1][1]), file=sys.stderr, flush=True)
-    target_s = curves[-1][1]
-    rs_ = rate_at(curves, target_s)
-    rn_s = rate_at(curven, target_s)
+    if args.skip_standalone:
+        lats, spans = None, None
+    else:
+        lats, spans = _run_standalone(pool, jobs, zero)
+        print("standalone saturated: %.2f s, %.2f jobs/s" % (spans, len(jobs) / spans), file=sys.stderr, flush=True)
+        caps = len(jobs) / spans
+        for f in (0.6, 0.8, 0.9):
+            ls, _ = _run_standalone(pool, jobs, unit / (f * caps))
+            curves.append((f * caps, float(np.percentile(ls, 99))))
+            print("standalone at %.2f jobs/s: p99 %.3f s" % (f * caps, curves[-1][1]), file=sys.stderr, flush=True)
+    target_s = curves[-1][1] if curves else None
+    rs_ = rate_at(curves, target_s) if curves else None
+    rn_s = rate_at(curven, target_s) if curves else None
 
     def q(a):
         return {"p50": round(float(np.percentile(a, 50)), 4), "p99": round(float(np.percentile(a, 99)), 4),
@@ -471,8 +480,9 @@ def service_bench(args):
                       "note": "service rate is a lower bound when it stays within the p99 at every swept rate"},
         "saturated_jobs_per_s": {"service": round(len(jobs) / spann, 4),
                                  "one_at_a_time": round(len(jobs) / span1, 4),
-                                 "standalone": round(len(jobs) / spans, 4)},
-        "standalone_equal_p99": {"p99_s": round(target_s, 3), "rate_standalone": rs_, "rate_service": rn_s,
+                                 "standalone": round(len(jobs) / spans, 4) if spans else None},
+        "standalone_equal_p99": {"p99_s": round(target_s, 3) if target_s else None, "rate_standalone": rs_,
+                                 "rate_service": rn_s,
                                  "ratio": (rn_s / rs_) if (rn_s and rs_) else None,
                                  "p99_vs_rate": [[round(r, 3), round(l, 3)] for r, l in curves],
                                  "note": "standalone = per job: scene upload + BVH build + render, one context"},
@@ -823,6 +833,8 @@ def main():
     ap.add_argument("--policy", choices=("fifo", "sjf"), default="fifo")
     ap.add_argument("--single-scene-passes", action="store_true",
                     help="service workload: only jobs of one scene share a pass")
+    ap.add_argument("--skip-standalone", action="store_true",
+                    help="service workload: no standalone (upload + build + render per job) baseline")
     ap.add_argument("--rate-sweep", type=float, nargs="*", default=[0.6, 0.8, 1.0, 1.2, 1.4],
                     help="service workload: arrival rates (x one-at-a-time capacity) of the p99 sweep")
     args = ap.parse_args()
