@@ -210,6 +210,10 @@ def test_cross_scene_layout_follows_resident_scenes(backend):
     _resident(backend, descs)
     md, rr = 6, 3
     item = lambda k, i, s=2, e=0: {"camera": descs[k - 1].camera, "spp": s, "seed": e, "film": i, "scene": k}
+    # one resident scene that is not current: its own layout, no union
+    one, _ = backend.render_batch([item(1, 0, 2, 3)], max_depth=md, rr_depth=rr)
+    ref = _reference(backend, 1, descs[0].camera, 2, 3, md, rr)
+    assert np.array_equal(one[0].view(np.uint32), ref.view(np.uint32))
     imgs, _ = backend.render_batch([item(1, 0), item(2, 1)], max_depth=md, rr_depth=rr)
     imgs2, st = backend.render_batch([item(1, 0), item(2, 1)], max_depth=md, rr_depth=rr, count_traversal=True)
     assert all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(imgs, imgs2))
