@@ -843,7 +843,7 @@ def main():
         return
     if args.summarize_counters:
         summarize_counters(args.summarize_counters[:-1], args.summarize_counters[-1], args.command,
-                           ["k_trace_queue<false>", "k_shadow_queue", "k_post<false>", "k_shade<0, false>",
+                           ["k_trace_queue<false>", "k_shadow_queue<false>", "k_post<false>", "k_shade<0, false>",
                             "k_shade<5, false>", "k_film_tiled"])
         return
     if args.summarize_pmc:
@@ -979,8 +979,8 @@ def main():
         },
         "roofline_shadow": {
             "bound": "hbm",
-            "limiter": limiter_from_profiles("k_shadow_queue", None, sh_ms_per_launch),
-            "kernel": "k_shadow_queue (any hit)",
+            "limiter": limiter_from_profiles("k_shadow_queue<false>", None, sh_ms_per_launch),
+            "kernel": "k_shadow_queue<false> (any hit)",
             "achieved": round(sh_achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
