@@ -888,10 +888,12 @@ def main():
     be.sync()
 
     agg = {}
+    step_ms = []  # HIP-event time of each step (each render call synchronises)
     t0 = time.perf_counter()
     for k in range(args.steps):
         _, st = be.render(spp=S, spp_per_pass=S, first_sample=(args.warmup + k) * S, accumulate=True,
                           want_image=False)
+        step_ms.append(st["ms_total"])
         for key, v in st.items():
             agg[key] = agg.get(key, 0.0) + v
     be.sync()
@@ -935,6 +937,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(1e3 * seconds / args.steps, 3),
+        "step_ms": [round(x, 3) for x in step_ms],
+        "mpaths_per_s_median_step": round(W * H * S / (float(np.median(step_ms)) * 1e-3) / 1e6, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -151,11 +151,11 @@ def test_pass_split_does_not_change_the_image(backend):
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), k
 
 
-def _resident(backend, descs):
+def _resident(backend, descs, builders=None):
     """Upload and keep each scene under keys 1, 2, ...; the last stays current."""
     for k, d in enumerate(descs, 1):
         backend.upload_scene(d)
-        backend.build_accel(drawlab_amd.BUILD_GPU)
+        backend.build_accel(builders[k - 1] if builders else drawlab_amd.BUILD_GPU)
         backend.scene_keep(k)
 
 
@@ -183,7 +183,9 @@ def test_pass_over_several_scenes_equals_each_scenes_own_render(backend):
     render of its scene alone bit for bit.  Two items of one scene and an
     item naming the current scene by key 0 are in the same pass."""
     descs = _cross_scene_set()
-    _resident(backend, descs)
+    # BVHs of all three builders side by side in the shared layout
+    _resident(backend, descs, [drawlab_amd.BUILD_GPU, drawlab_amd.BUILD_HOST_SAH, drawlab_amd.BUILD_GPU_LBVH,
+                               drawlab_amd.BUILD_GPU, drawlab_amd.BUILD_HOST_SAH])
     md, rr = 7, 3
     keys = [1, 2, 3, 4, 0, 1]
     cams = [descs[0].camera, descs[1].camera, descs[2].camera, descs[3].camera, descs[4].camera,
