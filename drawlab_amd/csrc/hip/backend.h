@@ -146,6 +146,7 @@ struct SceneUnion {
     DevScene ds;
     SceneRec* recs = nullptr;  // device, one per member
     int32_t* roots = nullptr;  // device, one per member
+    std::vector<SceneRec> host_recs;  // source of the last upload of recs (outlives its asynchronous copy)
     std::vector<void*> allocs;
 };
 

@@ -1571,7 +1571,6 @@ dl_status render_batch(Ctx* c, const dl_render_params* prm, const dl_batch_item*
     SceneView sv;
     BvhView bvh;
     std::vector<uint32_t> item_rec(n, 0);  // scene record of each item's paths
-    std::vector<SceneRec> recs;
     if (!uni) {
         const SceneParts& s = parts[0];
         sv = layout_view(*s.ds, (int)s.lights->size(), s.env_light, s.bg);
@@ -1579,6 +1578,7 @@ dl_status render_batch(Ctx* c, const dl_render_params* prm, const dl_batch_item*
         rs->b.roots = nullptr;
     } else {
         sv = layout_view(uni->ds, 0, -1, parts[0].bg);
+        std::vector<SceneRec>& recs = uni->host_recs;
         recs.resize(uni->keys.size());
         for (size_t m = 0; m < uni->keys.size(); m++) {
             SceneParts mp;
