@@ -817,6 +817,9 @@ def main():
     ap.add_argument("--command", default="", help="command line recorded in a summary")
     ap.add_argument("--title", default="", help="title of a kernel-time summary")
     ap.add_argument("--copy-mib", type=int, default=1024, help="calibration copy size used by the PMC runs")
+    ap.add_argument("--scene", choices=("displaced_sphere", "cornell_box", "mis_plates"), default="displaced_sphere",
+                    help="plain workload: the scene (1920x1080 displaced sphere, 512x512 Cornell box, 1280x720 "
+                         "MIS plates, at their configured depths)")
     ap.add_argument("--workload", choices=("displaced_sphere", "service", "guided", "trace"),
                     default="displaced_sphere")
     ap.add_argument("--guided-res", default="480x270", help="guided workload: image size WxH")
@@ -866,7 +869,11 @@ def main():
     import drawlab_amd
     from drawlab_amd import scenes
 
-    desc = scenes.displaced_sphere()
+    # the plain integrator on one of the configured scenes: the 2M-triangle
+    # displaced sphere (the headline workload), the Cornell box or the MIS plates
+    desc = {"displaced_sphere": scenes.displaced_sphere, "cornell_box": scenes.cornell_box,
+            "mis_plates": scenes.mis_plates}[args.scene]()
+    workload = WORKLOAD if args.scene == "displaced_sphere" else args.scene
     W, H = desc.camera.width, desc.camera.height
     S = args.spp_per_step
     be = drawlab_amd.Backend(0)
@@ -905,7 +912,9 @@ def main():
     bytes_per_launch = agg["closest_rays"] * bytes_per_ray / launches
     ms_per_launch = agg["ms_closest"] / launches
     achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
-    traffic = traffic_from_profiles()
+    # counter profiles exist for the headline workload only
+    headline = args.scene == "displaced_sphere"
+    traffic = traffic_from_profiles() if headline else None
     sh_launches = max(1.0, agg["launches_shadow"])
     sh_bytes_per_launch = agg["shadow_rays"] * bytes_per_shadow / sh_launches
     sh_ms_per_launch = agg["ms_shadow"] / sh_launches
@@ -945,14 +954,16 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene generated in code)",
         "config": {
-            "workload": WORKLOAD,
+            "workload": workload,
             "triangles": int(desc.num_triangles),
             "width": W,
             "height": H,
             "spp": args.steps * S,
             "spp_per_step": S,
             "max_depth": desc.max_depth,
-            "env_map": "1024x512",
+            "rr_depth": desc.rr_depth,
+            "env_map": "%dx%d" % (desc.envmap.rgb.shape[1], desc.envmap.rgb.shape[0]) if desc.envmap is not None
+            else None,
             "bvh": "gpu-ploc",
             "bvh_build_ms": round(accel["build_ms"], 2),
         },
@@ -967,7 +978,7 @@ def main():
                             "frac": round(value / (HBM_PEAK_GBS * 1e3 / bpath["total"]), 4)},
         "roofline": {
             "bound": "hbm",
-            "limiter": limiter_from_profiles(TRACE_KERNEL, traffic, ms_per_launch),
+            "limiter": limiter_from_profiles(TRACE_KERNEL, traffic, ms_per_launch) if headline else None,
             "kernel": TRACE_KERNEL + " (closest hit)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
@@ -983,7 +994,7 @@ def main():
         },
         "roofline_shadow": {
             "bound": "hbm",
-            "limiter": limiter_from_profiles("k_shadow_queue<false>", None, sh_ms_per_launch),
+            "limiter": limiter_from_profiles("k_shadow_queue<false>", None, sh_ms_per_launch) if headline else None,
             "kernel": "k_shadow_queue<false> (any hit)",
             "achieved": round(sh_achieved, 1),
             "peak": HBM_PEAK_GBS,
