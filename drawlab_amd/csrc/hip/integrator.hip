@@ -343,8 +343,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(BvhView bvh, Render
     TraceCounters cnt;
     traverse_persistent<false, COUNT>(bvh, n, &rs.cnt->work_ray, stk, io, cnt);
     if (COUNT) {
-        atomicAdd(&rs.cnt->nodes, (unsigned long long)cnt.nodes);
-        atomicAdd(&rs.cnt->tris, (unsigned long long)cnt.tris);
+        wave_add(&rs.cnt->nodes, cnt.nodes);
+        wave_add(&rs.cnt->tris, cnt.tris);
     }
 }
 
@@ -358,8 +358,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_queue(BvhView bvh, Rende
     TraceCounters cnt;
     traverse_persistent<true, COUNT>(bvh, n, &rs.cnt->work_shadow, stk, io, cnt);
     if (COUNT) {
-        atomicAdd(&rs.cnt->nodes_sh, (unsigned long long)cnt.nodes);
-        atomicAdd(&rs.cnt->tris_sh, (unsigned long long)cnt.tris);
+        wave_add(&rs.cnt->nodes_sh, cnt.nodes);
+        wave_add(&rs.cnt->tris_sh, cnt.tris);
     }
 }
 
@@ -707,9 +707,18 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
 // the throughput before Russian roulette, kept in the path's NEE slot (the
 // NEE contribution has been consumed) -- this keeps the rarely needed
 // emitter pdf out of k_post's registers.
+// The lanes of a wave that call it together take consecutive queue slots
+// with one atomic (the queue's order does not matter: each entry is one
+// path's own emission).
 __device__ __forceinline__ void defer_emission(const RenderBufs& rs, uint32_t p, vec3 thr, int light) {
     rs.nee[p] = f4(thr, __int_as_float(light));
-    rs.emq[atomicAdd(&rs.cnt->emit, 1u)] = p;
+    const unsigned long long mask = __ballot(1);
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&rs.cnt->emit, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    rs.emq[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = p;
 }
 
 // Emission of the continuations k_post deferred: rad += thr * mis * Le, in
@@ -745,7 +754,11 @@ __global__ __launch_bounds__(kStageBlock) void k_post_emit(SceneView s, RenderBu
 template <bool HIDE>
 __global__ __launch_bounds__(kStageBlock) void k_post(PassParams pp, SceneView s, RenderBufs rs, int cur,
                                                       float* __restrict__ grec) {
-    const uint32_t* counts = rs.cnt->mat[cur];
+    // the lengths of the queues being read, loaded once: the counters'
+    // cache line takes every block's appends to the next queues meanwhile
+    uint32_t counts[kTypes];
+#pragma unroll
+    for (int t = 0; t < kTypes; t++) counts[t] = rs.cnt->mat[cur][t];
     const uint32_t n = queue_total(counts);
     const int next = cur ^ 1;
     __shared__ uint32_t s_p[kStageItems][kStageBlock];
