@@ -121,12 +121,13 @@ __device__ __forceinline__ bool mt_test(const float4& a, const float4& b, const 
     return !(parallel | out_u | out_v);
 }
 
-// Adds v of every lane of a converged wave to *dst with one atomic.
+// Adds v of every lane of a converged wave to *dst with one atomic (none
+// when the sum is 0: the waves of a short launch mostly traced nothing).
 __device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
     unsigned long long sum = v;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-    if (__lane_id() == 0) atomicAdd(dst, sum);
+    if (__lane_id() == 0 && sum != 0) atomicAdd(dst, sum);
 }
 
 // Wave-aggregated claim of the next work item: one atomic per wave.
