@@ -62,8 +62,8 @@ OCC_BYTES = 1  # shadow query result
 # Path state each stage reads and writes per path or per shading vertex
 # (structure-of-arrays records of the integrator, 16-byte vectors unless
 # stated): camera ray generation writes ray origin and direction,
-# throughput, radiance, aux, the pcg32 state (8), sample index (4), film
-# position (8) and a queue entry (4); the first-hit stage reads the queue
+# throughput, aux, the pcg32 state (8), sample index (4), film position
+# (8), scene byte (1) and a queue entry (4); the first-hit stage reads the queue
 # entry, hit and direction and writes radiance and a material-queue entry;
 # a shading vertex reads its queue entry, hit, direction, throughput,
 # radiance, aux, pcg32 state and sample index and writes the shadow ray's
@@ -73,7 +73,7 @@ OCC_BYTES = 1  # shadow query result
 # pcg32 state and sample index and writes radiance, aux, throughput, pcg32
 # state and a queue entry; the film gather reads each sample's position and
 # radiance once and writes a pixel per 64 samples.
-PATH_STATE_BYTES = {"raygen": 5 * 16 + 8 + 4 + 8 + 4,                         # 104
+PATH_STATE_BYTES = {"raygen": 4 * 16 + 8 + 4 + 8 + 1 + 4,                     # 89
                     "first_hit": 4 + 2 * 16 + 16 + 4,                          # 56
                     "shade": (4 + 5 * 16 + 8 + 4) + (8 * 16 + 8 + 2 * 4),      # 96 read + 144 written
                     "post": (4 + 6 * 16 + 1 + 8 + 4) + (3 * 16 + 8 + 4),       # 113 read + 60 written

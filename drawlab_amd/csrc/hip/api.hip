@@ -254,6 +254,7 @@ dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
     c->tri_verts.assign(9 * nf, 0.0f);
     c->mesh_recs.assign(sd->num_meshes, MeshRec{});
     std::vector<float> cdf_all;
+    std::vector<uint32_t> guide_all;
     uint64_t vo = 0, fo = 0;
     for (uint32_t m = 0; m < sd->num_meshes; m++) {
         const dl_mesh_desc& md = sd->meshes[m];
@@ -285,6 +286,19 @@ dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
             r.area_norm = area_cdf(md.positions, md.indices, md.num_triangles, cdf);
             r.cdf_offset = (uint32_t)cdf_all.size();
             cdf_all.insert(cdf_all.end(), cdf.begin(), cdf.end());
+            // bucket index of the CDF; a CDF that rounding left non-monotone
+            // (a degenerate last triangle) gets one bucket spanning every
+            // entry, so its searches stay those of the whole CDF
+            const bool monotone = std::is_sorted(cdf.begin(), cdf.end());
+            r.guide_bits = monotone ? cdf_guide_bits(md.num_triangles) : 0;
+            r.guide_offset = (uint32_t)guide_all.size();
+            const uint32_t G = 1u << r.guide_bits;
+            for (uint32_t g = 0; g <= G; g++) {
+                const float key = (float)g / (float)G;  // exact: G is a power of two
+                guide_all.push_back(!monotone ? (g == 0 ? 0u : md.num_triangles)
+                                              : (uint32_t)(std::lower_bound(cdf.begin(), cdf.end(), key) -
+                                                           cdf.begin()));
+            }
         }
         vo += md.num_vertices;
         fo += md.num_triangles;
@@ -366,6 +380,7 @@ dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
     DL_TRY(upload(c, &dl_, c->light_recs.data(), c->light_recs.size()));
     ds.lights = dl_;
     DL_TRY(upload(c, &ds.light_cdf, cdf_all.data(), cdf_all.size()));
+    DL_TRY(upload(c, &ds.light_guide, guide_all.data(), guide_all.size()));
     DL_TRY(upload(c, &ds.env_rgb, env_rgb.data(), env_rgb.size()));
     DL_TRY(upload(c, &ds.env_q, env_q.data(), env_q.size()));
     DL_TRY(upload(c, &ds.env_alias, env_alias.data(), env_alias.size()));

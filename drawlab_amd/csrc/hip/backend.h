@@ -38,6 +38,7 @@ struct DevScene {
     void* materials = nullptr;
     void* lights = nullptr;
     float* light_cdf = nullptr;  // concatenated area CDFs (dpdf.h)
+    uint32_t* light_guide = nullptr;  // per emitter CDF: bucket index of its lower_bound (cdf_guide)
     // environment map
     float4* env_rgb = nullptr;   // w * h texels (rgb, 0)
     float* env_q = nullptr;
@@ -63,8 +64,20 @@ struct MeshRec {
     int32_t has_texcoords;
     float area_norm;   // 1 / total area (DiscretePDF::getNormalization)
     uint32_t cdf_offset;  // into light_cdf (num_prims + 1 entries), emitters only
-    int32_t pad[3];
+    uint32_t guide_offset;  // into light_guide (2^guide_bits + 1 entries), emitters only
+    uint32_t guide_bits;
+    int32_t pad;
 };
+
+// Bucket index of an area CDF c[0..n] (c[n] = 1): entry g, for g = 0 ..
+// 2^bits, is lower_bound(c, g / 2^bits), so the lower_bound of any x in
+// [g, g + 1) / 2^bits lies in [guide[g], guide[g + 1]] -- a search of that
+// range finds the same index as a search of the whole CDF.
+inline uint32_t cdf_guide_bits(uint32_t n) {
+    uint32_t b = 0;
+    while (b < 12 && (1u << b) < n) b++;
+    return b;
+}
 
 struct LightRec {
     int32_t type;

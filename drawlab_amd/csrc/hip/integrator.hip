@@ -295,8 +295,9 @@ __global__ __launch_bounds__(kStageBlock) void k_raygen(PassParams pp, dl_camera
             rs.ray_o[p] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], pp.eps);
             rs.ray_d[p] = f4(d, INFINITY);
         }
+        // (the radiance starts at the first hit: k_post_camera / k_normals
+        // write it for every camera ray)
         rs.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
-        rs.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
         rs.aux[p] = make_int4(0, 0, 0, 0);
         rs.rng[p] = rng.state;
         rs.seq[p] = pp.sample_base + j;
@@ -1197,6 +1198,7 @@ SceneView layout_view(const DevScene& d, int num_lights, int env_light, const fl
     s.mats = (const dl_material_desc*)d.materials;
     s.lights = (const LightRec*)d.lights;
     s.light_cdf = d.light_cdf;
+    s.light_guide = d.light_guide;
     s.env_rgb = d.env_rgb;
     s.env_q = d.env_q;
     s.env_alias = d.env_alias;

@@ -5,7 +5,7 @@
 // Every per-scene array is concatenated in member order and every index it
 // holds is rebased on the way: vertex indices by the vertices before the
 // scene, primitive -> mesh by the meshes before it, mesh records (first
-// primitive, material, light, CDF offset), materials (texture slots), lights
+// primitive, material, light, CDF and CDF-index offsets), materials (texture slots), lights
 // (mesh, first environment texel) and texture records (first texel) on the
 // host, and on the device the 4-wide BVH nodes (interior child refs by the
 // nodes before the scene, leaf refs by the triangles before it) and the
@@ -100,7 +100,7 @@ dl_status copy_at(Ctx* c, T* dst, size_t at, const T* src, size_t count) {
 
 // Entries of each per-scene array (the sizes the scene's upload gave them).
 struct Extent {
-    size_t verts, prims, meshes, mats, lights, cdf, env, textures, texels, nodes, tris;
+    size_t verts, prims, meshes, mats, lights, cdf, guide, env, textures, texels, nodes, tris;
 };
 
 Extent extent_of(const SceneParts& s) {
@@ -111,7 +111,10 @@ Extent extent_of(const SceneParts& s) {
     e.mats = s.mats->size();
     e.lights = s.lights->size();
     for (const MeshRec& m : *s.meshes)
-        if (m.light >= 0) e.cdf = std::max<size_t>(e.cdf, (size_t)m.cdf_offset + m.num_prims + 1);
+        if (m.light >= 0) {
+            e.cdf = std::max<size_t>(e.cdf, (size_t)m.cdf_offset + m.num_prims + 1);
+            e.guide = std::max<size_t>(e.guide, (size_t)m.guide_offset + (1u << m.guide_bits) + 1);
+        }
     for (const LightRec& l : *s.lights)
         if (l.type == DL_LIGHT_ENV) e.env = (size_t)l.env_w * (size_t)l.env_h;
     e.textures = s.num_textures;
@@ -201,6 +204,7 @@ dl_status scene_union_for(Ctx* c, const std::vector<uint64_t>& keys, SceneUnion*
         tot.mats += e.mats;
         tot.lights += e.lights;
         tot.cdf += e.cdf;
+        tot.guide += e.guide;
         tot.env += e.env;
         tot.textures += e.textures;
         tot.texels += e.texels;
@@ -241,6 +245,7 @@ dl_status build_union(Ctx* c, SceneUnion* u, const std::vector<uint64_t>& member
     DL_TRY(union_alloc(c, u, &d_mats, tot.mats));
     DL_TRY(union_alloc(c, u, &d_lights, tot.lights));
     DL_TRY(union_alloc(c, u, &d.light_cdf, tot.cdf));
+    DL_TRY(union_alloc(c, u, &d.light_guide, tot.guide));
     DL_TRY(union_alloc(c, u, &d.env_rgb, tot.env));
     DL_TRY(union_alloc(c, u, &d.env_q, tot.env));
     DL_TRY(union_alloc(c, u, &d.env_alias, tot.env));
@@ -269,6 +274,7 @@ dl_status build_union(Ctx* c, SceneUnion* u, const std::vector<uint64_t>& member
             r.material += (int32_t)o.mats;
             if (r.light >= 0) r.light += (int32_t)o.lights;
             r.cdf_offset += (uint32_t)o.cdf;
+            r.guide_offset += (uint32_t)o.guide;  // its entries index the mesh's own CDF
             meshes.push_back(r);
         }
         for (dl_material_desc md : *s.mats) {
@@ -295,6 +301,7 @@ dl_status build_union(Ctx* c, SceneUnion* u, const std::vector<uint64_t>& member
         DL_TRY(copy_at(c, d.uv, 2 * o.verts, sd.uv, 2 * e.verts));
         DL_TRY(copy_at(c, d.prim_info, o.prims, sd.prim_info, e.prims));
         DL_TRY(copy_at(c, d.light_cdf, o.cdf, sd.light_cdf, e.cdf));
+        DL_TRY(copy_at(c, d.light_guide, o.guide, sd.light_guide, e.guide));
         DL_TRY(copy_at(c, d.env_rgb, o.env, sd.env_rgb, e.env));
         DL_TRY(copy_at(c, d.env_q, o.env, sd.env_q, e.env));
         DL_TRY(copy_at(c, d.env_alias, o.env, sd.env_alias, e.env));  // alias entries are within the environment
@@ -321,6 +328,7 @@ dl_status build_union(Ctx* c, SceneUnion* u, const std::vector<uint64_t>& member
         o.mats += e.mats;
         o.lights += e.lights;
         o.cdf += e.cdf;
+        o.guide += e.guide;
         o.env += e.env;
         o.textures += e.textures;
         o.texels += e.texels;

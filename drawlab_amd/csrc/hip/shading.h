@@ -18,6 +18,7 @@ struct SceneView {
     const dl_material_desc* __restrict__ mats;
     const LightRec* __restrict__ lights;
     const float* __restrict__ light_cdf;
+    const uint32_t* __restrict__ light_guide;
     const float4* __restrict__ env_rgb;
     const float* __restrict__ env_q;
     const uint32_t* __restrict__ env_alias;
@@ -610,9 +611,14 @@ __device__ __forceinline__ vec3 env_eval_pdf(const SceneView& s, const LightRec&
 }
 
 // DiscretePDF::sampleReuse on a mesh CDF (include/core/math/dpdf.h:95-136).
-__device__ __forceinline__ uint32_t cdf_sample_reuse(const float* __restrict__ cdf, uint32_t n, float& x) {
-    // lower_bound over cdf[0..n] (n + 1 entries)
-    uint32_t lo = 0, count = n + 1;
+// The lower_bound over cdf[0..n] searches only the range the CDF's bucket
+// index gives for x (backend.h cdf_guide_bits): the same index as a search
+// of all n + 1 entries, from a few dependent loads instead of log2(n).
+__device__ __forceinline__ uint32_t cdf_sample_reuse(const float* __restrict__ cdf, uint32_t n,
+                                                     const uint32_t* __restrict__ guide, uint32_t bits, float& x) {
+    const uint32_t G = 1u << bits;
+    const uint32_t g = min((uint32_t)(x * (float)G), G - 1u);  // x in [0, 1]; x * G is exact
+    uint32_t lo = guide[g], count = guide[g + 1] - lo + 1;
     while (count > 0) {
         const uint32_t step = count / 2;
         const uint32_t it = lo + step;
@@ -637,7 +643,9 @@ __device__ inline vec3 emitter_sample(const SceneView& s, const LightRec& e, flo
     if (e.type == DL_LIGHT_AREA) {
         const MeshRec& m = s.meshes[e.mesh];
         float x = sx;
-        const uint32_t f = cdf_sample_reuse(s.light_cdf + m.cdf_offset, m.num_prims, x) + m.prim_offset;
+        const uint32_t f = cdf_sample_reuse(s.light_cdf + m.cdf_offset, m.num_prims, s.light_guide + m.guide_offset,
+                                            m.guide_bits, x) +
+                           m.prim_offset;
         const uint32_t i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
         const vec3 p0 = ld3(s.pos, i0), p1 = ld3(s.pos, i1), p2 = ld3(s.pos, i2);
         const vec3 b = sq_triangle(x, sy);
