@@ -70,11 +70,6 @@ __device__ __forceinline__ void cswap(float& ka, int& ra, float& kb, int& rb) {
     ra = r;
 }
 
-// Moller-Trumbore on a pre-gathered triangle; returns false on a miss.
-// Branch-free: every quantity is computed and the reference's early-out
-// tests (mesh.cpp:195, 204, 213) are combined into the returned predicate,
-// which accepts exactly the rays the early-out sequence accepts.  The triangle
-// is therefore fetched by three loads issued together.
 // Slab test of one child box (lo, hi per axis): entry distance n and
 // whether [n, far] meets [tmin, tmax] (far widened by kFarSlack).
 __device__ __forceinline__ bool slab1(float lx, float hx, float ly, float hy, float lz, float hz, const vec3& o,
@@ -102,6 +97,11 @@ __device__ __forceinline__ bool slab_q(const float4& r0, const uint4& r1, const 
                  tmax, n);
 }
 
+// Moller-Trumbore on a pre-gathered triangle; returns false on a miss.
+// Branch-free: every quantity is computed and the reference's early-out
+// tests (mesh.cpp:195, 204, 213) are combined into the returned predicate,
+// which accepts exactly the rays the early-out sequence accepts.  The triangle
+// is therefore fetched by three loads issued together.
 __device__ __forceinline__ bool mt_test(const float4& a, const float4& b, const float4& c, const vec3& o,
                                         const vec3& d, float& u, float& v, float& t) {
     const vec3 p0 = v3(a.x, a.y, a.z);
