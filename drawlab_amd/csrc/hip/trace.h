@@ -130,6 +130,12 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
     if (__lane_id() == 0 && sum != 0) atomicAdd(dst, sum);
 }
 
+// Relative VALU cost of the traversal's two step kinds, triangle test :
+// node visit (their instruction counts are about 130 : 300; 2 : 5 measured
+// best on the 2M-triangle displaced-sphere render, 1 : 3 and 1 : 2 within
+// 0.4%).
+constexpr uint32_t kTriCost = 2, kNodeCost = 5;
+
 // Wave-aggregated claim of the next work item: one atomic per wave.
 __device__ __forceinline__ uint32_t claim_next(uint32_t* counter, bool want) {
     const unsigned long long mask = __ballot(want);
@@ -147,16 +153,17 @@ __device__ __forceinline__ uint32_t claim_next(uint32_t* counter, bool want) {
 // ...: no atomic round trip on the refill path), so a wave never idles
 // behind its slowest ray.
 //
-// Each iteration a lane takes one step: it visits one node, or tests one of
-// the triangles of the leaves its last node visit hit.  A node visit
-// decides the next node (the nearest hit internal child, or a stack pop)
-// with the best t of that moment, then the lane works through that visit's
-// triangles, one per iteration, before it visits the next node -- the
-// sequence of node visits and triangle tests of a ray is the one a
-// node-by-node loop makes (and the CPU replay follows), but a wave no longer
-// waits at every node for its lane with the most leaf triangles: a
-// triangle test costs one iteration of the lane that needs it, one
-// dependent fetch each.
+// A step visits one node, or tests one of the triangles of the leaves the
+// lane's last node visit hit.  A node visit decides the next node (the
+// nearest hit internal child, or a stack pop) with the best t of that
+// moment, then the lane works through that visit's triangles, one per step,
+// before it visits the next node -- the sequence of node visits and
+// triangle tests of a ray is the one a node-by-node loop makes (and the CPU
+// replay follows), but a wave does not wait at every node for its lane with
+// the most leaf triangles: a triangle test costs one step of the lane that
+// needs it, one dependent fetch each.  Each iteration the wave takes steps
+// of one kind: the lanes whose next step is of that kind take it, the
+// others hold theirs.
 //
 // IO provides: load(i, o, d, tmin, tmax), root(i) (the node traversal
 // starts from), store_hit(i, prim, t, u, v, nodes, tris) for closest hits and
@@ -220,10 +227,21 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
             }
         }
         if (!__ballot(active)) break;
-        if (!active) continue;
+        // One kind of step per wave and iteration: the lanes whose next
+        // step is a triangle test and those whose next step is a node visit
+        // would otherwise run both branches every iteration under partial
+        // masks.  Triangle steps run when enough lanes wait for one (a
+        // triangle step costs about kTriCost / kNodeCost of a node visit);
+        // the other lanes hold their step, so every ray still makes its own
+        // sequence of visits and tests.
+        const bool tri_step = lj < ltri;
+        const uint32_t ntri = (uint32_t)__popcll(__ballot(active && tri_step));
+        const uint32_t nnode = (uint32_t)__popcll(__ballot(active && !tri_step));
+        const bool tri_mode = ntri != 0 && ntri * kNodeCost >= nnode * kTriCost;
+        if (!active || tri_step != tri_mode) continue;
 
         bool done = false, hit_any = false;
-        if (lj < ltri) {
+        if (tri_step) {
             // one triangle of the last visit's hit leaves, in slot order
             const int ls0 = (int)(lcnt & 0xffu);
             const int ls1 = ls0 + (int)((lcnt >> 8) & 0xffu);
