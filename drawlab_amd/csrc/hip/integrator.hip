@@ -308,7 +308,8 @@ __global__ __launch_bounds__(kStageBlock) void k_raygen(PassParams pp, dl_camera
     if (blockIdx.x == 0 && threadIdx.x == 0) rs.cnt->ray = total;
 }
 
-// Queued path rays: the ray of path q[i] lives in ray_o/ray_d (shadow: sh_o/sh_d).
+// Queued path rays: work slot i holds path q[i], whose ray lives in
+// ray_o/ray_d (shadow: sh_o/sh_d); the traversal's entries are path indices.
 struct QueueIO {
     const uint32_t* __restrict__ q;
     const float4* __restrict__ ro;
@@ -317,21 +318,21 @@ struct QueueIO {
     uint8_t* __restrict__ occ;
     const uint8_t* __restrict__ sid;
     const int32_t* __restrict__ roots;
+    __device__ __forceinline__ uint32_t entry(uint32_t i) const { return q[i]; }
     // first node of the ray's scene
-    __device__ __forceinline__ int root(uint32_t i) const { return roots ? roots[sid[q[i]]] : 0; }
-    __device__ __forceinline__ void load(uint32_t i, vec3& o, vec3& d, float& tmin, float& tmax) const {
-        const uint32_t p = q[i];
+    __device__ __forceinline__ int root(uint32_t p) const { return roots ? roots[sid[p]] : 0; }
+    __device__ __forceinline__ void load(uint32_t p, vec3& o, vec3& d, float& tmin, float& tmax) const {
         const float4 a = ro[p], b = rd[p];
         o = v3(a.x, a.y, a.z);
         d = v3(b.x, b.y, b.z);
         tmin = a.w;
         tmax = b.w;
     }
-    __device__ __forceinline__ void store_hit(uint32_t i, uint32_t prim, float t, float u, float v, uint32_t,
+    __device__ __forceinline__ void store_hit(uint32_t p, uint32_t prim, float t, float u, float v, uint32_t,
                                               uint32_t) const {
-        hit[q[i]] = make_float4(__uint_as_float(prim), t, u, v);
+        hit[p] = make_float4(__uint_as_float(prim), t, u, v);
     }
-    __device__ __forceinline__ void store_occ(uint32_t i, bool o, uint32_t, uint32_t) const { occ[q[i]] = o ? 1 : 0; }
+    __device__ __forceinline__ void store_occ(uint32_t p, bool o, uint32_t, uint32_t) const { occ[p] = o ? 1 : 0; }
 };
 
 template <bool COUNT>

@@ -17,6 +17,7 @@ struct BatchIO {
     float *__restrict__ th, *__restrict__ uh, *__restrict__ vh;
     uint32_t *__restrict__ cn, *__restrict__ ct;
     uint8_t* __restrict__ occ;
+    __device__ __forceinline__ uint32_t entry(uint32_t slot) const { return slot; }
     __device__ __forceinline__ void load(uint32_t i, vec3& o, vec3& d, float& tmin, float& tmax) const {
         o = v3(rays[i], rays[N + i], rays[2 * N + i]);
         d = v3(rays[3 * N + i], rays[4 * N + i], rays[5 * N + i]);
