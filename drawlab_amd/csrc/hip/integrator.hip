@@ -988,35 +988,39 @@ __global__ __launch_bounds__(kStageBlock) void k_film(PassParams pp, uint32_t k,
     }
 }
 
-// Tiled gather for filter radii up to kFilmHalo - 1: one 16x16 pixel tile per
-// block; each sample layer's tile plus halo is staged in LDS once instead of
-// being fetched by each of the (2R+1)^2 pixels it can reach.  Staging also
-// computes what film_add derives from the sample alone -- its 32x32 block
-// origin, position relative to it and the range of film pixels its
-// footprint covers (empty for an invalid sample) -- so a (pixel, sample)
-// pair costs a range test, two table lookups and the accumulation, with
-// film_add's arithmetic and order.
-constexpr int kFilmTile = 16;
+// Tiled gather for the filters whose footprint reaches at most kFilmHalo - 1
+// pixels from a sample's own pixel: one 16x16 pixel tile per block; each
+// sample layer's tile plus halo is staged in LDS once, with film_add's
+// weights already computed -- for each sample, the x weight of each of the
+// 2R + 1 film columns and the y weight of each of the 2R + 1 film rows it
+// can reach (0 where its block-local footprint does not cover the pixel, and
+// radiance 0 for an invalid sample or a position outside the image).  A
+// (pixel, sample) pair then costs three LDS reads and film_add's products
+// and sums, without a test: a zero weight adds +0 or -0, which leaves the
+// accumulator unchanged (it is never -0).  Samples are added in k_film's
+// order (layers, then rows, then columns), so the pixel sums are those of
+// film_add bit for bit.
 constexpr int kFilmHalo = 5;
-constexpr int kFilmSpan = kFilmTile + 2 * kFilmHalo;
+constexpr int kFilmTile = 16;
+template <int R>
 __global__ __launch_bounds__(kFilmTile* kFilmTile) void k_film_tiled(PassParams pp, uint32_t k, dl_film_desc film,
                                                                      RenderBufs rs) {
-    __shared__ float4 s_rad[kFilmSpan * kFilmSpan];
-    __shared__ float2 s_rel[kFilmSpan * kFilmSpan];  // plx, ply
-    // covered film pixels relative to the tile origin: x lo, x hi, y lo, y hi
-    __shared__ short4 s_cov[kFilmSpan * kFilmSpan];
-    __shared__ float s_table[33];                    // the filter table, indexed per lane
+    constexpr int S = kFilmTile + 2 * R;  // staged span
+    constexpr int D = 2 * R + 1;          // film columns (rows) a sample can reach
+    __shared__ float4 s_rad[S * S];
+    // s_wx[d][e]: weight of staged sample e for film column px - R + d
+    // (s_wy: row py - R + d)
+    __shared__ float s_wx[D][S * S];
+    __shared__ float s_wy[D][S * S];
+    __shared__ float s_table[33];  // the filter table, indexed per lane
     if (threadIdx.x < 33) s_table[threadIdx.x] = film.table[threadIdx.x];
+    __syncthreads();
     const uint32_t W = pp.W, H = pp.H, WH = W * H;
-    // A sample at P in [px, px + 1] reaches the film pixels with
-    // ceil(P - 1/2 - r) <= q <= floor(P - 1/2 + r), i.e. within
-    // floor(r + 1/2) of its own pixel px.
-    const int R = (int)floorf(film.radius + 0.5f);
-    const int S = kFilmTile + 2 * R;  // staged span
     const int b = film.border;
     const float r = film.radius;
     const int tx0 = (int)blockIdx.x * kFilmTile, ty0 = (int)blockIdx.y * kFilmTile;
-    const int qx = tx0 + (int)(threadIdx.x % kFilmTile), qy = ty0 + (int)(threadIdx.x / kFilmTile);
+    const int lx = (int)(threadIdx.x % kFilmTile), ly = (int)(threadIdx.x / kFilmTile);
+    const int qx = tx0 + lx, qy = ty0 + ly;
     const bool inside = qx < (int)W && qy < (int)H;
     const uint32_t q = inside ? (uint32_t)qy * W + (uint32_t)qx : 0u;
     float4 acc = inside ? rs.film[q] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1024,50 +1028,84 @@ __global__ __launch_bounds__(kFilmTile* kFilmTile) void k_film_tiled(PassParams 
     for (uint32_t j = 0; j < k; ++j) {
         for (int e = (int)threadIdx.x; e < S * S; e += (int)blockDim.x) {
             const int gx = tx0 - R + e % S, gy = ty0 - R + e / S;
+            float4 L = make_float4(0.f, 0.f, 0.f, 0.f);
+            float wx[D], wy[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) wx[d] = wy[d] = 0.0f;
             if (gx >= 0 && gx < (int)W && gy >= 0 && gy < (int)H) {
                 const uint32_t sp = j * WH + (uint32_t)gy * W + (uint32_t)gx;
-                const float4 L = rs.rad[sp];
+                const float4 Ls = rs.rad[sp];
                 const float2 P = rs.pos[sp];
-                // film_add's sample-only terms (block.cpp:68-128)
-                const int oy = (gy / 32) * 32, sy = min(32, (int)H - oy);
-                const int ox = (gx / 32) * 32, sx = min(32, (int)W - ox);
-                const float plx = P.x - 0.5f - (float)(ox - b);
-                const float ply = P.y - 0.5f - (float)(oy - b);
-                const int x0 = max((int)ceilf(plx - r), 0), x1 = min((int)floorf(plx + r), sx + 2 * b - 1);
-                const int y0 = max((int)ceilf(ply - r), 0), y1 = min((int)floorf(ply + r), sy + 2 * b - 1);
-                const bool ok = sample_valid(L);
-                if (!ok && gx >= tx0 && gx < tx0 + kFilmTile && gy >= ty0 && gy < ty0 + kFilmTile) invalid++;
-                s_rad[e] = L;
-                s_rel[e] = make_float2(plx, ply);
-                s_cov[e] = ok ? make_short4((short)(x0 + ox - b - tx0), (short)(x1 + ox - b - tx0),
-                                            (short)(y0 + oy - b - ty0), (short)(y1 + oy - b - ty0))
-                              : make_short4(1, 0, 1, 0);
+                if (sample_valid(Ls)) {
+                    L = Ls;
+                    // film_add's terms (block.cpp:68-128)
+                    const int oy = (gy / 32) * 32, sy = min(32, (int)H - oy);
+                    const int ox = (gx / 32) * 32, sx = min(32, (int)W - ox);
+                    const float plx = P.x - 0.5f - (float)(ox - b);
+                    const float ply = P.y - 0.5f - (float)(oy - b);
+                    const int x0 = max((int)ceilf(plx - r), 0), x1 = min((int)floorf(plx + r), sx + 2 * b - 1);
+                    const int y0 = max((int)ceilf(ply - r), 0), y1 = min((int)floorf(ply + r), sy + 2 * b - 1);
+#pragma unroll
+                    for (int d = 0; d < D; d++) {
+                        const int xl = gx - R + d - (ox - b), yl = gy - R + d - (oy - b);
+                        if (xl >= x0 && xl <= x1) wx[d] = s_table[(int)(fabsf((float)xl - plx) * film.lookup)];
+                        if (yl >= y0 && yl <= y1) wy[d] = s_table[(int)(fabsf((float)yl - ply) * film.lookup)];
+                    }
+                } else if (gx >= tx0 && gx < tx0 + kFilmTile && gy >= ty0 && gy < ty0 + kFilmTile) {
+                    invalid++;
+                }
+            }
+            s_rad[e] = L;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                s_wx[d][e] = wx[d];
+                s_wy[d][e] = wy[d];
             }
         }
         __syncthreads();
-        if (inside) {
-            const int lx = qx - tx0, ly = qy - ty0;
-            for (int py = max(0, qy - R); py <= min((int)H - 1, qy + R); ++py)
-                for (int px = max(0, qx - R); px <= min((int)W - 1, qx + R); ++px) {
-                    const int e = (py - (ty0 - R)) * S + (px - (tx0 - R));
-                    const short4 cv = s_cov[e];
-                    if (lx < cv.x || lx > cv.y || ly < cv.z || ly > cv.w) continue;
-                    const float4 L = s_rad[e];
-                    const float2 rel = s_rel[e];
-                    // the sample's 32x32 image block origin, as staged
-                    const int xl = qx - ((px / 32) * 32 - b), yl = qy - ((py / 32) * 32 - b);
-                    const float wx = s_table[(int)(fabsf((float)xl - rel.x) * film.lookup)];
-                    const float wy = s_table[(int)(fabsf((float)yl - rel.y) * film.lookup)];
-                    acc.x += (L.x * wx) * wy;
-                    acc.y += (L.y * wx) * wy;
-                    acc.z += (L.z * wx) * wy;
-                    acc.w += (1.0f * wx) * wy;
-                }
+        // the sample at staged column lx + cx lies R - cx columns left of
+        // the pixel: the pixel is its film column px - R + (2R - cx)
+#pragma unroll
+        for (int cy = 0; cy < D; cy++) {
+#pragma unroll
+            for (int cx = 0; cx < D; cx++) {
+                const int e = (ly + cy) * S + lx + cx;
+                const float4 L = s_rad[e];
+                const float wx = s_wx[2 * R - cx][e], wy = s_wy[2 * R - cy][e];
+                acc.x += (L.x * wx) * wy;
+                acc.y += (L.y * wx) * wy;
+                acc.z += (L.z * wx) * wy;
+                acc.w += (1.0f * wx) * wy;
+            }
         }
         __syncthreads();
     }
     if (inside) rs.film[q] = acc;
     if (invalid) atomicAdd(&rs.cnt->invalid, (unsigned long long)invalid);
+}
+
+// The film gather of one pass's k sample layers: the tiled gather for
+// footprints of up to kFilmHalo - 1 pixels, else the generic one.
+static void film_gather(const PassParams& pp, uint32_t k, const dl_film_desc& film, const RenderBufs& bufs,
+                        int stage_grid, hipStream_t stream) {
+    // A sample at P in [px, px + 1] reaches the film pixels with
+    // ceil(P - 1/2 - r) <= q <= floor(P - 1/2 + r), i.e. within
+    // floor(r + 1/2) of its own pixel px.  P -/+ r is rounded once in
+    // binary32 (block-relative, |P| < 64: by at most 2^-19), which can reach
+    // one pixel further when r + 1/2 lies just below an integer.
+    const float reach = film.radius + 0.5f;
+    int R = (int)floorf(reach);
+    if (reach - (float)R > 0.9999f) R++;
+    const dim3 fgrid((pp.W + kFilmTile - 1) / kFilmTile, (pp.H + kFilmTile - 1) / kFilmTile);
+    const int threads = kFilmTile * kFilmTile;
+    switch (R) {
+        case 0: k_film_tiled<0><<<fgrid, threads, 0, stream>>>(pp, k, film, bufs); break;
+        case 1: k_film_tiled<1><<<fgrid, threads, 0, stream>>>(pp, k, film, bufs); break;
+        case 2: k_film_tiled<2><<<fgrid, threads, 0, stream>>>(pp, k, film, bufs); break;
+        case 3: k_film_tiled<3><<<fgrid, threads, 0, stream>>>(pp, k, film, bufs); break;
+        case 4: k_film_tiled<4><<<fgrid, threads, 0, stream>>>(pp, k, film, bufs); break;
+        default: k_film<<<stage_grid, kStageBlock, 0, stream>>>(pp, k, film, bufs); break;
+    }
 }
 
 // ImageBlock::toBitmap / Color4f::divideByFilterWeight  block.cpp:58-66, color.h:120-125
@@ -1470,12 +1508,7 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
             train_steps += steps_per_pass;
         }
         DL_TRY(tm.start(kTagFilm));
-        if ((int)ceilf(c->film.radius) + 1 <= kFilmHalo) {
-            const dim3 fgrid((W + kFilmTile - 1) / kFilmTile, (H + kFilmTile - 1) / kFilmTile);
-            k_film_tiled<<<fgrid, kFilmTile * kFilmTile, 0, c->stream>>>(pp, kk, c->film, rs->b);
-        } else {
-            k_film<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, kk, c->film, rs->b);
-        }
+        film_gather(pp, kk, c->film, rs->b, stage_grid, c->stream);
         DL_TRY(tm.stop());
         done += kk;
     }
@@ -1677,12 +1710,7 @@ dl_status render_batch(Ctx* c, const dl_render_params* prm, const dl_batch_item*
         fb.pos += base[i];
         fb.film = (float4*)c->film_slots[it.film].film;
         const dl_film_desc& film = *parts[item_key[i]].film;  // the image's scene's reconstruction filter
-        if ((int)ceilf(film.radius) + 1 <= kFilmHalo) {
-            const dim3 fgrid((q.W + kFilmTile - 1) / kFilmTile, (q.H + kFilmTile - 1) / kFilmTile);
-            k_film_tiled<<<fgrid, kFilmTile * kFilmTile, 0, c->stream>>>(q, (uint32_t)it.spp, film, fb);
-        } else {
-            k_film<<<stage_grid, kStageBlock, 0, c->stream>>>(q, (uint32_t)it.spp, film, fb);
-        }
+        film_gather(q, (uint32_t)it.spp, film, fb, stage_grid, c->stream);
         if (it.rgb_out)
             k_resolve<<<stage_grid, kStageBlock, 0, c->stream>>>((const float4*)c->film_slots[it.film].film,
                                                                   c->film_slots[it.film].rgb, q.W * q.H);
