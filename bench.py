@@ -847,7 +847,7 @@ def main():
     if args.summarize_counters:
         summarize_counters(args.summarize_counters[:-1], args.summarize_counters[-1], args.command,
                            ["k_trace_queue<false>", "k_shadow_queue<false>", "k_post<false>", "k_shade<0, false>",
-                            "k_shade<5, false>", "k_film_tiled"])
+                            "k_shade<5, false>", "k_film_tiled<2>"])
         return
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, copy_bytes=args.copy_mib << 20, commands=args.command)
