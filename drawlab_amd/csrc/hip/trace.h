@@ -253,21 +253,35 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
 
         bool done = false, hit_any = false;
         if (tri_step) {
-            // one triangle of the last visit's hit leaves, in slot order
+            // the next one or two triangles of the last visit's hit leaves, in
+            // slot order: both fetched before the first is tested, so a pair
+            // costs one dependent wait; the second is accepted against the
+            // best t the first left, as a sequential loop would
             const int ls0 = (int)(lcnt & 0xffu);
             const int ls1 = ls0 + (int)((lcnt >> 8) & 0xffu);
             const int ls2 = ls1 + (int)((lcnt >> 16) & 0xffu);
-            const int j = lj++;
-            const int k = j < ls0 ? ~lref.x + j : j < ls1 ? ~lref.y + (j - ls0) : j < ls2 ? ~lref.z + (j - ls1)
-                                                                                    : ~lref.w + (j - ls2);
-            const float4* tp = bvh.tris + 3 * k;
-            const float4 ta = tp[0], tb = tp[1], tc = tp[2];
-            if (COUNT) cnt.tris++;
+            const int j = lj;
+            const bool two = j + 1 < ltri;
+            const int k0 = j < ls0 ? ~lref.x + j : j < ls1 ? ~lref.y + (j - ls0) : j < ls2 ? ~lref.z + (j - ls1)
+                                                                                     : ~lref.w + (j - ls2);
+            const int j1 = two ? j + 1 : j;
+            const int k1 = j1 < ls0 ? ~lref.x + j1 : j1 < ls1 ? ~lref.y + (j1 - ls0) : j1 < ls2 ? ~lref.z + (j1 - ls1)
+                                                                                          : ~lref.w + (j1 - ls2);
+            const float4* tp0 = bvh.tris + 3 * k0;
+            const float4* tp1 = bvh.tris + 3 * k1;
+            const float4 ta = tp0[0], tb = tp0[1], tc = tp0[2];
+            const float4 sa = tp1[0], sb = tp1[1], sc = tp1[2];
             float u, v, t;
-            if (mt_test(ta, tb, tc, o, d, u, v, t) && t >= tmin && t <= best_t) {
-                if (ANY) {
-                    hit_any = true;
-                } else {
+            const bool h0 = mt_test(ta, tb, tc, o, d, u, v, t) && t >= tmin && t <= best_t;
+            if (ANY) {
+                hit_any = h0;
+                if (two && !h0) {
+                    hit_any = mt_test(sa, sb, sc, o, d, u, v, t) && t >= tmin && t <= best_t;
+                    lj++;
+                    if (COUNT) cnt.tris++;
+                }
+            } else {
+                if (h0) {
                     const uint32_t id = __float_as_uint(tc.y);
                     if (t < best_t || id < best) {
                         best_t = t;
@@ -276,7 +290,22 @@ __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t
                         bv = v;
                     }
                 }
+                if (two) {
+                    lj++;
+                    if (COUNT) cnt.tris++;
+                    if (mt_test(sa, sb, sc, o, d, u, v, t) && t >= tmin && t <= best_t) {
+                        const uint32_t id = __float_as_uint(sc.y);
+                        if (t < best_t || id < best) {
+                            best_t = t;
+                            best = id;
+                            bu = u;
+                            bv = v;
+                        }
+                    }
+                }
             }
+            lj++;
+            if (COUNT) cnt.tris++;
             done = hit_any || (last && lj == ltri);
         } else {
             const float4* np = bvh.nodes + 4 * (size_t)cur;
