@@ -153,15 +153,15 @@ __device__ __forceinline__ uint32_t claim_next(uint32_t* counter, bool want) {
 // takes rays g, g + G, g + 2G, ...: no atomic round trip on the refill
 // path), so a wave never idles behind its slowest ray.
 //
-// A step visits one node, or tests one of the triangles of the leaves the
-// lane's last node visit hit.  A node visit decides the next node (the
-// nearest hit internal child, or a stack pop) with the best t of that
-// moment, then the lane works through that visit's triangles, one per step,
-// before it visits the next node -- the sequence of node visits and
+// A step visits one node, or tests the next one or two of the triangles of
+// the leaves the lane's last node visit hit.  A node visit decides the next
+// node (the nearest hit internal child, or a stack pop) with the best t of
+// that moment, then the lane works through that visit's triangles, two per
+// step, before it visits the next node -- the sequence of node visits and
 // triangle tests of a ray is the one a node-by-node loop makes (and the CPU
 // replay follows), but a wave does not wait at every node for its lane with
-// the most leaf triangles: a triangle test costs one step of the lane that
-// needs it, one dependent fetch each.  Each iteration the wave takes steps
+// the most leaf triangles: a pair of triangle tests costs one step of the
+// lane that needs it, one dependent fetch.  Each iteration the wave takes steps
 // of one kind: the lanes whose next step is of that kind take it, the
 // others hold theirs.
 //
