@@ -61,23 +61,25 @@ NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES = 64, 48, 32, 16  # 4-wide node, tri
 OCC_BYTES = 1  # shadow query result
 # Path state each stage reads and writes per path or per shading vertex
 # (structure-of-arrays records of the integrator, 16-byte vectors unless
-# stated): camera ray generation writes ray origin and direction,
-# throughput, aux, the pcg32 state (8), sample index (4), film position
-# (8), scene byte (1) and a queue entry (4); the first-hit stage reads the queue
+# stated): camera ray generation writes ray origin and direction, the
+# pcg32 state (8), sample index (4), film position (8), scene byte (1) and
+# a queue entry (4); the first-hit stage reads the queue
 # entry, hit and direction and writes radiance and a material-queue entry;
 # a shading vertex reads its queue entry, hit, direction, throughput,
-# radiance, aux, pcg32 state and sample index and writes the shadow ray's
+# radiance, aux, pcg32 state and sample index (a path's first vertex: not
+# throughput and aux, which are implied) and writes the shadow ray's
 # origin and direction and its NEE term, the continuation ray, throughput,
 # radiance, pcg32 state, aux and two queue entries; its post-processing
 # reads aux, radiance, NEE term, occlusion byte, throughput, hit, direction,
 # pcg32 state and sample index and writes radiance, aux, throughput, pcg32
 # state and a queue entry; the film gather reads each sample's position and
 # radiance once and writes a pixel per 64 samples.
-PATH_STATE_BYTES = {"raygen": 4 * 16 + 8 + 4 + 8 + 1 + 4,                     # 89
+PATH_STATE_BYTES = {"raygen": 2 * 16 + 8 + 4 + 8 + 1 + 4,                     # 57
                     "first_hit": 4 + 2 * 16 + 16 + 4,                          # 56
                     "shade": (4 + 5 * 16 + 8 + 4) + (8 * 16 + 8 + 2 * 4),      # 96 read + 144 written
                     "post": (4 + 6 * 16 + 1 + 8 + 4) + (3 * 16 + 8 + 4),       # 113 read + 60 written
-                    "film": 8 + 16 + 16 / 64.0}
+                    "film": 8 + 16 + 16 / 64.0,
+                    "first_vertex_skip": 2 * 16}                               # per path: an undercount
 TRACE_KERNEL = "k_trace_queue<false>"
 COPY_KERNEL = "k_copy"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_queue_%s_traffic.json" % WORKLOAD)
@@ -150,7 +152,11 @@ def path_bytes(agg, bytes_closest, bytes_shadow):
     paths = agg["paths"]
     vertices = agg["closest_rays"] - paths
     trav = (agg["closest_rays"] * (bytes_closest + 4) + agg["shadow_rays"] * (bytes_shadow + 4)) / paths
-    state = (paths * (PATH_STATE_BYTES["raygen"] + PATH_STATE_BYTES["first_hit"] + PATH_STATE_BYTES["film"]) +
+    # the throughput and aux a first shading vertex does not read are taken
+    # off once per path, also for paths that never reach one (so the sum is
+    # a lower bound)
+    state = (paths * (PATH_STATE_BYTES["raygen"] + PATH_STATE_BYTES["first_hit"] + PATH_STATE_BYTES["film"] -
+                      PATH_STATE_BYTES["first_vertex_skip"]) +
              vertices * (PATH_STATE_BYTES["shade"] + PATH_STATE_BYTES["post"])) / paths
     return {"traversal": round(trav, 1), "path_state": round(state, 1), "total": trav + state,
             "closest_rays_per_path": round(agg["closest_rays"] / paths, 3),

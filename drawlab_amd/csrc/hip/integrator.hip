@@ -296,9 +296,9 @@ __global__ __launch_bounds__(kStageBlock) void k_raygen(PassParams pp, dl_camera
             rs.ray_d[p] = f4(d, INFINITY);
         }
         // (the radiance starts at the first hit: k_post_camera / k_normals
-        // write it for every camera ray)
-        rs.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
-        rs.aux[p] = make_int4(0, 0, 0, 0);
+        // write it for every camera ray; throughput 1, eta 1 and aux 0 are
+        // implied at the first shading vertex, and written by k_post_camera
+        // for the paths that pass through a hidden light first)
         rs.rng[p] = rng.state;
         rs.seq[p] = pp.sample_base + j;
         rs.pos[p] = make_float2(px, py);
@@ -424,6 +424,7 @@ __global__ __launch_bounds__(kStageBlock) void k_post_camera(PassParams pp, Scen
                 } else {
                     const uint32_t info = prim_info(s, prim);
                     if (HIDE && info_light(info)) {
+                        rs.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
                         rs.aux[p] = make_int4(0, (int)(kFlagCont | kFlagCamera), 0, 0);
                         alive = true;
                         type = kPassQ;
@@ -521,7 +522,7 @@ __global__ __launch_bounds__(kStageBlock) void k_normals(SceneView s, RenderBufs
 // type), so each carries only its own BSDF's code and registers.
 template <int TYPE, bool GUIDED>
 __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView s, RenderBufs rs, int cur,
-                                                       GuideView gv) {
+                                                       GuideView gv, bool first) {
     const uint32_t n = rs.cnt->mat[cur][TYPE];
     const uint32_t* __restrict__ queue = rs.matq[cur] + (size_t)TYPE * pp.cap;
     constexpr int type = TYPE;
@@ -541,11 +542,17 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                 const float4 h = rs.hit[p];
                 const uint32_t prim = __float_as_uint(h.x);
                 const vec3 d = f3(rs.ray_d[p]);
-                const float4 th4 = rs.thr[p];
+                // the first shading vertex of a camera path: throughput 1,
+                // eta 1, aux 0 (not stored by k_raygen)
+                float4 th4 = make_float4(1.f, 1.f, 1.f, 1.f);
+                int4 aux = make_int4(0, 0, 0, 0);
+                if (!first) {
+                    th4 = rs.thr[p];
+                    aux = rs.aux[p];
+                }
                 vec3 thr = f3(th4);
                 float eta = th4.w;
                 vec3 rad = f3(rs.rad[p]);
-                int4 aux = rs.aux[p];
                 Pcg32 rng;
                 rng.state = rs.rng[p];
                 rng.inc = ((uint64_t)rs.seq[p] << 1u) | 1u;
@@ -1311,11 +1318,12 @@ dl_status run_bounces(Ctx* c, RenderState* rs, const PassParams& pp, const Scene
     DL_HIP_CHECK(c, hipGetLastError());
     if (normals) return DL_OK;  // no bounces
     // One bounce: shading of the paths in queue `cur` (whose lengths per
-    // material type are `len`), their continuation and shadow traces, and
-    // post (which queues the survivors in cur ^ 1).  Every kernel reads its
+    // material type are `len`; `first`: the camera paths' first shading
+    // vertices), their continuation and shadow traces, and post (which
+    // queues the survivors in cur ^ 1).  Every kernel reads its
     // queue length from the device counters; the host skips the shading
     // kernels of empty queues.
-    auto bounce = [&](int cur, const uint32_t* len) -> dl_status {
+    auto bounce = [&](int cur, const uint32_t* len, bool first) -> dl_status {
         k_reset<<<1, kStageBlock, 0, c->stream>>>(rs->b.cnt, cur ^ 1);
         DL_TRY(tm.start(kTagShade));
         if (hide && len[kPassQ] > 0) k_pass<<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur);
@@ -1333,20 +1341,20 @@ dl_status run_bounces(Ctx* c, RenderState* rs, const PassParams& pp, const Scene
                 DL_TRY(tm.stop());
                 DL_TRY(tm.start(kTagShade));
                 switch (t) {
-                    case DL_BSDF_DIFFUSE: k_shade<DL_BSDF_DIFFUSE, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                    case DL_BSDF_MICROFACET: k_shade<DL_BSDF_MICROFACET, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                    case DL_BSDF_ANISO_GGX: k_shade<DL_BSDF_ANISO_GGX, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                    case DL_BSDF_CONDUCTOR: k_shade<DL_BSDF_CONDUCTOR, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                    case DL_BSDF_DIFFUSE: k_shade<DL_BSDF_DIFFUSE, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv, first); break;
+                    case DL_BSDF_MICROFACET: k_shade<DL_BSDF_MICROFACET, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv, first); break;
+                    case DL_BSDF_ANISO_GGX: k_shade<DL_BSDF_ANISO_GGX, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv, first); break;
+                    case DL_BSDF_CONDUCTOR: k_shade<DL_BSDF_CONDUCTOR, true><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv, first); break;
                 }
                 continue;
             }
             switch (t) {
-                case DL_BSDF_DIFFUSE: k_shade<DL_BSDF_DIFFUSE, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                case DL_BSDF_MIRROR: k_shade<DL_BSDF_MIRROR, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                case DL_BSDF_DIELECTRIC: k_shade<DL_BSDF_DIELECTRIC, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                case DL_BSDF_MICROFACET: k_shade<DL_BSDF_MICROFACET, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                case DL_BSDF_ANISO_GGX: k_shade<DL_BSDF_ANISO_GGX, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
-                case DL_BSDF_CONDUCTOR: k_shade<DL_BSDF_CONDUCTOR, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv); break;
+                case DL_BSDF_DIFFUSE: k_shade<DL_BSDF_DIFFUSE, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv, first); break;
+                case DL_BSDF_MIRROR: k_shade<DL_BSDF_MIRROR, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv, first); break;
+                case DL_BSDF_DIELECTRIC: k_shade<DL_BSDF_DIELECTRIC, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv, first); break;
+                case DL_BSDF_MICROFACET: k_shade<DL_BSDF_MICROFACET, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv, first); break;
+                case DL_BSDF_ANISO_GGX: k_shade<DL_BSDF_ANISO_GGX, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv, first); break;
+                case DL_BSDF_CONDUCTOR: k_shade<DL_BSDF_CONDUCTOR, false><<<stage_grid, kStageBlock, 0, c->stream>>>(pp, sv, rs->b, cur, gv, first); break;
             }
         }
         DL_TRY(tm.stop());
@@ -1393,7 +1401,7 @@ dl_status run_bounces(Ctx* c, RenderState* rs, const PassParams& pp, const Scene
         if (live == 0) break;
         uint32_t len[kTypes];
         for (int t = 0; t < kTypes; t++) len[t] = s.mat[cur][t];
-        DL_TRY(bounce(cur, len));
+        DL_TRY(bounce(cur, len, k == 0));
     }
     return DL_OK;
 }
