@@ -66,17 +66,18 @@ OCC_BYTES = 1  # shadow query result
 # a queue entry (4); the first-hit stage reads the queue
 # entry, hit and direction and writes radiance and a material-queue entry;
 # a shading vertex reads its queue entry, hit, direction, throughput,
-# radiance, aux, pcg32 state and sample index (a path's first vertex: not
+# aux, pcg32 state and sample index (a path's first vertex: not
 # throughput and aux, which are implied) and writes the shadow ray's
 # origin and direction and its NEE term, the continuation ray, throughput,
-# radiance, pcg32 state, aux and two queue entries; its post-processing
+# pcg32 state, aux and two queue entries (radiance only when an emitter
+# sample that casts no shadow ray adds to it); its post-processing
 # reads aux, radiance, NEE term, occlusion byte, throughput, hit, direction,
 # pcg32 state and sample index and writes radiance, aux, throughput, pcg32
 # state and a queue entry; the film gather reads each sample's position and
 # radiance once and writes a pixel per 64 samples.
 PATH_STATE_BYTES = {"raygen": 2 * 16 + 8 + 4 + 8 + 1 + 4,                     # 57
                     "first_hit": 4 + 2 * 16 + 16 + 4,                          # 56
-                    "shade": (4 + 5 * 16 + 8 + 4) + (8 * 16 + 8 + 2 * 4),      # 96 read + 144 written
+                    "shade": (4 + 4 * 16 + 8 + 4) + (7 * 16 + 8 + 2 * 4),      # 80 read + 128 written
                     "post": (4 + 6 * 16 + 1 + 8 + 4) + (3 * 16 + 8 + 4),       # 113 read + 60 written
                     "film": 8 + 16 + 16 / 64.0,
                     "first_vertex_skip": 2 * 16}                               # per path: an undercount

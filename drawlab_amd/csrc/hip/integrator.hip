@@ -552,7 +552,6 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                 }
                 vec3 thr = f3(th4);
                 float eta = th4.w;
-                vec3 rad = f3(rs.rad[p]);
                 Pcg32 rng;
                 rng.state = rs.rng[p];
                 rng.inc = ((uint64_t)rs.seq[p] << 1u) | 1u;
@@ -604,7 +603,9 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                         flags |= kFlagNee;
                         want_shadow = true;
                     } else {
-                        rad = add(rad, contrib);
+                        // (the path's radiance is read and written only here:
+                        // post reads it next)
+                        rs.rad[p] = f4(add(f3(rs.rad[p]), contrib), 0.f);
                     }
                 }
                 // ---- BSDF sampling
@@ -688,7 +689,6 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                     }
                 }
                 rs.thr[p] = f4(thr, eta);
-                rs.rad[p] = f4(rad, 0.f);
                 rs.rng[p] = rng.state;
                 aux.x = __float_as_int(prev_pdf);
                 aux.y = (int)flags;
