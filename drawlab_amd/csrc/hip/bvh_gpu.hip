@@ -447,10 +447,12 @@ dl_status build_bvh_lbvh(Ctx* c, dl_accel_stats* stats) {
 namespace {
 
 constexpr int kPlocRadius = 24;
-// a visit of the 4-wide node that the binary node becomes costs about half
-// a triangle test (measured on the 2M-triangle workload: 0.5 beats 0.7, 1.0
-// and 1.5 by 1-4%)
-constexpr float kSahNode = 0.5f, kSahTri = 1.0f;
+// a visit of the 4-wide node that the binary node becomes costs about a
+// triangle test, the traversal testing a leaf's triangles two at a time
+// (measured on the 2M-triangle workload: 1.0 beats 0.35, 0.5, 0.7, 1.5 and
+// 2.0 by 1-3%, with 8.48 node visits and 3.16 triangle tests per ray
+// against 8.87 and 2.59 at 0.5)
+constexpr float kSahNode = 1.0f, kSahTri = 1.0f;
 
 __device__ __forceinline__ float half_area(const float4& lo, const float4& hi) {
     const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
