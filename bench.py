@@ -68,7 +68,8 @@ OCC_BYTES = 1  # shadow query result
 # a shading vertex reads its queue entry, hit, direction, throughput,
 # aux, pcg32 state and sample index (a path's first vertex: not
 # throughput and aux, which are implied) and writes the shadow ray's
-# origin and direction and its NEE term, the continuation ray, throughput,
+# direction and its NEE term, the continuation ray (whose origin the shadow
+# ray shares), throughput,
 # pcg32 state, aux and two queue entries (radiance only when an emitter
 # sample that casts no shadow ray adds to it); its post-processing
 # reads aux, radiance, NEE term, occlusion byte, throughput, hit, direction,
@@ -77,7 +78,7 @@ OCC_BYTES = 1  # shadow query result
 # radiance once and writes a pixel per 64 samples.
 PATH_STATE_BYTES = {"raygen": 2 * 16 + 8 + 4 + 8 + 1 + 4,                     # 57
                     "first_hit": 4 + 2 * 16 + 16 + 4,                          # 56
-                    "shade": (4 + 4 * 16 + 8 + 4) + (7 * 16 + 8 + 2 * 4),      # 80 read + 128 written
+                    "shade": (4 + 4 * 16 + 8 + 4) + (6 * 16 + 8 + 2 * 4),      # 80 read + 112 written
                     "post": (4 + 6 * 16 + 1 + 8 + 4) + (3 * 16 + 8 + 4),       # 113 read + 60 written
                     "film": 8 + 16 + 16 / 64.0,
                     "first_vertex_skip": 2 * 16}                               # per path: an undercount

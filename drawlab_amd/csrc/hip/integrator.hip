@@ -62,7 +62,7 @@ struct Counters {
 
 // Device buffers of a render (plain pointers: passed by value to kernels).
 struct RenderBufs {
-    float4 *ray_o, *ray_d, *hit, *sh_o, *sh_d;
+    float4 *ray_o, *ray_d, *hit, *sh_d;  // a shadow ray starts at ray_o
     float4 *thr, *rad, *nee;
     int4* aux;
     uint64_t* rng;
@@ -309,7 +309,9 @@ __global__ __launch_bounds__(kStageBlock) void k_raygen(PassParams pp, dl_camera
 }
 
 // Queued path rays: work slot i holds path q[i], whose ray lives in
-// ray_o/ray_d (shadow: sh_o/sh_d); the traversal's entries are path indices.
+// ray_o/ray_d (shadow: ray_o/sh_d -- a shading vertex's shadow ray and
+// continuation start at the same point); the traversal's entries are path
+// indices.
 struct QueueIO {
     const uint32_t* __restrict__ q;
     const float4* __restrict__ ro;
@@ -360,7 +362,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_queue(BvhView bvh, Rende
     const uint32_t n = rs.cnt->shadow;
     const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x, G = gridDim.x * kTraceBlock;
     LaneStack stk{lds, (int)threadIdx.x, kTraceBlock, ovf + g, G};
-    QueueIO io{rs.shq, rs.sh_o, rs.sh_d, nullptr, rs.occ, rs.sid, rs.roots};
+    QueueIO io{rs.shq, rs.ray_o, rs.sh_d, nullptr, rs.occ, rs.sid, rs.roots};
     TraceCounters cnt;
     traverse_persistent<true, COUNT>(bvh, n, &rs.cnt->work_shadow, stk, io, cnt);
     if (COUNT) {
@@ -598,8 +600,7 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                     const vec3 contrib = mul(mul(scale(thr, mis), bv), spectrum);
                     if (csum(spectrum) > 0) {
                         rs.nee[p] = f4(contrib, 0.f);
-                        rs.sh_o[p] = f4(its.p, pp.eps);
-                        rs.sh_d[p] = f4(ds.d, ds.dist - pp.eps);
+                        rs.sh_d[p] = f4(ds.d, ds.dist - pp.eps);  // from ray_o, written below
                         flags |= kFlagNee;
                         want_shadow = true;
                     } else {
@@ -651,8 +652,9 @@ __global__ __launch_bounds__(kStageBlock) void k_shade(PassParams pp, SceneView 
                 thr = mul(thr, fr);
                 eta *= br.eta;
                 if (bsdf_is_diffuse(type)) flags |= kFlagDiffuse;
+                // the origin of the continuation and of the shadow ray
+                if (want_shadow || !is_zero(thr)) rs.ray_o[p] = f4(its.p, pp.eps);
                 if (!is_zero(thr)) {
-                    rs.ray_o[p] = f4(its.p, pp.eps);
                     rs.ray_d[p] = f4(to_world(its.sh, br.wo), INFINITY);
                     flags |= kFlagCont;
                     want_cont = true;
@@ -1188,7 +1190,6 @@ dl_status ensure_state(Ctx* c, uint32_t P, uint32_t pixels) {
     DL_TRY(alloc(c, rs, &rs->b.ray_o, P));
     DL_TRY(alloc(c, rs, &rs->b.ray_d, P));
     DL_TRY(alloc(c, rs, &rs->b.hit, P));
-    DL_TRY(alloc(c, rs, &rs->b.sh_o, P));
     DL_TRY(alloc(c, rs, &rs->b.sh_d, P));
     DL_TRY(alloc(c, rs, &rs->b.thr, P));
     DL_TRY(alloc(c, rs, &rs->b.rad, P));
@@ -1215,7 +1216,7 @@ constexpr uint32_t kGuideRecords = dl_sizes::kGuideRecordSlots;  // training rec
 static_assert(kGRecFloats == dl_sizes::kGuideRecordFloats && kGMix == dl_sizes::kGuideMixFloats,
               "guided buffer sizes the render service reserves");
 // the per-path bytes ensure_state allocates, as the render service reserves them
-static_assert(9 * sizeof(float4) + sizeof(uint64_t) + sizeof(uint32_t) + sizeof(float2) + 2 * sizeof(uint8_t) +
+static_assert(8 * sizeof(float4) + sizeof(uint64_t) + sizeof(uint32_t) + sizeof(float2) + 2 * sizeof(uint8_t) +
                       2 * kTypes * sizeof(uint32_t) + 3 * sizeof(uint32_t) ==
                   dl_sizes::kStateBytesPerPath &&
               kTypes == dl_sizes::kMatQueueSegs && sizeof(float4) + 3 * sizeof(float) == dl_sizes::kFilmBytesPerPixel,

@@ -8,14 +8,14 @@
 namespace dl_sizes {
 
 // Path state, per path of a pass: ray origin and direction, hit record,
-// shadow-ray origin and direction, throughput, radiance, pending NEE term
-// and the aux word (one 16-byte vector each), the pcg32 state (8), the
+// shadow-ray direction (a shadow ray starts at the ray origin), throughput,
+// radiance, pending NEE term and the aux word (one 16-byte vector each), the pcg32 state (8), the
 // sample index (4), the film position (8), the occlusion byte (1), the
 // scene index (1), two double-buffered material queues of kMatQueueSegs
 // segments (4 bytes per path per segment each), and the ray, shadow and
 // emission queues (4 each).
 constexpr int kMatQueueSegs = 8;
-constexpr uint64_t kStateBytesPerPath = 9 * 16 + 8 + 4 + 8 + 1 + 1 + 2 * kMatQueueSegs * 4 + 3 * 4;  // 242
+constexpr uint64_t kStateBytesPerPath = 8 * 16 + 8 + 4 + 8 + 1 + 1 + 2 * kMatQueueSegs * 4 + 3 * 4;  // 226
 
 // Film: float4 accumulation (sum w L, sum w) + resolved RGB per pixel.
 constexpr uint64_t kFilmBytesPerPixel = 16 + 12;

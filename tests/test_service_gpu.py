@@ -134,13 +134,27 @@ def test_cancel_and_errors():
         svc.stop()
 
 
+def _state_bytes_per_path():
+    """Path-state bytes per path that the render service reserves (the
+    constant of drawlab_amd/csrc/render_sizes.h)."""
+    import os
+    import re
+    text = open(os.path.join(os.path.dirname(drawlab_amd.__file__), "csrc", "render_sizes.h")).read()
+    segs = int(re.search(r"kMatQueueSegs = (\d+);", text).group(1))
+    expr = re.search(r"kStateBytesPerPath = ([^;]+);", text).group(1).replace("kMatQueueSegs", str(segs))
+    assert re.fullmatch(r"[\d\s+*]+", expr), expr
+    return eval(expr)
+
+
 def test_memory_budget_is_never_exceeded():
     fixed = _fixed_context_bytes()
     cbox = scenes.cornell_box(96, 96, spp=8, max_depth=8, rr_depth=3)
-    # room for the two lanes' fixed buffers and one running job, not two
-    # (the service's estimate: scene + state + slack, see service.cpp)
-    one_job = (4 << 20) + 96 * 96 * 8 * 240 + 96 * 96 * 28 + (4 << 20) + 64 * 36 + 40 * 384
-    budget = 2 * fixed + int(1.5 * one_job)
+    # room for the two lanes' fixed buffers and one running job, not two,
+    # whether on two lanes or sharing a pass (the service's estimate: scene +
+    # state + slack, see service.cpp; two jobs in one pass hold about 1.7
+    # times one job's bytes)
+    one_job = (4 << 20) + 96 * 96 * 8 * _state_bytes_per_path() + 96 * 96 * 28 + (4 << 20) + 64 * 36 + 40 * 384
+    budget = 2 * fixed + int(1.25 * one_job)
     svc = drawlab_amd.Service(lanes=2, budget_bytes=budget)
     try:
         s = svc.add_scene(cbox)
