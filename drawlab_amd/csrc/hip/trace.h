@@ -130,10 +130,11 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
     if (__lane_id() == 0 && sum != 0) atomicAdd(dst, sum);
 }
 
-// Relative VALU cost of the traversal's two step kinds, triangle test :
-// node visit (their instruction counts are about 130 : 300; 2 : 5 measured
-// best on the 2M-triangle displaced-sphere render, 1 : 3 and 1 : 2 within
-// 0.4%).
+// Relative VALU cost of the traversal's two step kinds, triangle step :
+// node visit (2 : 5 measured best on the 2M-triangle displaced-sphere
+// render with one triangle per step, whose instruction count to a node
+// visit's is about 130 : 300; with two triangles per step 1 : 2, 3 : 5 and
+// 4 : 5 measure within 0.5% of it).
 constexpr uint32_t kTriCost = 2, kNodeCost = 5;
 
 // Wave-aggregated claim of the next work item: one atomic per wave.
