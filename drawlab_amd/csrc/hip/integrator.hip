@@ -121,17 +121,14 @@ __device__ __forceinline__ void block_append(uint32_t* counters, const uint32_t 
     const int lane = __lane_id();
     const int wave = threadIdx.x >> 6;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    unsigned long long m[NQ][N];
-    uint32_t pre[NQ][N];
+    // the ballots are taken twice (for the counts, then for the slots)
+    // rather than kept: NQ * N masks and prefixes would not fit the scalar
+    // registers
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
         uint32_t tot = 0;
 #pragma unroll
-        for (int u = 0; u < N; u++) {
-            m[q][u] = __ballot((bits[u] >> q) & 1u);
-            pre[q][u] = tot;
-            tot += (uint32_t)__popcll(m[q][u]);
-        }
+        for (int u = 0; u < N; u++) tot += (uint32_t)__popcll(__ballot((bits[u] >> q) & 1u));
         if (lane == 0) s_cnt[wave][q] = tot;
     }
     __syncthreads();
@@ -148,10 +145,16 @@ __device__ __forceinline__ void block_append(uint32_t* counters, const uint32_t 
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < NQ; q++)
+    for (int q = 0; q < NQ; q++) {
+        uint32_t pre = s_base[wave][q];
 #pragma unroll
-        for (int u = 0; u < N; u++)
-            if ((bits[u] >> q) & 1u) emit(q, u, s_base[wave][q] + pre[q][u] + (uint32_t)__popcll(m[q][u] & lt));
+        for (int u = 0; u < N; u++) {
+            const bool want = (bits[u] >> q) & 1u;
+            const unsigned long long m = __ballot(want);
+            if (want) emit(q, u, pre + (uint32_t)__popcll(m & lt));
+            pre += (uint32_t)__popcll(m);
+        }
+    }
     __syncthreads();
 }
 
