@@ -218,6 +218,17 @@ dl_status dl_hip_upload_scene(void* h, const dl_scene_desc* sd) {
     release_current(c);
     if (sd->num_meshes == 0) return set_error(c, DL_ERR_INVALID, "scene has no meshes");
     DL_TRY(check_camera(c, sd->camera));
+    {
+        // the reconstruction filter as dl_film_filter makes it: the film
+        // gathers size their footprints from the radius and border, and
+        // index the table with |offset| * lookup <= radius * lookup <= 32
+        const dl_film_desc& f = sd->film;
+        bool ok = std::isfinite(f.radius) && f.radius > 0.0f && f.radius <= 32.0f &&
+                  f.border == (int32_t)std::ceil(f.radius - 0.5f) && std::isfinite(f.lookup) && f.lookup > 0.0f &&
+                  f.lookup * f.radius <= 32.001f;
+        for (int k = 0; k < 33 && ok; k++) ok = std::isfinite(f.table[k]);
+        if (!ok) return set_error(c, DL_ERR_INVALID, "bad reconstruction filter");
+    }
     for (uint32_t i = 0; i < sd->num_materials; i++) {
         if (sd->materials[i].type < 0 || sd->materials[i].type >= DL_BSDF_COUNT)
             return set_error(c, DL_ERR_INVALID, "unknown material type");

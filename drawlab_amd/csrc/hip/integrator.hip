@@ -1109,9 +1109,13 @@ static void film_gather(const PassParams& pp, uint32_t k, const dl_film_desc& fi
     // floor(r + 1/2) of its own pixel px.  P -/+ r is rounded once in
     // binary32 (block-relative, |P| < 64: by at most 2^-19), which can reach
     // one pixel further when r + 1/2 lies just below an integer.
+    // (the radius is finite and positive: scene upload checks it)
     const float reach = film.radius + 0.5f;
-    int R = (int)floorf(reach);
-    if (reach - (float)R > 0.9999f) R++;
+    int R = kFilmHalo;
+    if (reach < (float)kFilmHalo) {
+        R = (int)floorf(reach);
+        if (reach - (float)R > 0.9999f) R++;
+    }
     const dim3 fgrid((pp.W + kFilmTile - 1) / kFilmTile, (pp.H + kFilmTile - 1) / kFilmTile);
     const int threads = kFilmTile * kFilmTile;
     switch (R) {

@@ -203,7 +203,9 @@ void dl_film_filter(int type, float radius, float p0, float p1, dl_film_desc* ou
     if (type == 1) radius = 0.5f;  // BoxFilter
     if (type == 2) radius = 1.0f;  // TentFilter
     out->radius = radius;
-    out->border = (int)std::ceil(radius - 0.5f);
+    // (a radius that is not finite or is far beyond any filter's stays
+    // unconverted: scene upload refuses it)
+    out->border = std::isfinite(radius) && std::fabs(radius) < 1e6f ? (int)std::ceil(radius - 0.5f) : 0;
     for (int i = 0; i < 32; ++i) {
         float x = (radius * i) / 32;
         float val;

@@ -55,6 +55,12 @@ def test_malformed_scenes_and_parameters(backend):
         (lambda x: setattr(x, "camera", dataclasses.replace(x.camera, width=0)), "bad camera size"),
         (lambda x: setattr(x, "camera", dataclasses.replace(x.camera, width=65536, height=65537)),
          "more than 2\\^27 pixels"),
+        (lambda x: setattr(x, "rfilter", RFilter(drawlab_amd.FILTER_GAUSSIAN, float("nan"), 0.5)),
+         "bad reconstruction filter"),
+        (lambda x: setattr(x, "rfilter", RFilter(drawlab_amd.FILTER_GAUSSIAN, 100.0, 0.5)),
+         "bad reconstruction filter"),
+        (lambda x: setattr(x, "rfilter", RFilter(drawlab_amd.FILTER_GAUSSIAN, -1.0, 0.5)),
+         "bad reconstruction filter"),
     ]:
         bad = _one_triangle()
         mutate(bad)
