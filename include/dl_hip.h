@@ -142,7 +142,9 @@ typedef struct dl_camera_desc {
 } dl_camera_desc;
 
 /* ImageBlock filter tabulation (src/core/bitmap/block.cpp:12-27):
-   table[i] = filter(radius * i / 32) for i < 32, table[32] = 0. */
+   table[i] = filter(radius * i / 32) for i < 32, table[32] = 0.
+   dl_hip_upload_scene accepts radii in (0, 32] with border and lookup as
+   dl_film_filter sets them (DL_ERR_INVALID "bad reconstruction filter"). */
 typedef struct dl_film_desc {
     float radius;
     int32_t border;            /* ceil(radius - 0.5) */
