@@ -220,7 +220,7 @@ def short_name(name):
 def summarize_stats(d, out_md, command, title=""):
     path = sorted(glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True))[0]
     rows = list(csv.DictReader(open(path)))
-    title = title or "%s bench (1920x1080, 128 spp per pass, max depth 16)" % WORKLOAD
+    title = title or "%s bench (1920x1080, 256 spp per pass, max depth 16)" % WORKLOAD
     lines = ["# Kernel time: %s" % title, "",
              "Command (repository root, MI355X):", "", "```", command, "```", "",
              "| kernel | calls | total ms | avg us | min us | max us | % |", "|---|---:|---:|---:|---:|---:|---:|"]
@@ -252,7 +252,7 @@ def summarize_pmc(fetch_dir, write_dir, out_json, copy_bytes, commands):
     write = w[TRACE_KERNEL][0] * 1024.0 * cw
     out = {
         "kernel": TRACE_KERNEL,
-        "workload": WORKLOAD + " (128 spp per pass)",
+        "workload": WORKLOAD + " (256 spp per pass)",
         "dispatches": f[TRACE_KERNEL][1],
         "fetch_kib_per_dispatch": f[TRACE_KERNEL][0],
         "write_kib_per_dispatch": w[TRACE_KERNEL][0],
@@ -291,7 +291,7 @@ def summarize_counters(dirs, out_md, command, kernels):
             s[0] += w * float(r["Counter_Value"])
             s[1] += w
             s[2] += 1
-    lines = ["# Hardware counters: %s bench (1920x1080, 128 spp per pass)" % WORKLOAD, "",
+    lines = ["# Hardware counters: %s bench (1920x1080, 256 spp per pass)" % WORKLOAD, "",
              "Commands (repository root, MI355X):", "", "```", command, "```", "",
              "Dispatch-time-weighted means over each kernel's dispatches.  VALUBusy: % of GPU time the vector ALUs "
              "issue; VALUUtilization: % of active lanes per VALU instruction; OccupancyPercent: resident waves as "
@@ -812,11 +812,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default: 2 on the displaced sphere, the 256-spp frame; else 4)")
+                    help="timed steps (default: 1 on the displaced sphere, the 256-spp frame; else 4)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp-per-step", type=int, default=None,
-                    help="samples per pixel of a step, one wavefront pass (default: 128 on the displaced "
-                         "sphere, 2^28 paths at 1080p; else 64)")
+                    help="samples per pixel of a step, one wavefront pass (default: 256 on the displaced "
+                         "sphere, 2^29 paths at 1080p; else 64)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -852,15 +852,15 @@ def main():
     ap.add_argument("--rate-sweep", type=float, nargs="*", default=[0.6, 0.8, 1.0, 1.2, 1.4],
                     help="service workload: arrival rates (x one-at-a-time capacity) of the p99 sweep")
     args = ap.parse_args()
-    # the 1080p displaced sphere renders its 256 spp in two passes of 2^28
-    # paths (about 60 GB of path state): larger passes amortise the sparse
-    # last bounces (128 spp per pass measured 4% faster than 64 there; the
-    # smaller scenes run faster at 64)
+    # the 1080p displaced sphere renders its 256 spp in one pass of 2^29
+    # paths (about 120 GB of path state): larger passes amortise the sparse
+    # last bounces (128 spp per pass measured 4% faster than 64 there, 256
+    # another 3-5%; the smaller scenes run faster at 64)
     big = args.workload == "displaced_sphere" and args.scene == "displaced_sphere"
     if args.spp_per_step is None:
-        args.spp_per_step = 128 if big else 64
+        args.spp_per_step = 256 if big else 64
     if args.steps is None:
-        args.steps = 2 if big else 4
+        args.steps = 1 if big else 4
     if args.summarize_stats:
         summarize_stats(args.summarize_stats[0], args.summarize_stats[1], args.command, args.title)
         return

@@ -1428,17 +1428,17 @@ dl_status render(Ctx* c, const dl_render_params* prm, float* rgb_host, dl_render
     (void)hipSetDevice(c->device);
     const uint32_t W = (uint32_t)c->cam.width, H = (uint32_t)c->cam.height, WH = W * H;
     // Default pass: at most 2^27 paths (64 spp at 1920x1080, ~30 GB of path
-    // state); a caller may ask for up to 2^28 (128 spp at 1080p, ~60 GB).
+    // state); a caller may ask for up to 2^29 (256 spp at 1080p, ~120 GB).
     // The last bounces of a pass hold few paths but still cost a traversal
     // of their slowest ray each, so larger passes amortise that tail
-    // (measured at 1080p on the 2M-triangle displaced sphere: 32, 64 and 128
-    // spp per pass give 1053, 1107 and -- with later kernels -- 1349 against
-    // 1407 Mpaths/s for the last two; at 1280x720 on the MIS plates 128 spp
-    // per pass is 2% slower than 64).
+    // (measured at 1080p on the 2M-triangle displaced sphere: 32 and 64 spp
+    // per pass give 1053 and 1107 Mpaths/s; with later kernels 64, 128 and
+    // 256 give 1349, 1407 and 1448-1493; at 1280x720 on the MIS plates 128
+    // spp per pass is 2% slower than 64).
     uint32_t k = prm->spp_per_pass > 0 ? (uint32_t)prm->spp_per_pass : std::max(1u, (128u << 20) / WH);
     k = std::min<uint32_t>(k, (uint32_t)prm->spp);
     if (paths_sample >= 0) k = 1;
-    if ((uint64_t)WH * k > (1ull << 28)) return set_error(c, DL_ERR_INVALID, "pass too large");
+    if ((uint64_t)WH * k > (1ull << 29)) return set_error(c, DL_ERR_INVALID, "pass too large");
     const uint32_t cap = WH * k;
     DL_TRY(ensure_state(c, cap, WH));
     RenderState* rs = (RenderState*)c->rs;

@@ -190,7 +190,7 @@ typedef struct dl_render_params {
     int32_t max_depth;         /* -1 = unlimited (reference behaviour) */
     int32_t rr_depth;          /* RR when bounce index >= rr_depth; 4 = reference */
     int32_t spp_per_pass;      /* samples per pixel per wavefront pass (0 = auto: at most 2^27 paths;
-                                  at most 2^28 paths when given) */
+                                  at most 2^29 paths when given) */
     int32_t first_sample;      /* index of the first sample (progressive rendering) */
     int32_t flags;             /* DL_RENDER_* */
     int32_t guide_steps;       /* DL_RENDER_GUIDED training: Adam steps after each pass (0 = 16 until
