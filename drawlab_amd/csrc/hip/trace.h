@@ -137,17 +137,6 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
 // 4 : 5 measure within 0.5% of it).
 constexpr uint32_t kTriCost = 2, kNodeCost = 5;
 
-// Wave-aggregated claim of the next work item: one atomic per wave.
-__device__ __forceinline__ uint32_t claim_next(uint32_t* counter, bool want) {
-    const unsigned long long mask = __ballot(want);
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if (want && lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-}
-
 // Persistent traversal with per-lane ray refill.  Every lane of a wave stays
 // in the loop until the whole wave has run out of rays; lanes whose rays
 // finished start their next rays as soon as enough of them wait (lane g
@@ -177,7 +166,7 @@ __device__ __forceinline__ uint32_t claim_next(uint32_t* counter, bool want) {
 template <bool ANY, bool COUNT, class IO>
 __device__ __forceinline__ void traverse_persistent(const BvhView& bvh, uint32_t n, uint32_t* work, LaneStack& stk,
                                                     IO& io, TraceCounters& total) {
-    (void)work;
+    (void)work;  // a claim counter the static-stride refill does not need (callers still zero it)
     const uint32_t G = gridDim.x * blockDim.x;
     uint32_t next = blockIdx.x * blockDim.x + threadIdx.x;
     // the IO entry of the lane's next slot, fetched one refill ahead (a
