@@ -5,9 +5,11 @@ Workload: the 2,097,152-triangle displaced sphere (GGX conductor) with a
 20,480-triangle glass sphere, a diffuse ground and a 1024x512 sky + sun
 environment map, rendered at 1920x1080 with max depth 16, 256 spp.  One step
 is one wavefront pass of the unidirectional MIS path tracer over the whole
-frame at 64 samples per pixel (132,710,400 paths, dl_hip_render's default
-pass size at this resolution), accumulated progressively; the default 4
-steps are the workload's 256 spp.
+frame at 256 samples per pixel (530,841,600 paths, the largest pass
+dl_hip_render takes, ~120 GB of path state), accumulated progressively; the
+default single step is the workload's 256 spp (--spp-per-step and --steps
+split it into smaller passes; the other scenes default to 64 spp per step
+and 4 steps).
 
 Prints one JSON line: Mpaths/s over the timed steps, the roofline of the
 dominant kernel (closest-hit traversal, k_trace_queue) and the CPU oracle's
